@@ -1,0 +1,365 @@
+/*
+ * gxy.h — C-ABI of the MI355X (gfx950) seed -> HSP -> gapped hot path.
+ *
+ * Every entry point replaces one public Rust function of lastz-gxy that the
+ * reference driver (src/driver.rs) calls on its hot path; the comment above
+ * each declaration names the function it replaces (file:line, relative to the
+ * reference crate).  Plain pointers and sizes only: no C++ or torch types.
+ *
+ * Conventions
+ *  - Return value: 0 on success, a negative GXY_E* code on failure.  No
+ *    exception or abort crosses the ABI; the message of the last failure on a
+ *    context is returned by gxy_last_error().
+ *  - Option<T> in Rust becomes an out-flag `found` (1 = Some, 0 = None).
+ *  - Inputs are borrowed for the duration of a call.  Objects and arrays the
+ *    library returns are freed with the matching gxy_*_free / gxy_free call,
+ *    never with the caller's allocator.
+ *  - One gxy_ctx owns one HIP stream and its scratch buffers.  Calls on one
+ *    context are serialised by the caller (one context per host thread);
+ *    distinct contexts may be used concurrently.
+ *  - Sequences use the reference's PackedSeq layout (src/sequences.rs:128-141):
+ *    2-bit codes A=0 C=1 G=2 T=3, four bases per byte, base i at bits
+ *    2*(i&3) of byte i>>2; `valid` and `mask` are one bit per base, base i at
+ *    bit (i&7) of byte i>>3.  N/ambiguous bases have code 0, valid 0, mask 0.
+ */
+#ifndef GXY_H
+#define GXY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GXY_OK 0
+#define GXY_EINVAL (-1)   /* invalid argument (bad size, null pointer, ...)  */
+#define GXY_ENOMEM (-2)   /* host or device allocation failed                */
+#define GXY_EDEVICE (-3)  /* HIP runtime / kernel launch error               */
+#define GXY_EUNSUP (-4)   /* feature outside the hot path (chain, tweener)   */
+#define GXY_ERANGE (-5)   /* input exceeds a documented limit                */
+
+/* ---- POD views ---------------------------------------------------------- */
+
+/* Borrowed view of a PackedSeq (src/sequences.rs:128-141). */
+typedef struct gxy_seq {
+    const uint8_t *codes; /* (len+3)/4 bytes */
+    const uint8_t *valid; /* (len+7)/8 bytes */
+    const uint8_t *mask;  /* (len+7)/8 bytes */
+    uint64_t len;
+} gxy_seq;
+
+/* SeedPattern (src/seeds.rs:20-23): `length` positions (1..32); bit k of
+ * `care` is set when offset k is a care position. */
+typedef struct gxy_seed_pattern {
+    uint32_t length;
+    uint32_t care;
+} gxy_seed_pattern;
+
+/* ScoringMatrix (src/scoring.rs:32-40): sub[target][query]. */
+typedef struct gxy_scoring {
+    int32_t sub[4][4];
+    int32_t gap_open;
+    int32_t gap_extend;
+    int32_t ambig;
+} gxy_scoring;
+
+/* Hsp (src/hsp.rs:19-24), same field order. */
+typedef struct gxy_hsp {
+    uint32_t t_start;
+    uint32_t q_start;
+    uint32_t length;
+    int32_t score;
+} gxy_hsp;
+
+/* HspParams (src/hsp.rs:46-49). */
+typedef struct gxy_hsp_params {
+    int32_t x_drop;
+    int32_t hsp_threshold;
+} gxy_hsp_params;
+
+/* SearchParams (src/seed_search.rs:19-30).  `entropy_threshold` is used
+ * only when `entropy_enabled` is non-zero (Option<i32>). */
+typedef struct gxy_search_params {
+    uint64_t step;
+    gxy_hsp_params hsp;
+    uint32_t transitions;
+    int32_t entropy_enabled;
+    int32_t entropy_threshold;
+} gxy_search_params;
+
+/* Anchor (src/anchor.rs:16-20). */
+typedef struct gxy_anchor {
+    uint32_t t_pos;
+    uint32_t q_pos;
+    gxy_hsp source;
+} gxy_anchor;
+
+/* GappedParams (src/gapped_extend.rs:28-31). */
+typedef struct gxy_gapped_params {
+    int32_t y_drop;
+    int32_t gapped_threshold;
+} gxy_gapped_params;
+
+/* EditOp (src/edit_script.rs:17-22) run: op 0 = Match (M), 1 = InsertQuery
+ * (I, gap in target), 2 = DeleteQuery (D, gap in query). */
+#define GXY_OP_MATCH 0u
+#define GXY_OP_INSERT 1u
+#define GXY_OP_DELETE 2u
+typedef struct gxy_edit_run {
+    uint32_t op;
+    uint32_t n;
+} gxy_edit_run;
+
+/* GappedAlignment (src/gapped_extend.rs:55-63).  The edit script is the run
+ * range [runs_offset, runs_offset + n_runs) of the owning result's run array. */
+typedef struct gxy_alignment {
+    uint32_t t_start;
+    uint32_t q_start;
+    uint32_t t_len;
+    uint32_t q_len;
+    int32_t score;
+    uint32_t n_runs;
+    uint64_t runs_offset;
+    gxy_anchor source;
+} gxy_alignment;
+
+/* Result of a batched gapped extension: found[k] says whether alignment k is
+ * Some; when it is None, alns[k] is zeroed and owns no runs. */
+typedef struct gxy_alignment_set {
+    uint64_t n;
+    gxy_alignment *alns;
+    uint8_t *found;
+    uint64_t n_runs;
+    gxy_edit_run *runs;
+    uint64_t cells; /* DP cells evaluated over the whole batch (both sides) */
+} gxy_alignment_set;
+
+/* Counters of one search call (algorithmic work, for GCUPS and roofline). */
+typedef struct gxy_search_stats {
+    uint64_t query_windows;  /* valid query seed windows                     */
+    uint64_t lookups;        /* variant-word lookups                         */
+    uint64_t hits;           /* seed hits returned by the lookups            */
+    uint64_t extended;       /* hits extended (GPU: every hit, speculatively)*/
+    uint64_t columns;        /* x-drop columns evaluated                     */
+    uint64_t passing;        /* extensions at or above hsp_threshold         */
+    uint64_t emitted;        /* HSPs after diagonal admission, before dedup  */
+    uint64_t tie_pairs;      /* adjacent equal-(diag,t_start) unequal HSPs   */
+} gxy_search_stats;
+
+/* ---- opaque handles ---------------------------------------------------- */
+typedef struct gxy_ctx gxy_ctx;           /* HIP stream + scratch + error   */
+typedef struct gxy_dseq gxy_dseq;         /* device-resident PackedSeq      */
+typedef struct gxy_postable gxy_postable; /* device-resident PosTable       */
+
+/* ---- context ----------------------------------------------------------- */
+/* Create a context on HIP device `device` (its own stream). */
+int gxy_ctx_create(int device, gxy_ctx **out);
+void gxy_ctx_destroy(gxy_ctx *ctx);
+/* Message of the last failed call on `ctx` ("" if none).  With ctx == NULL,
+ * the message of the last failed gxy_ctx_create on this thread. */
+const char *gxy_last_error(const gxy_ctx *ctx);
+/* Wait for all work queued on the context's stream. */
+int gxy_ctx_synchronize(gxy_ctx *ctx);
+/* Library version string. */
+const char *gxy_version(void);
+
+/* ---- sequences ---------------------------------------------------------- */
+/* PackedSeq::from_ascii (src/sequences.rs:164-206) into caller buffers sized
+ * as in gxy_seq.  Host-only utility; needs no context. */
+int gxy_pack_ascii(const char *ascii, uint64_t len, uint8_t *codes, uint8_t *valid,
+                   uint8_t *mask);
+/* Upload a PackedSeq to device memory (kept resident until gxy_dseq_free). */
+int gxy_dseq_upload(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out);
+/* Device-side PackedSeq::reverse_complement (src/sequences.rs:303-317). */
+int gxy_dseq_reverse_complement(gxy_ctx *ctx, const gxy_dseq *seq, gxy_dseq **out);
+/* Device-side PackedSeq::slice_to_new (src/sequences.rs:277-298). */
+int gxy_dseq_slice(gxy_ctx *ctx, const gxy_dseq *seq, uint64_t start, uint64_t end,
+                   gxy_dseq **out);
+uint64_t gxy_dseq_len(const gxy_dseq *seq);
+/* Copy a device PackedSeq back into caller buffers sized as in gxy_seq. */
+int gxy_dseq_download(gxy_ctx *ctx, const gxy_dseq *seq, uint8_t *codes, uint8_t *valid,
+                      uint8_t *mask);
+void gxy_dseq_free(gxy_dseq *seq);
+
+/* ---- position table ---------------------------------------------------- */
+/* PosTable::build(&PackedSeq, &SeedPattern, step) (src/pos_table.rs:35). */
+int gxy_postable_build(gxy_ctx *ctx, const gxy_seq *target, const gxy_seed_pattern *pattern,
+                       uint64_t step, gxy_postable **out);
+/* Same, from a device-resident target (the table keeps a reference to it:
+ * free the table before the sequence). */
+int gxy_postable_build_dev(gxy_ctx *ctx, const gxy_dseq *target,
+                           const gxy_seed_pattern *pattern, uint64_t step,
+                           gxy_postable **out);
+/* PosTable::prune_hot_words(&mut self, limit) (src/pos_table.rs:113). */
+int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *table, uint32_t limit);
+/* PosTable::lookup(&self, word) -> &[u32] (src/pos_table.rs:102).  Writes up to
+ * `cap` positions into `out`; `*n_out` always receives the full count. */
+int gxy_postable_lookup(gxy_ctx *ctx, const gxy_postable *table, uint64_t word, uint32_t *out,
+                        uint64_t cap, uint64_t *n_out);
+/* PosTable::total_positions (src/pos_table.rs:95). */
+uint64_t gxy_postable_total_positions(const gxy_postable *table);
+/* Copy `starts` (4^weight + 1 entries) and `positions` to host buffers of
+ * the given capacities (either pointer may be NULL to skip it). */
+int gxy_postable_export(gxy_ctx *ctx, const gxy_postable *table, uint32_t *starts,
+                        uint64_t starts_cap, uint32_t *positions, uint64_t positions_cap);
+void gxy_postable_free(gxy_postable *table);
+
+/* ---- seed search -------------------------------------------------------- */
+/* search(&PosTable, &PackedSeq target, &PackedSeq query, &ScoringMatrix,
+ *        &SearchParams) -> Vec<Hsp>  (src/seed_search.rs:45-51).
+ * `target` must be the sequence the table was built from.  Output sorted by
+ * (diagonal, t_start); equal keys keep seed-scan order.  `*out` is freed with
+ * gxy_free.  `stats` may be NULL. */
+int gxy_search(gxy_ctx *ctx, const gxy_postable *table, const gxy_seq *target,
+               const gxy_seq *query, const gxy_scoring *scoring,
+               const gxy_search_params *params, gxy_hsp **out, uint64_t *n_out,
+               gxy_search_stats *stats);
+/* Same with a device-resident query; the target is the table's own. */
+int gxy_search_dev(gxy_ctx *ctx, const gxy_postable *table, const gxy_dseq *query,
+                   const gxy_scoring *scoring, const gxy_search_params *params,
+                   gxy_hsp **out, uint64_t *n_out, gxy_search_stats *stats);
+
+/* ---- ungapped x-drop ---------------------------------------------------- */
+/* extend_hit(&PackedSeq, &PackedSeq, &ScoringMatrix, &HspParams, t_pos, q_pos,
+ *            seed_len) -> Option<Hsp>  (src/hsp.rs:66-74). */
+int gxy_extend_hit(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *query,
+                   const gxy_scoring *scoring, const gxy_hsp_params *params, uint32_t t_pos,
+                   uint32_t q_pos, uint32_t seed_len, gxy_hsp *out, int *found);
+/* Batched extend_hit over n hits of one (target, query) pair. */
+int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *query,
+                         const gxy_scoring *scoring, const gxy_hsp_params *params,
+                         const uint32_t *t_pos, const uint32_t *q_pos, uint64_t n,
+                         uint32_t seed_len, gxy_hsp *out, uint8_t *found);
+
+/* ---- anchor ------------------------------------------------------------- */
+/* choose_anchor(Hsp, &PackedSeq, &PackedSeq, &ScoringMatrix, window) -> Anchor
+ * (src/anchor.rs:25-31). */
+int gxy_choose_anchor(gxy_ctx *ctx, const gxy_hsp *hsp, const gxy_seq *target,
+                      const gxy_seq *query, const gxy_scoring *scoring, uint32_t window,
+                      gxy_anchor *out);
+/* Batched choose_anchor over n HSPs of one (target, query) pair. */
+int gxy_choose_anchor_batch(gxy_ctx *ctx, const gxy_hsp *hsps, uint64_t n,
+                            const gxy_seq *target, const gxy_seq *query,
+                            const gxy_scoring *scoring, uint32_t window, gxy_anchor *out);
+
+/* ---- gapped extension --------------------------------------------------- */
+/* gapped_extend::extend(Anchor, &PackedSeq, &PackedSeq, &ScoringMatrix,
+ *                       &GappedParams) -> Option<GappedAlignment>
+ * (src/gapped_extend.rs:78-84).  Writes the edit script into `runs` (capacity
+ * `runs_cap`); `out->n_runs` is the full run count even when it exceeds the
+ * capacity (then the call returns GXY_ERANGE). */
+int gxy_gapped_extend(gxy_ctx *ctx, const gxy_anchor *anchor, const gxy_seq *target,
+                      const gxy_seq *query, const gxy_scoring *scoring,
+                      const gxy_gapped_params *params, gxy_alignment *out, gxy_edit_run *runs,
+                      uint64_t runs_cap, int *found);
+/* Batched extend over n anchors of one (target, query) pair. */
+int gxy_gapped_extend_batch(gxy_ctx *ctx, const gxy_anchor *anchors, uint64_t n,
+                            const gxy_seq *target, const gxy_seq *query,
+                            const gxy_scoring *scoring, const gxy_gapped_params *params,
+                            gxy_alignment_set **out);
+/* Batched extend over n independent (target_k, query_k, anchor_k) problems
+ * held in device memory: targets[k]/queries[k] are device sequences. */
+int gxy_gapped_extend_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *targets,
+                                const gxy_dseq *const *queries, const gxy_anchor *anchors,
+                                uint64_t n, const gxy_scoring *scoring,
+                                const gxy_gapped_params *params, gxy_alignment_set **out);
+void gxy_alignment_set_free(gxy_alignment_set *set);
+
+/* ---- driver (src/driver.rs:50-111, 199-392) ----------------------------- */
+typedef struct gxy_named_seq {
+    const char *name;
+    gxy_seq seq;
+} gxy_named_seq;
+
+#define GXY_STRAND_PLUS 0u
+#define GXY_STRAND_MINUS 1u
+#define GXY_STRAND_BOTH 2u
+
+/* driver::Config (src/driver.rs:50-89).  chain_enabled / tweener_enabled are
+ * outside this library's path: run() rejects them with GXY_EUNSUP. */
+typedef struct gxy_config {
+    gxy_seed_pattern pattern;
+    gxy_scoring scoring;
+    uint64_t step;
+    gxy_hsp_params hsp;
+    gxy_gapped_params gapped;
+    uint32_t strand;
+    uint32_t max_word_count;
+    int32_t respect_masking;
+    int32_t gapped_enabled;
+    int32_t chain_enabled;
+    int32_t tweener_enabled;
+    uint32_t anchor_window;
+    uint32_t transitions;
+    uint32_t chunk_primary_bp;
+    uint32_t chunk_halo_bp;
+    int32_t entropy_enabled;
+    int32_t entropy_threshold;
+} gxy_config;
+
+/* Record (src/output/mod.rs:17-31) without the owned strings: names and base
+ * slices are taken from the input sequences when rendering. */
+typedef struct gxy_record {
+    uint32_t target_index;
+    uint32_t query_index;
+    uint32_t strand; /* GXY_STRAND_PLUS / GXY_STRAND_MINUS */
+    uint32_t t_start;
+    uint32_t q_start;
+    uint32_t t_span;
+    uint32_t q_span;
+    int32_t score;
+    uint32_t n_runs;
+    uint32_t reserved;
+    uint64_t runs_offset;
+} gxy_record;
+
+typedef struct gxy_record_set {
+    uint64_t n;
+    gxy_record *records;
+    uint64_t n_runs;
+    gxy_edit_run *runs;
+} gxy_record_set;
+
+/* Counters of one run() call. */
+typedef struct gxy_run_stats {
+    uint64_t chunks;
+    uint64_t query_bp;       /* total query bases scanned per strand pass   */
+    uint64_t hits;
+    uint64_t columns;
+    uint64_t hsps;
+    uint64_t dp_problems;    /* gapped extensions computed                  */
+    uint64_t dp_cells;
+    uint64_t tie_pairs;      /* order-sensitive ties (see gxy_search_stats) */
+    uint64_t score_ties;     /* equal-score neighbours in the gapped order  */
+} gxy_run_stats;
+
+/* Config::default() (src/driver.rs:91-111) with HOXD70 (src/scoring.rs:44-58). */
+void gxy_config_default(gxy_config *cfg);
+/* SeedPattern from "12of19", "19of20", "match<k>" or a 0/1 spec
+ * (src/cli.rs:225-240). */
+int gxy_pattern_parse(const char *spec, gxy_seed_pattern *out);
+
+/* driver::run(targets, queries, &Config) -> Vec<Record> (src/driver.rs:199). */
+int gxy_run(gxy_ctx *ctx, const gxy_named_seq *targets, uint64_t n_targets,
+            const gxy_named_seq *queries, uint64_t n_queries, const gxy_config *cfg,
+            gxy_record_set **out, gxy_run_stats *stats);
+/* Render records as MAF (src/output/maf.rs:33-74, header as src/main.rs:42)
+ * or PAF (src/output/paf.rs:23-89).  `*text` is NUL-terminated; free with
+ * gxy_free. */
+int gxy_format_maf(const gxy_named_seq *targets, uint64_t n_targets,
+                   const gxy_named_seq *queries, uint64_t n_queries,
+                   const gxy_record_set *records, char **text, uint64_t *len);
+int gxy_format_paf(const gxy_named_seq *targets, uint64_t n_targets,
+                   const gxy_named_seq *queries, uint64_t n_queries,
+                   const gxy_record_set *records, char **text, uint64_t *len);
+void gxy_record_set_free(gxy_record_set *set);
+
+/* Free an array returned by the library (gxy_search output, texts). */
+void gxy_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GXY_H */

@@ -1,0 +1,232 @@
+"""ctypes binding of oracle/liboracle.so — TEST INFRASTRUCTURE ONLY.
+
+The CPU restatement of the reference path, used as the parity checker by
+tests/, by __graft_entry__.smoke() and as the CPU baseline of bench.py.  The
+product (lastz-gxy_amd/) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "lastz-gxy_amd"))
+import gxy_types as T  # noqa: E402
+
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_P = C.POINTER
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "gxo_pack_ascii": (C.c_int, [C.c_char_p, C.c_uint64, vp, vp, vp]),
+        "gxo_seed_word": (C.c_int, [_P(T.Seq), _P(T.SeedPatternC), C.c_uint64, _P(C.c_uint64)]),
+        "gxo_postable_build": (C.c_int, [_P(T.Seq), _P(T.SeedPatternC), C.c_uint64, _P(vp)]),
+        "gxo_postable_prune": (C.c_int, [vp, C.c_uint32]),
+        "gxo_postable_num_words": (C.c_uint64, [vp]),
+        "gxo_postable_total_positions": (C.c_uint64, [vp]),
+        "gxo_postable_starts": (_P(C.c_uint32), [vp]),
+        "gxo_postable_positions": (_P(C.c_uint32), [vp]),
+        "gxo_postable_free": (None, [vp]),
+        "gxo_extend_hit": (C.c_int, [_P(T.Seq), _P(T.Seq), _P(T.Scoring), _P(T.HspParams),
+                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _P(T.Hsp),
+                                     _P(C.c_int)]),
+        "gxo_search": (C.c_int, [vp, _P(T.Seq), _P(T.Seq), _P(T.Scoring), _P(T.SearchParams),
+                                 C.c_int, _P(_P(T.Hsp)), _P(C.c_uint64), _P(T.SearchStats)]),
+        "gxo_choose_anchor": (C.c_int, [_P(T.Hsp), _P(T.Seq), _P(T.Seq), _P(T.Scoring),
+                                        C.c_uint32, _P(T.Anchor)]),
+        "gxo_gapped_extend": (C.c_int, [_P(T.Anchor), _P(T.Seq), _P(T.Seq), _P(T.Scoring),
+                                        _P(T.GappedParams), C.c_int, _P(T.Alignment),
+                                        _P(_P(T.EditRun)), _P(C.c_uint64), _P(C.c_int)]),
+        "gxo_shannon_entropy": (C.c_double, [_P(T.Seq), C.c_uint64, C.c_uint64]),
+        "gxo_passes_entropy_gate": (C.c_int, [_P(T.Seq), C.c_uint32, C.c_uint32, C.c_int32,
+                                              C.c_int32]),
+        "gxo_run": (C.c_int, [_P(T.NamedSeq), C.c_uint64, _P(T.NamedSeq), C.c_uint64,
+                              _P(T.Config), C.c_int, C.c_int, C.c_int, _P(_P(T.RecordSet)),
+                              _P(T.RunStats)]),
+        "gxo_format_maf": (C.c_int, [_P(T.NamedSeq), C.c_uint64, _P(T.NamedSeq), C.c_uint64,
+                                     _P(T.RecordSet), _P(C.c_void_p), _P(C.c_uint64)]),
+        "gxo_format_paf": (C.c_int, [_P(T.NamedSeq), C.c_uint64, _P(T.NamedSeq), C.c_uint64,
+                                     _P(T.RecordSet), _P(C.c_void_p), _P(C.c_uint64)]),
+        "gxo_record_set_free": (None, [_P(T.RecordSet)]),
+        "gxo_free": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with status {rc}")
+
+
+def cigar(runs):
+    return "".join(f"{n}{'MID'[op]}" for op, n in runs)
+
+
+class PosTable:
+    """PosTable (src/pos_table.rs:24-137) built by the oracle."""
+
+    def __init__(self, seq: T.PackedSeq, pattern: T.SeedPatternC, step: int = 1):
+        self._h = C.c_void_p()
+        _check(lib().gxo_postable_build(C.byref(seq.view()), C.byref(pattern), step,
+                                        C.byref(self._h)), "gxo_postable_build")
+        self.pattern = pattern
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gxo_postable_free(self._h)
+            self._h = None
+
+    @property
+    def num_words(self):
+        return int(lib().gxo_postable_num_words(self._h))
+
+    @property
+    def total_positions(self):
+        return int(lib().gxo_postable_total_positions(self._h))
+
+    def starts(self) -> np.ndarray:
+        p = lib().gxo_postable_starts(self._h)
+        return np.ctypeslib.as_array(p, shape=(self.num_words + 1,)).copy()
+
+    def positions(self) -> np.ndarray:
+        n = self.total_positions
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        p = lib().gxo_postable_positions(self._h)
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def lookup(self, word: int):
+        s = self.starts()
+        return self.positions()[s[word]:s[word + 1]].tolist()
+
+    def prune_hot_words(self, limit: int):
+        _check(lib().gxo_postable_prune(self._h, limit), "gxo_postable_prune")
+
+
+def seed_word(seq: T.PackedSeq, pattern: T.SeedPatternC, pos: int):
+    w = C.c_uint64()
+    r = lib().gxo_seed_word(C.byref(seq.view()), C.byref(pattern), pos, C.byref(w))
+    if r < 0:
+        raise RuntimeError("gxo_seed_word failed")
+    return int(w.value) if r == 1 else None
+
+
+def extend_hit(t, q, sc, params, t_pos, q_pos, seed_len, simd=False):
+    out = T.Hsp()
+    found = C.c_int()
+    _check(lib().gxo_extend_hit(C.byref(t.view()), C.byref(q.view()), C.byref(sc),
+                                C.byref(params), t_pos, q_pos, seed_len, 1 if simd else 0,
+                                C.byref(out), C.byref(found)), "gxo_extend_hit")
+    return out.astuple() if found.value else None
+
+
+def search(table: PosTable, t, q, sc, sp, simd=True):
+    out = _P(T.Hsp)()
+    n = C.c_uint64()
+    st = T.SearchStats()
+    _check(lib().gxo_search(table._h, C.byref(t.view()), C.byref(q.view()), C.byref(sc),
+                            C.byref(sp), 1 if simd else 0, C.byref(out), C.byref(n),
+                            C.byref(st)), "gxo_search")
+    res = [out[i].astuple() for i in range(n.value)]
+    lib().gxo_free(out)
+    return res, st
+
+
+def choose_anchor(hsp, t, q, sc, window):
+    h = T.Hsp(*hsp)
+    a = T.Anchor()
+    _check(lib().gxo_choose_anchor(C.byref(h), C.byref(t.view()), C.byref(q.view()),
+                                   C.byref(sc), window, C.byref(a)), "gxo_choose_anchor")
+    return (a.t_pos, a.q_pos)
+
+
+def gapped_extend(anchor, t, q, sc, gp, lazy=True):
+    """Returns (t_start, q_start, t_len, q_len, score, runs) or None, and cells."""
+    a = T.Anchor()
+    a.t_pos, a.q_pos = anchor[0], anchor[1]
+    if len(anchor) > 2:
+        a.source = T.Hsp(*anchor[2])
+    out = T.Alignment()
+    runs = _P(T.EditRun)()
+    cells = C.c_uint64()
+    found = C.c_int()
+    _check(lib().gxo_gapped_extend(C.byref(a), C.byref(t.view()), C.byref(q.view()),
+                                   C.byref(sc), C.byref(gp), 1 if lazy else 0, C.byref(out),
+                                   C.byref(runs), C.byref(cells), C.byref(found)),
+           "gxo_gapped_extend")
+    if not found.value:
+        return None, cells.value
+    rr = [(runs[i].op, runs[i].n) for i in range(out.n_runs)]
+    lib().gxo_free(runs)
+    return (out.t_start, out.q_start, out.t_len, out.q_len, out.score, rr), cells.value
+
+
+class Records:
+    """Owning wrapper of a gxy_record_set produced by the oracle."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.gxo_record_set_free(self.ptr)
+            self.ptr = None
+
+    def tuples(self):
+        return T.record_tuples(self.ptr.contents)
+
+
+def run(targets, queries, cfg: T.Config, threads=0, lazy=True, simd=True):
+    """driver::run (src/driver.rs:199); targets/queries are [(name, PackedSeq)]."""
+    ta, tk = T.named_seq_array(targets)
+    qa, qk = T.named_seq_array(queries)
+    out = _P(T.RecordSet)()
+    st = T.RunStats()
+    _check(lib().gxo_run(ta, len(targets), qa, len(queries), C.byref(cfg), threads,
+                         1 if lazy else 0, 1 if simd else 0, C.byref(out), C.byref(st)),
+           "gxo_run")
+    del tk, qk
+    return Records(out), st
+
+
+def _format(fn, targets, queries, recs: Records) -> bytes:
+    ta, tk = T.named_seq_array(targets)
+    qa, qk = T.named_seq_array(queries)
+    txt = C.c_void_p()
+    n = C.c_uint64()
+    _check(fn(ta, len(targets), qa, len(queries), recs.ptr, C.byref(txt), C.byref(n)), "format")
+    data = C.string_at(txt, n.value)
+    lib().gxo_free(txt)
+    del tk, qk
+    return data
+
+
+def format_maf(targets, queries, recs: Records) -> bytes:
+    return _format(lib().gxo_format_maf, targets, queries, recs)
+
+
+def format_paf(targets, queries, recs: Records) -> bytes:
+    return _format(lib().gxo_format_paf, targets, queries, recs)

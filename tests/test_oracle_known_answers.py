@@ -1,0 +1,444 @@
+"""Pins the oracle (CPU restatement) to the reference's own known answers.
+
+Every test restates one test of the reference crate (cited) and, where the
+reference only asserts an inequality, also checks the closed form the
+arithmetic fixes (e.g. a perfect-match HSP scores the sum of HOXD70's
+diagonal).  CPU only.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import gxy_types as T
+import oracle_py as O
+
+HOX = T.scoring_hoxd70()
+PS = T.PackedSeq.from_ascii
+
+
+def solid(k):
+    return T.pattern_from_spec(f"match{k}")
+
+
+def diag_sum(s):
+    d = {"A": 91, "C": 100, "G": 100, "T": 91}
+    return sum(d[c] for c in s)
+
+
+# ---- sequences.rs / dna.rs ---------------------------------------------------
+
+def test_pack_layout_matches_python_packer():
+    text = b"ACGTacgtNNnRYACGUu" * 7
+    ps = PS(text)
+    codes = np.zeros(len(ps.codes), np.uint8)
+    valid = np.zeros(len(ps.valid), np.uint8)
+    mask = np.zeros(len(ps.mask), np.uint8)
+    assert O.lib().gxo_pack_ascii(text, len(text), codes.ctypes.data, valid.ctypes.data,
+                                  mask.ctypes.data) == 0
+    assert codes.tolist() == ps.codes.tolist()
+    assert valid.tolist() == ps.valid.tolist()
+    assert mask.tolist() == ps.mask.tolist()
+
+
+def test_roundtrip_and_rc():  # sequences.rs tests packed_roundtrip_ascii / rc_random
+    assert PS(b"ACGTACGTNACG").to_ascii() == b"ACGTACGTNACG"
+    assert PS(b"ACCGTTNAG").reverse_complement().to_ascii() == b"CTNAACGGT"
+    rc = PS(b"ACgtAC").reverse_complement()
+    _, _, m = rc.unpack()
+    assert m.tolist() == [False, False, True, True, False, False]
+    sl = PS(b"ACgtNACGT").slice(2, 7)
+    code, valid, mask = sl.unpack()
+    assert sl.to_ascii() == b"GTNAC"
+    assert mask.tolist() == [True, True, False, False, False]
+    assert valid.tolist() == [True, True, False, True, True]
+
+
+# ---- seeds.rs ----------------------------------------------------------------
+
+def test_seed_words():
+    assert O.seed_word(PS(b"ACGT"), solid(4), 0) == 0x1B            # seeds.rs:217-224
+    assert O.seed_word(PS(b"ACGT"), T.pattern_from_spec("1001"), 0) == 0b0011  # :227-234
+    assert O.seed_word(PS(b"AcG"), solid(3), 0) is None              # :237-243
+    assert O.seed_word(PS(b"AcG"), T.pattern_from_spec("101"), 0) == 0b0010  # :246-253
+    assert O.seed_word(PS(b"ACG"), solid(4), 0) is None              # :256-261
+    p = T.pattern_from_spec("12of19")                                # :186-192
+    assert T.pattern_span_weight(p) == (19, 12)
+
+
+def test_iter_with_stride_via_table():  # seeds.rs:264-281
+    t = O.PosTable(PS(b"ACGNNACGT"), solid(3), 1)
+    assert sorted(t.positions().tolist()) == [0, 5, 6]
+    t = O.PosTable(PS(b"ACGTACGT"), solid(2), 3)
+    assert sorted(t.positions().tolist()) == [0, 3, 6]
+
+
+# ---- pos_table.rs --------------------------------------------------------------
+
+def test_postable_known_answers():
+    seq = PS(b"ACGTACGT")
+    t = O.PosTable(seq, solid(3), 1)  # pos_table.rs:143-161
+    w = lambda s: O.seed_word(PS(s), solid(3), 0)
+    assert t.lookup(w(b"ACG")) == [0, 4]
+    assert t.lookup(w(b"CGT")) == [1, 5]
+    assert t.lookup(w(b"GTA")) == [2]
+    assert t.lookup(w(b"TAC")) == [3]
+    t3 = O.PosTable(PS(b"ACGT" * 4), solid(3), 3)  # :177-187
+    assert t3.lookup(w(b"ACG")) == [0, 12]
+    assert t3.total_positions < O.PosTable(PS(b"ACGT" * 4), solid(3), 1).total_positions
+    tn = O.PosTable(PS(b"ACGNACG"), solid(3), 1)  # :190-197
+    assert tn.lookup(w(b"ACG")) == [0, 4]
+    tp = O.PosTable(PS(b"AAAAAAAAAAACG"), solid(3), 1)  # :200-211
+    assert len(tp.lookup(w(b"AAA"))) >= 9
+    tp.prune_hot_words(4)
+    assert tp.lookup(w(b"AAA")) == []
+    assert tp.lookup(w(b"ACG")) == [10]
+
+
+def test_postable_invariants_random():  # pos_table.rs:164-174, 214-225
+    rng = np.random.default_rng(5)
+    text = bytes(rng.choice(list(b"ACGTacgtN"), 3000))
+    seq = PS(text)
+    pat = T.pattern_from_spec("1101011")
+    t = O.PosTable(seq, pat, 1)
+    s = t.starts()
+    assert np.all(np.diff(s.astype(np.int64)) >= 0)
+    assert s[-1] == t.total_positions
+    pos = t.positions()
+    # brute force: every window's word, ascending positions per word
+    expect = {}
+    for p in range(len(text) - 7 + 1):
+        wd = O.seed_word(seq, pat, p)
+        if wd is not None:
+            expect.setdefault(wd, []).append(p)
+    for wd, lst in expect.items():
+        assert pos[s[wd]:s[wd + 1]].tolist() == lst
+    assert sum(len(v) for v in expect.values()) == t.total_positions
+
+
+# ---- diag_hash.rs (through search admission) ----------------------------------
+
+def test_search_known_answers():
+    sp = T.SearchParams(1, T.HspParams(500, 1000), 0, 0, 0)
+    t = PS(b"AAAAAGATTACACATGGCATGAAAAA")  # seed_search.rs:137-161
+    q = PS(b"GATTACACATGGCATG")
+    hs, _ = O.search(O.PosTable(t, solid(8)), t, q, HOX, sp)
+    assert len(hs) == 1
+    assert hs[0][:3] == (5, 0, 16)
+    assert hs[0][3] == diag_sum("GATTACACATGGCATG") == 1519
+
+    t = PS(b"AAAAACGTAAAAA")  # :164-200
+    q = PS(b"ACATA")
+    sp0 = T.SearchParams(1, T.HspParams(500, 50), 0, 0, 0)
+    sp1 = T.SearchParams(1, T.HspParams(500, 50), 1, 0, 0)
+    assert O.search(O.PosTable(t, solid(5)), t, q, HOX, sp0)[0] == []
+    assert len(O.search(O.PosTable(t, solid(5)), t, q, HOX, sp1)[0]) >= 1
+
+    t = PS(b"AAAAAAAAAA")  # :203-221
+    q = PS(b"CCCCCCCCCC")
+    assert O.search(O.PosTable(t, solid(5)), t, q, HOX,
+                    T.SearchParams(1, T.HspParams(100, 100), 0, 0, 0))[0] == []
+
+    t = PS(b"AAAAAGATTACACATGGCATGTCGAAAAAA")  # :224-253
+    q = PS(b"GATTACACATGGCATGTCGA")
+    hs, _ = O.search(O.PosTable(t, solid(7)), t, q, HOX,
+                     T.SearchParams(1, T.HspParams(500, 500), 0, 0, 0))
+    diags = {h[0] - h[1] for h in hs}
+    assert len(diags) == len(hs)
+    assert any(h[:3] == (5, 0, 20) for h in hs)
+
+    t = PS(b"AAAAGATTACACATGGAAAAAAAAAAAAGATTACACATGGTTTTT")  # :256-280
+    q = PS(b"GATTACACATGG")
+    hs, _ = O.search(O.PosTable(t, solid(8)), t, q, HOX,
+                     T.SearchParams(1, T.HspParams(500, 500), 0, 0, 0))
+    assert len({h[0] - h[1] for h in hs}) >= 2
+
+
+# ---- hsp.rs ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("simd", [False, True])
+def test_extend_hit_known_answers(simd):
+    s = PS(b"ACGTACGTACGTACGT")  # hsp.rs:222-232
+    h = O.extend_hit(s, s, HOX, T.HspParams(100, 100), 2, 2, 4, simd)
+    assert h == (0, 0, 16, 4 * 382)
+    t, q = PS(b"TTTACGTTTT"), PS(b"AAAACGAAAA")  # :235-245
+    h = O.extend_hit(t, q, HOX, T.HspParams(50, 50), 3, 3, 3, simd)
+    assert h == (3, 3, 3, 91 + 100 + 100)
+    assert O.extend_hit(PS(b"ACGT"), PS(b"TTTT"), HOX, T.HspParams(100, 10000), 0, 0, 4,
+                        simd) is None  # :248-253
+    assert O.extend_hit(PS(b"ACGT"), PS(b"ACGT"), HOX, T.HspParams(910, 3000), 2, 2, 10,
+                        simd) is None  # :256-261
+    h = O.extend_hit(PS(b"ACGTACGT"), PS(b"ACNTACGT"), HOX, T.HspParams(500, 200), 4, 4, 4,
+                     simd)  # :264-271
+    assert h is not None and h[2] >= 4
+    s = PS(b"CCCCCCAAAAAACCCCCC")  # :317-324
+    h = O.extend_hit(s, s, HOX, T.HspParams(500, 100), 8, 8, 2, simd)
+    assert h == (0, 0, 18, 12 * 100 + 6 * 91)
+
+
+def _naive_xdrop(tb, qb, t_pos, q_pos, seed_len, xdrop, thresh):
+    """Brute force restatement of the x-drop definition on ASCII strings."""
+    def col(i, j):
+        a, b = tb[i], qb[j]
+        if a not in "ACGT" or b not in "ACGT":
+            return -100
+        return T.HOXD70["ACGT".index(a)]["ACGT".index(b)]
+    if t_pos + seed_len > len(tb) or q_pos + seed_len > len(qb):
+        return None
+    seed = sum(col(t_pos + k, q_pos + k) for k in range(seed_len))
+    def side(i, j, d):
+        run = best = ext = k = 0
+        while 0 <= i < len(tb) and 0 <= j < len(qb):
+            k += 1
+            run += col(i, j)
+            if run > best:
+                best, ext = run, k
+            elif best - run > xdrop:
+                break
+            i += d
+            j += d
+        return best, ext
+    rb, re_ = side(t_pos + seed_len, q_pos + seed_len, 1)
+    lb, le = side(t_pos - 1, q_pos - 1, -1)
+    score = seed + lb + rb
+    if score < thresh:
+        return None
+    return (t_pos - le, q_pos - le, seed_len + le + re_, score)
+
+
+def test_extend_hit_simd_matches_scalar_and_brute_force():  # hsp.rs:283-314 proptest
+    rng = np.random.default_rng(1234)
+    for case in range(300):
+        tb = "".join(rng.choice(list("ACGTN"), rng.integers(32, 200)))
+        qb = "".join(rng.choice(list("ACGTN"), rng.integers(32, 200)))
+        seed_len = int(rng.integers(4, 12))
+        xdrop = int(rng.integers(50, 2000))
+        t, q = PS(tb), PS(qb)
+        t_max, q_max = max(0, len(tb) - seed_len), max(0, len(qb) - seed_len)
+        if t_max == 0 or q_max == 0:
+            continue
+        tp, qp = t_max // 2, q_max // 2
+        p = T.HspParams(xdrop, 0)
+        a = O.extend_hit(t, q, HOX, p, tp, qp, seed_len, False)
+        b = O.extend_hit(t, q, HOX, p, tp, qp, seed_len, True)
+        c = _naive_xdrop(tb, qb, tp, qp, seed_len, xdrop, 0)
+        assert a == b == c, (case, a, b, c)
+
+
+# ---- anchor.rs ---------------------------------------------------------------------
+
+def test_choose_anchor_known_answers():
+    s = PS(b"ACGTACGTACGTACGT")  # anchor.rs:80-88
+    assert O.choose_anchor((0, 0, 16, 0), s, s, HOX, 4) == (2, 2)
+    assert O.choose_anchor((0, 0, 8, 0), PS(b"AACCCCAA"), PS(b"TTCCCCTT"), HOX, 4) == (4, 4)
+    s = PS(b"ACGT")  # :104-111
+    assert O.choose_anchor((0, 0, 4, 0), s, s, HOX, 1000)[0] == 2
+
+
+# ---- gapped_extend.rs -----------------------------------------------------------------
+
+@pytest.mark.parametrize("lazy", [False, True])
+def test_gapped_known_answers(lazy):
+    s = PS(b"ACGTACGTACGTACGT")  # gapped_extend.rs:482-495
+    r, _ = O.gapped_extend((8, 8), s, s, HOX, T.GappedParams(500, 100), lazy)
+    assert r[:4] == (0, 0, 16, 16) and O.cigar(r[5]) == "16M" and r[4] == 4 * 382
+
+    t, q = PS(b"ACGTACGTACGT"), PS(b"ACGTACGGTACGT")  # :497-513
+    r, _ = O.gapped_extend((3, 3), t, q, HOX, T.GappedParams(1000, 100), lazy)
+    assert (r[2], r[3]) == (12, 13) and "1I" in O.cigar(r[5])
+    assert r[4] == 3 * 382 - 430  # 12 matched columns minus one gap open+extend
+
+    t, q = PS(b"ACGTACGGTACGT"), PS(b"ACGTACGTACGT")  # :516-529
+    r, _ = O.gapped_extend((3, 3), t, q, HOX, T.GappedParams(1000, 100), lazy)
+    assert (r[2], r[3]) == (13, 12) and "1D" in O.cigar(r[5])
+    assert r[4] == 3 * 382 - 430
+
+    s = PS(b"ACGT")  # :532-538
+    assert O.gapped_extend((1, 1), s, s, HOX, T.GappedParams(500, 100000), lazy)[0] is None
+
+    t = PS(b"TTTTTTTTACGTACGTACGTACGTACGTACGTTTTTTTTT")  # :540-555
+    q = PS(b"AAAAAAAAACGTACGTACGTACGTACGTACGTAAAAAAAA")
+    r, _ = O.gapped_extend((20, 20), t, q, HOX, T.GappedParams(300, 500), lazy)
+    assert 24 <= r[2] <= 32
+
+    core = b"GATTACACATGGCATGTCGAATGCCTAGCATGTCA"  # :556-587
+    t = PS(core + core)
+    q = PS(core + b"AGCTAGCTAGCTAGCTAGCTAGCTAGCTAGCTAGCTAGCTAGCTAGCTAG" + core)
+    r, _ = O.gapped_extend((5, 5), t, q, HOX, T.GappedParams(9400, 500), lazy)
+    assert r[2] >= 60 and r[3] >= 110 and "I" in O.cigar(r[5])
+
+    t = PS(b"AAAGATTACACATGGCATGTCGATTTT")  # :589-601
+    q = PS(b"GATACACATGGCAGTCGA")
+    r, _ = O.gapped_extend((6, 3), t, q, HOX, T.GappedParams(5000, 100), lazy)
+    tspan = sum(n for op, n in r[5] if op in (0, 2))
+    qspan = sum(n for op, n in r[5] if op in (0, 1))
+    assert (tspan, qspan) == (r[2], r[3])
+
+
+def _gotoh_full(tb, qb, oe, e):
+    """Full-matrix 3-state affine DP (no y-drop): best score over all cells."""
+    NEG = -(1 << 29)
+    n, m = len(tb), len(qb)
+    def s(i, j):
+        a, b = tb[i], qb[j]
+        if a not in "ACGT" or b not in "ACGT":
+            return -100
+        return T.HOXD70["ACGT".index(a)]["ACGT".index(b)]
+    M = [[NEG] * (m + 1) for _ in range(n + 1)]
+    X = [[NEG] * (m + 1) for _ in range(n + 1)]
+    Y = [[NEG] * (m + 1) for _ in range(n + 1)]
+    M[0][0] = 0
+    best = 0
+    for i in range(n + 1):
+        for j in range(m + 1):
+            if i == 0 and j == 0:
+                continue
+            if i:
+                X[i][j] = max(M[i - 1][j] - oe, X[i - 1][j] - e)
+            if j:
+                Y[i][j] = max(M[i][j - 1] - oe, Y[i][j - 1] - e)
+            if i and j:
+                M[i][j] = max(M[i - 1][j - 1], X[i - 1][j - 1], Y[i - 1][j - 1]) + s(i - 1, j - 1)
+            best = max(best, M[i][j], X[i][j], Y[i][j])
+    return best
+
+
+def test_gapped_matches_full_gotoh_when_band_covers_all():
+    """With a huge y-drop the band covers the whole (tiny) matrix, so each side
+    must reach the full-matrix Gotoh optimum; the script must re-score to it."""
+    rng = np.random.default_rng(7)
+    for case in range(25):
+        n = int(rng.integers(5, 40))
+        tb = "".join(rng.choice(list("ACGT"), n))
+        qb = list(tb)
+        for _ in range(int(rng.integers(0, 5))):
+            k = int(rng.integers(0, len(qb)))
+            op = rng.integers(0, 3)
+            if op == 0:
+                qb[k] = "ACGT"[int(rng.integers(0, 4))]
+            elif op == 1:
+                qb.insert(k, "ACGT"[int(rng.integers(0, 4))])
+            elif len(qb) > 2:
+                del qb[k]
+        qb = "".join(qb)
+        at, aq = len(tb) // 2, min(len(qb) - 1, len(tb) // 2)
+        t, q = PS(tb), PS(qb)
+        r, _ = O.gapped_extend((at, aq), t, q, HOX, T.GappedParams(10**7, -10**9), True)
+        right = _gotoh_full(tb[at + 1:], qb[aq + 1:], 430, 30)
+        left = _gotoh_full(tb[:at][::-1], qb[:aq][::-1], 430, 30)
+        col = T.HOXD70["ACGT".index(tb[at])]["ACGT".index(qb[aq])]
+        assert r[4] == col + left + right, case
+        # rescore the edit script
+        i, j, sc, gap = r[0], r[1], 0, None
+        for op, k in r[5]:
+            if op == 0:
+                for _ in range(k):
+                    sc += T.HOXD70["ACGT".index(tb[i])]["ACGT".index(qb[j])]
+                    i += 1
+                    j += 1
+                gap = None
+            else:
+                sc -= 400 + 30 * k
+                if op == 1:
+                    j += k
+                else:
+                    i += k
+        assert sc == r[4], case
+        assert (i - r[0], j - r[1]) == (r[2], r[3])
+
+
+def test_gapped_lazy_equals_as_written_random():
+    rng = np.random.default_rng(99)
+    for case in range(20):
+        n = int(rng.integers(50, 600))
+        tb = "".join(rng.choice(list("ACGTN"), n, p=[.24, .24, .24, .24, .04]))
+        qb = "".join(c if rng.random() > 0.15 else "ACGT"[int(rng.integers(0, 4))] for c in tb)
+        t, q = PS(tb), PS(qb)
+        a = (n // 2, n // 2)
+        r1 = O.gapped_extend(a, t, q, HOX, T.GappedParams(9400, 0), False)
+        r2 = O.gapped_extend(a, t, q, HOX, T.GappedParams(9400, 0), True)
+        assert r1 == r2
+
+
+# ---- driver.rs / fixture_smoke.rs / maf ---------------------------------------------
+
+def _cfg(**kw):
+    c = T.config_default()
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def test_driver_known_answers():
+    # tests/fixture_smoke.rs:14-51 and the MAF block of src/parity.rs:405-411
+    targets = [("chrT", PS(b"AAAAAAAAAAAGATTACACATGGCATGTCGAATTTTTTTTTTT"))]
+    queries = [("chrQ", PS(b"GATTACACATGGCATGTCGA"))]
+    cfg = _cfg(pattern=solid(12), hsp=T.HspParams(910, 500), gapped=T.GappedParams(9400, 500))
+    recs, _ = O.run(targets, queries, cfg)
+    tup = recs.tuples()
+    assert tup == [(0, 0, 0, 11, 0, 20, 20, diag_sum("GATTACACATGGCATGTCGA"), "20M")]
+    assert tup[0][7] == 1901
+    maf = O.format_maf(targets, queries, recs).decode()
+    assert maf == ("##maf version=1 program=lastz-gxy\n# scoring=HOXD70\n"
+                   "a score=1901\n"
+                   "s chrT                         11     20 +         43 GATTACACATGGCATGTCGA\n"
+                   "s chrQ                          0     20 +         20 GATTACACATGGCATGTCGA\n\n")
+    paf = O.format_paf(targets, queries, recs).decode()
+    assert paf == ("chrQ\t20\t0\t20\t+\tchrT\t43\t11\t31\t20\t20\t60\tNM:i:0\tAS:i:1901"
+                   "\ttp:A:P\tcg:Z:20M\n")
+
+    # driver.rs:479-506 single indel
+    targets = [("t", PS(b"AAAAAAAAAAGATTACACATGGCATGTCGAAAAAAAAAA"))]
+    queries = [("q", PS(b"GATTACAACATGGCATGTCGA"))]
+    cfg = _cfg(pattern=solid(8), strand=T.STRAND_PLUS, hsp=T.HspParams(910, 500),
+               gapped=T.GappedParams(3000, 500))
+    recs, _ = O.run(targets, queries, cfg)
+    r = max(recs.tuples(), key=lambda x: x[6])
+    assert (r[6], r[5]) == (21, 20) and "1I" in r[8]
+
+    # driver.rs:508-525 nogapped
+    targets = [("t", PS(b"AAAAAAAAAAAGATTACACATGGCATGTCGAAAAAAAAAAA"))]
+    queries = [("q", PS(b"GATTACACATGGCATGTCGA"))]
+    cfg = _cfg(pattern=solid(8), strand=T.STRAND_PLUS, hsp=T.HspParams(910, 500),
+               gapped_enabled=0)
+    recs, _ = O.run(targets, queries, cfg)
+    assert [x[8] for x in recs.tuples()] == ["20M"]
+
+
+def _lcg_filler(seed, n):
+    s = seed
+    out = bytearray()
+    for _ in range(n):
+        s = (s * 6364136223846793005 + 1442695040888963407) & ((1 << 64) - 1)
+        out.append(b"ACGT"[(s >> 33) & 3])
+    return bytes(out)
+
+
+def test_chunking_preserves_record_set():  # tests/fixture_smoke.rs:160-236
+    p1 = b"GATTACACATGGCATGTCGACCCCCCCCCCCC"
+    p2 = b"AGCTAGCTAGCTAGCTAGCGCATCATCATGCA"
+    tb = _lcg_filler(1, 400) + p1 + _lcg_filler(2, 800) + p2 + _lcg_filler(3, 400)
+    qb = _lcg_filler(4, 100) + p1 + _lcg_filler(5, 200) + p2 + _lcg_filler(6, 100)
+    targets, queries = [("t", PS(tb))], [("q", PS(qb))]
+    base = dict(pattern=solid(12), strand=T.STRAND_PLUS, hsp=T.HspParams(910, 500),
+                gapped=T.GappedParams(3000, 500), transitions=0)
+    a, _ = O.run(targets, queries, _cfg(**base))
+    b, st = O.run(targets, queries, _cfg(chunk_primary_bp=600, chunk_halo_bp=200, **base))
+    assert st.chunks >= 3
+    sig = lambda rs: sorted((x[3], x[5], x[4], x[6], x[2]) for x in rs.tuples())
+    assert sig(a) == sig(b) and sig(a)
+
+
+def test_run_thread_count_invariant():
+    rng = np.random.default_rng(3)
+    tb = bytes(rng.choice(list(b"ACGT"), 20000))
+    qb = bytearray(tb[5000:15000])
+    for k in range(0, len(qb), 9):
+        qb[k] = b"ACGT"[int(rng.integers(0, 4))]
+    targets, queries = [("t", PS(tb))], [("q", PS(bytes(qb)))]
+    cfg = _cfg(chunk_primary_bp=4000, chunk_halo_bp=1000)
+    a, _ = O.run(targets, queries, cfg, threads=1)
+    b, _ = O.run(targets, queries, cfg, threads=8)
+    assert O.format_maf(targets, queries, a) == O.format_maf(targets, queries, b)
+    c, _ = O.run(targets, queries, cfg, threads=4, simd=False, lazy=False)
+    assert O.format_maf(targets, queries, a) == O.format_maf(targets, queries, c)
