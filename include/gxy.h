@@ -164,6 +164,15 @@ const char *gxy_last_error(const gxy_ctx *ctx);
 int gxy_ctx_synchronize(gxy_ctx *ctx);
 /* Library version string. */
 const char *gxy_version(void);
+/* Kernel timing (no counterpart in the reference; for measurement only).
+ * When enabled, the library brackets its main kernels with HIP events on the
+ * context's stream; gxy_ctx_kernel_time returns the summed milliseconds and
+ * launch count of kernel `name` since the last reset (it synchronises the
+ * stream).  Names: "table_items", "table_sort", "table_starts", "qcount",
+ * "qexpand", "extend", "pass_sort", "admit", "dp_forward", "dp_traceback". */
+int gxy_ctx_set_timing(gxy_ctx *ctx, int enabled);
+int gxy_ctx_kernel_time(gxy_ctx *ctx, const char *name, double *ms, uint64_t *launches);
+int gxy_ctx_reset_timing(gxy_ctx *ctx);
 
 /* ---- sequences ---------------------------------------------------------- */
 /* PackedSeq::from_ascii (src/sequences.rs:164-206) into caller buffers sized
@@ -333,6 +342,7 @@ typedef struct gxy_run_stats {
     uint64_t dp_cells;
     uint64_t tie_pairs;      /* order-sensitive ties (see gxy_search_stats) */
     uint64_t score_ties;     /* equal-score neighbours in the gapped order  */
+    uint64_t passing;        /* x-drop extensions reaching hsp_threshold    */
 } gxy_run_stats;
 
 /* Config::default() (src/driver.rs:91-111) with HOXD70 (src/scoring.rs:44-58). */
@@ -345,6 +355,12 @@ int gxy_pattern_parse(const char *spec, gxy_seed_pattern *out);
 int gxy_run(gxy_ctx *ctx, const gxy_named_seq *targets, uint64_t n_targets,
             const gxy_named_seq *queries, uint64_t n_queries, const gxy_config *cfg,
             gxy_record_set **out, gxy_run_stats *stats);
+/* driver::run over device-resident sequences (names are not needed: records
+ * carry indices).  The target/query arrays hold n_targets / n_queries
+ * handles.  gxy_run is gxy_dseq_upload of every sequence followed by this. */
+int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets,
+                const gxy_dseq *const *queries, uint64_t n_queries, const gxy_config *cfg,
+                gxy_record_set **out, gxy_run_stats *stats);
 /* Render records as MAF (src/output/maf.rs:33-74, header as src/main.rs:42)
  * or PAF (src/output/paf.rs:23-89).  `*text` is NUL-terminated; free with
  * gxy_free. */

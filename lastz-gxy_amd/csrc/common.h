@@ -1,0 +1,293 @@
+// common.h — shared host/device definitions of the gfx950 hot-path library.
+//
+// Device sequence layout ("block32"): a PackedSeq (src/sequences.rs:128-141)
+// is re-laid in HBM as 16-byte blocks of 32 bases,
+//     struct Blk { uint64 codes; uint32 valid; uint32 mask; }
+// with base i at bits 2*(i&31) of codes[i>>5] and bit (i&31) of valid/mask —
+// the same bit positions as the reference's byte arrays read as little-endian
+// words, so the re-layout is a pure interleave.  One 16-byte load brings
+// everything a lane needs for 32 bases.  Every device sequence carries one
+// zero block past its end so a 32-base window read at any in-range position
+// may touch block (i>>5)+1 without a bounds branch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gxy.h"
+
+namespace gxy {
+
+struct Blk {
+    uint64_t codes;
+    uint32_t valid;
+    uint32_t mask;
+};
+static_assert(sizeof(Blk) == 16, "Blk must be 16 bytes");
+
+constexpr uint32_t kMaxWeight = 14;          // dense 4^w table; 4^14+1 starts = 1 GiB
+constexpr int32_t kNegInf = INT32_MIN / 4;   // src/gapped_extend.rs:44
+constexpr uint32_t kInvalidWord = 0xFFFFFFFFu;
+
+// Care offsets of a seed pattern (src/seeds.rs:112-120) in kernel form.
+struct DevPattern {
+    uint32_t span;
+    uint32_t weight;
+    uint32_t care_mask;     // bit k set when offset k is a care position
+    uint8_t off[32];        // ascending care offsets
+};
+
+inline bool make_dev_pattern(const gxy_seed_pattern *p, DevPattern *d) {
+    if (!p || p->length == 0 || p->length > 32) return false;
+    uint32_t care = p->length == 32 ? p->care : (p->care & ((1u << p->length) - 1u));
+    d->span = p->length;
+    d->care_mask = care;
+    d->weight = 0;
+    std::memset(d->off, 0, sizeof d->off);
+    for (uint32_t k = 0; k < p->length; k++)
+        if ((care >> k) & 1u) d->off[d->weight++] = (uint8_t)k;
+    return d->weight > 0;
+}
+
+// ---- device helpers --------------------------------------------------------
+
+// 32-base window starting at `pos` (any alignment): codes in bits 2k..2k+1 and
+// valid/mask in bit k for base pos+k.
+struct Win {
+    uint64_t codes;
+    uint32_t valid;
+    uint32_t mask;
+};
+
+__device__ __forceinline__ Win load_win(const Blk *__restrict__ b, uint64_t pos) {
+    const uint64_t k = pos >> 5;
+    const uint32_t r = (uint32_t)(pos & 31);
+    const Blk lo = b[k];
+    if (r == 0) return {lo.codes, lo.valid, lo.mask};
+    const Blk hi = b[k + 1];
+    Win w;
+    w.codes = (lo.codes >> (2 * r)) | (hi.codes << (64 - 2 * r));
+    w.valid = (lo.valid >> r) | (hi.valid << (32 - r));
+    w.mask = (lo.mask >> r) | (hi.mask << (32 - r));
+    return w;
+}
+
+__device__ __forceinline__ uint32_t base_code(const Blk *__restrict__ b, uint64_t i) {
+    return (uint32_t)(b[i >> 5].codes >> (2 * (i & 31))) & 3u;
+}
+__device__ __forceinline__ uint32_t base_valid(const Blk *__restrict__ b, uint64_t i) {
+    return (b[i >> 5].valid >> (i & 31)) & 1u;
+}
+
+// Reverse the order of the 32 2-bit groups of a 64-bit word.
+__host__ __device__ __forceinline__ uint64_t rev2(uint64_t x) {
+    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+    x = ((x >> 8) & 0x00FF00FF00FF00FFull) | ((x & 0x00FF00FF00FF00FFull) << 8);
+    x = ((x >> 16) & 0x0000FFFF0000FFFFull) | ((x & 0x0000FFFF0000FFFFull) << 16);
+    return (x >> 32) | (x << 32);
+}
+
+// Spread 32 validity bits to the 2-bit groups of a 64-bit code word.
+__host__ __device__ __forceinline__ uint64_t spread2(uint32_t v) {
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x | (x << 1);
+}
+
+__device__ __forceinline__ int32_t sat_add(int32_t a, int32_t b) {
+    return __builtin_elementwise_add_sat(a, b);
+}
+__device__ __forceinline__ int32_t sat_sub(int32_t a, int32_t b) {
+    return __builtin_elementwise_sub_sat(a, b);
+}
+
+// Column score lookup (src/hsp.rs:198-211): sub[t][q] when both bases are
+// valid, ambig otherwise.  Rows of sub[][] are packed as four signed bytes
+// when every entry fits in int8 (HOXD70 does; src/scoring.rs:93-96), which
+// makes a lookup three selects and one sign-extending shift.  Matrices with
+// wider entries use the 16-entry int32 table.
+struct ScorerI8 {
+    uint32_t r0, r1, r2, r3;
+    int32_t ambig;
+    __device__ __forceinline__ int32_t operator()(uint32_t tc, uint32_t qc, uint32_t v) const {
+        const uint32_t lo = (tc & 1u) ? r1 : r0;
+        const uint32_t hi = (tc & 1u) ? r3 : r2;
+        const uint32_t row = (tc & 2u) ? hi : lo;
+        const int32_t s = (int32_t)(row << (24u - 8u * qc)) >> 24;
+        return v ? s : ambig;
+    }
+};
+
+struct ScorerI32 {
+    int32_t s[16];
+    int32_t ambig;
+    __device__ __forceinline__ int32_t operator()(uint32_t tc, uint32_t qc, uint32_t v) const {
+        // Selects only: a dynamically indexed register array would spill.
+        int32_t row[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int32_t a = (qc & 1u) ? s[t * 4 + 1] : s[t * 4 + 0];
+            const int32_t b = (qc & 1u) ? s[t * 4 + 3] : s[t * 4 + 2];
+            row[t] = (qc & 2u) ? b : a;
+        }
+        const int32_t lo = (tc & 1u) ? row[1] : row[0];
+        const int32_t hi = (tc & 1u) ? row[3] : row[2];
+        const int32_t x = (tc & 2u) ? hi : lo;
+        return v ? x : ambig;
+    }
+};
+
+inline bool scoring_is_i8(const gxy_scoring &m) {
+    auto in = [](int32_t v) { return v >= -128 && v <= 127; };
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++)
+            if (!in(m.sub[a][b])) return false;
+    return true;
+}
+
+inline ScorerI8 make_scorer_i8(const gxy_scoring &m) {
+    ScorerI8 s;
+    uint32_t r[4];
+    for (int a = 0; a < 4; a++) {
+        r[a] = 0;
+        for (int b = 0; b < 4; b++) r[a] |= (uint32_t)(uint8_t)(int8_t)m.sub[a][b] << (8 * b);
+    }
+    s.r0 = r[0];
+    s.r1 = r[1];
+    s.r2 = r[2];
+    s.r3 = r[3];
+    s.ambig = m.ambig;
+    return s;
+}
+
+inline ScorerI32 make_scorer_i32(const gxy_scoring &m) {
+    ScorerI32 s;
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) s.s[a * 4 + b] = m.sub[a][b];
+    s.ambig = m.ambig;
+    return s;
+}
+
+// ---- host side ---------------------------------------------------------------
+
+inline uint64_t nblocks(uint64_t len) { return (len + 31) / 32 + 1; }
+
+}  // namespace gxy
+
+// Opaque handle definitions (shared by all translation units).
+struct gxy_dseq {
+    uint64_t len = 0;
+    uint64_t nblk = 0;
+    gxy::Blk *d = nullptr;  // nblk blocks (last one zero)
+    int device = 0;
+};
+
+struct gxy_postable {
+    gxy::DevPattern pattern;
+    uint64_t step = 1;
+    uint64_t num_words = 0;
+    uint64_t npos = 0;
+    uint32_t *d_starts = nullptr;     // num_words + 1
+    uint32_t *d_positions = nullptr;  // npos (>= 1 allocated)
+    const gxy_dseq *target = nullptr; // sequence the table indexes
+    gxy_dseq *owned_target = nullptr; // set when the table uploaded it itself
+    const void *host_codes = nullptr; // identity of the host target (gxy_search)
+    uint64_t host_len = 0;
+};
+
+struct gxy_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // Growable scratch slots (device memory), reused across calls.
+    static constexpr int kSlots = 24;
+    void *slot[kSlots] = {};
+    size_t slot_bytes[kSlots] = {};
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    // Kernel timing (gxy_ctx_set_timing): event pairs pending on the stream and
+    // per-name totals.
+    struct Span {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    bool timing = false;
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::pair<std::string, std::pair<double, uint64_t>>> totals;
+};
+
+namespace gxy {
+
+int set_err(gxy_ctx *ctx, int code, const char *fmt, ...);
+
+// Brackets the launches issued in its scope with timing events when the
+// context's timing is enabled.
+class KernelTimer {
+  public:
+    KernelTimer(gxy_ctx *ctx, const char *name);
+    ~KernelTimer();
+
+  private:
+    gxy_ctx *ctx_;
+    const char *name_;
+    hipEvent_t a_ = nullptr;
+};
+void timing_flush(gxy_ctx *ctx);
+// Device scratch slot `k` of at least `bytes` bytes (contents undefined).
+void *scratch(gxy_ctx *ctx, int k, size_t bytes);
+int dseq_upload_impl(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out);
+
+// Scratch slot assignment.
+enum Slot {
+    kSlotTmp = 0,      // hipcub temporary storage
+    kSlotWords,
+    kSlotCounts,
+    kSlotOffsets,
+    kSlotHits,
+    kSlotPassKeys,
+    kSlotPassKeys2,
+    kSlotPassIdx,
+    kSlotPassIdx2,
+    kSlotPassHsp,
+    kSlotFlags,
+    kSlotEmit,
+    kSlotCounters,
+    kSlotMisc0,
+    kSlotMisc1,
+    kSlotMisc2,
+    kSlotMisc3,
+    kSlotDpProb,
+    kSlotDpOut,
+    kSlotDpTb,
+    kSlotDpRows,
+    kSlotDpRuns,
+};
+
+}  // namespace gxy
+
+#define GXY_HIP(ctx, call)                                                              \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return gxy::set_err((ctx), GXY_EDEVICE, "%s failed: %s (%s:%d)", #call,    \
+                                hipGetErrorString(_e), __FILE__, __LINE__);             \
+    } while (0)
+
+#define GXY_LAUNCH_CHECK(ctx)                                                           \
+    do {                                                                                \
+        hipError_t _e = hipGetLastError();                                              \
+        if (_e != hipSuccess)                                                           \
+            return gxy::set_err((ctx), GXY_EDEVICE, "kernel launch failed: %s (%s:%d)",   \
+                                hipGetErrorString(_e), __FILE__, __LINE__);             \
+    } while (0)

@@ -1,0 +1,575 @@
+// driver.hip — host driver over the device hot path: driver::run
+// (src/driver.rs:199-392) and the MAF/PAF writers (src/output/maf.rs:33-74,
+// src/output/paf.rs:23-89).
+//
+// All sequences are uploaded once and stay resident; chunks are device slices
+// (src/driver.rs:156-194), the minus strand is a device reverse complement.
+// Per (chunk, query, strand) unit: device search -> primary-region filter ->
+// either ungapped records or the gapped stage.
+//
+// Gapped stage.  The reference extends HSPs one by one in descending score
+// order and skips an HSP whose seed lies in the rectangle of an alignment
+// already emitted (src/driver.rs:269-338), a sequential dependency.  Every
+// extension's result depends only on its own HSP, so the exact batched form
+// is speculative: gather the next K HSPs (score order) that no emitted record
+// covers, extend them all in one device batch, then resolve in order on the
+// host — skip if covered by what has been emitted so far, else emit the
+// precomputed result — until an HSP without a result is reached.  K doubles
+// each round, so a unit with few surviving HSPs costs few DPs and one with
+// many costs few rounds.
+//
+// Canonical order where the reference's is unspecified: equal scores keep
+// (diagonal, t_start) order (sort_unstable_by_key(Reverse(score)),
+// src/driver.rs:270) and groups of one (target, query, strand) key are merged
+// in ascending chunk order (rayon push order, src/driver.rs:361-372).
+#include <algorithm>
+#include <charconv>
+#include <memory>
+#include <string>
+
+#include "internal.h"
+
+using namespace gxy;
+
+namespace {
+
+struct Rec {
+    uint32_t ti, qi, strand;
+    uint32_t t_start, q_start, t_span, q_span;
+    int32_t score;
+    std::vector<gxy_edit_run> runs;
+};
+
+struct DevSeqDeleter {
+    void operator()(gxy_dseq *s) const { gxy_dseq_free(s); }
+};
+struct TableDeleter {
+    void operator()(gxy_postable *t) const { gxy_postable_free(t); }
+};
+using DSeq = std::unique_ptr<gxy_dseq, DevSeqDeleter>;
+using Table = std::unique_ptr<gxy_postable, TableDeleter>;
+
+struct Chunk {
+    uint32_t parent;
+    uint32_t global_start;
+    const gxy_dseq *seq;   // the whole target or an owned slice
+    DSeq owned;
+    uint32_t primary_start_local, primary_end_local;
+};
+
+}  // namespace
+
+extern "C" {
+
+void gxy_config_default(gxy_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    gxy_pattern_parse("12of19", &c->pattern);
+    static const int32_t hox[4][4] = {{91, -114, -31, -123}, {-114, 100, -125, -31},
+                                      {-31, -125, 100, -114}, {-123, -31, -114, 91}};  // src/scoring.rs:17-23
+    std::memcpy(c->scoring.sub, hox, sizeof hox);
+    c->scoring.gap_open = 400;
+    c->scoring.gap_extend = 30;
+    c->scoring.ambig = -100;
+    c->step = 1;
+    c->hsp = {910, 3000};
+    c->gapped = {9400, 3000};
+    c->strand = GXY_STRAND_BOTH;
+    c->respect_masking = 1;
+    c->gapped_enabled = 1;
+    c->anchor_window = 31;
+    c->transitions = 1;
+    c->chunk_halo_bp = 50000;
+}
+
+int gxy_pattern_parse(const char *spec, gxy_seed_pattern *out) {
+    if (!spec || !out) return GXY_EINVAL;
+    std::string s(spec), bits;
+    if (s == "12of19") bits = "1110100110010101111";
+    else if (s == "19of20") bits = "11111111101111111111";
+    else if (s.rfind("match", 0) == 0) {
+        const std::string k = s.substr(5);
+        if (k.empty() || k.find_first_not_of("0123456789") != std::string::npos) return GXY_EINVAL;
+        const unsigned long v = std::stoul(k);
+        if (v == 0 || v > 32) return GXY_EINVAL;
+        bits.assign(v, '1');
+    } else if (!s.empty() && s.find_first_not_of("01") == std::string::npos) {
+        bits = s;
+    } else {
+        return GXY_EINVAL;
+    }
+    if (bits.empty() || bits.size() > 32) return GXY_EINVAL;
+    out->length = (uint32_t)bits.size();
+    out->care = 0;
+    for (size_t k = 0; k < bits.size(); k++)
+        if (bits[k] == '1') out->care |= 1u << k;
+    return GXY_OK;
+}
+
+int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets, const gxy_dseq *const *queries,
+                uint64_t n_queries, const gxy_config *cfg, gxy_record_set **out, gxy_run_stats *stats) {
+    if (!ctx) return GXY_EINVAL;
+    if ((!targets && n_targets) || (!queries && n_queries) || !cfg || !out)
+        return set_err(ctx, GXY_EINVAL, "null argument");
+    for (uint64_t i = 0; i < n_targets; i++)
+        if (!targets[i]) return set_err(ctx, GXY_EINVAL, "null target %llu", (unsigned long long)i);
+    for (uint64_t i = 0; i < n_queries; i++)
+        if (!queries[i]) return set_err(ctx, GXY_EINVAL, "null query %llu", (unsigned long long)i);
+    if (cfg->chain_enabled || cfg->tweener_enabled)
+        return set_err(ctx, GXY_EUNSUP, "chaining and the tweener are not on this library's path");
+    if (cfg->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    gxy_run_stats rs;
+    std::memset(&rs, 0, sizeof rs);
+    const int respect = cfg->respect_masking ? 1 : 0;
+
+    std::vector<uint32_t> strands;
+    if (cfg->strand == GXY_STRAND_PLUS || cfg->strand == GXY_STRAND_BOTH) strands.push_back(GXY_STRAND_PLUS);
+    if (cfg->strand == GXY_STRAND_MINUS || cfg->strand == GXY_STRAND_BOTH) strands.push_back(GXY_STRAND_MINUS);
+    if (strands.empty()) return set_err(ctx, GXY_EINVAL, "invalid strand selector");
+    // minus strand: one device reverse complement per query (src/driver.rs:221)
+    std::vector<DSeq> drc(n_queries);
+    if (cfg->strand != GXY_STRAND_PLUS)
+        for (uint64_t i = 0; i < n_queries; i++) {
+            gxy_dseq *d;
+            int rc = gxy_dseq_reverse_complement(ctx, queries[i], &d);
+            if (rc) return rc;
+            drc[i].reset(d);
+        }
+
+    // chunk_target (src/driver.rs:156-194)
+    std::vector<Chunk> chunks;
+    for (uint64_t ti = 0; ti < n_targets; ti++) {
+        const uint32_t t_len = (uint32_t)targets[ti]->len;
+        const uint32_t P = cfg->chunk_primary_bp, H = cfg->chunk_halo_bp;
+        if (P == 0 || (uint64_t)t_len <= (uint64_t)P + H) {
+            chunks.push_back({(uint32_t)ti, 0, targets[ti], nullptr, 0, t_len});
+            continue;
+        }
+        uint32_t ps = 0;
+        while (ps < t_len) {
+            const uint32_t pe = (uint32_t)std::min<uint64_t>((uint64_t)ps + P, t_len);
+            const uint32_t lo = ps >= H ? ps - H : 0;
+            const uint32_t hi = (uint32_t)std::min<uint64_t>((uint64_t)pe + H, t_len);
+            gxy_dseq *s;
+            int rc = gxy_dseq_slice(ctx, targets[ti], lo, hi, &s);
+            if (rc) return rc;
+            Chunk c{(uint32_t)ti, lo, s, DSeq(s), ps - lo, pe - lo};
+            chunks.push_back(std::move(c));
+            if (pe == t_len) break;
+            ps = pe;
+        }
+    }
+    rs.chunks = chunks.size();
+
+    gxy_search_params sp;
+    sp.step = cfg->step;
+    sp.hsp = cfg->hsp;
+    sp.transitions = cfg->transitions;
+    sp.entropy_enabled = cfg->entropy_enabled;
+    sp.entropy_threshold = cfg->entropy_threshold;
+
+    struct Group {
+        uint32_t ti, qi, strand;
+        size_t chunk;
+        std::vector<Rec> recs;
+    };
+    std::vector<Group> groups;
+
+    for (size_t c = 0; c < chunks.size(); c++) {
+        const Chunk &ch = chunks[c];
+        gxy_postable *tp;
+        int rc = postable_build_impl(ctx, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
+        if (rc) return rc;
+        Table table(tp);
+        if (cfg->max_word_count > 0) {
+            rc = gxy_postable_prune(ctx, tp, cfg->max_word_count);
+            if (rc) return rc;
+        }
+        for (uint32_t qi = 0; qi < n_queries; qi++) {
+            for (uint32_t strand : strands) {
+                const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
+                std::vector<gxy_hsp> hsps;
+                gxy_search_stats st;
+                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st);
+                if (rc) return rc;
+                rs.hits += st.hits;
+                rs.columns += st.columns;
+                rs.hsps += st.emitted;
+                rs.tie_pairs += st.tie_pairs;
+                rs.passing += st.passing;
+                if (hsps.empty()) continue;
+                std::vector<gxy_hsp> kept;  // primary-region filter (src/driver.rs:250-256)
+                kept.reserve(hsps.size());
+                for (const auto &h : hsps)
+                    if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local) kept.push_back(h);
+                Group g{ch.parent, qi, strand, c, {}};
+                if (!cfg->gapped_enabled) {
+                    g.recs.reserve(kept.size());
+                    for (const auto &h : kept) {  // build_ungapped_record (src/driver.rs:437-468)
+                        Rec r{ch.parent, qi, strand, h.t_start + ch.global_start, h.q_start, h.length, h.length,
+                              h.score, {{GXY_OP_MATCH, h.length}}};
+                        g.recs.push_back(std::move(r));
+                    }
+                } else {
+                    std::stable_sort(kept.begin(), kept.end(),
+                                     [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
+                    for (size_t k = 1; k < kept.size(); k++)
+                        if (kept[k].score == kept[k - 1].score) rs.score_ties++;
+                    struct Cov {
+                        uint32_t t_lo, t_hi, q_lo, q_hi;
+                    };
+                    std::vector<Cov> covered;
+                    // src/driver.rs:288-297 compares chunk-local HSP coordinates
+                    // with global record coordinates; kept as written.
+                    auto is_covered = [&](const gxy_hsp &h) {
+                        for (const auto &cv : covered)
+                            if (h.t_start >= cv.t_lo && h.t_start < cv.t_hi && h.q_start >= cv.q_lo &&
+                                h.q_start < cv.q_hi)
+                                return true;
+                        return false;
+                    };
+                    const size_t K = kept.size();
+                    std::vector<char> have(K, 0);
+                    std::vector<GappedResult> res(K);
+                    std::vector<std::vector<gxy_edit_run>> scripts(K);
+                    size_t pos = 0, batch = 8;
+                    while (pos < K) {
+                        while (pos < K) {
+                            if (is_covered(kept[pos])) {
+                                pos++;
+                                continue;
+                            }
+                            if (!have[pos]) break;
+                            if (res[pos].found) {  // build_gapped_record (src/driver.rs:395-434)
+                                const gxy_alignment &a = res[pos].aln;
+                                Rec r{ch.parent, qi, strand, a.t_start + ch.global_start, a.q_start, a.t_len,
+                                      a.q_len, a.score, std::move(scripts[pos])};
+                                covered.push_back({r.t_start, r.t_start + r.t_span, r.q_start, r.q_start + r.q_span});
+                                g.recs.push_back(std::move(r));
+                            }
+                            pos++;
+                        }
+                        if (pos >= K) break;
+                        std::vector<size_t> cand;
+                        for (size_t k = pos; k < K && cand.size() < batch; k++)
+                            if (!have[k] && !is_covered(kept[k])) cand.push_back(k);
+                        std::vector<gxy_hsp> ch_h(cand.size());
+                        for (size_t x = 0; x < cand.size(); x++) ch_h[x] = kept[cand[x]];
+                        std::vector<gxy_anchor> anc(cand.size());
+                        rc = choose_anchor_dev(ctx, ch.seq, qs, ch_h.data(), ch_h.size(), cfg->scoring,
+                                               cfg->anchor_window, anc.data());
+                        if (rc) return rc;
+                        std::vector<GappedItem> items(cand.size());
+                        for (size_t x = 0; x < cand.size(); x++) items[x] = {ch.seq, qs, anc[x]};
+                        std::vector<GappedResult> br;
+                        std::vector<gxy_edit_run> runs;
+                        rc = gapped_batch_dev(ctx, items, cfg->scoring, cfg->gapped, &br, &runs, &rs.dp_cells);
+                        if (rc) return rc;
+                        rs.dp_problems += cand.size();
+                        for (size_t x = 0; x < cand.size(); x++) {
+                            const size_t k = cand[x];
+                            have[k] = 1;
+                            res[k] = br[x];
+                            if (br[x].found)
+                                scripts[k].assign(runs.begin() + br[x].aln.runs_offset,
+                                                  runs.begin() + br[x].aln.runs_offset + br[x].aln.n_runs);
+                        }
+                        batch = std::min<size_t>(batch * 2, 1024);
+                    }
+                }
+                if (!g.recs.empty()) groups.push_back(std::move(g));
+            }
+        }
+    }
+    for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
+
+    // group merge (src/driver.rs:371-389)
+    std::stable_sort(groups.begin(), groups.end(), [](const Group &a, const Group &b) {
+        if (a.ti != b.ti) return a.ti < b.ti;
+        if (a.qi != b.qi) return a.qi < b.qi;
+        if (a.strand != b.strand) return a.strand < b.strand;
+        return a.chunk < b.chunk;
+    });
+    std::vector<Rec> all;
+    for (auto &g : groups) {
+        std::stable_sort(g.recs.begin(), g.recs.end(), [](const Rec &a, const Rec &b) {
+            if (a.t_start != b.t_start) return a.t_start < b.t_start;
+            if (a.q_start != b.q_start) return a.q_start < b.q_start;
+            if (a.t_span != b.t_span) return a.t_span < b.t_span;
+            return a.q_span < b.q_span;
+        });
+        const size_t first = all.size();
+        for (auto &r : g.recs) {
+            if (all.size() > first) {
+                const Rec &b = all.back();
+                if (r.t_start == b.t_start && r.q_start == b.q_start && r.t_span == b.t_span &&
+                    r.q_span == b.q_span && r.strand == b.strand)
+                    continue;
+            }
+            all.push_back(std::move(r));
+        }
+    }
+    auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
+    if (!set) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    size_t nruns = 0;
+    for (const auto &r : all) nruns += r.runs.size();
+    set->n = all.size();
+    set->records = (gxy_record *)std::calloc(std::max<size_t>(1, all.size()), sizeof(gxy_record));
+    set->n_runs = nruns;
+    set->runs = (gxy_edit_run *)std::calloc(std::max<size_t>(1, nruns), sizeof(gxy_edit_run));
+    if (!set->records || !set->runs) {
+        gxy_record_set_free(set);
+        return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    }
+    size_t ro = 0;
+    for (size_t k = 0; k < all.size(); k++) {
+        const Rec &r = all[k];
+        gxy_record &o = set->records[k];
+        o.target_index = r.ti;
+        o.query_index = r.qi;
+        o.strand = r.strand;
+        o.t_start = r.t_start;
+        o.q_start = r.q_start;
+        o.t_span = r.t_span;
+        o.q_span = r.q_span;
+        o.score = r.score;
+        o.n_runs = (uint32_t)r.runs.size();
+        o.runs_offset = ro;
+        for (const auto &x : r.runs) set->runs[ro++] = x;
+    }
+    *out = set;
+    if (stats) *stats = rs;
+    return GXY_OK;
+}
+
+int gxy_run(gxy_ctx *ctx, const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,
+            uint64_t n_queries, const gxy_config *cfg, gxy_record_set **out, gxy_run_stats *stats) {
+    if (!ctx) return GXY_EINVAL;
+    if ((!targets && n_targets) || (!queries && n_queries) || !cfg || !out)
+        return set_err(ctx, GXY_EINVAL, "null argument");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<DSeq> dt(n_targets), dq(n_queries);
+    std::vector<const gxy_dseq *> tp(n_targets), qp(n_queries);
+    for (uint64_t i = 0; i < n_targets; i++) {
+        gxy_dseq *d;
+        int rc = dseq_upload_impl(ctx, &targets[i].seq, &d);
+        if (rc) return rc;
+        dt[i].reset(d);
+        tp[i] = d;
+    }
+    for (uint64_t i = 0; i < n_queries; i++) {
+        gxy_dseq *d;
+        int rc = dseq_upload_impl(ctx, &queries[i].seq, &d);
+        if (rc) return rc;
+        dq[i].reset(d);
+        qp[i] = d;
+    }
+    return gxy_run_dev(ctx, tp.data(), n_targets, qp.data(), n_queries, cfg, out, stats);
+}
+
+void gxy_record_set_free(gxy_record_set *set) {
+    if (!set) return;
+    std::free(set->records);
+    std::free(set->runs);
+    std::free(set);
+}
+
+}  // extern "C"
+
+// ---- writers ---------------------------------------------------------------
+
+namespace {
+
+// Uppercase ASCII with N for invalid bases (PackedSeq::to_ascii,
+// src/sequences.rs:268-272) of seq[start, start+len), optionally from the
+// reverse complement of the whole sequence (src/sequences.rs:303-317).
+void ascii_slice(const gxy_seq &s, bool rc, uint64_t start, uint64_t len, std::string &out) {
+    static const char dec[4] = {'A', 'C', 'G', 'T'};
+    out.resize(len);
+    for (uint64_t k = 0; k < len; k++) {
+        uint64_t i = start + k;
+        if (rc) i = s.len - 1 - i;
+        const bool v = (s.valid[i >> 3] >> (i & 7)) & 1;
+        uint32_t c = (s.codes[i >> 2] >> (2 * (i & 3))) & 3u;
+        if (rc) c = 3u - c;
+        out[k] = v ? dec[c] : 'N';
+    }
+}
+
+void append_u(std::string &o, uint64_t v) {
+    char b[24];
+    auto r = std::to_chars(b, b + sizeof b, v);
+    o.append(b, r.ptr);
+}
+void append_i(std::string &o, int64_t v) {
+    char b[24];
+    auto r = std::to_chars(b, b + sizeof b, v);
+    o.append(b, r.ptr);
+}
+// Rust `{:>w}` for integers: right-aligned, space padded, never truncated.
+void append_right(std::string &o, uint64_t v, size_t w) {
+    char b[24];
+    auto r = std::to_chars(b, b + sizeof b, v);
+    const size_t n = (size_t)(r.ptr - b);
+    if (n < w) o.append(w - n, ' ');
+    o.append(b, r.ptr);
+}
+// Rust `{:<w}` for strings: left-aligned, padded by characters.
+void append_left(std::string &o, const char *s, size_t w) {
+    const size_t n = std::strlen(s);
+    o.append(s, n);
+    // pad by Unicode scalar values like Rust's formatter
+    size_t chars = 0;
+    for (size_t k = 0; k < n; k++)
+        if (((unsigned char)s[k] & 0xC0) != 0x80) chars++;
+    if (chars < w) o.append(w - chars, ' ');
+}
+
+bool records_ok(uint64_t nt, uint64_t nq, const gxy_record_set *rs, const gxy_named_seq *ts,
+                const gxy_named_seq *qs) {
+    for (uint64_t k = 0; k < rs->n; k++) {
+        const gxy_record &r = rs->records[k];
+        if (r.target_index >= nt || r.query_index >= nq) return false;
+        if (r.runs_offset + r.n_runs > rs->n_runs) return false;
+        if ((uint64_t)r.t_start + r.t_span > ts[r.target_index].seq.len) return false;
+        if ((uint64_t)r.q_start + r.q_span > qs[r.query_index].seq.len) return false;
+    }
+    return true;
+}
+
+char *finish(const std::string &s, uint64_t *len) {
+    char *p = (char *)std::malloc(s.size() + 1);
+    if (!p) return nullptr;
+    std::memcpy(p, s.data(), s.size());
+    p[s.size()] = 0;
+    *len = s.size();
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gxy_format_maf(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,
+                   uint64_t n_queries, const gxy_record_set *records, char **text, uint64_t *len) {
+    if (!records || !text || !len || !records_ok(n_targets, n_queries, records, targets, queries))
+        return GXY_EINVAL;
+    std::string o, ta, qa, tt, qq;
+    if (records->n > 0) o += "##maf version=1 program=lastz-gxy\n# scoring=HOXD70\n";
+    for (uint64_t k = 0; k < records->n; k++) {
+        const gxy_record &r = records->records[k];
+        ascii_slice(targets[r.target_index].seq, false, r.t_start, r.t_span, ta);
+        ascii_slice(queries[r.query_index].seq, r.strand == GXY_STRAND_MINUS, r.q_start, r.q_span, qa);
+        // render_aligned_bases (src/edit_script.rs:140-172)
+        tt.clear();
+        qq.clear();
+        size_t ti = 0, qi = 0;
+        for (uint32_t x = 0; x < r.n_runs; x++) {
+            const gxy_edit_run &run = records->runs[r.runs_offset + x];
+            if (run.op == GXY_OP_MATCH) {
+                tt.append(ta, ti, run.n);
+                qq.append(qa, qi, run.n);
+                ti += run.n;
+                qi += run.n;
+            } else if (run.op == GXY_OP_INSERT) {
+                tt.append(run.n, '-');
+                qq.append(qa, qi, run.n);
+                qi += run.n;
+            } else {
+                tt.append(ta, ti, run.n);
+                qq.append(run.n, '-');
+                ti += run.n;
+            }
+        }
+        o += "a score=";
+        append_i(o, r.score);
+        o += "\ns ";
+        append_left(o, targets[r.target_index].name, 20);
+        o += ' ';
+        append_right(o, r.t_start, 10);
+        o += ' ';
+        append_right(o, r.t_span, 6);
+        o += " + ";
+        append_right(o, (uint32_t)targets[r.target_index].seq.len, 10);
+        o += ' ';
+        o += tt;
+        o += "\ns ";
+        append_left(o, queries[r.query_index].name, 20);
+        o += ' ';
+        append_right(o, r.q_start, 10);
+        o += ' ';
+        append_right(o, r.q_span, 6);
+        o += r.strand == GXY_STRAND_MINUS ? " - " : " + ";
+        append_right(o, (uint32_t)queries[r.query_index].seq.len, 10);
+        o += ' ';
+        o += qq;
+        o += "\n\n";
+    }
+    *text = finish(o, len);
+    return *text ? GXY_OK : GXY_ENOMEM;
+}
+
+int gxy_format_paf(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,
+                   uint64_t n_queries, const gxy_record_set *records, char **text, uint64_t *len) {
+    if (!records || !text || !len || !records_ok(n_targets, n_queries, records, targets, queries))
+        return GXY_EINVAL;
+    std::string o, ta, qa, cig;
+    for (uint64_t k = 0; k < records->n; k++) {
+        const gxy_record &r = records->records[k];
+        ascii_slice(targets[r.target_index].seq, false, r.t_start, r.t_span, ta);
+        ascii_slice(queries[r.query_index].seq, r.strand == GXY_STRAND_MINUS, r.q_start, r.q_span, qa);
+        // score_script (src/output/paf.rs:59-89); positions past a slice are None
+        uint64_t mt = 0, mm = 0, ind = 0;
+        size_t ti = 0, qi = 0;
+        cig.clear();
+        for (uint32_t x = 0; x < r.n_runs; x++) {
+            const gxy_edit_run &run = records->runs[r.runs_offset + x];
+            append_u(cig, run.n);
+            cig += run.op == GXY_OP_MATCH ? 'M' : (run.op == GXY_OP_INSERT ? 'I' : 'D');
+            if (run.op == GXY_OP_MATCH) {
+                for (uint32_t c = 0; c < run.n; c++, ti++, qi++) {
+                    const bool th = ti < ta.size(), qh = qi < qa.size();
+                    const bool eq = (th && qh) ? ta[ti] == qa[qi] : th == qh;
+                    if (eq) mt++; else mm++;
+                }
+            } else if (run.op == GXY_OP_INSERT) {
+                ind += run.n;
+                qi += run.n;
+            } else {
+                ind += run.n;
+                ti += run.n;
+            }
+        }
+        o += queries[r.query_index].name;
+        o += '\t';
+        append_u(o, (uint32_t)queries[r.query_index].seq.len);
+        o += '\t';
+        append_u(o, r.q_start);
+        o += '\t';
+        append_u(o, (uint64_t)r.q_start + r.q_span);
+        o += r.strand == GXY_STRAND_MINUS ? "\t-\t" : "\t+\t";
+        o += targets[r.target_index].name;
+        o += '\t';
+        append_u(o, (uint32_t)targets[r.target_index].seq.len);
+        o += '\t';
+        append_u(o, r.t_start);
+        o += '\t';
+        append_u(o, (uint64_t)r.t_start + r.t_span);
+        o += '\t';
+        append_u(o, mt);
+        o += '\t';
+        append_u(o, mt + mm + ind);
+        o += "\t60\tNM:i:";
+        append_u(o, mm + ind);
+        o += "\tAS:i:";
+        append_i(o, r.score);
+        o += "\ttp:A:P\tcg:Z:";
+        o += cig;
+        o += '\n';
+    }
+    *text = finish(o, len);
+    return *text ? GXY_OK : GXY_ENOMEM;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// internal.h — entry points shared between the library's translation units
+// (not part of the C-ABI).
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+namespace gxy {
+
+// PosTable::build (src/pos_table.rs:35-83) on a device sequence; with
+// respect_mask == 0 soft-masked bases seed like unmasked ones (--nomasking,
+// src/main.rs:26-30).
+int postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern *pattern, uint64_t step,
+                        int respect_mask, gxy_postable **out);
+
+int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
+                    const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
+                    gxy_search_stats *stats);
+
+struct GappedItem {
+    const gxy_dseq *t;
+    const gxy_dseq *q;
+    gxy_anchor a;
+};
+
+struct GappedResult {
+    bool found;
+    gxy_alignment aln;  // runs_offset indexes the call's run vector
+};
+
+int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const gxy_scoring &sc,
+                     const gxy_gapped_params &gp, std::vector<GappedResult> *results,
+                     std::vector<gxy_edit_run> *runs, uint64_t *cells_out);
+
+int choose_anchor_dev(gxy_ctx *ctx, const gxy_dseq *t, const gxy_dseq *q, const gxy_hsp *hsps_host, uint64_t n,
+                      const gxy_scoring &sc, uint32_t window, gxy_anchor *out_host);
+
+}  // namespace gxy

@@ -1,0 +1,272 @@
+// postable.hip — device PosTable (src/pos_table.rs:24-137).
+//
+// Build = one pass that turns every window (0, step, 2*step, ... <= len-span)
+// into a 64-bit item (word << 32 | pos) — invalid windows get the sentinel
+// word 4^w so they sort last — then a stable LSD radix sort on the word bits
+// only.  Items enter in ascending position, so the stable sort leaves each
+// word's positions ascending exactly as the reference's counting-sort scatter
+// does (src/pos_table.rs:60-75).  `starts` is then filled by one thread per
+// sorted item writing the run of words (key[i-1], key[i]] — every entry written
+// exactly once, no atomics.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.h"
+
+using namespace gxy;
+
+namespace {
+
+__global__ void k_table_items(const Blk *__restrict__ seq, uint64_t len, DevPattern pat,
+                              uint64_t step, uint64_t nwin, int respect_mask,
+                              uint64_t *__restrict__ items, unsigned long long *__restrict__ nvalid) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = false;
+    if (w < nwin) {
+        const uint64_t pos = w * step;
+        const uint64_t sentinel = 1ull << (2 * pat.weight);
+        uint64_t word = sentinel;
+        // SeedExtractor::word_at (src/seeds.rs:133-147)
+        if (pos + pat.span <= len) {
+            const Win x = load_win(seq, pos);
+            const uint32_t care = pat.care_mask;
+            const uint32_t msk = respect_mask ? x.mask : 0u;
+            if ((x.valid & care) == care && (msk & care) == 0) {
+                uint64_t v = 0;
+                for (uint32_t k = 0; k < pat.weight; k++)
+                    v = (v << 2) | ((x.codes >> (2 * pat.off[k])) & 3ull);
+                word = v;
+                ok = true;
+            }
+        }
+        items[w] = (word << 32) | (pos & 0xFFFFFFFFull);
+    }
+    // one atomic per wavefront
+    const unsigned long long b = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(nvalid, (unsigned long long)__popcll(b));
+}
+
+__global__ void k_table_starts(const uint64_t *__restrict__ items, uint64_t nvalid, uint64_t num_words,
+                               uint32_t *__restrict__ starts, uint32_t *__restrict__ positions) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nvalid) return;
+    const uint64_t key = i < nvalid ? (items[i] >> 32) : num_words;
+    const int64_t prev = i == 0 ? -1 : (int64_t)(items[i - 1] >> 32);
+    for (int64_t w = prev + 1; w <= (int64_t)key; w++) starts[w] = (uint32_t)i;
+    if (i < nvalid) positions[i] = (uint32_t)items[i];
+}
+
+__global__ void k_prune_counts(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t limit,
+                               uint32_t *__restrict__ keep_cnt) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= num_words) return;
+    const uint32_t c = starts[w + 1] - starts[w];
+    keep_cnt[w] = c <= limit ? c : 0u;  // src/pos_table.rs:128
+}
+
+__global__ void k_prune_copy(const uint32_t *__restrict__ starts, const uint32_t *__restrict__ pos,
+                             const uint32_t *__restrict__ new_starts, const uint32_t *__restrict__ keep_cnt,
+                             uint64_t num_words, uint32_t *__restrict__ new_pos) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= num_words) return;
+    const uint32_t c = keep_cnt[w];
+    const uint32_t s = starts[w], d = new_starts[w];
+    for (uint32_t k = 0; k < c; k++) new_pos[d + k] = pos[s + k];
+}
+
+__global__ void k_set_last(uint32_t *__restrict__ new_starts, const uint32_t *__restrict__ keep_cnt,
+                           uint64_t num_words) {
+    new_starts[num_words] = new_starts[num_words - 1] + keep_cnt[num_words - 1];
+}
+
+inline unsigned grid_for(uint64_t n, unsigned th) { return (unsigned)((n + th - 1) / th); }
+
+}  // namespace
+
+int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern *pattern, uint64_t step,
+                             int respect_mask, gxy_postable **out) {
+    DevPattern pat;
+    if (!make_dev_pattern(pattern, &pat)) return set_err(ctx, GXY_EINVAL, "invalid seed pattern");
+    if (step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");  // src/pos_table.rs:36
+    if (pat.weight > kMaxWeight)
+        return set_err(ctx, GXY_ERANGE, "seed weight %u exceeds the dense-table limit %u", pat.weight,
+                       kMaxWeight);
+    if (t->len > 0xFFFFFFFFull)
+        return set_err(ctx, GXY_ERANGE, "pos_table overflow (> 4 Gbp)");  // src/pos_table.rs:56
+    const uint64_t max_start = t->len >= pat.span ? t->len - pat.span : 0;
+    const uint64_t nwin = max_start / step + 1;
+    if (nwin > 0x7FFFFFFFull) return set_err(ctx, GXY_ERANGE, "too many windows");
+    auto *tab = new (std::nothrow) gxy_postable();
+    if (!tab) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    tab->pattern = pat;
+    tab->step = step;
+    tab->num_words = 1ull << (2 * pat.weight);
+    tab->target = t;
+
+    auto *items = (uint64_t *)scratch(ctx, kSlotHits, nwin * sizeof(uint64_t));
+    auto *items2 = (uint64_t *)scratch(ctx, kSlotPassKeys, nwin * sizeof(uint64_t));
+    auto *cnt = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
+    if (!items || !items2 || !cnt) {
+        delete tab;
+        return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    }
+    auto fail = [&](int code, const char *msg, hipError_t e) {
+        gxy_postable_free(tab);
+        return set_err(ctx, code, "%s: %s", msg, hipGetErrorString(e));
+    };
+    hipError_t e = hipMemsetAsync(cnt, 0, 8, ctx->stream);
+    if (e != hipSuccess) return fail(GXY_EDEVICE, "memset", e);
+    {
+        KernelTimer kt_(ctx, "table_items");
+        k_table_items<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(t->d, t->len, pat, step, nwin, respect_mask,
+                                                                    items, cnt);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_table_items", e);
+    size_t tmp_bytes = 0;
+    const int end_bit = 32 + 2 * (int)pat.weight + 1;
+    (void)hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, items, items2, (int)nwin, 32, end_bit,
+                                      ctx->stream);
+    void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
+    if (!tmp) {
+        gxy_postable_free(tab);
+        return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    }
+    {
+        KernelTimer kt_(ctx, "table_sort");
+        e = hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, items, items2, (int)nwin, 32, end_bit,
+                                              ctx->stream);
+    }
+    if (e != hipSuccess) return fail(GXY_EDEVICE, "radix sort", e);
+    unsigned long long nvalid = 0;
+    if ((e = hipMemcpyAsync(&nvalid, cnt, 8, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return fail(GXY_EDEVICE, "count download", e);
+    tab->npos = nvalid;
+    if (hipMalloc(&tab->d_starts, (tab->num_words + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&tab->d_positions, std::max<uint64_t>(1, nvalid) * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        gxy_postable_free(tab);
+        return set_err(ctx, GXY_ENOMEM, "table allocation failed");
+    }
+    {
+        KernelTimer kt_(ctx, "table_starts");
+        k_table_starts<<<grid_for(nvalid + 1, 256), 256, 0, ctx->stream>>>(items2, nvalid, tab->num_words,
+                                                                            tab->d_starts, tab->d_positions);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_table_starts", e);
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(GXY_EDEVICE, "build", e);
+    *out = tab;
+    return GXY_OK;
+}
+
+extern "C" {
+
+int gxy_postable_build_dev(gxy_ctx *ctx, const gxy_dseq *target, const gxy_seed_pattern *pattern,
+                           uint64_t step, gxy_postable **out) {
+    if (!ctx) return GXY_EINVAL;
+    if (!target || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    return postable_build_impl(ctx, target, pattern, step, 1, out);
+}
+
+int gxy_postable_build(gxy_ctx *ctx, const gxy_seq *target, const gxy_seed_pattern *pattern,
+                       uint64_t step, gxy_postable **out) {
+    if (!ctx) return GXY_EINVAL;
+    if (!target || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    gxy_dseq *d;
+    int rc = dseq_upload_impl(ctx, target, &d);
+    if (rc) return rc;
+    rc = postable_build_impl(ctx, d, pattern, step, 1, out);
+    if (rc) {
+        gxy_dseq_free(d);
+        return rc;
+    }
+    (*out)->owned_target = d;
+    (*out)->host_codes = target->codes;
+    (*out)->host_len = target->len;
+    return GXY_OK;
+}
+
+int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
+    // PosTable::prune_hot_words (src/pos_table.rs:113-136)
+    if (!ctx) return GXY_EINVAL;
+    if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
+    const uint64_t nw = tab->num_words;
+    auto *keep = (uint32_t *)scratch(ctx, kSlotCounts, nw * sizeof(uint32_t));
+    uint32_t *ns = nullptr, *np = nullptr;
+    if (!keep) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    GXY_HIP(ctx, hipMalloc(&ns, (nw + 1) * sizeof(uint32_t)));
+    k_prune_counts<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, limit, keep);
+    GXY_LAUNCH_CHECK(ctx);
+    size_t tmp_bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep, ns, (int)nw, ctx->stream);
+    void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
+    if (!tmp) {
+        (void)hipFree(ns);
+        return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    }
+    GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, keep, ns, (int)nw, ctx->stream));
+    k_set_last<<<1, 1, 0, ctx->stream>>>(ns, keep, nw);
+    GXY_LAUNCH_CHECK(ctx);
+    uint32_t total = 0;
+    GXY_HIP(ctx, hipMemcpyAsync(&total, ns + nw, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GXY_HIP(ctx, hipMalloc(&np, std::max<uint32_t>(1, total) * sizeof(uint32_t)));
+    k_prune_copy<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, ns, keep,
+                                                             nw, np);
+    GXY_LAUNCH_CHECK(ctx);
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(tab->d_starts);
+    (void)hipFree(tab->d_positions);
+    tab->d_starts = ns;
+    tab->d_positions = np;
+    tab->npos = total;
+    return GXY_OK;
+}
+
+int gxy_postable_lookup(gxy_ctx *ctx, const gxy_postable *tab, uint64_t word, uint32_t *out, uint64_t cap,
+                        uint64_t *n_out) {
+    // PosTable::lookup (src/pos_table.rs:102-108)
+    if (!ctx) return GXY_EINVAL;
+    if (!tab || !n_out || (cap && !out)) return set_err(ctx, GXY_EINVAL, "null argument");
+    if (word >= tab->num_words) return set_err(ctx, GXY_EINVAL, "word out of range");
+    uint32_t se[2];
+    GXY_HIP(ctx, hipMemcpyAsync(se, tab->d_starts + word, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint64_t n = se[1] - se[0];
+    *n_out = n;
+    const uint64_t c = std::min(n, cap);
+    if (c) {
+        GXY_HIP(ctx, hipMemcpyAsync(out, tab->d_positions + se[0], c * 4, hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return GXY_OK;
+}
+
+uint64_t gxy_postable_total_positions(const gxy_postable *tab) { return tab ? tab->npos : 0; }
+
+int gxy_postable_export(gxy_ctx *ctx, const gxy_postable *tab, uint32_t *starts, uint64_t starts_cap,
+                        uint32_t *positions, uint64_t positions_cap) {
+    if (!ctx) return GXY_EINVAL;
+    if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
+    if (starts) {
+        const uint64_t n = std::min(starts_cap, tab->num_words + 1);
+        GXY_HIP(ctx, hipMemcpyAsync(starts, tab->d_starts, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (positions) {
+        const uint64_t n = std::min(positions_cap, tab->npos);
+        if (n) GXY_HIP(ctx, hipMemcpyAsync(positions, tab->d_positions, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GXY_OK;
+}
+
+void gxy_postable_free(gxy_postable *tab) {
+    if (!tab) return;
+    if (tab->d_starts) (void)hipFree(tab->d_starts);
+    if (tab->d_positions) (void)hipFree(tab->d_positions);
+    if (tab->owned_target) gxy_dseq_free(tab->owned_target);
+    delete tab;
+}
+
+}  // extern "C"

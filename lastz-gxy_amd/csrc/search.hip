@@ -1,0 +1,634 @@
+// search.hip — seed search and ungapped x-drop (src/seed_search.rs:45-125,
+// src/diag_hash.rs, src/hsp.rs:66-211).
+//
+// Exact parallel form of the reference's sequential scan.  DiagHash state
+// changes only when an extension passes the threshold (and the entropy gate),
+// it is keyed by diagonal, and hits on one diagonal reach it in ascending
+// query position.  So:
+//   1. every seed hit is extended speculatively (its result does not depend on
+//      any state);
+//   2. the passing ones are sorted by (diagonal, q) and each diagonal is
+//      scanned in order: emit iff no earlier emitted HSP on that diagonal ends
+//      beyond the seed's t, then move the cover to max(cover, t_end);
+//   3. the emitted HSPs are stably sorted by (diagonal, t_start), which keeps
+//      seed-scan order among equal keys.
+// Pipeline per call: k_qcount (word + lookup counts per query window) ->
+// exclusive scan -> k_qexpand (materialise (q,t) hits) -> k_extend (one lane
+// per hit; passing hits appended with one atomic per wavefront) -> radix sort
+// by (diag,q) -> k_admit (one lane per diagonal run) -> compaction -> radix
+// sort by (diag,t_start) -> download.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+using namespace gxy;
+
+namespace {
+
+inline unsigned grid_for(uint64_t n, unsigned th) { return (unsigned)((n + th - 1) / th); }
+
+// Variant words of a seed word (src/seed_search.rs:63-90): the word itself,
+// then each single transition (XOR 0b10 at a care position, leftmost care
+// first), then each pair i<j when transitions >= 2.
+struct Variants {
+    uint32_t weight, transitions;
+    __device__ __forceinline__ uint32_t count() const {
+        return transitions == 0 ? 1u : (transitions == 1 ? 1u + weight : 1u + weight + weight * (weight - 1) / 2);
+    }
+    template <class F>
+    __device__ __forceinline__ void each(uint64_t word, F &&f) const {
+        f(word);
+        if (transitions >= 1)
+            for (uint32_t i = 0; i < weight; i++) f(word ^ (2ull << (2 * (weight - 1 - i))));
+        if (transitions >= 2)
+            for (uint32_t i = 0; i < weight; i++)
+                for (uint32_t j = i + 1; j < weight; j++)
+                    f(word ^ (2ull << (2 * (weight - 1 - i))) ^ (2ull << (2 * (weight - 1 - j))));
+    }
+};
+
+__device__ __forceinline__ bool seed_word(const Blk *__restrict__ q, uint64_t qlen, const DevPattern &pat,
+                                          uint64_t pos, int respect_mask, uint64_t *word) {
+    if (pos + pat.span > qlen) return false;
+    const Win x = load_win(q, pos);
+    const uint32_t care = pat.care_mask;
+    const uint32_t msk = respect_mask ? x.mask : 0u;
+    if ((x.valid & care) != care || (msk & care) != 0) return false;
+    uint64_t v = 0;
+    for (uint32_t k = 0; k < pat.weight; k++) v = (v << 2) | ((x.codes >> (2 * pat.off[k])) & 3ull);
+    *word = v;
+    return true;
+}
+
+__global__ void k_qcount(const Blk *__restrict__ q, uint64_t qlen, DevPattern pat, uint64_t step,
+                         uint64_t nwin, int respect_mask, Variants var, const uint32_t *__restrict__ starts,
+                         uint32_t *__restrict__ words, uint32_t *__restrict__ cnt,
+                         unsigned long long *__restrict__ nvalid) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = false;
+    if (w < nwin) {
+        uint64_t word;
+        uint32_t c = 0;
+        if (seed_word(q, qlen, pat, w * step, respect_mask, &word)) {
+            ok = true;
+            var.each(word, [&](uint64_t v) { c += starts[v + 1] - starts[v]; });
+            words[w] = (uint32_t)word;
+        } else {
+            words[w] = kInvalidWord;
+        }
+        cnt[w] = c;
+    }
+    const unsigned long long b = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(nvalid, (unsigned long long)__popcll(b));
+}
+
+struct ToU64 {
+    __host__ __device__ __forceinline__ uint64_t operator()(const uint32_t &x) const { return x; }
+};
+
+__global__ void k_qexpand(uint64_t w0, uint64_t w1, uint64_t step, const uint32_t *__restrict__ words,
+                          const uint64_t *__restrict__ off, uint64_t base, Variants var,
+                          const uint32_t *__restrict__ starts, const uint32_t *__restrict__ positions,
+                          uint64_t *__restrict__ hits) {
+    const uint64_t w = w0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= w1) return;
+    const uint32_t word = words[w];
+    if (word == kInvalidWord) return;
+    uint64_t o = off[w] - base;
+    const uint64_t qhi = (w * step) << 32;
+    var.each(word, [&](uint64_t v) {
+        const uint32_t b = starts[v], e = starts[v + 1];
+        for (uint32_t k = b; k < e; k++) hits[o++] = qhi | positions[k];
+    });
+}
+
+// ---- x-drop extension (src/hsp.rs:98-195) -------------------------------------
+
+// Window of 32 bases ending at p, reversed: index k <-> position p-k.
+__device__ __forceinline__ Win load_win_rev(const Blk *__restrict__ b, uint64_t p) {
+    Win w;
+    if (p >= 31) {
+        const Win x = load_win(b, p - 31);
+        w.codes = rev2(x.codes);
+        w.valid = __builtin_bitreverse32(x.valid);
+    } else {
+        const Win x = load_win(b, 0);
+        const uint32_t sh = 31 - (uint32_t)p;
+        w.codes = rev2(x.codes) >> (2 * sh);
+        w.valid = __builtin_bitreverse32(x.valid) >> sh;
+    }
+    w.mask = 0;
+    return w;
+}
+
+struct SideRes {
+    int32_t best;
+    uint32_t ext;
+};
+
+// One side of the x-drop walk: `maxn` columns available starting at (t0,q0)
+// going forward (dir>0) or backward.  Returns best score and its extent.
+template <class S, bool kFwd>
+__device__ __forceinline__ SideRes xdrop_side(const Blk *__restrict__ T, const Blk *__restrict__ Q,
+                                              const S &sc, int32_t xdrop, uint64_t t0, uint64_t q0,
+                                              uint64_t maxn, uint32_t *cols) {
+    int32_t running = 0, best = 0;
+    uint32_t best_ext = 0, ext = 0;
+    bool stop = false;
+    for (uint64_t off = 0; off < maxn && !stop; off += 32) {
+        const uint32_t n = (uint32_t)min<uint64_t>(32, maxn - off);
+        Win tw, qw;
+        if (kFwd) {
+            tw = load_win(T, t0 + off);
+            qw = load_win(Q, q0 + off);
+        } else {
+            tw = load_win_rev(T, t0 - off);
+            qw = load_win_rev(Q, q0 - off);
+        }
+        const uint32_t vv = tw.valid & qw.valid;
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t tc = (uint32_t)(tw.codes >> (2 * k)) & 3u;
+            const uint32_t qc = (uint32_t)(qw.codes >> (2 * k)) & 3u;
+            const int32_t s = sc(tc, qc, (vv >> k) & 1u);
+            ext++;
+            running = sat_add(running, s);
+            if (running > best) {
+                best = running;
+                best_ext = ext;
+            } else if (sat_sub(best, running) > xdrop) {
+                stop = true;
+                break;
+            }
+        }
+    }
+    *cols += ext;
+    return {best, best_ext};
+}
+
+template <class S>
+__device__ __forceinline__ bool xdrop_hit(const Blk *__restrict__ T, uint64_t tlen, const Blk *__restrict__ Q,
+                                          uint64_t qlen, const S &sc, int32_t xdrop, int32_t thresh,
+                                          uint32_t t_pos, uint32_t q_pos, uint32_t span, gxy_hsp *out,
+                                          uint32_t *cols) {
+    if ((uint64_t)t_pos + span > tlen || (uint64_t)q_pos + span > qlen) return false;  // src/hsp.rs:113-115
+    int32_t seed = 0;
+    for (uint32_t k0 = 0; k0 < span; k0 += 32) {
+        const Win tw = load_win(T, (uint64_t)t_pos + k0);
+        const Win qw = load_win(Q, (uint64_t)q_pos + k0);
+        const uint32_t n = min(32u, span - k0), vv = tw.valid & qw.valid;
+        for (uint32_t k = 0; k < n; k++)
+            seed = sat_add(seed, sc((uint32_t)(tw.codes >> (2 * k)) & 3u, (uint32_t)(qw.codes >> (2 * k)) & 3u,
+                                    (vv >> k) & 1u));
+    }
+    *cols += span;
+    const uint64_t rt = (uint64_t)t_pos + span, rq = (uint64_t)q_pos + span;
+    const uint64_t rmax = (rt < tlen && rq < qlen) ? min(tlen - rt, qlen - rq) : 0;
+    const SideRes r = xdrop_side<S, true>(T, Q, sc, xdrop, rt, rq, rmax, cols);
+    const uint64_t lmax = min((uint64_t)t_pos, (uint64_t)q_pos);
+    const SideRes l = lmax ? xdrop_side<S, false>(T, Q, sc, xdrop, (uint64_t)t_pos - 1, (uint64_t)q_pos - 1,
+                                                  lmax, cols)
+                           : SideRes{0, 0};
+    const int32_t score = sat_add(sat_add(seed, l.best), r.best);
+    if (score < thresh) return false;
+    out->t_start = t_pos - l.ext;
+    out->q_start = q_pos - l.ext;
+    out->length = span + l.ext + r.ext;
+    out->score = score;
+    return true;
+}
+
+// shannon_entropy / passes_entropy_gate (src/entropy.rs:27-66) on the device.
+__device__ bool entropy_pass(const Blk *__restrict__ T, uint64_t tlen, uint32_t t_start, uint32_t t_end,
+                             int32_t score, int32_t threshold) {
+    uint32_t c[4] = {0, 0, 0, 0};
+    uint32_t total = 0;
+    const uint64_t e = min((uint64_t)t_end, tlen);
+    for (uint64_t p = t_start; p < e; p += 32) {
+        const Win w = load_win(T, p);
+        const uint32_t n = (uint32_t)min<uint64_t>(32, e - p);
+        const uint32_t vm = n == 32 ? w.valid : (w.valid & ((1u << n) - 1u));
+        // count codes among valid bases
+        const uint64_t sp = spread2(vm);
+        const uint64_t cc = w.codes & sp;
+        const uint64_t lo = cc & 0x5555555555555555ull, hi = (cc >> 1) & 0x5555555555555555ull;
+        const uint64_t vlo = sp & 0x5555555555555555ull;
+        const uint32_t n3 = __popcll(lo & hi), n1 = __popcll(lo & ~hi), n2 = __popcll(~lo & hi & vlo);
+        const uint32_t nv = __popc(vm);
+        c[1] += n1;
+        c[2] += n2;
+        c[3] += n3;
+        c[0] += nv - n1 - n2 - n3;
+        total += nv;
+    }
+    if (total == 0) return 0.0 * score >= (double)threshold;
+    const double tt = (double)total;
+    double h = 0.0;
+    for (int k = 0; k < 4; k++) {
+        if (c[k] == 0) continue;
+        const double p = (double)c[k] / tt;
+        h -= p * log2(p);
+    }
+    return (double)score * (h / 2.0) >= (double)threshold;
+}
+
+struct PassOut {
+    uint64_t *keys;
+    gxy_hsp *hsp;
+    unsigned long long *count;
+};
+
+template <class S>
+__global__ __launch_bounds__(256) void k_extend(const uint64_t *__restrict__ hits, uint64_t nh,
+                                                const Blk *__restrict__ T, uint64_t tlen,
+                                                const Blk *__restrict__ Q, uint64_t qlen, S sc,
+                                                int32_t xdrop, int32_t thresh, uint32_t span, int entropy_on,
+                                                int32_t entropy_thresh, PassOut po,
+                                                unsigned long long *__restrict__ colsum) {
+    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cols = 0;
+    bool pass = false;
+    gxy_hsp r;
+    uint32_t qp = 0, tp = 0;
+    if (h < nh) {
+        const uint64_t hit = hits[h];
+        qp = (uint32_t)(hit >> 32);
+        tp = (uint32_t)hit;
+        pass = xdrop_hit(T, tlen, Q, qlen, sc, xdrop, thresh, tp, qp, span, &r, &cols);
+        if (pass && entropy_on) pass = entropy_pass(T, tlen, r.t_start, r.t_start + r.length, r.score, entropy_thresh);
+    }
+    // wavefront-aggregated append of passing hits
+    const unsigned long long b = __ballot(pass);
+    unsigned long long base = 0;
+    const int lane = threadIdx.x & 63;
+    if (b) {
+        if (lane == __ffsll((long long)b) - 1) base = atomicAdd(po.count, (unsigned long long)__popcll(b));
+        base = __shfl(base, __ffsll((long long)b) - 1);
+        if (pass) {
+            const uint64_t slot = base + __popcll(b & ((1ull << lane) - 1ull));
+            const uint64_t d = (uint64_t)((int64_t)tp - (int64_t)qp + (int64_t)qlen);
+            po.keys[slot] = (d << 32) | qp;
+            po.hsp[slot] = r;
+        }
+    }
+    // column counter: wave reduction then one atomic
+    unsigned long long c = cols;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0 && c) atomicAdd(colsum, c);
+}
+
+__global__ void k_iota(uint32_t *__restrict__ v, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+// Diagonal admission scan (src/diag_hash.rs:34-53 as used by
+// src/seed_search.rs:95-114) over passing hits sorted by (diag, q).
+__global__ void k_admit(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx,
+                        const gxy_hsp *__restrict__ hsp, uint64_t n, uint64_t qlen, uint32_t *__restrict__ flag) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t d = keys[i] >> 32;
+    if (i > 0 && (keys[i - 1] >> 32) == d) return;  // not the head of its diagonal run
+    bool has = false;
+    uint32_t cover = 0;
+    for (uint64_t j = i; j < n && (keys[j] >> 32) == d; j++) {
+        const uint32_t q = (uint32_t)keys[j];
+        const uint32_t t = (uint32_t)((int64_t)d - (int64_t)qlen + (int64_t)q);
+        const bool emit = !has || t >= cover;
+        flag[j] = emit ? 1u : 0u;
+        if (emit) {
+            const gxy_hsp r = hsp[idx[j]];
+            const uint32_t te = r.t_start + r.length;
+            cover = has ? max(cover, te) : te;
+            has = true;
+        }
+    }
+}
+
+__global__ void k_emit(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx,
+                       const gxy_hsp *__restrict__ hsp, const uint32_t *__restrict__ flag,
+                       const uint32_t *__restrict__ pos, uint64_t n, uint64_t *__restrict__ out_keys,
+                       uint32_t *__restrict__ out_idx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const uint32_t k = idx[i];
+    const uint64_t d = keys[i] >> 32;
+    out_keys[pos[i]] = (d << 32) | hsp[k].t_start;
+    out_idx[pos[i]] = k;
+}
+
+__global__ void k_gather_hsp(const uint32_t *__restrict__ idx, const gxy_hsp *__restrict__ hsp, uint64_t n,
+                             gxy_hsp *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = hsp[idx[i]];
+}
+
+template <class S>
+__global__ void k_extend_batch(const uint32_t *__restrict__ tpos, const uint32_t *__restrict__ qpos, uint64_t n,
+                               const Blk *__restrict__ T, uint64_t tlen, const Blk *__restrict__ Q, uint64_t qlen,
+                               S sc, int32_t xdrop, int32_t thresh, uint32_t span, gxy_hsp *__restrict__ out,
+                               uint8_t *__restrict__ found) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t cols = 0;
+    gxy_hsp r = {0, 0, 0, 0};
+    const bool ok = xdrop_hit(T, tlen, Q, qlen, sc, xdrop, thresh, tpos[i], qpos[i], span, &r, &cols);
+    out[i] = ok ? r : gxy_hsp{0, 0, 0, 0};
+    found[i] = ok ? 1 : 0;
+}
+
+__global__ void k_find_bounds(const uint64_t *__restrict__ off, uint64_t nwin, uint64_t total, uint32_t nb,
+                              uint64_t *__restrict__ bounds) {
+    // bounds[k] = first window w with off[w] >= k*total/nb (k = 0..nb)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nb) return;
+    if (k == nb) {
+        bounds[k] = nwin;
+        return;
+    }
+    const uint64_t target = (uint64_t)((unsigned __int128)total * k / nb);
+    uint64_t lo = 0, hi = nwin;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (off[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    bounds[k] = lo;
+}
+
+}  // namespace
+
+namespace gxy {
+
+// Seed search against a device table and device query.  `respect_mask`
+// mirrors the driver's --nomasking (masks cleared before seeding).
+int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
+                    const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
+                    gxy_search_stats *stats) {
+    if (!tab || !query || !scm || !sp || !result) return set_err(ctx, GXY_EINVAL, "null argument");
+    if (sp->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
+    const gxy_dseq *target = tab->target;
+    const uint64_t tlen = target->len, qlen = query->len;
+    if (tlen + qlen >= (1ull << 32)) return set_err(ctx, GXY_ERANGE, "target+query length must be < 4 Gbp");
+    const DevPattern &pat = tab->pattern;
+    Variants var{pat.weight, std::min<uint32_t>(sp->transitions, 2u)};
+    const uint32_t nvar = var.transitions == 0 ? 1 : (var.transitions == 1 ? 1 + pat.weight
+                                                                            : 1 + pat.weight + pat.weight * (pat.weight - 1) / 2);
+    gxy_search_stats st;
+    std::memset(&st, 0, sizeof st);
+    result->clear();
+
+    const uint64_t max_start = qlen >= pat.span ? qlen - pat.span : 0;
+    const uint64_t nwin = max_start / sp->step + 1;
+    if (nwin > 0x7FFFFFFFull) return set_err(ctx, GXY_ERANGE, "too many query windows");
+    auto *words = (uint32_t *)scratch(ctx, kSlotWords, nwin * 4);
+    auto *cnt = (uint32_t *)scratch(ctx, kSlotCounts, nwin * 4);
+    auto *off = (uint64_t *)scratch(ctx, kSlotOffsets, (nwin + 1) * 8);
+    auto *ctr = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
+    if (!words || !cnt || !off || !ctr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    GXY_HIP(ctx, hipMemsetAsync(ctr, 0, 64, ctx->stream));
+    {
+        KernelTimer kt_(ctx, "qcount");
+        k_qcount<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin, respect_mask, var,
+                                                               tab->d_starts, words, cnt, ctr + 0);
+    }
+    GXY_LAUNCH_CHECK(ctx);
+    size_t tmp_bytes = 0;
+    hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> cnt64(cnt, ToU64());
+    (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, cnt64, off + 1, (int)nwin, ctx->stream);
+    void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
+    if (!tmp) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    GXY_HIP(ctx, hipMemsetAsync(off, 0, 8, ctx->stream));
+    GXY_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, cnt64, off + 1, (int)nwin, ctx->stream));
+    uint64_t H = 0;
+    unsigned long long nvalid = 0;
+    GXY_HIP(ctx, hipMemcpyAsync(&H, off + nwin, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GXY_HIP(ctx, hipMemcpyAsync(&nvalid, ctr + 0, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    st.query_windows = nvalid;
+    st.lookups = nvalid * nvar;
+    st.hits = H;
+    st.extended = H;
+
+    const bool i8 = scoring_is_i8(*scm);
+    const ScorerI8 s8 = make_scorer_i8(*scm);
+    const ScorerI32 s32 = make_scorer_i32(*scm);
+    uint64_t npass = 0;
+    if (H > 0) {
+        // Hits are expanded and extended in batches of at most kCap.
+        const uint64_t kCap = 1ull << 27;
+        const uint32_t nb = (uint32_t)std::max<uint64_t>(1, (H + kCap / 2 - 1) / (kCap / 2));
+        std::vector<uint64_t> bounds(nb + 1);
+        if (nb == 1) {
+            bounds[0] = 0;
+            bounds[1] = nwin;
+        } else {
+            auto *db = (uint64_t *)scratch(ctx, kSlotMisc0, (nb + 1) * 8);
+            if (!db) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+            k_find_bounds<<<grid_for(nb + 1, 64), 64, 0, ctx->stream>>>(off, nwin, H, nb, db);
+            GXY_LAUNCH_CHECK(ctx);
+            GXY_HIP(ctx, hipMemcpyAsync(bounds.data(), db, (nb + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+            GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        std::vector<uint64_t> boff(nb + 1);
+        uint64_t maxb = 0;
+        for (uint32_t k = 0; k <= nb; k++) {
+            GXY_HIP(ctx, hipMemcpyAsync(&boff[k], off + bounds[k], 8, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint32_t k = 0; k < nb; k++) maxb = std::max(maxb, boff[k + 1] - boff[k]);
+        auto *hits = (uint64_t *)scratch(ctx, kSlotHits, maxb * 8);
+        auto *pkeys = (uint64_t *)scratch(ctx, kSlotPassKeys, H * 8);
+        auto *phsp = (gxy_hsp *)scratch(ctx, kSlotPassHsp, H * sizeof(gxy_hsp));
+        if (!hits || !pkeys || !phsp) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)",
+                                                     (unsigned long long)H);
+        PassOut po{pkeys, phsp, ctr + 1};
+        for (uint32_t k = 0; k < nb; k++) {
+            const uint64_t w0 = bounds[k], w1 = bounds[k + 1], nh = boff[k + 1] - boff[k];
+            if (nh == 0) continue;
+            {
+                KernelTimer kt_(ctx, "qexpand");
+                k_qexpand<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(w0, w1, sp->step, words, off, boff[k], var,
+                                                                          tab->d_starts, tab->d_positions, hits);
+            }
+            GXY_LAUNCH_CHECK(ctx);
+            KernelTimer kt_ext(ctx, "extend");
+            if (i8)
+                k_extend<ScorerI8><<<grid_for(nh, 256), 256, 0, ctx->stream>>>(
+                    hits, nh, target->d, tlen, query->d, qlen, s8, sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
+                    sp->entropy_enabled, sp->entropy_threshold, po, ctr + 2);
+            else
+                k_extend<ScorerI32><<<grid_for(nh, 256), 256, 0, ctx->stream>>>(
+                    hits, nh, target->d, tlen, query->d, qlen, s32, sp->hsp.x_drop, sp->hsp.hsp_threshold,
+                    pat.span, sp->entropy_enabled, sp->entropy_threshold, po, ctr + 2);
+            GXY_LAUNCH_CHECK(ctx);
+        }
+        unsigned long long c2[2];
+        GXY_HIP(ctx, hipMemcpyAsync(c2, ctr + 1, 16, hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        npass = c2[0];
+        st.columns = c2[1];
+    }
+    st.passing = npass;
+    if (npass > 0) {
+        auto *pkeys = (uint64_t *)ctx->slot[kSlotPassKeys];
+        auto *phsp = (gxy_hsp *)ctx->slot[kSlotPassHsp];
+        auto *k2 = (uint64_t *)scratch(ctx, kSlotPassKeys2, npass * 8);
+        auto *i1 = (uint32_t *)scratch(ctx, kSlotPassIdx, npass * 4);
+        auto *i2 = (uint32_t *)scratch(ctx, kSlotPassIdx2, npass * 4);
+        auto *flag = (uint32_t *)scratch(ctx, kSlotFlags, npass * 4);
+        auto *pos = (uint32_t *)scratch(ctx, kSlotEmit, (npass + 1) * 4);
+        if (!k2 || !i1 || !i2 || !flag || !pos) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        k_iota<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(i1, npass);
+        GXY_LAUNCH_CHECK(ctx);
+        size_t tb = 0;
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, pkeys, k2, i1, i2, (int)npass, 0, 64, ctx->stream);
+        size_t tb2 = 0;
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, flag, pos, (int)npass + 1, ctx->stream);
+        void *t2 = scratch(ctx, kSlotTmp, std::max(tb, tb2));
+        if (!t2) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        {
+            KernelTimer kt_(ctx, "pass_sort");
+            GXY_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(t2, tb, pkeys, k2, i1, i2, (int)npass, 0, 64, ctx->stream));
+        }
+        {
+            KernelTimer kt_(ctx, "admit");
+            k_admit<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(k2, i2, phsp, npass, qlen, flag);
+        }
+        GXY_LAUNCH_CHECK(ctx);
+        GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(t2, tb2, flag, pos, (int)npass, ctx->stream));
+        uint32_t last_pos = 0, last_flag = 0;
+        GXY_HIP(ctx, hipMemcpyAsync(&last_pos, pos + npass - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipMemcpyAsync(&last_flag, flag + npass - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const uint64_t nemit = (uint64_t)last_pos + last_flag;
+        st.emitted = nemit;
+        // emitted (diag, t_start) keys into pkeys/i1 (free now), sort into k2/i2
+        k_emit<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(k2, i2, phsp, flag, pos, npass, pkeys, i1);
+        GXY_LAUNCH_CHECK(ctx);
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, pkeys, k2, i1, i2, (int)nemit, 0, 64, ctx->stream);
+        t2 = scratch(ctx, kSlotTmp, tb);
+        if (!t2) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        GXY_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(t2, tb, pkeys, k2, i1, i2, (int)nemit, 0, 64, ctx->stream));
+        auto *outd = (gxy_hsp *)scratch(ctx, kSlotMisc1, nemit * sizeof(gxy_hsp));
+        if (!outd) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        k_gather_hsp<<<grid_for(nemit, 256), 256, 0, ctx->stream>>>(i2, phsp, nemit, outd);
+        GXY_LAUNCH_CHECK(ctx);
+        std::vector<gxy_hsp> v(nemit);
+        GXY_HIP(ctx, hipMemcpyAsync(v.data(), outd, nemit * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // Vec::dedup (src/seed_search.rs:123) and the tie count.
+        result->reserve(nemit);
+        for (uint64_t k = 0; k < nemit; k++) {
+            const gxy_hsp &h = v[k];
+            if (!result->empty()) {
+                const gxy_hsp &b = result->back();
+                const bool same_key = (int64_t)h.t_start - h.q_start == (int64_t)b.t_start - b.q_start &&
+                                      h.t_start == b.t_start;
+                const bool equal = same_key && h.length == b.length && h.score == b.score;
+                if (equal) continue;
+                if (same_key) st.tie_pairs++;
+            }
+            result->push_back(h);
+        }
+    }
+    if (stats) *stats = st;
+    return GXY_OK;
+}
+
+}  // namespace gxy
+
+extern "C" {
+
+int gxy_search_dev(gxy_ctx *ctx, const gxy_postable *table, const gxy_dseq *query, const gxy_scoring *scoring,
+                   const gxy_search_params *params, gxy_hsp **out, uint64_t *n_out, gxy_search_stats *stats) {
+    if (!ctx) return GXY_EINVAL;
+    if (!out || !n_out) return set_err(ctx, GXY_EINVAL, "null output");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<gxy_hsp> v;
+    int rc = search_dev_impl(ctx, table, query, scoring, params, 1, &v, stats);
+    if (rc) return rc;
+    *n_out = v.size();
+    *out = (gxy_hsp *)std::malloc(std::max<size_t>(1, v.size()) * sizeof(gxy_hsp));
+    if (!*out) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    if (!v.empty()) std::memcpy(*out, v.data(), v.size() * sizeof(gxy_hsp));
+    return GXY_OK;
+}
+
+int gxy_search(gxy_ctx *ctx, const gxy_postable *table, const gxy_seq *target, const gxy_seq *query,
+               const gxy_scoring *scoring, const gxy_search_params *params, gxy_hsp **out, uint64_t *n_out,
+               gxy_search_stats *stats) {
+    if (!ctx) return GXY_EINVAL;
+    if (!table || !target || !query) return set_err(ctx, GXY_EINVAL, "null argument");
+    if (target->len != table->target->len)
+        return set_err(ctx, GXY_EINVAL, "target is not the sequence the table was built from");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    gxy_dseq *dq;
+    int rc = dseq_upload_impl(ctx, query, &dq);
+    if (rc) return rc;
+    rc = gxy_search_dev(ctx, table, dq, scoring, params, out, n_out, stats);
+    gxy_dseq_free(dq);
+    return rc;
+}
+
+int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *query, const gxy_scoring *scoring,
+                         const gxy_hsp_params *params, const uint32_t *t_pos, const uint32_t *q_pos, uint64_t n,
+                         uint32_t seed_len, gxy_hsp *out, uint8_t *found) {
+    if (!ctx) return GXY_EINVAL;
+    if (!target || !query || !scoring || !params || (n && (!t_pos || !q_pos || !out || !found)))
+        return set_err(ctx, GXY_EINVAL, "null argument");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    if (n == 0) return GXY_OK;
+    gxy_dseq *dt = nullptr, *dq = nullptr;
+    int rc = dseq_upload_impl(ctx, target, &dt);
+    if (rc) return rc;
+    rc = dseq_upload_impl(ctx, query, &dq);
+    if (rc) {
+        gxy_dseq_free(dt);
+        return rc;
+    }
+    auto *dtp = (uint32_t *)scratch(ctx, kSlotMisc0, n * 4);
+    auto *dqp = (uint32_t *)scratch(ctx, kSlotMisc1, n * 4);
+    auto *dout = (gxy_hsp *)scratch(ctx, kSlotMisc2, n * sizeof(gxy_hsp));
+    auto *dfound = (uint8_t *)scratch(ctx, kSlotMisc3, n);
+    hipError_t e = hipSuccess;
+    if (!dtp || !dqp || !dout || !dfound) {
+        rc = set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    } else {
+        e = hipMemcpyAsync(dtp, t_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dqp, q_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) {
+            if (scoring_is_i8(*scoring))
+                k_extend_batch<ScorerI8><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+                    dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i8(*scoring), params->x_drop,
+                    params->hsp_threshold, seed_len, dout, dfound);
+            else
+                k_extend_batch<ScorerI32><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+                    dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i32(*scoring), params->x_drop,
+                    params->hsp_threshold, seed_len, dout, dfound);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(found, dfound, n, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = set_err(ctx, GXY_EDEVICE, "extend_hit_batch: %s", hipGetErrorString(e));
+    }
+    gxy_dseq_free(dt);
+    gxy_dseq_free(dq);
+    return rc;
+}
+
+int gxy_extend_hit(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *query, const gxy_scoring *scoring,
+                   const gxy_hsp_params *params, uint32_t t_pos, uint32_t q_pos, uint32_t seed_len, gxy_hsp *out,
+                   int *found) {
+    if (!ctx) return GXY_EINVAL;
+    if (!out || !found) return set_err(ctx, GXY_EINVAL, "null output");
+    uint8_t f = 0;
+    int rc = gxy_extend_hit_batch(ctx, target, query, scoring, params, &t_pos, &q_pos, 1, seed_len, out, &f);
+    *found = f;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,410 @@
+// seq.hip — context, scratch, and device-resident PackedSeq (block32 layout).
+//
+// Reference: PackedSeq (src/sequences.rs:128-318).  reverse_complement and
+// slice_to_new run as one thread per 32-base output block: each thread loads
+// the (at most two) source blocks its window spans, shifts them into place and
+// for RC reverses the 2-bit groups, so both kernels are single-pass,
+// coalesced and HBM-bound.
+#include <cstdarg>
+#include <mutex>
+
+#include "common.h"
+
+namespace gxy {
+
+static thread_local std::string g_create_err;
+
+int set_err(gxy_ctx *ctx, int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx)
+        ctx->err = buf;
+    else
+        g_create_err = buf;
+    return code;
+}
+
+void *scratch(gxy_ctx *ctx, int k, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (ctx->slot_bytes[k] >= bytes) return ctx->slot[k];
+    if (ctx->slot[k]) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->slot[k]);
+        ctx->slot[k] = nullptr;
+        ctx->slot_bytes[k] = 0;
+    }
+    size_t sz = bytes + bytes / 8;  // headroom so slowly growing inputs do not reallocate
+    sz = (sz + 255) & ~(size_t)255;
+    if (hipMalloc(&ctx->slot[k], sz) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->slot[k] = nullptr;
+        return nullptr;
+    }
+    ctx->slot_bytes[k] = sz;
+    return ctx->slot[k];
+}
+
+static hipEvent_t take_event(gxy_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+KernelTimer::KernelTimer(gxy_ctx *ctx, const char *name) : ctx_(ctx), name_(name) {
+    if (!ctx_->timing) return;
+    a_ = take_event(ctx_);
+    if (a_) (void)hipEventRecord(a_, ctx_->stream);
+}
+
+KernelTimer::~KernelTimer() {
+    if (!a_) return;
+    hipEvent_t b = take_event(ctx_);
+    if (!b) {
+        ctx_->event_pool.push_back(a_);
+        return;
+    }
+    (void)hipEventRecord(b, ctx_->stream);
+    ctx_->spans.push_back({name_, a_, b});
+    if (ctx_->spans.size() > 4096) timing_flush(ctx_);
+}
+
+void timing_flush(gxy_ctx *ctx) {
+    if (ctx->spans.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &sp : ctx->spans) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, sp.a, sp.b);
+        bool found = false;
+        for (auto &t : ctx->totals)
+            if (t.first == sp.name) {
+                t.second.first += ms;
+                t.second.second += 1;
+                found = true;
+                break;
+            }
+        if (!found) ctx->totals.push_back({sp.name, {(double)ms, 1}});
+        ctx->event_pool.push_back(sp.a);
+        ctx->event_pool.push_back(sp.b);
+    }
+    ctx->spans.clear();
+}
+
+// Interleave the reference byte arrays into Blk words on the host.
+static void interleave(const gxy_seq *s, Blk *out, uint64_t nblk) {
+    const uint64_t len = s->len;
+    const uint64_t ncode = (len + 3) / 4, nbit = (len + 7) / 8;
+    for (uint64_t k = 0; k < nblk; k++) {
+        uint64_t c = 0;
+        uint32_t v = 0, m = 0;
+        const uint64_t cb = 8 * k, bb = 4 * k;
+        if (cb + 8 <= ncode) {
+            std::memcpy(&c, s->codes + cb, 8);
+        } else {
+            for (uint64_t x = cb; x < ncode && x < cb + 8; x++) c |= (uint64_t)s->codes[x] << (8 * (x - cb));
+        }
+        if (bb + 4 <= nbit) {
+            std::memcpy(&v, s->valid + bb, 4);
+            std::memcpy(&m, s->mask + bb, 4);
+        } else {
+            for (uint64_t x = bb; x < nbit && x < bb + 4; x++) {
+                v |= (uint32_t)s->valid[x] << (8 * (x - bb));
+                m |= (uint32_t)s->mask[x] << (8 * (x - bb));
+            }
+        }
+        // Clear everything at or past `len` (block carries 32 bases).
+        const uint64_t first = 32 * k;
+        if (first >= len) {
+            c = 0;
+            v = m = 0;
+        } else if (len - first < 32) {
+            const uint32_t keep = (uint32_t)(len - first);
+            v &= (1u << keep) - 1u;
+            m &= (1u << keep) - 1u;
+            c &= (keep == 32) ? ~0ull : ((1ull << (2 * keep)) - 1ull);
+        }
+        m &= v;                  // mask is only ever set on valid bases
+        c &= spread2(v);    // invalid bases carry code 0
+        out[k] = {c, v, m};
+    }
+}
+
+}  // namespace gxy
+
+using namespace gxy;
+
+namespace {
+
+__global__ void k_rc(const Blk *__restrict__ src, Blk *__restrict__ dst, uint64_t len, uint64_t nblk) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    const uint64_t o0 = 32 * b;  // first output position of this block
+    if (o0 >= len) {
+        dst[b] = {0, 0, 0};
+        return;
+    }
+    // Output k (k < 32) comes from source position len-1-(o0+k).  The 32
+    // sources are the window [s0, s0+32) with s0 = len-32-o0 (may be < 0).
+    const int64_t s0 = (int64_t)len - 32 - (int64_t)o0;
+    Win w;
+    if (s0 >= 0) {
+        w = load_win(src, (uint64_t)s0);
+    } else {
+        // Fewer than 32 sources remain: they sit at [0, 32+s0), shifted up.
+        const Win z = load_win(src, 0);
+        const uint32_t sh = (uint32_t)(-s0);
+        w.codes = z.codes << (2 * sh);
+        w.valid = z.valid << sh;
+        w.mask = z.mask << sh;
+    }
+    Blk out;
+    out.valid = __builtin_bitreverse32(w.valid);
+    out.mask = __builtin_bitreverse32(w.mask) & out.valid;
+    out.codes = (~rev2(w.codes)) & spread2(out.valid);  // complement2 (src/dna.rs:35-37)
+    const uint64_t rem = len - o0;
+    if (rem < 32) {
+        const uint32_t keep = (uint32_t)rem;
+        out.valid &= (1u << keep) - 1u;
+        out.mask &= (1u << keep) - 1u;
+        out.codes &= (1ull << (2 * keep)) - 1ull;
+    }
+    dst[b] = out;
+}
+
+__global__ void k_slice(const Blk *__restrict__ src, Blk *__restrict__ dst, uint64_t start,
+                        uint64_t len, uint64_t nblk) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    const uint64_t o0 = 32 * b;
+    if (o0 >= len) {
+        dst[b] = {0, 0, 0};
+        return;
+    }
+    Win w = load_win(src, start + o0);
+    const uint64_t rem = len - o0;
+    if (rem < 32) {
+        const uint32_t keep = (uint32_t)rem;
+        w.valid &= (1u << keep) - 1u;
+        w.mask &= (1u << keep) - 1u;
+        w.codes &= (1ull << (2 * keep)) - 1ull;
+    }
+    dst[b] = {w.codes, w.valid, w.mask};
+}
+
+int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
+    auto *s = new (std::nothrow) gxy_dseq();
+    if (!s) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    s->len = len;
+    s->nblk = nblocks(len);
+    s->device = ctx->device;
+    const size_t bytes = s->nblk * sizeof(Blk);
+    if (hipMalloc(&s->d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete s;
+        return set_err(ctx, GXY_ENOMEM, "device allocation of %llu bytes failed",
+                       (unsigned long long)bytes);
+    }
+    *out = s;
+    return GXY_OK;
+}
+
+}  // namespace
+
+int gxy::dseq_upload_impl(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out) {
+    if (!seq || !out || (seq->len && (!seq->codes || !seq->valid || !seq->mask)))
+        return set_err(ctx, GXY_EINVAL, "gxy_dseq_upload: invalid sequence view");
+    gxy_dseq *s;
+    int rc = dseq_alloc(ctx, seq->len, &s);
+    if (rc) return rc;
+    std::vector<Blk> host(s->nblk);
+    interleave(seq, host.data(), s->nblk);
+    hipError_t e = hipMemcpyAsync(s->d, host.data(), s->nblk * sizeof(Blk), hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        gxy_dseq_free(s);
+        return set_err(ctx, GXY_EDEVICE, "upload failed: %s", hipGetErrorString(e));
+    }
+    *out = s;
+    return GXY_OK;
+}
+
+extern "C" {
+
+const char *gxy_version(void) { return "gxy-mi355x 0.1 (gfx950)"; }
+
+int gxy_ctx_create(int device, gxy_ctx **out) {
+    if (!out) return set_err(nullptr, GXY_EINVAL, "gxy_ctx_create: out is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return set_err(nullptr, GXY_EDEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return set_err(nullptr, GXY_EINVAL, "device %d out of range", device);
+    if (hipSetDevice(device) != hipSuccess) return set_err(nullptr, GXY_EDEVICE, "hipSetDevice failed");
+    auto *c = new (std::nothrow) gxy_ctx();
+    if (!c) return set_err(nullptr, GXY_ENOMEM, "host allocation failed");
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_err(nullptr, GXY_EDEVICE, "hipStreamCreate failed");
+    }
+    *out = c;
+    return GXY_OK;
+}
+
+int gxy_ctx_set_timing(gxy_ctx *ctx, int enabled) {
+    if (!ctx) return GXY_EINVAL;
+    ctx->timing = enabled != 0;
+    return GXY_OK;
+}
+
+int gxy_ctx_kernel_time(gxy_ctx *ctx, const char *name, double *ms, uint64_t *launches) {
+    if (!ctx || !name || !ms || !launches) return GXY_EINVAL;
+    timing_flush(ctx);
+    *ms = 0.0;
+    *launches = 0;
+    for (auto &t : ctx->totals)
+        if (t.first == name) {
+            *ms = t.second.first;
+            *launches = t.second.second;
+        }
+    return GXY_OK;
+}
+
+int gxy_ctx_reset_timing(gxy_ctx *ctx) {
+    if (!ctx) return GXY_EINVAL;
+    timing_flush(ctx);
+    ctx->totals.clear();
+    return GXY_OK;
+}
+
+void gxy_ctx_destroy(gxy_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    timing_flush(ctx);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (int k = 0; k < gxy_ctx::kSlots; k++)
+        if (ctx->slot[k]) (void)hipFree(ctx->slot[k]);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *gxy_last_error(const gxy_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int gxy_ctx_synchronize(gxy_ctx *ctx) {
+    if (!ctx) return GXY_EINVAL;
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GXY_OK;
+}
+
+int gxy_pack_ascii(const char *ascii, uint64_t len, uint8_t *codes, uint8_t *valid, uint8_t *mask) {
+    // PackedSeq::from_ascii / push_ascii (src/sequences.rs:164-206) with
+    // encode_base (src/dna.rs:12-20).
+    if ((len && !ascii) || !codes || !valid || !mask) return GXY_EINVAL;
+    std::memset(codes, 0, (len + 3) / 4);
+    std::memset(valid, 0, (len + 7) / 8);
+    std::memset(mask, 0, (len + 7) / 8);
+    for (uint64_t i = 0; i < len; i++) {
+        const unsigned char b = (unsigned char)ascii[i];
+        uint8_t c = 0;
+        bool v = true;
+        switch (b) {
+            case 'A': case 'a': c = 0; break;
+            case 'C': case 'c': c = 1; break;
+            case 'G': case 'g': c = 2; break;
+            case 'T': case 't': case 'U': case 'u': c = 3; break;
+            default: v = false; break;
+        }
+        if (!v) continue;
+        codes[i >> 2] |= (uint8_t)(c << (2 * (i & 3)));
+        valid[i >> 3] |= (uint8_t)(1u << (i & 7));
+        if (b >= 'a' && b <= 'z') mask[i >> 3] |= (uint8_t)(1u << (i & 7));
+    }
+    return GXY_OK;
+}
+
+int gxy_dseq_upload(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out) {
+    if (!ctx) return GXY_EINVAL;
+    return dseq_upload_impl(ctx, seq, out);
+}
+
+int gxy_dseq_reverse_complement(gxy_ctx *ctx, const gxy_dseq *seq, gxy_dseq **out) {
+    if (!ctx || !seq || !out) return ctx ? set_err(ctx, GXY_EINVAL, "null argument") : GXY_EINVAL;
+    gxy_dseq *r;
+    int rc = dseq_alloc(ctx, seq->len, &r);
+    if (rc) return rc;
+    const unsigned th = 256;
+    k_rc<<<(unsigned)((r->nblk + th - 1) / th), th, 0, ctx->stream>>>(seq->d, r->d, seq->len, r->nblk);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        gxy_dseq_free(r);
+        return set_err(ctx, GXY_EDEVICE, "reverse complement failed: %s", hipGetErrorString(e));
+    }
+    *out = r;
+    return GXY_OK;
+}
+
+int gxy_dseq_slice(gxy_ctx *ctx, const gxy_dseq *seq, uint64_t start, uint64_t end, gxy_dseq **out) {
+    if (!ctx || !seq || !out) return ctx ? set_err(ctx, GXY_EINVAL, "null argument") : GXY_EINVAL;
+    if (start > end || end > seq->len)  // src/sequences.rs:278
+        return set_err(ctx, GXY_EINVAL, "slice [%llu, %llu) out of range (len %llu)",
+                       (unsigned long long)start, (unsigned long long)end,
+                       (unsigned long long)seq->len);
+    gxy_dseq *r;
+    int rc = dseq_alloc(ctx, end - start, &r);
+    if (rc) return rc;
+    const unsigned th = 256;
+    k_slice<<<(unsigned)((r->nblk + th - 1) / th), th, 0, ctx->stream>>>(seq->d, r->d, start, end - start,
+                                                                          r->nblk);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        gxy_dseq_free(r);
+        return set_err(ctx, GXY_EDEVICE, "slice failed: %s", hipGetErrorString(e));
+    }
+    *out = r;
+    return GXY_OK;
+}
+
+uint64_t gxy_dseq_len(const gxy_dseq *seq) { return seq ? seq->len : 0; }
+
+int gxy_dseq_download(gxy_ctx *ctx, const gxy_dseq *seq, uint8_t *codes, uint8_t *valid, uint8_t *mask) {
+    if (!ctx || !seq || !codes || !valid || !mask) return GXY_EINVAL;
+    std::vector<Blk> host(seq->nblk);
+    GXY_HIP(ctx, hipMemcpyAsync(host.data(), seq->d, seq->nblk * sizeof(Blk), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint64_t len = seq->len, ncode = (len + 3) / 4, nbit = (len + 7) / 8;
+    for (uint64_t x = 0; x < ncode; x++) codes[x] = (uint8_t)(host[x / 8].codes >> (8 * (x % 8)));
+    for (uint64_t x = 0; x < nbit; x++) {
+        valid[x] = (uint8_t)(host[x / 4].valid >> (8 * (x % 4)));
+        mask[x] = (uint8_t)(host[x / 4].mask >> (8 * (x % 4)));
+    }
+    return GXY_OK;
+}
+
+void gxy_dseq_free(gxy_dseq *seq) {
+    if (!seq) return;
+    if (seq->d) (void)hipFree(seq->d);
+    delete seq;
+}
+
+void gxy_free(void *p) { std::free(p); }
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+"""Deterministic synthetic workloads (numpy, seeded).
+
+* `lcg_dna` — the reference bench generator (benches/hotpath.rs:22-30):
+  x <- x*6364136223846793005 + 1442695040888963407 (mod 2^64), base
+  ACGT[(x >> 33) & 3].
+* `mutate` — per source base: with probability `indel` an indel event
+  (insertion or deletion with equal odds, length Geometric(1/3), mean 3,
+  inserted bases uniform uppercase); otherwise a substitution with probability
+  `sub` to one of the other three bases.  Case and N carry over from source
+  bases.
+* `pair_30k` — 30 kbp LCG target, query = target at 20 % substitutions and 1 %
+  indel events (the small gapped pair).
+* `chromosome_pair` — uniform target with 50 % soft-masked blocks
+  (LogNormal median 300, sigma 1) and 2 % N-runs (LogNormal median 1 kbp,
+  sigma 1); query = target plus segmental duplications (copies of target
+  segments, LogNormal median 5 kbp, half reverse-complemented, inserted at
+  random points until the query is 1.05x), then 15 % substitutions and 2 %
+  indel events.  The 48 Mbp default is the chromosome-scale HSP-only workload.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from gxy_types import PackedSeq
+
+_M64 = (1 << 64) - 1
+
+
+def lcg_dna(n: int, seed: int) -> bytes:
+    """synthetic_dna (benches/hotpath.rs:22-30); vectorised by jumping the LCG."""
+    a, c = 6364136223846793005, 1442695040888963407
+    out = np.empty(n, np.uint8)
+    # advance in blocks of 4096 lanes: x_{k+B} = A_B x_k + C_B
+    B = min(4096, max(1, n))
+    xs = np.empty(B, np.uint64)
+    x = seed & _M64
+    for k in range(B):
+        x = (x * a + c) & _M64
+        xs[k] = x
+    AB, CB = 1, 0
+    for _ in range(B):
+        AB, CB = (AB * a) & _M64, (CB * a + c) & _M64
+    AB, CB = np.uint64(AB), np.uint64(CB)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    with np.errstate(over="ignore"):
+        for s in range(0, n, B):
+            m = min(B, n - s)
+            out[s:s + m] = acgt[((xs[:m] >> np.uint64(33)) & np.uint64(3)).astype(np.int64)]
+            xs = xs * AB + CB
+    return out.tobytes()
+
+
+def mutate(code, valid, mask, rng, sub: float, indel: float):
+    """Return mutated (code, valid, mask) arrays (see module docstring)."""
+    n = len(code)
+    u = rng.random(n)
+    is_indel = u < indel
+    is_sub = (~is_indel) & (rng.random(n) < sub)
+    new = code.copy()
+    off = rng.integers(1, 4, n).astype(np.uint8)
+    new[is_sub] = (code[is_sub] + off[is_sub]) % 4
+    is_ins = is_indel & (rng.random(n) < 0.5)
+    is_del = is_indel & ~is_ins
+    L = rng.geometric(1.0 / 3.0, n)
+    cov = np.zeros(n + 1, np.int64)
+    di = np.nonzero(is_del)[0]
+    np.add.at(cov, di, 1)
+    np.add.at(cov, np.minimum(di + L[di], n), -1)
+    keep = np.cumsum(cov[:n]) == 0
+    ins = np.where(is_ins, L, 0)
+    counts = keep.astype(np.int64) + ins
+    total = int(counts.sum())
+    idx = np.repeat(np.arange(n), counts)
+    starts = np.cumsum(counts) - counts
+    rank = np.arange(total) - starts[idx]
+    is_src = keep[idx] & (rank == 0)
+    rnd = rng.integers(0, 4, total).astype(np.uint8)
+    oc = np.where(is_src, new[idx], rnd).astype(np.uint8)
+    ov = np.where(is_src, valid[idx], True)
+    om = np.where(is_src, mask[idx], False)
+    return oc, ov, om
+
+
+def pair_30k(seed: int = 1, n: int = 30000):
+    """Small gapped pair: (target, query) PackedSeqs."""
+    t = PackedSeq.from_ascii(lcg_dna(n, seed))
+    rng = np.random.default_rng(seed)
+    c, v, m = t.unpack()
+    qc, qv, qm = mutate(c, v, m, rng, 0.20, 0.01)
+    return t, PackedSeq.from_arrays(qc, qv, qm)
+
+
+def _lognormal_runs(rng, median: float, sigma: float, total: int):
+    lens = []
+    s = 0
+    while s < total:
+        x = np.maximum(1, rng.lognormal(np.log(median), sigma, 4096).astype(np.int64))
+        lens.append(x)
+        s += int(x.sum())
+    return np.concatenate(lens)
+
+
+def chromosome_pair(n: int = 48_000_000, seed: int = 7, masked: float = 0.5, nfrac: float = 0.02,
+                    dup_frac: float = 0.05, sub: float = 0.15, indel: float = 0.02):
+    """Chromosome-scale pair (target, query) PackedSeqs (see module docstring)."""
+    rng = np.random.default_rng(seed)
+    code = rng.integers(0, 4, n).astype(np.uint8)
+    valid = np.ones(n, bool)
+    mask = np.zeros(n, bool)
+    if masked > 0:
+        # alternate unmasked / masked blocks; masked share set by scaling
+        runs = _lognormal_runs(rng, 300.0, 1.0, 2 * n)
+        scale_m = masked / (1 - masked)
+        ends = np.cumsum(np.where(np.arange(len(runs)) % 2 == 1,
+                                  np.maximum(1, (runs * scale_m).astype(np.int64)), runs))
+        ends = ends[ends < n]
+        flags = np.zeros(n + 1, np.int8)
+        flags[ends] = 1
+        state = np.cumsum(flags[:n]) % 2 == 1
+        mask = state
+    if nfrac > 0:
+        mean_len = 1000.0 * np.exp(0.5)
+        k = max(1, int(nfrac * n / mean_len))
+        starts = rng.integers(0, n, k)
+        lens = np.maximum(1, rng.lognormal(np.log(1000.0), 1.0, k).astype(np.int64))
+        cov = np.zeros(n + 1, np.int64)
+        np.add.at(cov, starts, 1)
+        np.add.at(cov, np.minimum(starts + lens, n), -1)
+        isn = np.cumsum(cov[:n]) > 0
+        valid = ~isn
+        code = np.where(valid, code, 0).astype(np.uint8)
+        mask = mask & valid
+    target = PackedSeq.from_arrays(code, valid, mask)
+    # segmental duplications
+    pieces_c, pieces_v, pieces_m = [], [], []
+    need = int(dup_frac * n)
+    dups = []
+    got = 0
+    while got < need:
+        L = int(max(100, min(n // 2, rng.lognormal(np.log(5000.0), 1.0))))
+        s = int(rng.integers(0, n - L))
+        rc = bool(rng.random() < 0.5)
+        at = int(rng.integers(0, n))
+        dups.append((at, s, L, rc))
+        got += L
+    dups.sort()
+    prev = 0
+    for at, s, L, rc in dups:
+        pieces_c.append(code[prev:at])
+        pieces_v.append(valid[prev:at])
+        pieces_m.append(mask[prev:at])
+        c, v, m = code[s:s + L], valid[s:s + L], mask[s:s + L]
+        if rc:
+            c, v, m = np.where(v, 3 - c, 0)[::-1].astype(np.uint8), v[::-1], m[::-1]
+        pieces_c.append(c)
+        pieces_v.append(v)
+        pieces_m.append(m)
+        prev = at
+    pieces_c.append(code[prev:])
+    pieces_v.append(valid[prev:])
+    pieces_m.append(mask[prev:])
+    qc0 = np.concatenate(pieces_c)
+    qv0 = np.concatenate(pieces_v)
+    qm0 = np.concatenate(pieces_m)
+    qc, qv, qm = mutate(qc0, qv0, qm0, rng, sub, indel)
+    return target, PackedSeq.from_arrays(qc, qv, qm)

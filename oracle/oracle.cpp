@@ -1147,6 +1147,7 @@ int gxo_run(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_se
         rs.hits += u.st.hits;
         rs.columns += u.st.columns;
         rs.tie_pairs += u.st.tie_pairs;
+        rs.passing += u.st.passing;
         rs.hsps += u.st.emitted;
         rs.dp_problems += u.dp;
         rs.dp_cells += u.cells;
