@@ -178,6 +178,54 @@ inline ScorerI32 make_scorer_i32(const gxy_scoring &m) {
     return s;
 }
 
+// ---- byte-per-base layout used by the x-drop kernels ---------------------------
+//
+// One byte per base: the 2-bit code for a valid base, 0x10 for an invalid one
+// (the flag survives the index arithmetic below).  Forward and reversed
+// copies let a backward walk read forward.  Arrays carry kBytePad bytes of
+// padding so a 32-base window read at any in-range position stays in bounds.
+constexpr uint64_t kBytePad = 64;
+constexpr uint8_t kInvalidByte = 0x10;
+
+// Scores of four columns from four target bytes and four query bytes, as four
+// signed bytes: idx = t*4+q selects from the packed int8 rows r0..r3 with two
+// byte permutes, invalid columns take `amb4` (ambig replicated).
+struct ByteLut {
+    uint32_t r0, r1, r2, r3, amb4;
+};
+
+__device__ __forceinline__ uint32_t score4(const ByteLut &L, uint32_t t4, uint32_t q4) {
+    const uint32_t idx = (t4 << 2) | q4;
+    const uint32_t sel = idx & 0x07070707u;
+    const uint32_t lo = __builtin_amdgcn_perm(L.r1, L.r0, sel);
+    const uint32_t hi = __builtin_amdgcn_perm(L.r3, L.r2, sel);
+    const uint32_t hm = __builtin_amdgcn_perm(0u, 0u, ((idx >> 3) & 0x01010101u) | 0x0C0C0C0Cu);
+    const uint32_t s = (hi & hm) | (lo & ~hm);
+    const uint32_t im = __builtin_amdgcn_perm(0u, 0u, (((idx >> 4) | (idx >> 6)) & 0x01010101u) | 0x0C0C0C0Cu);
+    return (L.amb4 & im) | (s & ~im);
+}
+
+// 32 consecutive bytes starting at any byte offset.
+__device__ __forceinline__ void load_bytes32(const uint8_t *__restrict__ b, uint64_t pos, uint32_t w[8]) {
+    const uint32_t *p = (const uint32_t *)(b + (pos & ~3ull));
+    const uint32_t sh = (uint32_t)(pos & 3);
+    uint32_t d[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) d[i] = p[i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+}
+
+inline ByteLut make_byte_lut(const gxy_scoring &m) {
+    const ScorerI8 s = make_scorer_i8(m);
+    const uint32_t a = (uint32_t)(uint8_t)(int8_t)m.ambig;
+    return ByteLut{s.r0, s.r1, s.r2, s.r3, a * 0x01010101u};
+}
+
+inline bool scoring_byte_ok(const gxy_scoring &m) {
+    return scoring_is_i8(m) && m.ambig >= -128 && m.ambig <= 127;
+}
+
 // ---- host side ---------------------------------------------------------------
 
 inline uint64_t nblocks(uint64_t len) { return (len + 31) / 32 + 1; }
@@ -203,6 +251,9 @@ struct gxy_postable {
     gxy_dseq *owned_target = nullptr; // set when the table uploaded it itself
     const void *host_codes = nullptr; // identity of the host target (gxy_search)
     uint64_t host_len = 0;
+    uint32_t *d_nonempty = nullptr;   // bit w set when word w has positions
+    uint8_t *d_tbytes = nullptr;      // target, one byte per base (x-drop layout)
+    uint8_t *d_tbytes_rev = nullptr;  // the same, reversed
 };
 
 struct gxy_ctx {
@@ -210,7 +261,7 @@ struct gxy_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     // Growable scratch slots (device memory), reused across calls.
-    static constexpr int kSlots = 24;
+    static constexpr int kSlots = 32;
     void *slot[kSlots] = {};
     size_t slot_bytes[kSlots] = {};
     void *pinned = nullptr;
@@ -272,6 +323,10 @@ enum Slot {
     kSlotDpTb,
     kSlotDpRows,
     kSlotDpRuns,
+    kSlotQBytes,
+    kSlotQBytesRev,
+    kSlotTBytes,
+    kSlotTBytesRev,
 };
 
 }  // namespace gxy

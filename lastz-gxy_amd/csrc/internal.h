@@ -14,6 +14,10 @@ namespace gxy {
 int postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern *pattern, uint64_t step,
                         int respect_mask, gxy_postable **out);
 
+// Forward and reversed byte-per-base copies of `s` (x-drop layout, see
+// common.h); each buffer must hold s->len + kBytePad bytes.
+int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *fwd, uint8_t *rev);
+
 int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
                     const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
                     gxy_search_stats *stats);

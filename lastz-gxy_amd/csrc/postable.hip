@@ -55,6 +55,21 @@ __global__ void k_table_starts(const uint64_t *__restrict__ items, uint64_t nval
     if (i < nvalid) positions[i] = (uint32_t)items[i];
 }
 
+// Bitmap of non-empty words: one thread per 32 words.
+__global__ void k_nonempty(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t *__restrict__ bits) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g * 32 >= num_words) return;
+    uint32_t m = 0;
+    const uint64_t w0 = g * 32;
+    uint32_t prev = starts[w0];
+    for (uint32_t k = 0; k < 32 && w0 + k < num_words; k++) {
+        const uint32_t nx = starts[w0 + k + 1];
+        if (nx != prev) m |= 1u << k;
+        prev = nx;
+    }
+    bits[g] = m;
+}
+
 __global__ void k_prune_counts(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t limit,
                                uint32_t *__restrict__ keep_cnt) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -153,6 +168,21 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
                                                                             tab->d_starts, tab->d_positions);
     }
     if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_table_starts", e);
+    const uint64_t nbw = (tab->num_words + 31) / 32;
+    const uint64_t nbytes = t->len + kBytePad;
+    if (hipMalloc(&tab->d_nonempty, nbw * 4) != hipSuccess || hipMalloc(&tab->d_tbytes, nbytes) != hipSuccess ||
+        hipMalloc(&tab->d_tbytes_rev, nbytes) != hipSuccess) {
+        (void)hipGetLastError();
+        gxy_postable_free(tab);
+        return set_err(ctx, GXY_ENOMEM, "table allocation failed");
+    }
+    k_nonempty<<<grid_for(nbw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words, tab->d_nonempty);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_nonempty", e);
+    int rc = seq_to_bytes(ctx, t, tab->d_tbytes, tab->d_tbytes_rev);
+    if (rc) {
+        gxy_postable_free(tab);
+        return rc;
+    }
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(GXY_EDEVICE, "build", e);
     *out = tab;
     return GXY_OK;
@@ -221,6 +251,9 @@ int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
     tab->d_starts = ns;
     tab->d_positions = np;
     tab->npos = total;
+    k_nonempty<<<grid_for((nw + 31) / 32, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_nonempty);
+    GXY_LAUNCH_CHECK(ctx);
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GXY_OK;
 }
 
@@ -265,6 +298,9 @@ void gxy_postable_free(gxy_postable *tab) {
     if (!tab) return;
     if (tab->d_starts) (void)hipFree(tab->d_starts);
     if (tab->d_positions) (void)hipFree(tab->d_positions);
+    if (tab->d_nonempty) (void)hipFree(tab->d_nonempty);
+    if (tab->d_tbytes) (void)hipFree(tab->d_tbytes);
+    if (tab->d_tbytes_rev) (void)hipFree(tab->d_tbytes_rev);
     if (tab->owned_target) gxy_dseq_free(tab->owned_target);
     delete tab;
 }

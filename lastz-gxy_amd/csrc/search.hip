@@ -30,6 +30,10 @@ namespace {
 
 inline unsigned grid_for(uint64_t n, unsigned th) { return (unsigned)((n + th - 1) / th); }
 
+// Persistent x-drop grid: enough 256-lane blocks to fill every CU several
+// times over; lanes stride over the hits.
+constexpr unsigned kXdropBlocks = 2048;
+
 // Variant words of a seed word (src/seed_search.rs:63-90): the word itself,
 // then each single transition (XOR 0b10 at a care position, leftmost care
 // first), then each pair i<j when transitions >= 2.
@@ -63,10 +67,16 @@ __device__ __forceinline__ bool seed_word(const Blk *__restrict__ q, uint64_t ql
     return true;
 }
 
+// The bitmap of non-empty words (4^w bits: 2 MiB for 12of19) stays in L2, so
+// the ~3/4 of variant lookups that find nothing never touch `starts`.
+__device__ __forceinline__ bool nonempty(const uint32_t *__restrict__ bits, uint64_t v) {
+    return (bits[v >> 5] >> (v & 31)) & 1u;
+}
+
 __global__ void k_qcount(const Blk *__restrict__ q, uint64_t qlen, DevPattern pat, uint64_t step,
                          uint64_t nwin, int respect_mask, Variants var, const uint32_t *__restrict__ starts,
-                         uint32_t *__restrict__ words, uint32_t *__restrict__ cnt,
-                         unsigned long long *__restrict__ nvalid) {
+                         const uint32_t *__restrict__ bits, uint32_t *__restrict__ words,
+                         uint32_t *__restrict__ cnt, unsigned long long *__restrict__ nvalid) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = false;
     if (w < nwin) {
@@ -74,7 +84,9 @@ __global__ void k_qcount(const Blk *__restrict__ q, uint64_t qlen, DevPattern pa
         uint32_t c = 0;
         if (seed_word(q, qlen, pat, w * step, respect_mask, &word)) {
             ok = true;
-            var.each(word, [&](uint64_t v) { c += starts[v + 1] - starts[v]; });
+            var.each(word, [&](uint64_t v) {
+                if (nonempty(bits, v)) c += starts[v + 1] - starts[v];
+            });
             words[w] = (uint32_t)word;
         } else {
             words[w] = kInvalidWord;
@@ -91,8 +103,8 @@ struct ToU64 {
 
 __global__ void k_qexpand(uint64_t w0, uint64_t w1, uint64_t step, const uint32_t *__restrict__ words,
                           const uint64_t *__restrict__ off, uint64_t base, Variants var,
-                          const uint32_t *__restrict__ starts, const uint32_t *__restrict__ positions,
-                          uint64_t *__restrict__ hits) {
+                          const uint32_t *__restrict__ starts, const uint32_t *__restrict__ bits,
+                          const uint32_t *__restrict__ positions, uint64_t *__restrict__ hits) {
     const uint64_t w = w0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= w1) return;
     const uint32_t word = words[w];
@@ -100,6 +112,7 @@ __global__ void k_qexpand(uint64_t w0, uint64_t w1, uint64_t step, const uint32_
     uint64_t o = off[w] - base;
     const uint64_t qhi = (w * step) << 32;
     var.each(word, [&](uint64_t v) {
+        if (!nonempty(bits, v)) return;
         const uint32_t b = starts[v], e = starts[v + 1];
         for (uint32_t k = b; k < e; k++) hits[o++] = qhi | positions[k];
     });
@@ -279,6 +292,204 @@ __global__ __launch_bounds__(256) void k_extend(const uint64_t *__restrict__ hit
     if (lane == 0 && c) atomicAdd(colsum, c);
 }
 
+// ---- persistent-lane x-drop over the byte layout ------------------------------
+//
+// Each lane owns hits h = lane, lane + stride, ... and steps through them as a
+// small state machine — seed, right walk, left walk — one 32-column chunk per
+// loop iteration, so a lane whose walk ends early starts its next hit instead
+// of idling while the rest of its wavefront finishes.  Lanes own ~stride-apart
+// hits, so per-lane totals even out and the wavefront stays full until the
+// end.  Per column: one signed byte extract, a saturating add, max/compare for
+// the best prefix and its extent, and the x-drop test; after the first stop the
+// running score is pinned to a large negative value so later columns of the
+// chunk cannot move the best (src/hsp.rs:179-193 semantics, branch-free).
+// Left walks read the reversed copies forward.
+enum : uint32_t { kPhSeed = 0, kPhRight = 1, kPhLeft = 2, kPhDone = 3 };
+
+struct XdropArgs {
+    const uint64_t *hits;      // (q << 32 | t) records, or nullptr with tpos/qpos
+    const uint32_t *tpos, *qpos;
+    uint64_t nh;
+    const uint8_t *tf, *tr, *qf, *qr;
+    uint64_t tlen, qlen;
+    const Blk *tblk;           // block layout of the target (entropy gate)
+    ByteLut lut;
+    int32_t xdrop, thresh;
+    uint32_t span;
+    int entropy_on;
+    int32_t entropy_thresh;
+};
+
+template <bool kAppend>
+__global__ __launch_bounds__(256) void k_xdrop(XdropArgs a, PassOut po, gxy_hsp *__restrict__ out,
+                                               uint8_t *__restrict__ found, unsigned long long *__restrict__ colsum) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    uint32_t tp = 0, qp = 0, phase = kPhDone;
+    uint64_t pt = 0, pq = 0, rem = 0;
+    int32_t seed = 0, run = 0, best = 0, rbest = 0;
+    uint32_t bext = 0, rext = 0, base = 0;
+    uint64_t cols = 0;
+    constexpr int32_t kPinned = -(1 << 30);
+
+    auto fetch = [&]() {
+        phase = kPhDone;
+        while (h < a.nh) {
+            if (a.hits) {
+                const uint64_t x = a.hits[h];
+                qp = (uint32_t)(x >> 32);
+                tp = (uint32_t)x;
+            } else {
+                tp = a.tpos[h];
+                qp = a.qpos[h];
+            }
+            if ((uint64_t)tp + a.span <= a.tlen && (uint64_t)qp + a.span <= a.qlen) {  // src/hsp.rs:113-115
+                phase = kPhSeed;
+                pt = tp;
+                pq = qp;
+                rem = a.span;
+                seed = 0;
+                cols += a.span;
+                return;
+            }
+            if (!kAppend) {
+                out[h] = gxy_hsp{0, 0, 0, 0};
+                found[h] = 0;
+            }
+            h += stride;
+        }
+    };
+    fetch();
+
+    while (__any(phase != kPhDone)) {
+        bool finished = false;  // this lane completed a hit this iteration
+        bool pass = false;
+        gxy_hsp res = {0, 0, 0, 0};
+        if (phase != kPhDone) {
+            const bool rv = phase == kPhLeft;
+            uint32_t tw[8], qw[8], sw[8];
+            load_bytes32(rv ? a.tr : a.tf, pt, tw);
+            load_bytes32(rv ? a.qr : a.qf, pq, qw);
+#pragma unroll
+            for (int w = 0; w < 8; w++) sw[w] = score4(a.lut, tw[w], qw[w]);
+            const uint32_t n = (uint32_t)min<uint64_t>(32, rem);
+            if (phase == kPhSeed) {
+#pragma unroll
+                for (int k = 0; k < 32; k++)
+                    if ((uint32_t)k < n) seed = sat_add(seed, __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8));
+                rem -= n;
+                pt += n;
+                pq += n;
+                if (rem == 0) {  // seed done: set up the right walk (src/hsp.rs:126-127)
+                    phase = kPhRight;
+                    pt = (uint64_t)tp + a.span;
+                    pq = (uint64_t)qp + a.span;
+                    rem = (pt < a.tlen && pq < a.qlen) ? min(a.tlen - pt, a.qlen - pq) : 0;
+                    run = best = 0;
+                    bext = base = 0;
+                }
+            } else {
+                if (n < 32) {  // columns past the sequence end must not move the best
+#pragma unroll
+                    for (int w = 0; w < 8; w++) {
+                        const uint32_t live = n > 4u * w ? min(4u, n - 4u * w) : 0u;
+                        const uint32_t keep = live >= 4 ? 0xFFFFFFFFu : ((1u << (8 * live)) - 1u);
+                        sw[w] = (sw[w] & keep) | (0x80808080u & ~keep);
+                    }
+                }
+                uint32_t stopbits = 0;
+#pragma unroll
+                for (int k = 0; k < 32; k++) {
+                    const int32_t sc = __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8);
+                    const int32_t r = sat_add(run, sc);
+                    const bool better = r > best;
+                    best = better ? r : best;
+                    bext = better ? base + (uint32_t)(k + 1) : bext;
+                    const bool stop = !better && sat_sub(best, r) > a.xdrop;
+                    stopbits |= stop ? (1u << k) : 0u;
+                    run = stop ? kPinned : r;
+                }
+                const uint32_t live = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+                const uint32_t sb = stopbits & live;
+                const bool ended = sb != 0 || rem <= 32;
+                cols += sb ? (uint32_t)__ffs(sb) : n;
+                rem -= n;
+                pt += n;
+                pq += n;
+                base += 32;
+                if (ended) {
+                    if (phase == kPhRight) {  // left walk from (t-1, q-1), src/hsp.rs:129
+                        rbest = best;
+                        rext = bext;
+                        phase = kPhLeft;
+                        pt = a.tlen - tp;  // reversed index of tp-1
+                        pq = a.qlen - qp;
+                        rem = min((uint64_t)tp, (uint64_t)qp);
+                        run = best = 0;
+                        bext = base = 0;
+                    } else {
+                        finished = true;
+                    }
+                }
+            }
+            // a side with no columns at all ends at once
+            if (!finished && phase == kPhRight && rem == 0) {
+                rbest = 0;
+                rext = 0;
+                phase = kPhLeft;
+                pt = a.tlen - tp;
+                pq = a.qlen - qp;
+                rem = min((uint64_t)tp, (uint64_t)qp);
+                run = best = 0;
+                bext = base = 0;
+            }
+            if (!finished && phase == kPhLeft && rem == 0) {
+                best = 0;
+                bext = 0;
+                finished = true;
+            }
+            if (finished) {  // src/hsp.rs:131-147
+                const int32_t score = sat_add(sat_add(seed, best), rbest);
+                res.t_start = tp - bext;
+                res.q_start = qp - bext;
+                res.length = a.span + bext + rext;
+                res.score = score;
+                pass = score >= a.thresh;
+                if (pass && a.entropy_on)
+                    pass = entropy_pass(a.tblk, a.tlen, res.t_start, res.t_start + res.length, res.score,
+                                        a.entropy_thresh);
+                if (!kAppend) {
+                    out[h] = pass ? res : gxy_hsp{0, 0, 0, 0};
+                    found[h] = pass ? 1 : 0;
+                }
+            }
+        }
+        if (kAppend) {
+            const unsigned long long b = __ballot(pass);
+            if (b) {
+                unsigned long long basei = 0;
+                const int first = __ffsll((long long)b) - 1;
+                if (lane == first) basei = atomicAdd(po.count, (unsigned long long)__popcll(b));
+                basei = __shfl(basei, first);
+                if (pass) {
+                    const uint64_t slot = basei + __popcll(b & ((1ull << lane) - 1ull));
+                    const uint64_t d = (uint64_t)((int64_t)tp - (int64_t)qp + (int64_t)a.qlen);
+                    po.keys[slot] = (d << 32) | qp;
+                    po.hsp[slot] = res;
+                }
+            }
+        }
+        if (finished) {
+            h += stride;
+            fetch();
+        }
+    }
+    unsigned long long c = cols;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0 && c) atomicAdd(colsum, c);
+}
+
 __global__ void k_iota(uint32_t *__restrict__ v, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v[i] = (uint32_t)i;
@@ -392,7 +603,8 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     {
         KernelTimer kt_(ctx, "qcount");
         k_qcount<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin, respect_mask, var,
-                                                               tab->d_starts, words, cnt, ctr + 0);
+                                                               tab->d_starts, tab->d_nonempty, words, cnt,
+                                                           ctr + 0);
     }
     GXY_LAUNCH_CHECK(ctx);
     size_t tmp_bytes = 0;
@@ -412,9 +624,16 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     st.hits = H;
     st.extended = H;
 
-    const bool i8 = scoring_is_i8(*scm);
-    const ScorerI8 s8 = make_scorer_i8(*scm);
+    const bool byte_ok = scoring_byte_ok(*scm) && tab->d_tbytes != nullptr;
     const ScorerI32 s32 = make_scorer_i32(*scm);
+    uint8_t *qb = nullptr, *qbr = nullptr;
+    if (byte_ok) {
+        qb = (uint8_t *)scratch(ctx, kSlotQBytes, qlen + kBytePad);
+        qbr = (uint8_t *)scratch(ctx, kSlotQBytesRev, qlen + kBytePad);
+        if (!qb || !qbr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        int rc = seq_to_bytes(ctx, query, qb, qbr);
+        if (rc) return rc;
+    }
     uint64_t npass = 0;
     if (H > 0) {
         // Hits are expanded and extended in batches of at most kCap.
@@ -451,18 +670,22 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             {
                 KernelTimer kt_(ctx, "qexpand");
                 k_qexpand<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(w0, w1, sp->step, words, off, boff[k], var,
-                                                                          tab->d_starts, tab->d_positions, hits);
+                                                                          tab->d_starts, tab->d_nonempty,
+                                                                          tab->d_positions, hits);
             }
             GXY_LAUNCH_CHECK(ctx);
             KernelTimer kt_ext(ctx, "extend");
-            if (i8)
-                k_extend<ScorerI8><<<grid_for(nh, 256), 256, 0, ctx->stream>>>(
-                    hits, nh, target->d, tlen, query->d, qlen, s8, sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
-                    sp->entropy_enabled, sp->entropy_threshold, po, ctr + 2);
-            else
+            if (byte_ok) {
+                XdropArgs xa{hits, nullptr, nullptr, nh, tab->d_tbytes, tab->d_tbytes_rev, qb, qbr, tlen, qlen,
+                             target->d, make_byte_lut(*scm), sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
+                             sp->entropy_enabled, sp->entropy_threshold};
+                const unsigned g = std::min<unsigned>(grid_for(nh, 256), kXdropBlocks);
+                k_xdrop<true><<<g, 256, 0, ctx->stream>>>(xa, po, nullptr, nullptr, ctr + 2);
+            } else {
                 k_extend<ScorerI32><<<grid_for(nh, 256), 256, 0, ctx->stream>>>(
                     hits, nh, target->d, tlen, query->d, qlen, s32, sp->hsp.x_drop, sp->hsp.hsp_threshold,
                     pat.span, sp->entropy_enabled, sp->entropy_threshold, po, ctr + 2);
+            }
             GXY_LAUNCH_CHECK(ctx);
         }
         unsigned long long c2[2];
@@ -600,11 +823,30 @@ int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *que
         e = hipMemcpyAsync(dtp, t_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dqp, q_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) {
-            if (scoring_is_i8(*scoring))
-                k_extend_batch<ScorerI8><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
-                    dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i8(*scoring), params->x_drop,
-                    params->hsp_threshold, seed_len, dout, dfound);
-            else
+            if (scoring_byte_ok(*scoring)) {
+                auto *tb = (uint8_t *)scratch(ctx, kSlotTBytes, dt->len + kBytePad);
+                auto *tbr = (uint8_t *)scratch(ctx, kSlotTBytesRev, dt->len + kBytePad);
+                auto *qb = (uint8_t *)scratch(ctx, kSlotQBytes, dq->len + kBytePad);
+                auto *qbr = (uint8_t *)scratch(ctx, kSlotQBytesRev, dq->len + kBytePad);
+                if (!tb || !tbr || !qb || !qbr) {
+                    gxy_dseq_free(dt);
+                    gxy_dseq_free(dq);
+                    return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+                }
+                rc = seq_to_bytes(ctx, dt, tb, tbr);
+                if (!rc) rc = seq_to_bytes(ctx, dq, qb, qbr);
+                if (rc) {
+                    gxy_dseq_free(dt);
+                    gxy_dseq_free(dq);
+                    return rc;
+                }
+                XdropArgs xa{nullptr, dtp, dqp, n, tb, tbr, qb, qbr, dt->len, dq->len, dt->d,
+                             make_byte_lut(*scoring), params->x_drop, params->hsp_threshold, seed_len, 0, 0};
+                PassOut none{nullptr, nullptr, nullptr};
+                auto *cs = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
+                k_xdrop<false><<<std::min<unsigned>(grid_for(n, 256), kXdropBlocks), 256, 0, ctx->stream>>>(
+                    xa, none, dout, dfound, cs);
+            } else
                 k_extend_batch<ScorerI32><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
                     dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i32(*scoring), params->x_drop,
                     params->hsp_threshold, seed_len, dout, dfound);

@@ -8,7 +8,7 @@
 #include <cstdarg>
 #include <mutex>
 
-#include "common.h"
+#include "internal.h"
 
 namespace gxy {
 
@@ -201,6 +201,27 @@ __global__ void k_slice(const Blk *__restrict__ src, Blk *__restrict__ dst, uint
     dst[b] = {w.codes, w.valid, w.mask};
 }
 
+// Byte-per-base copies (x-drop layout): one thread per output dword.
+__global__ void k_to_bytes(const Blk *__restrict__ src, uint64_t len, uint64_t ndw, uint32_t *__restrict__ fwd,
+                           uint32_t *__restrict__ rev) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndw) return;
+    uint32_t f = 0, r = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const uint64_t p = 4 * i + b;
+        if (p < len) {
+            const uint32_t v = base_valid(src, p) ? base_code(src, p) : (uint32_t)kInvalidByte;
+            f |= v << (8 * b);
+            const uint64_t pr = len - 1 - p;
+            const uint32_t w = base_valid(src, pr) ? base_code(src, pr) : (uint32_t)kInvalidByte;
+            r |= w << (8 * b);
+        }
+    }
+    fwd[i] = f;
+    rev[i] = r;
+}
+
 int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
     auto *s = new (std::nothrow) gxy_dseq();
     if (!s) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
@@ -219,6 +240,14 @@ int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
 }
 
 }  // namespace
+
+int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *fwd, uint8_t *rev) {
+    const uint64_t ndw = (s->len + kBytePad) / 4;
+    k_to_bytes<<<(unsigned)((ndw + 255) / 256), 256, 0, ctx->stream>>>(s->d, s->len, ndw, (uint32_t *)fwd,
+                                                                       (uint32_t *)rev);
+    GXY_LAUNCH_CHECK(ctx);
+    return GXY_OK;
+}
 
 int gxy::dseq_upload_impl(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out) {
     if (!seq || !out || (seq->len && (!seq->codes || !seq->valid || !seq->mask)))
