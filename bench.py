@@ -169,6 +169,7 @@ def main():
         "records": len(recs), "seed_hits_per_step": st.hits, "xdrop_columns_per_step": st.columns,
         "pcie_inclusive_query_mbps": round(qbp / 1e6 / pcie_s, 3),
         "kernel_ms_per_step": {n: round(v[0] / args.steps, 3) for n, v in ktime.items() if v[1]},
+        "timed_kernels_ms_per_step": round(sum(v[0] for v in ktime.values()) / args.steps, 3),
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -56,6 +56,40 @@ inline bool make_dev_pattern(const gxy_seed_pattern *p, DevPattern *d) {
 
 // ---- device helpers --------------------------------------------------------
 
+// Table index of a seed word.  The reference's word (src/seeds.rs:139-145)
+// concatenates the 2-bit codes at the care offsets, leftmost first.  The
+// device tables are indexed by a bit permutation of it,
+//     index = (low bits of the codes, leftmost first) << W | (high bits ...),
+// because a transition (A<->G, C<->T) flips only a code's high bit: the 1+W
+// single-transition variants of a word are then index ^ (1 << b), b < W — all
+// inside one 4^w/2^W-entry block (512 B of the non-empty bitmap, 16 KiB of
+// `starts` for 12of19) instead of 13 random places in a 64 MiB table.
+// Positions per word are unchanged; only the order of the word buckets is.
+// Fully unrolled so the care offsets stay in scalar registers.
+__device__ __forceinline__ uint64_t gather_index(uint64_t codes, const DevPattern &pat) {
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+        if ((uint32_t)k < pat.weight) {
+            const uint32_t c = (uint32_t)(codes >> (2 * pat.off[k])) & 3u;
+            lo = (lo << 1) | (c & 1u);
+            hi = (hi << 1) | (c >> 1);
+        }
+    }
+    return ((uint64_t)lo << pat.weight) | hi;
+}
+
+// Reference word <-> table index (see gather_index).
+__host__ __device__ __forceinline__ uint64_t word_to_index(uint64_t word, uint32_t weight) {
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t k = 0; k < weight; k++) {
+        const uint64_t c = (word >> (2 * (weight - 1 - k))) & 3u;
+        lo = (lo << 1) | (c & 1u);
+        hi = (hi << 1) | (c >> 1);
+    }
+    return (lo << weight) | hi;
+}
+
 // 32-base window starting at `pos` (any alignment): codes in bits 2k..2k+1 and
 // valid/mask in bit k for base pos+k.
 struct Win {
@@ -226,6 +260,21 @@ inline bool scoring_byte_ok(const gxy_scoring &m) {
     return scoring_is_i8(m) && m.ambig >= -128 && m.ambig <= 127;
 }
 
+// ---- spread counters ------------------------------------------------------------
+//
+// A counter incremented once per wavefront by every wavefront of a large grid
+// must not be one address: same-address atomics serialise in one L2 channel.
+// kSpread counters, one per 128-byte line, are added to by wavefront index and
+// summed on the host.
+constexpr int kSpread = 64;
+constexpr int kSpreadStride = 16;  // in 8-byte words: one counter per 128-byte line
+constexpr size_t kSpreadBytes = (size_t)kSpread * kSpreadStride * 8;
+
+__device__ __forceinline__ void spread_add(unsigned long long *c, unsigned long long v) {
+    const unsigned slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kSpread - 1);
+    atomicAdd(c + (size_t)slot * kSpreadStride, v);
+}
+
 // ---- host side ---------------------------------------------------------------
 
 inline uint64_t nblocks(uint64_t len) { return (len + 31) / 32 + 1; }
@@ -281,6 +330,8 @@ struct gxy_ctx {
 namespace gxy {
 
 int set_err(gxy_ctx *ctx, int code, const char *fmt, ...);
+// Sum of a spread counter block (synchronises the stream).
+int spread_sum(gxy_ctx *ctx, const unsigned long long *c, uint64_t *out);
 
 // Brackets the launches issued in its scope with timing events when the
 // context's timing is enabled.
@@ -327,6 +378,8 @@ enum Slot {
     kSlotQBytesRev,
     kSlotTBytes,
     kSlotTBytesRev,
+    kSlotVMask,
+    kSlotSpread,
 };
 
 }  // namespace gxy

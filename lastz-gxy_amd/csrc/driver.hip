@@ -33,11 +33,15 @@ using namespace gxy;
 
 namespace {
 
+// A record (src/output/mod.rs:17-31) as a plain value: an ungapped record's
+// script is one implicit Match run of t_span (n_runs == 0); a gapped record's
+// runs live in the run's shared run store.
 struct Rec {
     uint32_t ti, qi, strand;
     uint32_t t_start, q_start, t_span, q_span;
     int32_t score;
-    std::vector<gxy_edit_run> runs;
+    uint32_t n_runs;
+    uint64_t runs_offset;
 };
 
 struct DevSeqDeleter {
@@ -137,6 +141,32 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             drc[i].reset(d);
         }
 
+    // byte copies of every query strand, shared by all chunk searches
+    struct DevBytes {
+        uint8_t *p = nullptr;
+        ~DevBytes() {
+            if (p) (void)hipFree(p);
+        }
+    };
+    const bool byte_ok = scoring_byte_ok(cfg->scoring);
+    std::vector<DevBytes> qbytes(byte_ok ? 4 * n_queries : 0);
+    if (byte_ok) {
+        for (uint64_t i = 0; i < n_queries; i++) {
+            for (int s = 0; s < 2; s++) {
+                const gxy_dseq *qs = s == 0 ? queries[i] : drc[i].get();
+                if (!qs) continue;
+                const size_t nb = qs->len + kBytePad;
+                if (hipMalloc(&qbytes[4 * i + 2 * s].p, nb) != hipSuccess ||
+                    hipMalloc(&qbytes[4 * i + 2 * s + 1].p, nb) != hipSuccess) {
+                    (void)hipGetLastError();
+                    return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
+                }
+                int rc = seq_to_bytes(ctx, qs, qbytes[4 * i + 2 * s].p, qbytes[4 * i + 2 * s + 1].p);
+                if (rc) return rc;
+            }
+        }
+    }
+
     // chunk_target (src/driver.rs:156-194)
     std::vector<Chunk> chunks;
     for (uint64_t ti = 0; ti < n_targets; ti++) {
@@ -175,6 +205,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         std::vector<Rec> recs;
     };
     std::vector<Group> groups;
+    std::vector<gxy_edit_run> run_store;  // scripts of gapped records
 
     for (size_t c = 0; c < chunks.size(); c++) {
         const Chunk &ch = chunks[c];
@@ -191,7 +222,10 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
                 std::vector<gxy_hsp> hsps;
                 gxy_search_stats st;
-                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st);
+                const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
+                const uint8_t *qf = byte_ok ? qbytes[4 * qi + 2 * si].p : nullptr;
+                const uint8_t *qr = byte_ok ? qbytes[4 * qi + 2 * si + 1].p : nullptr;
+                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st, qf, qr);
                 if (rc) return rc;
                 rs.hits += st.hits;
                 rs.columns += st.columns;
@@ -207,9 +241,8 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 if (!cfg->gapped_enabled) {
                     g.recs.reserve(kept.size());
                     for (const auto &h : kept) {  // build_ungapped_record (src/driver.rs:437-468)
-                        Rec r{ch.parent, qi, strand, h.t_start + ch.global_start, h.q_start, h.length, h.length,
-                              h.score, {{GXY_OP_MATCH, h.length}}};
-                        g.recs.push_back(std::move(r));
+                        g.recs.push_back(Rec{ch.parent, qi, strand, h.t_start + ch.global_start, h.q_start,
+                                             h.length, h.length, h.score, 0, 0});
                     }
                 } else {
                     std::stable_sort(kept.begin(), kept.end(),
@@ -244,9 +277,10 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                             if (res[pos].found) {  // build_gapped_record (src/driver.rs:395-434)
                                 const gxy_alignment &a = res[pos].aln;
                                 Rec r{ch.parent, qi, strand, a.t_start + ch.global_start, a.q_start, a.t_len,
-                                      a.q_len, a.score, std::move(scripts[pos])};
+                                      a.q_len, a.score, (uint32_t)scripts[pos].size(), run_store.size()};
+                                run_store.insert(run_store.end(), scripts[pos].begin(), scripts[pos].end());
                                 covered.push_back({r.t_start, r.t_start + r.t_span, r.q_start, r.q_start + r.q_span});
-                                g.recs.push_back(std::move(r));
+                                g.recs.push_back(r);
                             }
                             pos++;
                         }
@@ -307,13 +341,13 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                     r.q_span == b.q_span && r.strand == b.strand)
                     continue;
             }
-            all.push_back(std::move(r));
+            all.push_back(r);
         }
     }
     auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
     if (!set) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
     size_t nruns = 0;
-    for (const auto &r : all) nruns += r.runs.size();
+    for (const auto &r : all) nruns += r.n_runs ? r.n_runs : 1;
     set->n = all.size();
     set->records = (gxy_record *)std::calloc(std::max<size_t>(1, all.size()), sizeof(gxy_record));
     set->n_runs = nruns;
@@ -334,9 +368,14 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         o.t_span = r.t_span;
         o.q_span = r.q_span;
         o.score = r.score;
-        o.n_runs = (uint32_t)r.runs.size();
         o.runs_offset = ro;
-        for (const auto &x : r.runs) set->runs[ro++] = x;
+        if (r.n_runs == 0) {  // build_ungapped_record: one Match run (src/driver.rs:451-452)
+            o.n_runs = 1;
+            set->runs[ro++] = {GXY_OP_MATCH, r.t_span};
+        } else {
+            o.n_runs = r.n_runs;
+            for (uint32_t x = 0; x < r.n_runs; x++) set->runs[ro++] = run_store[r.runs_offset + x];
+        }
     }
     *out = set;
     if (stats) *stats = rs;

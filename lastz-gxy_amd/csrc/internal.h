@@ -18,9 +18,12 @@ int postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern 
 // common.h); each buffer must hold s->len + kBytePad bytes.
 int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *fwd, uint8_t *rev);
 
+// Seed search on device sequences.  `qf`/`qr` may hold the query's byte
+// copies (seq_to_bytes) when the caller searches the same query repeatedly;
+// otherwise they are built per call.
 int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
                     const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
-                    gxy_search_stats *stats);
+                    gxy_search_stats *stats, const uint8_t *qf = nullptr, const uint8_t *qr = nullptr);
 
 struct GappedItem {
     const gxy_dseq *t;

@@ -1,9 +1,9 @@
 // postable.hip — device PosTable (src/pos_table.rs:24-137).
 //
 // Build = one pass that turns every window (0, step, 2*step, ... <= len-span)
-// into a 64-bit item (word << 32 | pos) — invalid windows get the sentinel
-// word 4^w so they sort last — then a stable LSD radix sort on the word bits
-// only.  Items enter in ascending position, so the stable sort leaves each
+// into a 64-bit item (index << 32 | pos) — `index` is the bit-permuted word of
+// gather_index; invalid windows get the sentinel 4^w so they sort last — then
+// a stable LSD radix sort on the index bits only.  Items enter in ascending position, so the stable sort leaves each
 // word's positions ascending exactly as the reference's counting-sort scatter
 // does (src/pos_table.rs:60-75).  `starts` is then filled by one thread per
 // sorted item writing the run of words (key[i-1], key[i]] — every entry written
@@ -31,10 +31,7 @@ __global__ void k_table_items(const Blk *__restrict__ seq, uint64_t len, DevPatt
             const uint32_t care = pat.care_mask;
             const uint32_t msk = respect_mask ? x.mask : 0u;
             if ((x.valid & care) == care && (msk & care) == 0) {
-                uint64_t v = 0;
-                for (uint32_t k = 0; k < pat.weight; k++)
-                    v = (v << 2) | ((x.codes >> (2 * pat.off[k])) & 3ull);
-                word = v;
+                word = gather_index(x.codes, pat);
                 ok = true;
             }
         }
@@ -42,7 +39,7 @@ __global__ void k_table_items(const Blk *__restrict__ seq, uint64_t len, DevPatt
     }
     // one atomic per wavefront
     const unsigned long long b = __ballot(ok);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(nvalid, (unsigned long long)__popcll(b));
+    if ((threadIdx.x & 63) == 0 && b) spread_add(nvalid, (unsigned long long)__popcll(b));
 }
 
 __global__ void k_table_starts(const uint64_t *__restrict__ items, uint64_t nvalid, uint64_t num_words,
@@ -119,7 +116,7 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
 
     auto *items = (uint64_t *)scratch(ctx, kSlotHits, nwin * sizeof(uint64_t));
     auto *items2 = (uint64_t *)scratch(ctx, kSlotPassKeys, nwin * sizeof(uint64_t));
-    auto *cnt = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
+    auto *cnt = (unsigned long long *)scratch(ctx, kSlotSpread, kSpreadBytes);
     if (!items || !items2 || !cnt) {
         delete tab;
         return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
@@ -128,7 +125,7 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         gxy_postable_free(tab);
         return set_err(ctx, code, "%s: %s", msg, hipGetErrorString(e));
     };
-    hipError_t e = hipMemsetAsync(cnt, 0, 8, ctx->stream);
+    hipError_t e = hipMemsetAsync(cnt, 0, kSpreadBytes, ctx->stream);
     if (e != hipSuccess) return fail(GXY_EDEVICE, "memset", e);
     {
         KernelTimer kt_(ctx, "table_items");
@@ -151,10 +148,11 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
                                               ctx->stream);
     }
     if (e != hipSuccess) return fail(GXY_EDEVICE, "radix sort", e);
-    unsigned long long nvalid = 0;
-    if ((e = hipMemcpyAsync(&nvalid, cnt, 8, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-        return fail(GXY_EDEVICE, "count download", e);
+    uint64_t nvalid = 0;
+    if (spread_sum(ctx, cnt, &nvalid) != GXY_OK) {
+        gxy_postable_free(tab);
+        return GXY_EDEVICE;
+    }
     tab->npos = nvalid;
     if (hipMalloc(&tab->d_starts, (tab->num_words + 1) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&tab->d_positions, std::max<uint64_t>(1, nvalid) * sizeof(uint32_t)) != hipSuccess) {
@@ -264,7 +262,8 @@ int gxy_postable_lookup(gxy_ctx *ctx, const gxy_postable *tab, uint64_t word, ui
     if (!tab || !n_out || (cap && !out)) return set_err(ctx, GXY_EINVAL, "null argument");
     if (word >= tab->num_words) return set_err(ctx, GXY_EINVAL, "word out of range");
     uint32_t se[2];
-    GXY_HIP(ctx, hipMemcpyAsync(se, tab->d_starts + word, 8, hipMemcpyDeviceToHost, ctx->stream));
+    const uint64_t idx = word_to_index(word, tab->pattern.weight);
+    GXY_HIP(ctx, hipMemcpyAsync(se, tab->d_starts + idx, 8, hipMemcpyDeviceToHost, ctx->stream));
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint64_t n = se[1] - se[0];
     *n_out = n;
@@ -280,17 +279,25 @@ uint64_t gxy_postable_total_positions(const gxy_postable *tab) { return tab ? ta
 
 int gxy_postable_export(gxy_ctx *ctx, const gxy_postable *tab, uint32_t *starts, uint64_t starts_cap,
                         uint32_t *positions, uint64_t positions_cap) {
+    // The device buckets are in index order (gather_index); the reference
+    // layout orders them by word.  Rebuild it on the host.
     if (!ctx) return GXY_EINVAL;
     if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
-    if (starts) {
-        const uint64_t n = std::min(starts_cap, tab->num_words + 1);
-        GXY_HIP(ctx, hipMemcpyAsync(starts, tab->d_starts, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    }
-    if (positions) {
-        const uint64_t n = std::min(positions_cap, tab->npos);
-        if (n) GXY_HIP(ctx, hipMemcpyAsync(positions, tab->d_positions, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    }
+    const uint64_t nw = tab->num_words;
+    std::vector<uint32_t> ds(nw + 1), dp(std::max<uint64_t>(1, tab->npos));
+    GXY_HIP(ctx, hipMemcpyAsync(ds.data(), tab->d_starts, (nw + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (tab->npos)
+        GXY_HIP(ctx, hipMemcpyAsync(dp.data(), tab->d_positions, tab->npos * 4, hipMemcpyDeviceToHost, ctx->stream));
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t acc = 0;
+    for (uint64_t w = 0; w < nw; w++) {
+        const uint64_t idx = word_to_index(w, tab->pattern.weight);
+        const uint32_t b = ds[idx], e = ds[idx + 1];
+        if (starts && w < starts_cap) starts[w] = (uint32_t)acc;
+        for (uint32_t k = b; k < e; k++, acc++)
+            if (positions && acc < positions_cap) positions[acc] = dp[k];
+    }
+    if (starts && nw < starts_cap) starts[nw] = (uint32_t)acc;
     return GXY_OK;
 }
 

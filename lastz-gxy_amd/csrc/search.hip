@@ -34,23 +34,21 @@ inline unsigned grid_for(uint64_t n, unsigned th) { return (unsigned)((n + th - 
 // times over; lanes stride over the hits.
 constexpr unsigned kXdropBlocks = 2048;
 
-// Variant words of a seed word (src/seed_search.rs:63-90): the word itself,
-// then each single transition (XOR 0b10 at a care position, leftmost care
-// first), then each pair i<j when transitions >= 2.
+// Variants of a seed word (src/seed_search.rs:63-90) in table-index form: the
+// word itself, then each single transition — a code's high bit, i.e. index bit
+// W-1-i for care position i (leftmost first) — then each pair i<j when
+// transitions >= 2.
 struct Variants {
     uint32_t weight, transitions;
-    __device__ __forceinline__ uint32_t count() const {
-        return transitions == 0 ? 1u : (transitions == 1 ? 1u + weight : 1u + weight + weight * (weight - 1) / 2);
-    }
     template <class F>
-    __device__ __forceinline__ void each(uint64_t word, F &&f) const {
-        f(word);
+    __device__ __forceinline__ void each(uint64_t idx, F &&f) const {
+        f(idx);
         if (transitions >= 1)
-            for (uint32_t i = 0; i < weight; i++) f(word ^ (2ull << (2 * (weight - 1 - i))));
+            for (uint32_t i = 0; i < weight; i++) f(idx ^ (1ull << (weight - 1 - i)));
         if (transitions >= 2)
             for (uint32_t i = 0; i < weight; i++)
                 for (uint32_t j = i + 1; j < weight; j++)
-                    f(word ^ (2ull << (2 * (weight - 1 - i))) ^ (2ull << (2 * (weight - 1 - j))));
+                    f(idx ^ (1ull << (weight - 1 - i)) ^ (1ull << (weight - 1 - j)));
     }
 };
 
@@ -61,9 +59,7 @@ __device__ __forceinline__ bool seed_word(const Blk *__restrict__ q, uint64_t ql
     const uint32_t care = pat.care_mask;
     const uint32_t msk = respect_mask ? x.mask : 0u;
     if ((x.valid & care) != care || (msk & care) != 0) return false;
-    uint64_t v = 0;
-    for (uint32_t k = 0; k < pat.weight; k++) v = (v << 2) | ((x.codes >> (2 * pat.off[k])) & 3ull);
-    *word = v;
+    *word = gather_index(x.codes, pat);
     return true;
 }
 
@@ -94,7 +90,82 @@ __global__ void k_qcount(const Blk *__restrict__ q, uint64_t qlen, DevPattern pa
         cnt[w] = c;
     }
     const unsigned long long b = __ballot(ok);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(nvalid, (unsigned long long)__popcll(b));
+    if ((threadIdx.x & 63) == 0 && b) spread_add(nvalid, (unsigned long long)__popcll(b));
+}
+
+// transitions <= 1 and weight <= 15: the (at most 16) variant words are held
+// in registers so all bitmap loads issue together; the mask of non-empty
+// variants is kept for k_qexpand_v so it revisits only those.
+constexpr int kMaxV1 = 16;
+
+__device__ __forceinline__ uint64_t variant1(uint64_t idx, uint32_t weight, int i) {
+    // i = 0: the word; 1 <= i <= weight: transition at care position i-1
+    // (src/seed_search.rs:79-83), index bit weight-i
+    const uint32_t sh = (uint32_t)i <= weight ? weight - (uint32_t)i : 0u;
+    return i == 0 ? idx : idx ^ (1ull << sh);
+}
+
+__global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern pat, uint64_t step,
+                           uint64_t nwin, int respect_mask, uint32_t nvar, const uint32_t *__restrict__ starts,
+                           const uint32_t *__restrict__ bits, uint32_t *__restrict__ words,
+                           uint32_t *__restrict__ cnt, uint16_t *__restrict__ vmask,
+                           unsigned long long *__restrict__ nvalid) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = false;
+    if (w < nwin) {
+        uint64_t word;
+        uint32_t c = 0, m = 0;
+        if (seed_word(q, qlen, pat, w * step, respect_mask, &word)) {
+            ok = true;
+            // all bitmap loads issue back to back: variants past nvar repeat the
+            // word and are masked afterwards
+            uint64_t vv[kMaxV1];
+            uint32_t bw[kMaxV1];
+#pragma unroll
+            for (int i = 0; i < kMaxV1; i++) {
+                vv[i] = (uint32_t)i < nvar ? variant1(word, pat.weight, i) : word;
+                bw[i] = bits[vv[i] >> 5];
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxV1; i++)
+                if ((uint32_t)i < nvar && ((bw[i] >> (vv[i] & 31)) & 1u)) m |= 1u << i;
+            // starts pairs of the non-empty variants (usually none or one)
+            uint32_t mm = m;
+            while (mm) {
+                const int i = __ffs(mm) - 1;
+                mm &= mm - 1;
+                const uint64_t v = variant1(word, pat.weight, i);
+                c += starts[v + 1] - starts[v];
+            }
+            words[w] = (uint32_t)word;
+        } else {
+            words[w] = kInvalidWord;
+        }
+        cnt[w] = c;
+        vmask[w] = (uint16_t)m;
+    }
+    const unsigned long long b = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && b) spread_add(nvalid, (unsigned long long)__popcll(b));
+}
+
+__global__ void k_qexpand_v(uint64_t w0, uint64_t w1, uint64_t step, const uint32_t *__restrict__ words,
+                            const uint16_t *__restrict__ vmask, const uint64_t *__restrict__ off, uint64_t base,
+                            uint32_t weight, const uint32_t *__restrict__ starts,
+                            const uint32_t *__restrict__ positions, uint64_t *__restrict__ hits) {
+    const uint64_t w = w0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= w1) return;
+    uint32_t m = vmask[w];
+    if (m == 0) return;
+    const uint32_t word = words[w];
+    uint64_t o = off[w] - base;
+    const uint64_t qhi = (w * step) << 32;
+    while (m) {
+        const int i = __ffs(m) - 1;
+        m &= m - 1;
+        const uint64_t v = variant1(word, weight, i);
+        const uint32_t b = starts[v], e = starts[v + 1];
+        for (uint32_t k = b; k < e; k++) hits[o++] = qhi | positions[k];
+    }
 }
 
 struct ToU64 {
@@ -577,7 +648,7 @@ namespace gxy {
 // mirrors the driver's --nomasking (masks cleared before seeding).
 int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
                     const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
-                    gxy_search_stats *stats) {
+                    gxy_search_stats *stats, const uint8_t *qf, const uint8_t *qr) {
     if (!tab || !query || !scm || !sp || !result) return set_err(ctx, GXY_EINVAL, "null argument");
     if (sp->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
     const gxy_dseq *target = tab->target;
@@ -598,13 +669,23 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     auto *cnt = (uint32_t *)scratch(ctx, kSlotCounts, nwin * 4);
     auto *off = (uint64_t *)scratch(ctx, kSlotOffsets, (nwin + 1) * 8);
     auto *ctr = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
-    if (!words || !cnt || !off || !ctr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    const bool fast_var = var.transitions <= 1 && nvar <= (uint32_t)kMaxV1;
+    auto *vmask = (uint16_t *)scratch(ctx, kSlotVMask, nwin * 2);
+    if (!words || !cnt || !off || !ctr || !vmask) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    auto *spr = (unsigned long long *)scratch(ctx, kSlotSpread, kSpreadBytes);
+    if (!spr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     GXY_HIP(ctx, hipMemsetAsync(ctr, 0, 64, ctx->stream));
+    GXY_HIP(ctx, hipMemsetAsync(spr, 0, kSpreadBytes, ctx->stream));
     {
         KernelTimer kt_(ctx, "qcount");
-        k_qcount<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin, respect_mask, var,
-                                                               tab->d_starts, tab->d_nonempty, words, cnt,
-                                                           ctr + 0);
+        if (fast_var)
+            k_qcount_v<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin,
+                                                                     respect_mask, nvar, tab->d_starts,
+                                                                     tab->d_nonempty, words, cnt, vmask, spr);
+        else
+            k_qcount<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin, respect_mask,
+                                                                   var, tab->d_starts, tab->d_nonempty, words, cnt,
+                                                                   spr);
     }
     GXY_LAUNCH_CHECK(ctx);
     size_t tmp_bytes = 0;
@@ -617,7 +698,12 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     uint64_t H = 0;
     unsigned long long nvalid = 0;
     GXY_HIP(ctx, hipMemcpyAsync(&H, off + nwin, 8, hipMemcpyDeviceToHost, ctx->stream));
-    GXY_HIP(ctx, hipMemcpyAsync(&nvalid, ctr + 0, 8, hipMemcpyDeviceToHost, ctx->stream));
+    {
+        uint64_t nv = 0;
+        int rc = spread_sum(ctx, spr, &nv);
+        if (rc) return rc;
+        nvalid = nv;
+    }
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     st.query_windows = nvalid;
     st.lookups = nvalid * nvar;
@@ -626,13 +712,15 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
 
     const bool byte_ok = scoring_byte_ok(*scm) && tab->d_tbytes != nullptr;
     const ScorerI32 s32 = make_scorer_i32(*scm);
-    uint8_t *qb = nullptr, *qbr = nullptr;
-    if (byte_ok) {
-        qb = (uint8_t *)scratch(ctx, kSlotQBytes, qlen + kBytePad);
-        qbr = (uint8_t *)scratch(ctx, kSlotQBytesRev, qlen + kBytePad);
-        if (!qb || !qbr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-        int rc = seq_to_bytes(ctx, query, qb, qbr);
+    const uint8_t *qb = qf, *qbr = qr;
+    if (byte_ok && !(qf && qr)) {
+        auto *b1 = (uint8_t *)scratch(ctx, kSlotQBytes, qlen + kBytePad);
+        auto *b2 = (uint8_t *)scratch(ctx, kSlotQBytesRev, qlen + kBytePad);
+        if (!b1 || !b2) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        int rc = seq_to_bytes(ctx, query, b1, b2);
         if (rc) return rc;
+        qb = b1;
+        qbr = b2;
     }
     uint64_t npass = 0;
     if (H > 0) {
@@ -669,9 +757,14 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             if (nh == 0) continue;
             {
                 KernelTimer kt_(ctx, "qexpand");
-                k_qexpand<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(w0, w1, sp->step, words, off, boff[k], var,
-                                                                          tab->d_starts, tab->d_nonempty,
-                                                                          tab->d_positions, hits);
+                if (fast_var)
+                    k_qexpand_v<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(
+                        w0, w1, sp->step, words, vmask, off, boff[k], pat.weight, tab->d_starts, tab->d_positions,
+                        hits);
+                else
+                    k_qexpand<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(w0, w1, sp->step, words, off, boff[k],
+                                                                              var, tab->d_starts, tab->d_nonempty,
+                                                                              tab->d_positions, hits);
             }
             GXY_LAUNCH_CHECK(ctx);
             KernelTimer kt_ext(ctx, "extend");

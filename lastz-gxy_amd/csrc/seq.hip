@@ -100,6 +100,16 @@ void timing_flush(gxy_ctx *ctx) {
     ctx->spans.clear();
 }
 
+int spread_sum(gxy_ctx *ctx, const unsigned long long *c, uint64_t *out) {
+    unsigned long long h[kSpread * kSpreadStride];
+    GXY_HIP(ctx, hipMemcpyAsync(h, c, kSpreadBytes, hipMemcpyDeviceToHost, ctx->stream));
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t t = 0;
+    for (int k = 0; k < kSpread; k++) t += h[k * kSpreadStride];
+    *out = t;
+    return GXY_OK;
+}
+
 // Interleave the reference byte arrays into Blk words on the host.
 static void interleave(const gxy_seq *s, Blk *out, uint64_t nblk) {
     const uint64_t len = s->len;
