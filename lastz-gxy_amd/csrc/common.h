@@ -13,8 +13,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -325,7 +327,33 @@ struct gxy_ctx {
     std::vector<Span> spans;
     std::vector<hipEvent_t> event_pool;
     std::vector<std::pair<std::string, std::pair<double, uint64_t>>> totals;
+    // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.
+    bool trace = std::getenv("GXY_TRACE") != nullptr;
+    double trace_ms[16] = {};
 };
+
+namespace gxy {
+
+// Adds the wall time since the previous lap to ctx->trace_ms[k] when tracing.
+struct PhaseClock {
+    gxy_ctx *ctx;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    explicit PhaseClock(gxy_ctx *c) : ctx(c) {}
+    void lap(int k) {
+        if (!ctx->trace) return;
+        auto n = std::chrono::steady_clock::now();
+        ctx->trace_ms[k] += std::chrono::duration<double, std::milli>(n - t).count();
+        t = n;
+    }
+};
+
+// Phase slots of the host clock.
+enum TracePhase : int {
+    kTrSetup = 0, kTrTable, kTrSearchCount, kTrSearchExtend, kTrSearchAdmit, kTrSearchCopy, kTrSearchDedup,
+    kTrRecords, kTrMerge, kTrSearchOther
+};
+
+}  // namespace gxy
 
 namespace gxy {
 

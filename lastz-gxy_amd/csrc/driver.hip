@@ -207,10 +207,14 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     std::vector<Group> groups;
     std::vector<gxy_edit_run> run_store;  // scripts of gapped records
 
+    PhaseClock clk(ctx);
+    clk.lap(kTrSetup);
     for (size_t c = 0; c < chunks.size(); c++) {
         const Chunk &ch = chunks[c];
         gxy_postable *tp;
+        clk.lap(kTrRecords);
         int rc = postable_build_impl(ctx, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
+        clk.lap(kTrTable);
         if (rc) return rc;
         Table table(tp);
         if (cfg->max_word_count > 0) {
@@ -225,8 +229,10 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
                 const uint8_t *qf = byte_ok ? qbytes[4 * qi + 2 * si].p : nullptr;
                 const uint8_t *qr = byte_ok ? qbytes[4 * qi + 2 * si + 1].p : nullptr;
+                clk.lap(kTrRecords);
                 rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st, qf, qr);
                 if (rc) return rc;
+                clk = PhaseClock(ctx);
                 rs.hits += st.hits;
                 rs.columns += st.columns;
                 rs.hsps += st.emitted;
@@ -318,6 +324,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     }
     for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
 
+    clk.lap(kTrRecords);
     // group merge (src/driver.rs:371-389)
     std::stable_sort(groups.begin(), groups.end(), [](const Group &a, const Group &b) {
         if (a.ti != b.ti) return a.ti < b.ti;
@@ -379,6 +386,15 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     }
     *out = set;
     if (stats) *stats = rs;
+    clk.lap(kTrMerge);
+    if (ctx->trace) {
+        static const char *names[] = {"setup", "table", "s.count", "s.extend", "s.admit", "s.copy", "s.dedup",
+                                      "records", "merge", "s.other"};
+        std::fprintf(stderr, "gxy_run phases ms:");
+        for (int k = 0; k < 10; k++) std::fprintf(stderr, " %s %.2f", names[k], ctx->trace_ms[k]);
+        std::fprintf(stderr, "\n");
+        std::memset(ctx->trace_ms, 0, sizeof ctx->trace_ms);
+    }
     return GXY_OK;
 }
 

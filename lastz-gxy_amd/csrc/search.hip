@@ -392,7 +392,7 @@ struct XdropArgs {
 };
 
 template <bool kAppend>
-__global__ __launch_bounds__(256) void k_xdrop(XdropArgs a, PassOut po, gxy_hsp *__restrict__ out,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop(XdropArgs a, PassOut po, gxy_hsp *__restrict__ out,
                                                uint8_t *__restrict__ found, unsigned long long *__restrict__ colsum) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,18 +469,21 @@ __global__ __launch_bounds__(256) void k_xdrop(XdropArgs a, PassOut po, gxy_hsp 
                         sw[w] = (sw[w] & keep) | (0x80808080u & ~keep);
                     }
                 }
-                uint32_t stopbits = 0;
+                // x_drop >= 0 here (the generic kernel takes the rest), so a
+                // column that improves the best can never satisfy the stop test:
+                // the reference's else-branch needs no separate guard.
+                uint32_t stopbits = 0, kbest = 0;
 #pragma unroll
                 for (int k = 0; k < 32; k++) {
                     const int32_t sc = __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8);
                     const int32_t r = sat_add(run, sc);
-                    const bool better = r > best;
-                    best = better ? r : best;
-                    bext = better ? base + (uint32_t)(k + 1) : bext;
-                    const bool stop = !better && sat_sub(best, r) > a.xdrop;
+                    kbest = r > best ? (uint32_t)(k + 1) : kbest;
+                    best = max(best, r);
+                    const bool stop = sat_sub(best, r) > a.xdrop;
                     stopbits |= stop ? (1u << k) : 0u;
                     run = stop ? kPinned : r;
                 }
+                if (kbest) bext = base + kbest;
                 const uint32_t live = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
                 const uint32_t sb = stopbits & live;
                 const bool ended = sb != 0 || rem <= 32;
@@ -661,6 +664,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     gxy_search_stats st;
     std::memset(&st, 0, sizeof st);
     result->clear();
+    PhaseClock clk(ctx);
 
     const uint64_t max_start = qlen >= pat.span ? qlen - pat.span : 0;
     const uint64_t nwin = max_start / sp->step + 1;
@@ -705,12 +709,13 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         nvalid = nv;
     }
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    clk.lap(kTrSearchCount);
     st.query_windows = nvalid;
     st.lookups = nvalid * nvar;
     st.hits = H;
     st.extended = H;
 
-    const bool byte_ok = scoring_byte_ok(*scm) && tab->d_tbytes != nullptr;
+    const bool byte_ok = scoring_byte_ok(*scm) && tab->d_tbytes != nullptr && sp->hsp.x_drop >= 0;
     const ScorerI32 s32 = make_scorer_i32(*scm);
     const uint8_t *qb = qf, *qbr = qr;
     if (byte_ok && !(qf && qr)) {
@@ -723,6 +728,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         qbr = b2;
     }
     uint64_t npass = 0;
+    clk.lap(kTrSearchOther);
     if (H > 0) {
         // Hits are expanded and extended in batches of at most kCap.
         const uint64_t kCap = 1ull << 27;
@@ -786,6 +792,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         npass = c2[0];
         st.columns = c2[1];
+        clk.lap(kTrSearchExtend);
     }
     st.passing = npass;
     if (npass > 0) {
@@ -819,6 +826,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         GXY_HIP(ctx, hipMemcpyAsync(&last_pos, pos + npass - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipMemcpyAsync(&last_flag, flag + npass - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        clk.lap(kTrSearchAdmit);
         const uint64_t nemit = (uint64_t)last_pos + last_flag;
         st.emitted = nemit;
         // emitted (diag, t_start) keys into pkeys/i1 (free now), sort into k2/i2
@@ -835,6 +843,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         std::vector<gxy_hsp> v(nemit);
         GXY_HIP(ctx, hipMemcpyAsync(v.data(), outd, nemit * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        clk.lap(kTrSearchCopy);
         // Vec::dedup (src/seed_search.rs:123) and the tie count.
         result->reserve(nemit);
         for (uint64_t k = 0; k < nemit; k++) {
@@ -849,6 +858,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             }
             result->push_back(h);
         }
+        clk.lap(kTrSearchDedup);
     }
     if (stats) *stats = st;
     return GXY_OK;
@@ -916,7 +926,7 @@ int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *que
         e = hipMemcpyAsync(dtp, t_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dqp, q_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) {
-            if (scoring_byte_ok(*scoring)) {
+            if (scoring_byte_ok(*scoring) && params->x_drop >= 0) {
                 auto *tb = (uint8_t *)scratch(ctx, kSlotTBytes, dt->len + kBytePad);
                 auto *tbr = (uint8_t *)scratch(ctx, kSlotTBytesRev, dt->len + kBytePad);
                 auto *qb = (uint8_t *)scratch(ctx, kSlotQBytes, dq->len + kBytePad);
