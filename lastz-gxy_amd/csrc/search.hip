@@ -366,16 +366,18 @@ __global__ __launch_bounds__(256) void k_extend(const uint64_t *__restrict__ hit
 // ---- persistent-lane x-drop over the byte layout ------------------------------
 //
 // Each lane owns hits h = lane, lane + stride, ... and steps through them as a
-// small state machine — seed, right walk, left walk — one 32-column chunk per
-// loop iteration, so a lane whose walk ends early starts its next hit instead
-// of idling while the rest of its wavefront finishes.  Lanes own ~stride-apart
+// small state machine — right walk, left walk — one 32-column chunk per loop
+// iteration, so a lane whose walk ends early starts its next hit instead of
+// idling while the rest of its wavefront finishes.  Lanes own ~stride-apart
 // hits, so per-lane totals even out and the wavefront stays full until the
-// end.  Per column: one signed byte extract, a saturating add, max/compare for
-// the best prefix and its extent, and the x-drop test; after the first stop the
-// running score is pinned to a large negative value so later columns of the
-// chunk cannot move the best (src/hsp.rs:179-193 semantics, branch-free).
-// Left walks read the reversed copies forward.
-enum : uint32_t { kPhSeed = 0, kPhRight = 1, kPhLeft = 2, kPhDone = 3 };
+// end.  The seed's own score (at most 32 columns) is taken when the hit is
+// fetched: score bytes masked to the seed span, summed four at a time with a
+// signed dot product.  Per walk column: one signed byte extract, a saturating
+// add, max/compare for the best prefix and its extent, and the x-drop test;
+// after the first stop the running score is pinned to a large negative value
+// so later columns of the chunk cannot move the best (src/hsp.rs:179-193
+// semantics, branch-free).  Left walks read the reversed copies forward.
+enum : uint32_t { kPhRight = 1, kPhLeft = 2, kPhDone = 3 };
 
 struct XdropArgs {
     const uint64_t *hits;      // (q << 32 | t) records, or nullptr with tpos/qpos
@@ -392,8 +394,9 @@ struct XdropArgs {
 };
 
 template <bool kAppend>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop(XdropArgs a, PassOut po, gxy_hsp *__restrict__ out,
-                                               uint8_t *__restrict__ found, unsigned long long *__restrict__ colsum) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop(
+    XdropArgs a, PassOut po, gxy_hsp *__restrict__ out, uint8_t *__restrict__ found,
+    unsigned long long *__restrict__ colsum) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -403,7 +406,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     uint32_t bext = 0, rext = 0, base = 0;
     uint64_t cols = 0;
     constexpr int32_t kPinned = -(1 << 30);
+    // seed span in score words; bytes of the last word inside the span
+    const uint32_t nsw = (a.span + 3) >> 2;
+    const uint32_t lastmask = (a.span & 3) ? ((1u << (8 * (a.span & 3))) - 1u) : 0xFFFFFFFFu;
 
+    auto begin_left = [&]() {  // left walk from (t-1, q-1), src/hsp.rs:129
+        phase = kPhLeft;
+        pt = a.tlen - tp;  // reversed index of tp-1
+        pq = a.qlen - qp;
+        rem = min((uint64_t)tp, (uint64_t)qp);
+        run = best = 0;
+        bext = base = 0;
+    };
     auto fetch = [&]() {
         phase = kPhDone;
         while (h < a.nh) {
@@ -416,12 +430,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                 qp = a.qpos[h];
             }
             if ((uint64_t)tp + a.span <= a.tlen && (uint64_t)qp + a.span <= a.qlen) {  // src/hsp.rs:113-115
-                phase = kPhSeed;
-                pt = tp;
-                pq = qp;
-                rem = a.span;
-                seed = 0;
+                // seed score (src/hsp.rs:118-124); |sum| <= 32*128, no saturation
+                uint32_t tw[8], qw[8];
+                load_bytes32(a.tf, tp, tw);
+                load_bytes32(a.qf, qp, qw);
+                int32_t sd = 0;
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    if ((uint32_t)w < nsw) {
+                        uint32_t sw = score4(a.lut, tw[w], qw[w]);
+                        if ((uint32_t)w == nsw - 1) sw &= lastmask;
+                        sd = __builtin_amdgcn_sdot4((int)sw, 0x01010101, sd, false);
+                    }
+                }
+                seed = sd;
                 cols += a.span;
+                // right walk from the seed end (src/hsp.rs:126-127)
+                phase = kPhRight;
+                pt = (uint64_t)tp + a.span;
+                pq = (uint64_t)qp + a.span;
+                rem = (pt < a.tlen && pq < a.qlen) ? min(a.tlen - pt, a.qlen - pq) : 0;
+                run = best = 0;
+                bext = base = 0;
+                if (rem == 0) {  // no right columns at all
+                    rbest = 0;
+                    rext = 0;
+                    begin_left();
+                }
                 return;
             }
             if (!kAppend) {
@@ -445,83 +480,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
 #pragma unroll
             for (int w = 0; w < 8; w++) sw[w] = score4(a.lut, tw[w], qw[w]);
             const uint32_t n = (uint32_t)min<uint64_t>(32, rem);
-            if (phase == kPhSeed) {
+            if (n < 32) {  // columns past the sequence end must not move the best
 #pragma unroll
-                for (int k = 0; k < 32; k++)
-                    if ((uint32_t)k < n) seed = sat_add(seed, __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8));
-                rem -= n;
-                pt += n;
-                pq += n;
-                if (rem == 0) {  // seed done: set up the right walk (src/hsp.rs:126-127)
-                    phase = kPhRight;
-                    pt = (uint64_t)tp + a.span;
-                    pq = (uint64_t)qp + a.span;
-                    rem = (pt < a.tlen && pq < a.qlen) ? min(a.tlen - pt, a.qlen - pq) : 0;
-                    run = best = 0;
-                    bext = base = 0;
+                for (int w = 0; w < 8; w++) {
+                    const uint32_t live = n > 4u * w ? min(4u, n - 4u * w) : 0u;
+                    const uint32_t keep = live >= 4 ? 0xFFFFFFFFu : ((1u << (8 * live)) - 1u);
+                    sw[w] = (sw[w] & keep) | (0x80808080u & ~keep);
                 }
-            } else {
-                if (n < 32) {  // columns past the sequence end must not move the best
+            }
+            // x_drop >= 0 here (the generic kernel takes the rest), so a
+            // column that improves the best can never satisfy the stop test:
+            // the reference's else-branch needs no separate guard.
+            uint32_t stopbits = 0, kbest = 0;
 #pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        const uint32_t live = n > 4u * w ? min(4u, n - 4u * w) : 0u;
-                        const uint32_t keep = live >= 4 ? 0xFFFFFFFFu : ((1u << (8 * live)) - 1u);
-                        sw[w] = (sw[w] & keep) | (0x80808080u & ~keep);
-                    }
-                }
-                // x_drop >= 0 here (the generic kernel takes the rest), so a
-                // column that improves the best can never satisfy the stop test:
-                // the reference's else-branch needs no separate guard.
-                uint32_t stopbits = 0, kbest = 0;
-#pragma unroll
-                for (int k = 0; k < 32; k++) {
-                    const int32_t sc = __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8);
-                    const int32_t r = sat_add(run, sc);
-                    kbest = r > best ? (uint32_t)(k + 1) : kbest;
-                    best = max(best, r);
-                    const bool stop = sat_sub(best, r) > a.xdrop;
-                    stopbits |= stop ? (1u << k) : 0u;
-                    run = stop ? kPinned : r;
-                }
-                if (kbest) bext = base + kbest;
-                const uint32_t live = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
-                const uint32_t sb = stopbits & live;
-                const bool ended = sb != 0 || rem <= 32;
-                cols += sb ? (uint32_t)__ffs(sb) : n;
-                rem -= n;
-                pt += n;
-                pq += n;
-                base += 32;
-                if (ended) {
-                    if (phase == kPhRight) {  // left walk from (t-1, q-1), src/hsp.rs:129
-                        rbest = best;
-                        rext = bext;
-                        phase = kPhLeft;
-                        pt = a.tlen - tp;  // reversed index of tp-1
-                        pq = a.qlen - qp;
-                        rem = min((uint64_t)tp, (uint64_t)qp);
-                        run = best = 0;
-                        bext = base = 0;
-                    } else {
+            for (int k = 0; k < 32; k++) {
+                const int32_t sc = __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8);
+                const int32_t r = sat_add(run, sc);
+                kbest = r > best ? (uint32_t)(k + 1) : kbest;
+                best = max(best, r);
+                const bool stop = sat_sub(best, r) > a.xdrop;
+                stopbits |= stop ? (1u << k) : 0u;
+                run = stop ? kPinned : r;
+            }
+            if (kbest) bext = base + kbest;
+            const uint32_t live = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+            const uint32_t sb = stopbits & live;
+            const bool ended = sb != 0 || rem <= 32;
+            cols += sb ? (uint32_t)__ffs(sb) : n;
+            rem -= n;
+            pt += n;
+            pq += n;
+            base += 32;
+            if (ended) {
+                if (phase == kPhRight) {
+                    rbest = best;
+                    rext = bext;
+                    begin_left();
+                    if (rem == 0) {  // no left columns at all
+                        best = 0;
+                        bext = 0;
                         finished = true;
                     }
+                } else {
+                    finished = true;
                 }
-            }
-            // a side with no columns at all ends at once
-            if (!finished && phase == kPhRight && rem == 0) {
-                rbest = 0;
-                rext = 0;
-                phase = kPhLeft;
-                pt = a.tlen - tp;
-                pq = a.qlen - qp;
-                rem = min((uint64_t)tp, (uint64_t)qp);
-                run = best = 0;
-                bext = base = 0;
-            }
-            if (!finished && phase == kPhLeft && rem == 0) {
-                best = 0;
-                bext = 0;
-                finished = true;
             }
             if (finished) {  // src/hsp.rs:131-147
                 const int32_t score = sat_add(sat_add(seed, best), rbest);
