@@ -95,7 +95,7 @@ __global__ void k_qcount(const Blk *__restrict__ q, uint64_t qlen, DevPattern pa
 
 // transitions <= 1 and weight <= 15: the (at most 16) variant words are held
 // in registers so all bitmap loads issue together; the mask of non-empty
-// variants is kept for k_qexpand_v so it revisits only those.
+// variants is kept for k_qexpand_w so it revisits only those.
 constexpr int kMaxV1 = 16;
 
 __device__ __forceinline__ uint64_t variant1(uint64_t idx, uint32_t weight, int i) {
@@ -148,23 +148,70 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
     if ((threadIdx.x & 63) == 0 && b) spread_add(nvalid, (unsigned long long)__popcll(b));
 }
 
-__global__ void k_qexpand_v(uint64_t w0, uint64_t w1, uint64_t step, const uint32_t *__restrict__ words,
-                            const uint16_t *__restrict__ vmask, const uint64_t *__restrict__ off, uint64_t base,
-                            uint32_t weight, const uint32_t *__restrict__ starts,
-                            const uint32_t *__restrict__ positions, uint64_t *__restrict__ hits) {
-    const uint64_t w = w0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= w1) return;
-    uint32_t m = vmask[w];
-    if (m == 0) return;
-    const uint32_t word = words[w];
-    uint64_t o = off[w] - base;
-    const uint64_t qhi = (w * step) << 32;
-    while (m) {
-        const int i = __ffs(m) - 1;
-        m &= m - 1;
-        const uint64_t v = variant1(word, weight, i);
-        const uint32_t b = starts[v], e = starts[v + 1];
-        for (uint32_t k = b; k < e; k++) hits[o++] = qhi | positions[k];
+// Load-balanced expansion (transitions <= 1): one wavefront per 64
+// consecutive windows.  Each lane lists its window's non-empty variant
+// buckets in LDS; the wavefront's output range [off[w0], off[w0+64]) is then
+// split evenly over the lanes, each output slot finding its window by binary
+// search over the 64 window offsets and its bucket by a short scan, so hit
+// stores are coalesced and a window with many hits is shared by all lanes.
+constexpr int kExpWaves = 4;
+
+__global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uint64_t step,
+                                                   const uint32_t *__restrict__ words,
+                                                   const uint16_t *__restrict__ vmask,
+                                                   const uint64_t *__restrict__ off, uint64_t base, uint32_t weight,
+                                                   const uint32_t *__restrict__ starts,
+                                                   const uint32_t *__restrict__ positions,
+                                                   uint64_t *__restrict__ hits) {
+    __shared__ uint32_t lbeg[kExpWaves][64][kMaxV1];
+    __shared__ uint32_t lcum[kExpWaves][64][kMaxV1];
+    __shared__ uint64_t lo[kExpWaves][64];
+    __shared__ uint32_t lnb[kExpWaves][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t wb = w0 + ((uint64_t)blockIdx.x * kExpWaves + wv) * 64;
+    const uint64_t wend = min(wb + 64, w1);
+    const uint64_t w = wb + lane;
+    const bool active = wb < w1;
+    uint64_t oend = 0;
+    if (active) {
+        oend = off[wend] - base;
+        uint64_t o = oend;
+        uint32_t nb = 0;
+        if (w < w1) {
+            o = off[w] - base;
+            uint32_t m = vmask[w];
+            if (m) {
+                const uint32_t word = words[w];
+                uint32_t cum = 0;
+                while (m) {
+                    const int i = __ffs(m) - 1;
+                    m &= m - 1;
+                    const uint64_t v = variant1(word, weight, i);
+                    const uint32_t b = starts[v], e = starts[v + 1];
+                    lbeg[wv][lane][nb] = b;
+                    lcum[wv][lane][nb] = cum;
+                    cum += e - b;
+                    nb++;
+                }
+            }
+        }
+        lo[wv][lane] = o;
+        lnb[wv][lane] = nb;
+    }
+    __syncthreads();
+    if (!active) return;
+    for (uint64_t j = lo[wv][0] + lane; j < oend; j += 64) {
+        // owning window: the last lane L with lo[L] <= j
+        int L = 0;
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1)
+            if (lo[wv][L + s] <= j) L += s;
+        const uint32_t r = (uint32_t)(j - lo[wv][L]);
+        const uint32_t nb = lnb[wv][L];
+        uint32_t i = 0;
+        while (i + 1 < nb && lcum[wv][L][i + 1] <= r) i++;
+        const uint32_t p = lbeg[wv][L][i] + (r - lcum[wv][L][i]);
+        hits[j] = (((wb + (uint64_t)L) * step) << 32) | positions[p];
     }
 }
 
@@ -380,9 +427,6 @@ __global__ __launch_bounds__(256) void k_extend(const uint64_t *__restrict__ hit
 enum : uint32_t { kPhRight = 1, kPhLeft = 2, kPhDone = 3 };
 
 struct XdropArgs {
-    const uint64_t *hits;      // (q << 32 | t) records, or nullptr with tpos/qpos
-    const uint32_t *tpos, *qpos;
-    uint64_t nh;
     const uint8_t *tf, *tr, *qf, *qr;
     uint64_t tlen, qlen;
     const Blk *tblk;           // block layout of the target (entropy gate)
@@ -393,12 +437,42 @@ struct XdropArgs {
     int32_t entropy_thresh;
 };
 
-template <bool kAppend>
+// Hit sources: next(t, q) yields the lane's next seed hit, `cur` its index
+// where results are written per hit.
+struct HitArraySrc {  // (q << 32 | t) records
+    const uint64_t *hits;
+    uint64_t n, h, stride, cur;
+    __device__ void init(uint64_t gid, uint64_t s) { h = gid; stride = s; }
+    __device__ __forceinline__ bool next(uint32_t &t, uint32_t &q) {
+        if (h >= n) return false;
+        const uint64_t x = hits[h];
+        q = (uint32_t)(x >> 32);
+        t = (uint32_t)x;
+        cur = h;
+        h += stride;
+        return true;
+    }
+};
+
+struct PairSrc {  // separate t and q position arrays
+    const uint32_t *tpos, *qpos;
+    uint64_t n, h, stride, cur;
+    __device__ void init(uint64_t gid, uint64_t s) { h = gid; stride = s; }
+    __device__ __forceinline__ bool next(uint32_t &t, uint32_t &q) {
+        if (h >= n) return false;
+        t = tpos[h];
+        q = qpos[h];
+        cur = h;
+        h += stride;
+        return true;
+    }
+};
+
+template <class Src, bool kAppend>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop(
-    XdropArgs a, PassOut po, gxy_hsp *__restrict__ out, uint8_t *__restrict__ found,
+    XdropArgs a, Src src, PassOut po, gxy_hsp *__restrict__ out, uint8_t *__restrict__ found,
     unsigned long long *__restrict__ colsum) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    src.init((uint64_t)blockIdx.x * blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
     const int lane = threadIdx.x & 63;
     uint32_t tp = 0, qp = 0, phase = kPhDone;
     uint64_t pt = 0, pq = 0, rem = 0;
@@ -420,15 +494,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     };
     auto fetch = [&]() {
         phase = kPhDone;
-        while (h < a.nh) {
-            if (a.hits) {
-                const uint64_t x = a.hits[h];
-                qp = (uint32_t)(x >> 32);
-                tp = (uint32_t)x;
-            } else {
-                tp = a.tpos[h];
-                qp = a.qpos[h];
-            }
+        while (src.next(tp, qp)) {
             if ((uint64_t)tp + a.span <= a.tlen && (uint64_t)qp + a.span <= a.qlen) {  // src/hsp.rs:113-115
                 // seed score (src/hsp.rs:118-124); |sum| <= 32*128, no saturation
                 uint32_t tw[8], qw[8];
@@ -460,10 +526,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                 return;
             }
             if (!kAppend) {
-                out[h] = gxy_hsp{0, 0, 0, 0};
-                found[h] = 0;
+                out[src.cur] = gxy_hsp{0, 0, 0, 0};
+                found[src.cur] = 0;
             }
-            h += stride;
         }
     };
     fetch();
@@ -536,8 +601,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                     pass = entropy_pass(a.tblk, a.tlen, res.t_start, res.t_start + res.length, res.score,
                                         a.entropy_thresh);
                 if (!kAppend) {
-                    out[h] = pass ? res : gxy_hsp{0, 0, 0, 0};
-                    found[h] = pass ? 1 : 0;
+                    out[src.cur] = pass ? res : gxy_hsp{0, 0, 0, 0};
+                    found[src.cur] = pass ? 1 : 0;
                 }
             }
         }
@@ -556,10 +621,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                 }
             }
         }
-        if (finished) {
-            h += stride;
-            fetch();
-        }
+        if (finished) fetch();
     }
     unsigned long long c = cols;
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
@@ -766,7 +828,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             {
                 KernelTimer kt_(ctx, "qexpand");
                 if (fast_var)
-                    k_qexpand_v<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(
+                    k_qexpand_w<<<grid_for(w1 - w0, 64 * kExpWaves), 64 * kExpWaves, 0, ctx->stream>>>(
                         w0, w1, sp->step, words, vmask, off, boff[k], pat.weight, tab->d_starts, tab->d_positions,
                         hits);
                 else
@@ -777,11 +839,12 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             GXY_LAUNCH_CHECK(ctx);
             KernelTimer kt_ext(ctx, "extend");
             if (byte_ok) {
-                XdropArgs xa{hits, nullptr, nullptr, nh, tab->d_tbytes, tab->d_tbytes_rev, qb, qbr, tlen, qlen,
-                             target->d, make_byte_lut(*scm), sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
+                XdropArgs xa{tab->d_tbytes, tab->d_tbytes_rev, qb, qbr, tlen, qlen, target->d,
+                             make_byte_lut(*scm), sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
                              sp->entropy_enabled, sp->entropy_threshold};
+                HitArraySrc src{hits, nh};
                 const unsigned g = std::min<unsigned>(grid_for(nh, 256), kXdropBlocks);
-                k_xdrop<true><<<g, 256, 0, ctx->stream>>>(xa, po, nullptr, nullptr, ctr + 2);
+                k_xdrop<HitArraySrc, true><<<g, 256, 0, ctx->stream>>>(xa, src, po, nullptr, nullptr, ctr + 2);
             } else {
                 k_extend<ScorerI32><<<grid_for(nh, 256), 256, 0, ctx->stream>>>(
                     hits, nh, target->d, tlen, query->d, qlen, s32, sp->hsp.x_drop, sp->hsp.hsp_threshold,
@@ -945,12 +1008,13 @@ int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *que
                     gxy_dseq_free(dq);
                     return rc;
                 }
-                XdropArgs xa{nullptr, dtp, dqp, n, tb, tbr, qb, qbr, dt->len, dq->len, dt->d,
-                             make_byte_lut(*scoring), params->x_drop, params->hsp_threshold, seed_len, 0, 0};
+                XdropArgs xa{tb, tbr, qb, qbr, dt->len, dq->len, dt->d, make_byte_lut(*scoring),
+                             params->x_drop, params->hsp_threshold, seed_len, 0, 0};
+                PairSrc src{dtp, dqp, n};
                 PassOut none{nullptr, nullptr, nullptr};
                 auto *cs = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
-                k_xdrop<false><<<std::min<unsigned>(grid_for(n, 256), kXdropBlocks), 256, 0, ctx->stream>>>(
-                    xa, none, dout, dfound, cs);
+                k_xdrop<PairSrc, false><<<std::min<unsigned>(grid_for(n, 256), kXdropBlocks), 256, 0,
+                                          ctx->stream>>>(xa, src, none, dout, dfound, cs);
             } else
                 k_extend_batch<ScorerI32><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
                     dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i32(*scoring), params->x_drop,
