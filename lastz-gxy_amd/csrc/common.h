@@ -18,7 +18,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/gxy.h"
@@ -289,6 +291,7 @@ struct gxy_dseq {
     uint64_t nblk = 0;
     gxy::Blk *d = nullptr;  // nblk blocks (last one zero)
     int device = 0;
+    gxy_ctx *pool = nullptr;  // block cache `d` came from (driver-internal sequences)
 };
 
 struct gxy_postable {
@@ -305,6 +308,7 @@ struct gxy_postable {
     uint32_t *d_nonempty = nullptr;   // bit w set when word w has positions
     uint8_t *d_tbytes = nullptr;      // target, one byte per base (x-drop layout)
     uint8_t *d_tbytes_rev = nullptr;  // the same, reversed
+    gxy_ctx *pool = nullptr;          // block cache the buffers came from (driver-internal tables)
 };
 
 struct gxy_ctx {
@@ -327,6 +331,13 @@ struct gxy_ctx {
     std::vector<Span> spans;
     std::vector<hipEvent_t> event_pool;
     std::vector<std::pair<std::string, std::pair<double, uint64_t>>> totals;
+    // Device block cache: while pool_on is set (inside the driver), sequence
+    // and table buffers are taken from and returned to these lists instead of
+    // hipMalloc/hipFree, so repeated runs of the same shapes allocate nothing.
+    // Reuse is ordered by the context's single stream.
+    bool pool_on = false;
+    std::multimap<size_t, void *> pool_free;
+    std::unordered_map<void *, size_t> pool_used;
     // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.
     bool trace = std::getenv("GXY_TRACE") != nullptr;
     double trace_ms[16] = {};
@@ -358,6 +369,16 @@ enum TracePhase : int {
 namespace gxy {
 
 int set_err(gxy_ctx *ctx, int code, const char *fmt, ...);
+// Device allocation for sequences and tables: from the context's block cache
+// when ctx->pool_on (then *owner = ctx), else hipMalloc (*owner = nullptr).
+// nullptr on failure.
+void *dev_alloc(gxy_ctx *ctx, size_t bytes, gxy_ctx **owner);
+// The same with the owner given: its cache when non-null, else hipMalloc.
+void *dev_alloc_from(gxy_ctx *owner, size_t bytes);
+// Returns p to owner's cache, or hipFree when owner is null.
+void dev_free(gxy_ctx *owner, void *p);
+// hipFree every cached block (after synchronising the stream).
+void pool_trim(gxy_ctx *ctx);
 // Sum of a spread counter block (synchronises the stream).
 int spread_sum(gxy_ctx *ctx, const unsigned long long *c, uint64_t *out);
 

@@ -33,6 +33,42 @@ using namespace gxy;
 
 namespace {
 
+// Sort key of a record within its group: (t_start, q_start) and
+// (t_span, q_span) packed, with the record's index.
+struct SortKey {
+    uint64_t k1, k2;
+    uint32_t i;
+};
+
+// Stable LSD radix sort by (k1, k2) — sort_by_key's order, equal keys kept in
+// input order (src/driver.rs:376).  11-bit digits, least significant first;
+// digits that are equal across the whole input are skipped.
+void radix_sort_keys(std::vector<SortKey> &a, std::vector<SortKey> &tmp) {
+    const size_t n = a.size();
+    if (n < 2) return;
+    tmp.resize(n);
+    constexpr int kBits = 11, kBuckets = 1 << kBits;
+    std::vector<uint32_t> cnt(kBuckets);
+    for (int word = 1; word >= 0; word--) {
+        for (int shift = 0; shift < 64; shift += kBits) {
+            auto digit = [&](const SortKey &x) {
+                return (uint32_t)(((word ? x.k2 : x.k1) >> shift) & (kBuckets - 1));
+            };
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (size_t k = 0; k < n; k++) cnt[digit(a[k])]++;
+            if (cnt[digit(a[0])] == n) continue;  // constant digit
+            uint32_t sum = 0;
+            for (int b = 0; b < kBuckets; b++) {
+                const uint32_t c = cnt[b];
+                cnt[b] = sum;
+                sum += c;
+            }
+            for (size_t k = 0; k < n; k++) tmp[cnt[digit(a[k])]++] = a[k];
+            a.swap(tmp);
+        }
+    }
+}
+
 // A record (src/output/mod.rs:17-31) as a plain value: an ungapped record's
 // script is one implicit Match run of t_span (n_runs == 0); a gapped record's
 // runs live in the run's shared run store.
@@ -123,6 +159,13 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         return set_err(ctx, GXY_EUNSUP, "chaining and the tweener are not on this library's path");
     if (cfg->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
     GXY_HIP(ctx, hipSetDevice(ctx->device));
+    PhaseClock clk(ctx);
+    // Internal sequences and tables come from the context's block cache.
+    struct PoolScope {
+        gxy_ctx *c;
+        explicit PoolScope(gxy_ctx *x) : c(x) { c->pool_on = true; }
+        ~PoolScope() { c->pool_on = false; }
+    } pool_scope(ctx);
     gxy_run_stats rs;
     std::memset(&rs, 0, sizeof rs);
     const int respect = cfg->respect_masking ? 1 : 0;
@@ -144,9 +187,8 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     // byte copies of every query strand, shared by all chunk searches
     struct DevBytes {
         uint8_t *p = nullptr;
-        ~DevBytes() {
-            if (p) (void)hipFree(p);
-        }
+        gxy_ctx *owner = nullptr;
+        ~DevBytes() { dev_free(owner, p); }
     };
     const bool byte_ok = scoring_byte_ok(cfg->scoring);
     std::vector<DevBytes> qbytes(byte_ok ? 4 * n_queries : 0);
@@ -156,11 +198,10 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 const gxy_dseq *qs = s == 0 ? queries[i] : drc[i].get();
                 if (!qs) continue;
                 const size_t nb = qs->len + kBytePad;
-                if (hipMalloc(&qbytes[4 * i + 2 * s].p, nb) != hipSuccess ||
-                    hipMalloc(&qbytes[4 * i + 2 * s + 1].p, nb) != hipSuccess) {
-                    (void)hipGetLastError();
-                    return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
-                }
+                DevBytes &bf = qbytes[4 * i + 2 * s], &br = qbytes[4 * i + 2 * s + 1];
+                bf.p = (uint8_t *)dev_alloc(ctx, nb, &bf.owner);
+                br.p = (uint8_t *)dev_alloc(ctx, nb, &br.owner);
+                if (!bf.p || !br.p) return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
                 int rc = seq_to_bytes(ctx, qs, qbytes[4 * i + 2 * s].p, qbytes[4 * i + 2 * s + 1].p);
                 if (rc) return rc;
             }
@@ -207,7 +248,6 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     std::vector<Group> groups;
     std::vector<gxy_edit_run> run_store;  // scripts of gapped records
 
-    PhaseClock clk(ctx);
     clk.lap(kTrSetup);
     for (size_t c = 0; c < chunks.size(); c++) {
         const Chunk &ch = chunks[c];
@@ -332,58 +372,59 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         if (a.strand != b.strand) return a.strand < b.strand;
         return a.chunk < b.chunk;
     });
-    std::vector<Rec> all;
-    for (auto &g : groups) {
-        std::stable_sort(g.recs.begin(), g.recs.end(), [](const Rec &a, const Rec &b) {
-            if (a.t_start != b.t_start) return a.t_start < b.t_start;
-            if (a.q_start != b.q_start) return a.q_start < b.q_start;
-            if (a.t_span != b.t_span) return a.t_span < b.t_span;
-            return a.q_span < b.q_span;
-        });
-        const size_t first = all.size();
-        for (auto &r : g.recs) {
-            if (all.size() > first) {
-                const Rec &b = all.back();
-                if (r.t_start == b.t_start && r.q_start == b.q_start && r.t_span == b.t_span &&
-                    r.q_span == b.q_span && r.strand == b.strand)
-                    continue;
-            }
-            all.push_back(r);
-        }
+    size_t total = 0, runs_ub = 0;
+    for (const auto &g : groups) {
+        total += g.recs.size();
+        for (const auto &r : g.recs) runs_ub += r.n_runs ? r.n_runs : 1;
     }
     auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
     if (!set) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
-    size_t nruns = 0;
-    for (const auto &r : all) nruns += r.n_runs ? r.n_runs : 1;
-    set->n = all.size();
-    set->records = (gxy_record *)std::calloc(std::max<size_t>(1, all.size()), sizeof(gxy_record));
-    set->n_runs = nruns;
-    set->runs = (gxy_edit_run *)std::calloc(std::max<size_t>(1, nruns), sizeof(gxy_edit_run));
+    set->records = (gxy_record *)std::malloc(std::max<size_t>(1, total) * sizeof(gxy_record));
+    set->runs = (gxy_edit_run *)std::malloc(std::max<size_t>(1, runs_ub) * sizeof(gxy_edit_run));
     if (!set->records || !set->runs) {
         gxy_record_set_free(set);
         return set_err(ctx, GXY_ENOMEM, "host allocation failed");
     }
-    size_t ro = 0;
-    for (size_t k = 0; k < all.size(); k++) {
-        const Rec &r = all[k];
-        gxy_record &o = set->records[k];
-        o.target_index = r.ti;
-        o.query_index = r.qi;
-        o.strand = r.strand;
-        o.t_start = r.t_start;
-        o.q_start = r.q_start;
-        o.t_span = r.t_span;
-        o.q_span = r.q_span;
-        o.score = r.score;
-        o.runs_offset = ro;
-        if (r.n_runs == 0) {  // build_ungapped_record: one Match run (src/driver.rs:451-452)
-            o.n_runs = 1;
-            set->runs[ro++] = {GXY_OP_MATCH, r.t_span};
-        } else {
-            o.n_runs = r.n_runs;
-            for (uint32_t x = 0; x < r.n_runs; x++) set->runs[ro++] = run_store[r.runs_offset + x];
+    std::vector<SortKey> keys, tmp;
+    size_t nrec = 0, ro = 0;
+    for (auto &g : groups) {
+        keys.resize(g.recs.size());
+        for (size_t k = 0; k < g.recs.size(); k++) {
+            const Rec &r = g.recs[k];
+            keys[k] = {(uint64_t)r.t_start << 32 | r.q_start, (uint64_t)r.t_span << 32 | r.q_span, (uint32_t)k};
+        }
+        radix_sort_keys(keys, tmp);
+        const size_t first = nrec;
+        for (const SortKey &sk : keys) {
+            const Rec &r = g.recs[sk.i];
+            if (nrec > first) {  // dedup_by on the sorted group (src/driver.rs:376-387)
+                const gxy_record &b = set->records[nrec - 1];
+                if (r.t_start == b.t_start && r.q_start == b.q_start && r.t_span == b.t_span &&
+                    r.q_span == b.q_span && r.strand == b.strand)
+                    continue;
+            }
+            gxy_record &o = set->records[nrec++];
+            o.target_index = r.ti;
+            o.query_index = r.qi;
+            o.strand = r.strand;
+            o.t_start = r.t_start;
+            o.q_start = r.q_start;
+            o.t_span = r.t_span;
+            o.q_span = r.q_span;
+            o.score = r.score;
+            o.reserved = 0;
+            o.runs_offset = ro;
+            if (r.n_runs == 0) {  // build_ungapped_record: one Match run (src/driver.rs:451-452)
+                o.n_runs = 1;
+                set->runs[ro++] = {GXY_OP_MATCH, r.t_span};
+            } else {
+                o.n_runs = r.n_runs;
+                for (uint32_t x = 0; x < r.n_runs; x++) set->runs[ro++] = run_store[r.runs_offset + x];
+            }
         }
     }
+    set->n = nrec;
+    set->n_runs = ro;
     *out = set;
     if (stats) *stats = rs;
     clk.lap(kTrMerge);

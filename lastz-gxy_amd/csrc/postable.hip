@@ -154,9 +154,9 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         return GXY_EDEVICE;
     }
     tab->npos = nvalid;
-    if (hipMalloc(&tab->d_starts, (tab->num_words + 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&tab->d_positions, std::max<uint64_t>(1, nvalid) * sizeof(uint32_t)) != hipSuccess) {
-        (void)hipGetLastError();
+    tab->d_starts = (uint32_t *)dev_alloc(ctx, (tab->num_words + 1) * sizeof(uint32_t), &tab->pool);
+    tab->d_positions = (uint32_t *)dev_alloc(ctx, std::max<uint64_t>(1, nvalid) * sizeof(uint32_t), &tab->pool);
+    if (!tab->d_starts || !tab->d_positions) {
         gxy_postable_free(tab);
         return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     }
@@ -168,9 +168,10 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_table_starts", e);
     const uint64_t nbw = (tab->num_words + 31) / 32;
     const uint64_t nbytes = t->len + kBytePad;
-    if (hipMalloc(&tab->d_nonempty, nbw * 4) != hipSuccess || hipMalloc(&tab->d_tbytes, nbytes) != hipSuccess ||
-        hipMalloc(&tab->d_tbytes_rev, nbytes) != hipSuccess) {
-        (void)hipGetLastError();
+    tab->d_nonempty = (uint32_t *)dev_alloc(ctx, nbw * 4, &tab->pool);
+    tab->d_tbytes = (uint8_t *)dev_alloc(ctx, nbytes, &tab->pool);
+    tab->d_tbytes_rev = (uint8_t *)dev_alloc(ctx, nbytes, &tab->pool);
+    if (!tab->d_nonempty || !tab->d_tbytes || !tab->d_tbytes_rev) {
         gxy_postable_free(tab);
         return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     }
@@ -223,14 +224,15 @@ int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
     auto *keep = (uint32_t *)scratch(ctx, kSlotCounts, nw * sizeof(uint32_t));
     uint32_t *ns = nullptr, *np = nullptr;
     if (!keep) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-    GXY_HIP(ctx, hipMalloc(&ns, (nw + 1) * sizeof(uint32_t)));
+    ns = (uint32_t *)dev_alloc_from(tab->pool, (nw + 1) * sizeof(uint32_t));
+    if (!ns) return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     k_prune_counts<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, limit, keep);
     GXY_LAUNCH_CHECK(ctx);
     size_t tmp_bytes = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep, ns, (int)nw, ctx->stream);
     void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
     if (!tmp) {
-        (void)hipFree(ns);
+        dev_free(tab->pool, ns);
         return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     }
     GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, keep, ns, (int)nw, ctx->stream));
@@ -239,13 +241,17 @@ int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
     uint32_t total = 0;
     GXY_HIP(ctx, hipMemcpyAsync(&total, ns + nw, 4, hipMemcpyDeviceToHost, ctx->stream));
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    GXY_HIP(ctx, hipMalloc(&np, std::max<uint32_t>(1, total) * sizeof(uint32_t)));
+    np = (uint32_t *)dev_alloc_from(tab->pool, std::max<uint32_t>(1, total) * sizeof(uint32_t));
+    if (!np) {
+        dev_free(tab->pool, ns);
+        return set_err(ctx, GXY_ENOMEM, "table allocation failed");
+    }
     k_prune_copy<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, ns, keep,
                                                              nw, np);
     GXY_LAUNCH_CHECK(ctx);
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(tab->d_starts);
-    (void)hipFree(tab->d_positions);
+    dev_free(tab->pool, tab->d_starts);
+    dev_free(tab->pool, tab->d_positions);
     tab->d_starts = ns;
     tab->d_positions = np;
     tab->npos = total;
@@ -303,11 +309,11 @@ int gxy_postable_export(gxy_ctx *ctx, const gxy_postable *tab, uint32_t *starts,
 
 void gxy_postable_free(gxy_postable *tab) {
     if (!tab) return;
-    if (tab->d_starts) (void)hipFree(tab->d_starts);
-    if (tab->d_positions) (void)hipFree(tab->d_positions);
-    if (tab->d_nonempty) (void)hipFree(tab->d_nonempty);
-    if (tab->d_tbytes) (void)hipFree(tab->d_tbytes);
-    if (tab->d_tbytes_rev) (void)hipFree(tab->d_tbytes_rev);
+    dev_free(tab->pool, tab->d_starts);
+    dev_free(tab->pool, tab->d_positions);
+    dev_free(tab->pool, tab->d_nonempty);
+    dev_free(tab->pool, tab->d_tbytes);
+    dev_free(tab->pool, tab->d_tbytes_rev);
     if (tab->owned_target) gxy_dseq_free(tab->owned_target);
     delete tab;
 }

@@ -47,6 +47,60 @@ void *scratch(gxy_ctx *ctx, int k, size_t bytes) {
     return ctx->slot[k];
 }
 
+void *dev_alloc(gxy_ctx *ctx, size_t bytes, gxy_ctx **owner) {
+    *owner = ctx->pool_on ? ctx : nullptr;
+    return dev_alloc_from(*owner, bytes);
+}
+
+void *dev_alloc_from(gxy_ctx *ctx, size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    void *p = nullptr;
+    if (!ctx) {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return p;
+    }
+    // smallest cached block that fits without wasting more than a quarter
+    auto it = ctx->pool_free.lower_bound(bytes);
+    if (it != ctx->pool_free.end() && it->first <= bytes + bytes / 4) {
+        p = it->second;
+        ctx->pool_used[p] = it->first;
+        ctx->pool_free.erase(it);
+        return p;
+    }
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim(ctx);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    ctx->pool_used[p] = bytes;
+    return p;
+}
+
+void dev_free(gxy_ctx *owner, void *p) {
+    if (!p) return;
+    if (!owner) {
+        (void)hipFree(p);
+        return;
+    }
+    auto it = owner->pool_used.find(p);
+    if (it == owner->pool_used.end()) return;
+    owner->pool_free.emplace(it->second, p);
+    owner->pool_used.erase(it);
+}
+
+void pool_trim(gxy_ctx *ctx) {
+    if (ctx->pool_free.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->pool_free) (void)hipFree(kv.second);
+    ctx->pool_free.clear();
+}
+
 static hipEvent_t take_event(gxy_ctx *ctx) {
     if (!ctx->event_pool.empty()) {
         hipEvent_t e = ctx->event_pool.back();
@@ -239,8 +293,8 @@ int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
     s->nblk = nblocks(len);
     s->device = ctx->device;
     const size_t bytes = s->nblk * sizeof(Blk);
-    if (hipMalloc(&s->d, bytes) != hipSuccess) {
-        (void)hipGetLastError();
+    s->d = (Blk *)dev_alloc(ctx, bytes, &s->pool);
+    if (!s->d) {
         delete s;
         return set_err(ctx, GXY_ENOMEM, "device allocation of %llu bytes failed",
                        (unsigned long long)bytes);
@@ -336,6 +390,7 @@ void gxy_ctx_destroy(gxy_ctx *ctx) {
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < gxy_ctx::kSlots; k++)
         if (ctx->slot[k]) (void)hipFree(ctx->slot[k]);
+    pool_trim(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -440,7 +495,7 @@ int gxy_dseq_download(gxy_ctx *ctx, const gxy_dseq *seq, uint8_t *codes, uint8_t
 
 void gxy_dseq_free(gxy_dseq *seq) {
     if (!seq) return;
-    if (seq->d) (void)hipFree(seq->d);
+    dev_free(seq->pool, seq->d);
     delete seq;
 }
 
