@@ -24,8 +24,13 @@
 // in ascending chunk order (rayon push order, src/driver.rs:361-372).
 #include <algorithm>
 #include <charconv>
+#include <condition_variable>
+#include <deque>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 
 #include "internal.h"
 
@@ -78,6 +83,92 @@ struct Rec {
     int32_t score;
     uint32_t n_runs;
     uint64_t runs_offset;
+};
+
+// A chunk x query x strand group of records (one entry of src/driver.rs:359-366).
+struct Group {
+    uint32_t ti, qi, strand;
+    size_t chunk;
+    std::vector<Rec> recs;
+};
+
+// Sorts a group by (t_start, q_start, t_span, q_span) and drops duplicates
+// (src/driver.rs:376-387).
+void finalize_group(Group &g) {
+    std::vector<SortKey> keys(g.recs.size()), tmp;
+    for (size_t k = 0; k < g.recs.size(); k++) {
+        const Rec &r = g.recs[k];
+        keys[k] = {(uint64_t)r.t_start << 32 | r.q_start, (uint64_t)r.t_span << 32 | r.q_span, (uint32_t)k};
+    }
+    radix_sort_keys(keys, tmp);
+    std::vector<Rec> out;
+    out.reserve(keys.size());
+    for (const SortKey &sk : keys) {
+        const Rec &r = g.recs[sk.i];
+        if (!out.empty()) {
+            const Rec &b = out.back();
+            if (r.t_start == b.t_start && r.q_start == b.q_start && r.t_span == b.t_span &&
+                r.q_span == b.q_span && r.strand == b.strand)
+                continue;
+        }
+        out.push_back(r);
+    }
+    g.recs.swap(out);
+}
+
+// Host jobs (record building and group sorting) run on worker threads while
+// the calling thread keeps the device busy with the next search.
+class HostJobs {
+  public:
+    explicit HostJobs(unsigned n) {
+        for (unsigned k = 0; k < n; k++) threads_.emplace_back([this] { loop(); });
+    }
+    ~HostJobs() {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+    void submit(std::function<void()> f) {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            q_.push_back(std::move(f));
+            pending_++;
+        }
+        cv_.notify_one();
+    }
+    void wait_all() {
+        std::unique_lock<std::mutex> l(m_);
+        done_cv_.wait(l, [this] { return pending_ == 0; });
+    }
+
+  private:
+    void loop() {
+        while (true) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [this] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                f = std::move(q_.front());
+                q_.pop_front();
+            }
+            f();
+            {
+                std::lock_guard<std::mutex> l(m_);
+                pending_--;
+            }
+            done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::deque<std::function<void()>> q_;
+    std::mutex m_;
+    std::condition_variable cv_, done_cv_;
+    size_t pending_ = 0;
+    bool stop_ = false;
 };
 
 struct DevSeqDeleter {
@@ -240,13 +331,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     sp.entropy_enabled = cfg->entropy_enabled;
     sp.entropy_threshold = cfg->entropy_threshold;
 
-    struct Group {
-        uint32_t ti, qi, strand;
-        size_t chunk;
-        std::vector<Rec> recs;
-    };
-    std::vector<Group> groups;
+    std::vector<std::unique_ptr<Group>> groups;
     std::vector<gxy_edit_run> run_store;  // scripts of gapped records
+    HostJobs jobs(2);                     // joined before groups are destroyed
 
     clk.lap(kTrSetup);
     for (size_t c = 0; c < chunks.size(); c++) {
@@ -279,18 +366,26 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 rs.tie_pairs += st.tie_pairs;
                 rs.passing += st.passing;
                 if (hsps.empty()) continue;
-                std::vector<gxy_hsp> kept;  // primary-region filter (src/driver.rs:250-256)
-                kept.reserve(hsps.size());
-                for (const auto &h : hsps)
-                    if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local) kept.push_back(h);
-                Group g{ch.parent, qi, strand, c, {}};
+                groups.push_back(std::make_unique<Group>(Group{ch.parent, qi, strand, c, {}}));
+                Group &g = *groups.back();
                 if (!cfg->gapped_enabled) {
-                    g.recs.reserve(kept.size());
-                    for (const auto &h : kept) {  // build_ungapped_record (src/driver.rs:437-468)
-                        g.recs.push_back(Rec{ch.parent, qi, strand, h.t_start + ch.global_start, h.q_start,
-                                             h.length, h.length, h.score, 0, 0});
-                    }
+                    // primary-region filter (src/driver.rs:250-256) and
+                    // build_ungapped_record (src/driver.rs:437-468), off the device path
+                    jobs.submit([&g, hsps = std::move(hsps), lo = ch.primary_start_local, hi = ch.primary_end_local,
+                                 gs = ch.global_start] {
+                        g.recs.reserve(hsps.size());
+                        for (const auto &h : hsps)
+                            if (h.t_start >= lo && h.t_start < hi)
+                                g.recs.push_back(Rec{g.ti, g.qi, g.strand, h.t_start + gs, h.q_start, h.length,
+                                                     h.length, h.score, 0, 0});
+                        finalize_group(g);
+                    });
                 } else {
+                    std::vector<gxy_hsp> kept;  // primary-region filter (src/driver.rs:250-256)
+                    kept.reserve(hsps.size());
+                    for (const auto &h : hsps)
+                        if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
+                            kept.push_back(h);
                     std::stable_sort(kept.begin(), kept.end(),
                                      [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
                     for (size_t k = 1; k < kept.size(); k++)
@@ -357,25 +452,27 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                         }
                         batch = std::min<size_t>(batch * 2, 1024);
                     }
+                    jobs.submit([&g] { finalize_group(g); });
                 }
-                if (!g.recs.empty()) groups.push_back(std::move(g));
             }
         }
     }
+    jobs.wait_all();
     for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
 
     clk.lap(kTrRecords);
     // group merge (src/driver.rs:371-389)
-    std::stable_sort(groups.begin(), groups.end(), [](const Group &a, const Group &b) {
-        if (a.ti != b.ti) return a.ti < b.ti;
-        if (a.qi != b.qi) return a.qi < b.qi;
-        if (a.strand != b.strand) return a.strand < b.strand;
-        return a.chunk < b.chunk;
-    });
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const std::unique_ptr<Group> &a, const std::unique_ptr<Group> &b) {
+                         if (a->ti != b->ti) return a->ti < b->ti;
+                         if (a->qi != b->qi) return a->qi < b->qi;
+                         if (a->strand != b->strand) return a->strand < b->strand;
+                         return a->chunk < b->chunk;
+                     });
     size_t total = 0, runs_ub = 0;
     for (const auto &g : groups) {
-        total += g.recs.size();
-        for (const auto &r : g.recs) runs_ub += r.n_runs ? r.n_runs : 1;
+        total += g->recs.size();
+        for (const auto &r : g->recs) runs_ub += r.n_runs ? r.n_runs : 1;
     }
     auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
     if (!set) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
@@ -385,24 +482,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         gxy_record_set_free(set);
         return set_err(ctx, GXY_ENOMEM, "host allocation failed");
     }
-    std::vector<SortKey> keys, tmp;
     size_t nrec = 0, ro = 0;
-    for (auto &g : groups) {
-        keys.resize(g.recs.size());
-        for (size_t k = 0; k < g.recs.size(); k++) {
-            const Rec &r = g.recs[k];
-            keys[k] = {(uint64_t)r.t_start << 32 | r.q_start, (uint64_t)r.t_span << 32 | r.q_span, (uint32_t)k};
-        }
-        radix_sort_keys(keys, tmp);
-        const size_t first = nrec;
-        for (const SortKey &sk : keys) {
-            const Rec &r = g.recs[sk.i];
-            if (nrec > first) {  // dedup_by on the sorted group (src/driver.rs:376-387)
-                const gxy_record &b = set->records[nrec - 1];
-                if (r.t_start == b.t_start && r.q_start == b.q_start && r.t_span == b.t_span &&
-                    r.q_span == b.q_span && r.strand == b.strand)
-                    continue;
-            }
+    for (const auto &g : groups) {
+        for (const Rec &r : g->recs) {
             gxy_record &o = set->records[nrec++];
             o.target_index = r.ti;
             o.query_index = r.qi;

@@ -762,7 +762,10 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
     if (!tmp) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     GXY_HIP(ctx, hipMemsetAsync(off, 0, 8, ctx->stream));
-    GXY_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, cnt64, off + 1, (int)nwin, ctx->stream));
+    {
+        KernelTimer kt_(ctx, "qscan");
+        GXY_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, cnt64, off + 1, (int)nwin, ctx->stream));
+    }
     uint64_t H = 0;
     unsigned long long nvalid = 0;
     GXY_HIP(ctx, hipMemcpyAsync(&H, off + nwin, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -869,7 +872,10 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         auto *flag = (uint32_t *)scratch(ctx, kSlotFlags, npass * 4);
         auto *pos = (uint32_t *)scratch(ctx, kSlotEmit, (npass + 1) * 4);
         if (!k2 || !i1 || !i2 || !flag || !pos) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-        k_iota<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(i1, npass);
+        {
+            KernelTimer kt_(ctx, "pass_sort");
+            k_iota<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(i1, npass);
+        }
         GXY_LAUNCH_CHECK(ctx);
         size_t tb = 0;
         (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, pkeys, k2, i1, i2, (int)npass, 0, 64, ctx->stream);
@@ -886,7 +892,10 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             k_admit<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(k2, i2, phsp, npass, qlen, flag);
         }
         GXY_LAUNCH_CHECK(ctx);
-        GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(t2, tb2, flag, pos, (int)npass, ctx->stream));
+        {
+            KernelTimer kt_(ctx, "admit");
+            GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(t2, tb2, flag, pos, (int)npass, ctx->stream));
+        }
         uint32_t last_pos = 0, last_flag = 0;
         GXY_HIP(ctx, hipMemcpyAsync(&last_pos, pos + npass - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipMemcpyAsync(&last_flag, flag + npass - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -895,16 +904,19 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         const uint64_t nemit = (uint64_t)last_pos + last_flag;
         st.emitted = nemit;
         // emitted (diag, t_start) keys into pkeys/i1 (free now), sort into k2/i2
-        k_emit<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(k2, i2, phsp, flag, pos, npass, pkeys, i1);
-        GXY_LAUNCH_CHECK(ctx);
         (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, pkeys, k2, i1, i2, (int)nemit, 0, 64, ctx->stream);
         t2 = scratch(ctx, kSlotTmp, tb);
-        if (!t2) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-        GXY_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(t2, tb, pkeys, k2, i1, i2, (int)nemit, 0, 64, ctx->stream));
         auto *outd = (gxy_hsp *)scratch(ctx, kSlotMisc1, nemit * sizeof(gxy_hsp));
-        if (!outd) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-        k_gather_hsp<<<grid_for(nemit, 256), 256, 0, ctx->stream>>>(i2, phsp, nemit, outd);
-        GXY_LAUNCH_CHECK(ctx);
+        if (!t2 || !outd) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        {
+            KernelTimer kt_(ctx, "emit");
+            k_emit<<<grid_for(npass, 256), 256, 0, ctx->stream>>>(k2, i2, phsp, flag, pos, npass, pkeys, i1);
+            GXY_LAUNCH_CHECK(ctx);
+            GXY_HIP(ctx,
+                    hipcub::DeviceRadixSort::SortPairs(t2, tb, pkeys, k2, i1, i2, (int)nemit, 0, 64, ctx->stream));
+            k_gather_hsp<<<grid_for(nemit, 256), 256, 0, ctx->stream>>>(i2, phsp, nemit, outd);
+            GXY_LAUNCH_CHECK(ctx);
+        }
         std::vector<gxy_hsp> v(nemit);
         GXY_HIP(ctx, hipMemcpyAsync(v.data(), outd, nemit * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
