@@ -219,11 +219,15 @@ inline ScorerI32 make_scorer_i32(const gxy_scoring &m) {
 // ---- byte-per-base layout used by the x-drop kernels ---------------------------
 //
 // One byte per base: the 2-bit code for a valid base, 0x10 for an invalid one
-// (the flag survives the index arithmetic below).  Forward and reversed
-// copies let a backward walk read forward.  Arrays carry kBytePad bytes of
-// padding so a 32-base window read at any in-range position stays in bounds.
+// (the flag survives the index arithmetic below).  A buffer holds kBytePad
+// bytes of padding (invalid bases) on each side of the sequence, so a 32-base
+// window read forward from any in-range position, or backward from any
+// position >= 0, stays in bounds; callers pass buffer + kBytePad as the
+// sequence's byte 0.  Backward walks read the forward bytes and reverse them
+// in registers, so both sides of a hit share cache lines.
 constexpr uint64_t kBytePad = 64;
 constexpr uint8_t kInvalidByte = 0x10;
+__host__ __device__ constexpr uint64_t byte_buffer_size(uint64_t len) { return len + 2 * kBytePad; }
 
 // Scores of four columns from four target bytes and four query bytes, as four
 // signed bytes: idx = t*4+q selects from the packed int8 rows r0..r3 with two
@@ -243,9 +247,10 @@ __device__ __forceinline__ uint32_t score4(const ByteLut &L, uint32_t t4, uint32
     return (L.amb4 & im) | (s & ~im);
 }
 
-// 32 consecutive bytes starting at any byte offset.
-__device__ __forceinline__ void load_bytes32(const uint8_t *__restrict__ b, uint64_t pos, uint32_t w[8]) {
-    const uint32_t *p = (const uint32_t *)(b + (pos & ~3ull));
+// 32 consecutive bytes starting at any byte offset (pos may be negative down
+// to -kBytePad).
+__device__ __forceinline__ void load_bytes32(const uint8_t *__restrict__ b, int64_t pos, uint32_t w[8]) {
+    const uint32_t *p = (const uint32_t *)(b + (pos & ~(int64_t)3));
     const uint32_t sh = (uint32_t)(pos & 3);
     uint32_t d[9];
 #pragma unroll
@@ -306,8 +311,7 @@ struct gxy_postable {
     const void *host_codes = nullptr; // identity of the host target (gxy_search)
     uint64_t host_len = 0;
     uint32_t *d_nonempty = nullptr;   // bit w set when word w has positions
-    uint8_t *d_tbytes = nullptr;      // target, one byte per base (x-drop layout)
-    uint8_t *d_tbytes_rev = nullptr;  // the same, reversed
+    uint8_t *d_tbytes = nullptr;      // target byte buffer (x-drop layout, sequence at +kBytePad)
     gxy_ctx *pool = nullptr;          // block cache the buffers came from (driver-internal tables)
 };
 
@@ -424,9 +428,7 @@ enum Slot {
     kSlotDpRows,
     kSlotDpRuns,
     kSlotQBytes,
-    kSlotQBytesRev,
     kSlotTBytes,
-    kSlotTBytesRev,
     kSlotVMask,
     kSlotSpread,
 };

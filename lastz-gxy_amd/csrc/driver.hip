@@ -282,18 +282,16 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         ~DevBytes() { dev_free(owner, p); }
     };
     const bool byte_ok = scoring_byte_ok(cfg->scoring);
-    std::vector<DevBytes> qbytes(byte_ok ? 4 * n_queries : 0);
+    std::vector<DevBytes> qbytes(byte_ok ? 2 * n_queries : 0);
     if (byte_ok) {
         for (uint64_t i = 0; i < n_queries; i++) {
             for (int s = 0; s < 2; s++) {
                 const gxy_dseq *qs = s == 0 ? queries[i] : drc[i].get();
                 if (!qs) continue;
-                const size_t nb = qs->len + kBytePad;
-                DevBytes &bf = qbytes[4 * i + 2 * s], &br = qbytes[4 * i + 2 * s + 1];
-                bf.p = (uint8_t *)dev_alloc(ctx, nb, &bf.owner);
-                br.p = (uint8_t *)dev_alloc(ctx, nb, &br.owner);
-                if (!bf.p || !br.p) return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
-                int rc = seq_to_bytes(ctx, qs, qbytes[4 * i + 2 * s].p, qbytes[4 * i + 2 * s + 1].p);
+                DevBytes &b = qbytes[2 * i + s];
+                b.p = (uint8_t *)dev_alloc(ctx, byte_buffer_size(qs->len), &b.owner);
+                if (!b.p) return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
+                int rc = seq_to_bytes(ctx, qs, b.p);
                 if (rc) return rc;
             }
         }
@@ -354,10 +352,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 std::vector<gxy_hsp> hsps;
                 gxy_search_stats st;
                 const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
-                const uint8_t *qf = byte_ok ? qbytes[4 * qi + 2 * si].p : nullptr;
-                const uint8_t *qr = byte_ok ? qbytes[4 * qi + 2 * si + 1].p : nullptr;
+                const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
                 clk.lap(kTrRecords);
-                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st, qf, qr);
+                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st, qb);
                 if (rc) return rc;
                 clk = PhaseClock(ctx);
                 rs.hits += st.hits;

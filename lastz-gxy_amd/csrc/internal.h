@@ -14,16 +14,16 @@ namespace gxy {
 int postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern *pattern, uint64_t step,
                         int respect_mask, gxy_postable **out);
 
-// Forward and reversed byte-per-base copies of `s` (x-drop layout, see
-// common.h); each buffer must hold s->len + kBytePad bytes.
-int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *fwd, uint8_t *rev);
+// Byte-per-base copy of `s` (x-drop layout, see common.h) into a buffer of
+// byte_buffer_size(s->len) bytes; the sequence starts at buf + kBytePad.
+int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *buf);
 
-// Seed search on device sequences.  `qf`/`qr` may hold the query's byte
-// copies (seq_to_bytes) when the caller searches the same query repeatedly;
-// otherwise they are built per call.
+// Seed search on device sequences.  `qbytes` may hold the query's byte copy
+// (seq_to_bytes buffer + kBytePad) when the caller searches the same query
+// repeatedly; otherwise it is built per call.
 int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
                     const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
-                    gxy_search_stats *stats, const uint8_t *qf = nullptr, const uint8_t *qr = nullptr);
+                    gxy_search_stats *stats, const uint8_t *qbytes = nullptr);
 
 struct GappedItem {
     const gxy_dseq *t;

@@ -167,11 +167,10 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     }
     if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_table_starts", e);
     const uint64_t nbw = (tab->num_words + 31) / 32;
-    const uint64_t nbytes = t->len + kBytePad;
+    const uint64_t nbytes = byte_buffer_size(t->len);
     tab->d_nonempty = (uint32_t *)dev_alloc(ctx, nbw * 4, &tab->pool);
     tab->d_tbytes = (uint8_t *)dev_alloc(ctx, nbytes, &tab->pool);
-    tab->d_tbytes_rev = (uint8_t *)dev_alloc(ctx, nbytes, &tab->pool);
-    if (!tab->d_nonempty || !tab->d_tbytes || !tab->d_tbytes_rev) {
+    if (!tab->d_nonempty || !tab->d_tbytes) {
         gxy_postable_free(tab);
         return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     }
@@ -180,7 +179,7 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         KernelTimer kt_(ctx, "table_bytes");
         k_nonempty<<<grid_for(nbw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words, tab->d_nonempty);
         if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_nonempty", e);
-        rc = seq_to_bytes(ctx, t, tab->d_tbytes, tab->d_tbytes_rev);
+        rc = seq_to_bytes(ctx, t, tab->d_tbytes);
     }
     if (rc) {
         gxy_postable_free(tab);
@@ -317,7 +316,6 @@ void gxy_postable_free(gxy_postable *tab) {
     dev_free(tab->pool, tab->d_positions);
     dev_free(tab->pool, tab->d_nonempty);
     dev_free(tab->pool, tab->d_tbytes);
-    dev_free(tab->pool, tab->d_tbytes_rev);
     if (tab->owned_target) gxy_dseq_free(tab->owned_target);
     delete tab;
 }

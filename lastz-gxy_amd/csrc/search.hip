@@ -423,11 +423,13 @@ __global__ __launch_bounds__(256) void k_extend(const uint64_t *__restrict__ hit
 // add, max/compare for the best prefix and its extent, and the x-drop test;
 // after the first stop the running score is pinned to a large negative value
 // so later columns of the chunk cannot move the best (src/hsp.rs:179-193
-// semantics, branch-free).  Left walks read the reversed copies forward.
+// semantics, branch-free).  A left walk loads the 32 bytes before its current
+// end and reverses the score words in registers, so it touches the same cache
+// lines as the seed and the right walk.
 enum : uint32_t { kPhRight = 1, kPhLeft = 2, kPhDone = 3 };
 
 struct XdropArgs {
-    const uint8_t *tf, *tr, *qf, *qr;
+    const uint8_t *tf, *qf;    // byte 0 of each sequence (padded buffers)
     uint64_t tlen, qlen;
     const Blk *tblk;           // block layout of the target (entropy gate)
     ByteLut lut;
@@ -486,8 +488,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
 
     auto begin_left = [&]() {  // left walk from (t-1, q-1), src/hsp.rs:129
         phase = kPhLeft;
-        pt = a.tlen - tp;  // reversed index of tp-1
-        pq = a.qlen - qp;
+        pt = tp;  // exclusive end: the next column is (pt-1, pq-1)
+        pq = qp;
         rem = min((uint64_t)tp, (uint64_t)qp);
         run = best = 0;
         bext = base = 0;
@@ -540,10 +542,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         if (phase != kPhDone) {
             const bool rv = phase == kPhLeft;
             uint32_t tw[8], qw[8], sw[8];
-            load_bytes32(rv ? a.tr : a.tf, pt, tw);
-            load_bytes32(rv ? a.qr : a.qf, pq, qw);
+            load_bytes32(a.tf, rv ? (int64_t)pt - 32 : (int64_t)pt, tw);
+            load_bytes32(a.qf, rv ? (int64_t)pq - 32 : (int64_t)pq, qw);
 #pragma unroll
             for (int w = 0; w < 8; w++) sw[w] = score4(a.lut, tw[w], qw[w]);
+            if (rv) {  // column k of a left walk is byte 31-k of the window
+                uint32_t r[8];
+#pragma unroll
+                for (int w = 0; w < 8; w++) r[w] = __builtin_bswap32(sw[7 - w]);
+#pragma unroll
+                for (int w = 0; w < 8; w++) sw[w] = r[w];
+            }
             const uint32_t n = (uint32_t)min<uint64_t>(32, rem);
             if (n < 32) {  // columns past the sequence end must not move the best
 #pragma unroll
@@ -573,8 +582,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             const bool ended = sb != 0 || rem <= 32;
             cols += sb ? (uint32_t)__ffs(sb) : n;
             rem -= n;
-            pt += n;
-            pq += n;
+            if (rv) {
+                pt -= n;
+                pq -= n;
+            } else {
+                pt += n;
+                pq += n;
+            }
             base += 32;
             if (ended) {
                 if (phase == kPhRight) {
@@ -715,7 +729,7 @@ namespace gxy {
 // mirrors the driver's --nomasking (masks cleared before seeding).
 int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
                     const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
-                    gxy_search_stats *stats, const uint8_t *qf, const uint8_t *qr) {
+                    gxy_search_stats *stats, const uint8_t *qbytes) {
     if (!tab || !query || !scm || !sp || !result) return set_err(ctx, GXY_EINVAL, "null argument");
     if (sp->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
     const gxy_dseq *target = tab->target;
@@ -784,15 +798,13 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
 
     const bool byte_ok = scoring_byte_ok(*scm) && tab->d_tbytes != nullptr && sp->hsp.x_drop >= 0;
     const ScorerI32 s32 = make_scorer_i32(*scm);
-    const uint8_t *qb = qf, *qbr = qr;
-    if (byte_ok && !(qf && qr)) {
-        auto *b1 = (uint8_t *)scratch(ctx, kSlotQBytes, qlen + kBytePad);
-        auto *b2 = (uint8_t *)scratch(ctx, kSlotQBytesRev, qlen + kBytePad);
-        if (!b1 || !b2) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-        int rc = seq_to_bytes(ctx, query, b1, b2);
+    const uint8_t *qb = qbytes;
+    if (byte_ok && !qb) {
+        auto *b = (uint8_t *)scratch(ctx, kSlotQBytes, byte_buffer_size(qlen));
+        if (!b) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        int rc = seq_to_bytes(ctx, query, b);
         if (rc) return rc;
-        qb = b1;
-        qbr = b2;
+        qb = b + kBytePad;
     }
     uint64_t npass = 0;
     clk.lap(kTrSearchOther);
@@ -842,7 +854,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             GXY_LAUNCH_CHECK(ctx);
             KernelTimer kt_ext(ctx, "extend");
             if (byte_ok) {
-                XdropArgs xa{tab->d_tbytes, tab->d_tbytes_rev, qb, qbr, tlen, qlen, target->d,
+                XdropArgs xa{tab->d_tbytes + kBytePad, qb, tlen, qlen, target->d,
                              make_byte_lut(*scm), sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
                              sp->entropy_enabled, sp->entropy_threshold};
                 HitArraySrc src{hits, nh};
@@ -1004,23 +1016,21 @@ int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *que
         if (e == hipSuccess) e = hipMemcpyAsync(dqp, q_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) {
             if (scoring_byte_ok(*scoring) && params->x_drop >= 0) {
-                auto *tb = (uint8_t *)scratch(ctx, kSlotTBytes, dt->len + kBytePad);
-                auto *tbr = (uint8_t *)scratch(ctx, kSlotTBytesRev, dt->len + kBytePad);
-                auto *qb = (uint8_t *)scratch(ctx, kSlotQBytes, dq->len + kBytePad);
-                auto *qbr = (uint8_t *)scratch(ctx, kSlotQBytesRev, dq->len + kBytePad);
-                if (!tb || !tbr || !qb || !qbr) {
+                auto *tb = (uint8_t *)scratch(ctx, kSlotTBytes, byte_buffer_size(dt->len));
+                auto *qb = (uint8_t *)scratch(ctx, kSlotQBytes, byte_buffer_size(dq->len));
+                if (!tb || !qb) {
                     gxy_dseq_free(dt);
                     gxy_dseq_free(dq);
                     return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
                 }
-                rc = seq_to_bytes(ctx, dt, tb, tbr);
-                if (!rc) rc = seq_to_bytes(ctx, dq, qb, qbr);
+                rc = seq_to_bytes(ctx, dt, tb);
+                if (!rc) rc = seq_to_bytes(ctx, dq, qb);
                 if (rc) {
                     gxy_dseq_free(dt);
                     gxy_dseq_free(dq);
                     return rc;
                 }
-                XdropArgs xa{tb, tbr, qb, qbr, dt->len, dq->len, dt->d, make_byte_lut(*scoring),
+                XdropArgs xa{tb + kBytePad, qb + kBytePad, dt->len, dq->len, dt->d, make_byte_lut(*scoring),
                              params->x_drop, params->hsp_threshold, seed_len, 0, 0};
                 PairSrc src{dtp, dqp, n};
                 PassOut none{nullptr, nullptr, nullptr};

@@ -265,25 +265,21 @@ __global__ void k_slice(const Blk *__restrict__ src, Blk *__restrict__ dst, uint
     dst[b] = {w.codes, w.valid, w.mask};
 }
 
-// Byte-per-base copies (x-drop layout): one thread per output dword.
-__global__ void k_to_bytes(const Blk *__restrict__ src, uint64_t len, uint64_t ndw, uint32_t *__restrict__ fwd,
-                           uint32_t *__restrict__ rev) {
+// Byte-per-base buffer (x-drop layout) including both pads: one thread per
+// output dword.
+__global__ void k_to_bytes(const Blk *__restrict__ src, uint64_t len, uint64_t ndw, uint32_t *__restrict__ buf) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndw) return;
-    uint32_t f = 0, r = 0;
+    uint32_t f = 0;
 #pragma unroll
     for (int b = 0; b < 4; b++) {
-        const uint64_t p = 4 * i + b;
-        if (p < len) {
-            const uint32_t v = base_valid(src, p) ? base_code(src, p) : (uint32_t)kInvalidByte;
-            f |= v << (8 * b);
-            const uint64_t pr = len - 1 - p;
-            const uint32_t w = base_valid(src, pr) ? base_code(src, pr) : (uint32_t)kInvalidByte;
-            r |= w << (8 * b);
-        }
+        const int64_t p = (int64_t)(4 * i + b) - (int64_t)kBytePad;
+        const uint32_t v = (p >= 0 && (uint64_t)p < len && base_valid(src, (uint64_t)p))
+                               ? base_code(src, (uint64_t)p)
+                               : (uint32_t)kInvalidByte;
+        f |= v << (8 * b);
     }
-    fwd[i] = f;
-    rev[i] = r;
+    buf[i] = f;
 }
 
 int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
@@ -305,10 +301,9 @@ int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
 
 }  // namespace
 
-int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *fwd, uint8_t *rev) {
-    const uint64_t ndw = (s->len + kBytePad) / 4;
-    k_to_bytes<<<(unsigned)((ndw + 255) / 256), 256, 0, ctx->stream>>>(s->d, s->len, ndw, (uint32_t *)fwd,
-                                                                       (uint32_t *)rev);
+int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *buf) {
+    const uint64_t ndw = (byte_buffer_size(s->len) + 3) / 4;
+    k_to_bytes<<<(unsigned)((ndw + 255) / 256), 256, 0, ctx->stream>>>(s->d, s->len, ndw, (uint32_t *)buf);
     GXY_LAUNCH_CHECK(ctx);
     return GXY_OK;
 }
