@@ -218,33 +218,49 @@ inline ScorerI32 make_scorer_i32(const gxy_scoring &m) {
 
 // ---- byte-per-base layout used by the x-drop kernels ---------------------------
 //
-// One byte per base: the 2-bit code for a valid base, 0x10 for an invalid one
-// (the flag survives the index arithmetic below).  A buffer holds kBytePad
-// bytes of padding (invalid bases) on each side of the sequence, so a 32-base
-// window read forward from any in-range position, or backward from any
-// position >= 0, stays in bounds; callers pass buffer + kBytePad as the
-// sequence's byte 0.  Backward walks read the forward bytes and reverse them
-// in registers, so both sides of a hit share cache lines.
+// Byte-per-base x-drop layout.  Target and query use different byte codes
+// so that a column's score is two byte permutes and one and-xor:
+//   target t in {0,1}: 0xD0 | 4*t         (low nibble selects table A)
+//   target t in {2,3}: 0x0D | 4*(t&1)<<4  (high nibble selects table B)
+//   target invalid:    0xDD
+//   query q valid:     q;   query invalid: 0x0D
+// Table A holds the rows of t=0 (bytes 0-3) and t=1 (bytes 4-7), table B
+// those of t=2 and t=3.  A permute selector of 13..15 yields 0xFF, so for a
+// column the table not holding its target row — and both tables when either
+// base is invalid — give 0xFF.  The rows are stored xor K, K = ~ambig in
+// every byte, so (A & B) ^ K is the score, and ambig for invalid columns.
+// A buffer holds kBytePad bytes of padding (invalid bases) on each side of
+// the sequence, so a 32-base window read forward from any in-range position,
+// or backward from any position >= 0, stays in bounds; callers pass
+// buffer + kBytePad as the sequence's byte 0.  Backward walks read the
+// forward bytes and reverse them in registers, so both sides of a hit share
+// cache lines.
 constexpr uint64_t kBytePad = 64;
-constexpr uint8_t kInvalidByte = 0x10;
+constexpr uint8_t kTargetInvalidByte = 0xDD;
+constexpr uint8_t kQueryInvalidByte = 0x0D;
 __host__ __device__ constexpr uint64_t byte_buffer_size(uint64_t len) { return len + 2 * kBytePad; }
 
-// Scores of four columns from four target bytes and four query bytes, as four
-// signed bytes: idx = t*4+q selects from the packed int8 rows r0..r3 with two
-// byte permutes, invalid columns take `amb4` (ambig replicated).
+__host__ __device__ __forceinline__ uint32_t target_byte(uint32_t code, bool valid) {
+    if (!valid) return kTargetInvalidByte;
+    return code < 2 ? 0xD0u | (4u * code) : 0x0Du | ((4u * (code & 1u)) << 4);
+}
+__host__ __device__ __forceinline__ uint32_t query_byte(uint32_t code, bool valid) {
+    return valid ? code : (uint32_t)kQueryInvalidByte;
+}
+
 struct ByteLut {
-    uint32_t r0, r1, r2, r3, amb4;
+    uint32_t a0, a1, b0, b1;  // rows of t = 0, 1, 2, 3, each xor k
+    uint32_t k;               // ~ambig in every byte
 };
 
+// Scores of four columns (four target bytes, four query bytes) as four
+// signed bytes.
 __device__ __forceinline__ uint32_t score4(const ByteLut &L, uint32_t t4, uint32_t q4) {
-    const uint32_t idx = (t4 << 2) | q4;
-    const uint32_t sel = idx & 0x07070707u;
-    const uint32_t lo = __builtin_amdgcn_perm(L.r1, L.r0, sel);
-    const uint32_t hi = __builtin_amdgcn_perm(L.r3, L.r2, sel);
-    const uint32_t hm = __builtin_amdgcn_perm(0u, 0u, ((idx >> 3) & 0x01010101u) | 0x0C0C0C0Cu);
-    const uint32_t s = (hi & hm) | (lo & ~hm);
-    const uint32_t im = __builtin_amdgcn_perm(0u, 0u, (((idx >> 4) | (idx >> 6)) & 0x01010101u) | 0x0C0C0C0Cu);
-    return (L.amb4 & im) | (s & ~im);
+    const uint32_t sa = (t4 & 0x0F0F0F0Fu) | q4;
+    const uint32_t sb = ((t4 >> 4) & 0x0F0F0F0Fu) | q4;
+    const uint32_t a = __builtin_amdgcn_perm(L.a1, L.a0, sa);
+    const uint32_t b = __builtin_amdgcn_perm(L.b1, L.b0, sb);
+    return (a & b) ^ L.k;
 }
 
 // 32 consecutive bytes starting at any byte offset (pos may be negative down
@@ -260,9 +276,9 @@ __device__ __forceinline__ void load_bytes32(const uint8_t *__restrict__ b, int6
 }
 
 inline ByteLut make_byte_lut(const gxy_scoring &m) {
-    const ScorerI8 s = make_scorer_i8(m);
-    const uint32_t a = (uint32_t)(uint8_t)(int8_t)m.ambig;
-    return ByteLut{s.r0, s.r1, s.r2, s.r3, a * 0x01010101u};
+    const ScorerI8 s = make_scorer_i8(m);  // row t: byte q = sub[t][q]
+    const uint32_t k = (uint32_t)(uint8_t)~(uint8_t)(int8_t)m.ambig * 0x01010101u;
+    return ByteLut{s.r0 ^ k, s.r1 ^ k, s.r2 ^ k, s.r3 ^ k, k};
 }
 
 inline bool scoring_byte_ok(const gxy_scoring &m) {

@@ -291,7 +291,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 DevBytes &b = qbytes[2 * i + s];
                 b.p = (uint8_t *)dev_alloc(ctx, byte_buffer_size(qs->len), &b.owner);
                 if (!b.p) return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
-                int rc = seq_to_bytes(ctx, qs, b.p);
+                int rc = seq_to_bytes(ctx, qs, false, b.p);
                 if (rc) return rc;
             }
         }

@@ -14,9 +14,10 @@ namespace gxy {
 int postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern *pattern, uint64_t step,
                         int respect_mask, gxy_postable **out);
 
-// Byte-per-base copy of `s` (x-drop layout, see common.h) into a buffer of
-// byte_buffer_size(s->len) bytes; the sequence starts at buf + kBytePad.
-int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *buf);
+// Byte-per-base copy of `s` (x-drop layout with target or query byte codes,
+// see common.h) into a buffer of byte_buffer_size(s->len) bytes; the sequence
+// starts at buf + kBytePad.
+int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf);
 
 // Seed search on device sequences.  `qbytes` may hold the query's byte copy
 // (seq_to_bytes buffer + kBytePad) when the caller searches the same query

@@ -179,7 +179,7 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         KernelTimer kt_(ctx, "table_bytes");
         k_nonempty<<<grid_for(nbw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words, tab->d_nonempty);
         if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_nonempty", e);
-        rc = seq_to_bytes(ctx, t, tab->d_tbytes);
+        rc = seq_to_bytes(ctx, t, true, tab->d_tbytes);
     }
     if (rc) {
         gxy_postable_free(tab);

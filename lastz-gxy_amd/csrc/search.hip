@@ -482,6 +482,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     uint32_t bext = 0, rext = 0, base = 0;
     uint64_t cols = 0;
     constexpr int32_t kPinned = -(1 << 30);
+    constexpr int32_t kSmall = 1 << 29;
+    const bool xd_small = a.xdrop < kSmall;
     // seed span in score words; bytes of the last word inside the span
     const uint32_t nsw = (a.span + 3) >> 2;
     const uint32_t lastmask = (a.span & 3) ? ((1u << (8 * (a.span & 3))) - 1u) : 0xFFFFFFFFu;
@@ -564,23 +566,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             }
             // x_drop >= 0 here (the generic kernel takes the rest), so a
             // column that improves the best can never satisfy the stop test:
-            // the reference's else-branch needs no separate guard.
-            uint32_t stopbits = 0, kbest = 0;
+            // the reference's else-branch needs no separate guard.  After the
+            // first stop every later column stops too (the pinned run stays
+            // far below best - x_drop), so the stopping columns are
+            // [kstop, 32).
+            uint32_t kbest = 0, kstop = 32;
+            const bool small = xd_small && run >= -kSmall && run <= kSmall && best <= kSmall;
+            if (__all(small)) {
+                // no sum can leave int32 within a chunk: plain adds, the byte
+                // extract folds into the add
+                uint32_t nst = 0;
 #pragma unroll
-            for (int k = 0; k < 32; k++) {
-                const int32_t sc = __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8);
-                const int32_t r = sat_add(run, sc);
-                kbest = r > best ? (uint32_t)(k + 1) : kbest;
-                best = max(best, r);
-                const bool stop = sat_sub(best, r) > a.xdrop;
-                stopbits |= stop ? (1u << k) : 0u;
-                run = stop ? kPinned : r;
+                for (int k = 0; k < 32; k++) {
+                    const int32_t sc = (int32_t)(int8_t)(sw[k >> 2] >> (8 * (k & 3)));
+                    const int32_t r = run + sc;
+                    kbest = r > best ? (uint32_t)(k + 1) : kbest;
+                    best = max(best, r);
+                    const bool stop = best - r > a.xdrop;
+                    nst += stop ? 1u : 0u;
+                    run = stop ? kPinned : r;
+                }
+                kstop = 32 - nst;
+            } else {  // saturating form of the same walk (src/hsp.rs:186)
+                uint32_t stopbits = 0;
+#pragma unroll
+                for (int k = 0; k < 32; k++) {
+                    const int32_t sc = __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8);
+                    const int32_t r = sat_add(run, sc);
+                    kbest = r > best ? (uint32_t)(k + 1) : kbest;
+                    best = max(best, r);
+                    const bool stop = sat_sub(best, r) > a.xdrop;
+                    stopbits |= stop ? (1u << k) : 0u;
+                    run = stop ? kPinned : r;
+                }
+                kstop = stopbits ? (uint32_t)__ffs(stopbits) - 1u : 32u;
             }
             if (kbest) bext = base + kbest;
-            const uint32_t live = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
-            const uint32_t sb = stopbits & live;
-            const bool ended = sb != 0 || rem <= 32;
-            cols += sb ? (uint32_t)__ffs(sb) : n;
+            const bool live_stop = kstop < n;
+            const bool ended = live_stop || rem <= 32;
+            cols += live_stop ? kstop + 1 : n;
             rem -= n;
             if (rv) {
                 pt -= n;
@@ -802,7 +826,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     if (byte_ok && !qb) {
         auto *b = (uint8_t *)scratch(ctx, kSlotQBytes, byte_buffer_size(qlen));
         if (!b) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-        int rc = seq_to_bytes(ctx, query, b);
+        int rc = seq_to_bytes(ctx, query, false, b);
         if (rc) return rc;
         qb = b + kBytePad;
     }
@@ -1023,8 +1047,8 @@ int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *que
                     gxy_dseq_free(dq);
                     return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
                 }
-                rc = seq_to_bytes(ctx, dt, tb);
-                if (!rc) rc = seq_to_bytes(ctx, dq, qb);
+                rc = seq_to_bytes(ctx, dt, true, tb);
+                if (!rc) rc = seq_to_bytes(ctx, dq, false, qb);
                 if (rc) {
                     gxy_dseq_free(dt);
                     gxy_dseq_free(dq);

@@ -265,19 +265,20 @@ __global__ void k_slice(const Blk *__restrict__ src, Blk *__restrict__ dst, uint
     dst[b] = {w.codes, w.valid, w.mask};
 }
 
-// Byte-per-base buffer (x-drop layout) including both pads: one thread per
-// output dword.
-__global__ void k_to_bytes(const Blk *__restrict__ src, uint64_t len, uint64_t ndw, uint32_t *__restrict__ buf) {
+// Byte-per-base buffer (x-drop layout, target or query codes) including both
+// pads: one thread per output dword.
+__global__ void k_to_bytes(const Blk *__restrict__ src, uint64_t len, uint64_t ndw, int as_target,
+                           uint32_t *__restrict__ buf) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndw) return;
     uint32_t f = 0;
 #pragma unroll
     for (int b = 0; b < 4; b++) {
         const int64_t p = (int64_t)(4 * i + b) - (int64_t)kBytePad;
-        const uint32_t v = (p >= 0 && (uint64_t)p < len && base_valid(src, (uint64_t)p))
-                               ? base_code(src, (uint64_t)p)
-                               : (uint32_t)kInvalidByte;
-        f |= v << (8 * b);
+        const bool in = p >= 0 && (uint64_t)p < len;
+        const bool valid = in && base_valid(src, (uint64_t)p);
+        const uint32_t code = in ? base_code(src, (uint64_t)p) : 0u;
+        f |= (as_target ? target_byte(code, valid) : query_byte(code, valid)) << (8 * b);
     }
     buf[i] = f;
 }
@@ -301,9 +302,10 @@ int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
 
 }  // namespace
 
-int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, uint8_t *buf) {
+int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf) {
     const uint64_t ndw = (byte_buffer_size(s->len) + 3) / 4;
-    k_to_bytes<<<(unsigned)((ndw + 255) / 256), 256, 0, ctx->stream>>>(s->d, s->len, ndw, (uint32_t *)buf);
+    k_to_bytes<<<(unsigned)((ndw + 255) / 256), 256, 0, ctx->stream>>>(s->d, s->len, ndw, as_target ? 1 : 0,
+                                                                       (uint32_t *)buf);
     GXY_LAUNCH_CHECK(ctx);
     return GXY_OK;
 }
