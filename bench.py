@@ -81,7 +81,7 @@ def main():
         recs, st = gxy.run_dev([dt], [dq], cfg, ctx)
         log(f"warmup {w}: {time.time() - t1:.3f}s, {len(recs)} records")
 
-    names = ["table_items", "table_sort", "table_starts", "table_bytes", "qcount", "qscan", "qexpand",
+    names = ["table_items", "table_sort", "table_starts", "table_bytes", "qcount", "qscan", "qexpand", "bin",
              "extend", "pass_sort", "admit", "emit", "dp_forward", "dp_traceback"]
     ctx.reset_timing()
     ctx.set_timing(True)

@@ -447,6 +447,9 @@ enum Slot {
     kSlotTBytes,
     kSlotVMask,
     kSlotSpread,
+    kSlotHits2,
+    kSlotBins,
+    kSlotBinOff,
 };
 
 }  // namespace gxy

@@ -156,13 +156,25 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
 // stores are coalesced and a window with many hits is shared by all lanes.
 constexpr int kExpWaves = 4;
 
+// Hits are binned by target position into one bin per XCD (workgroup i runs
+// on XCD i % 8), so the x-drop lanes of one XCD read one eighth of the target
+// bytes — a slice its L2 holds.  bin = t * mbin >> 32, mbin = 2^35 / tlen.
+constexpr uint32_t kBins = 8;
+
+inline uint32_t bin_multiplier(uint64_t tlen) {
+    return (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, (8ull << 32) / std::max<uint64_t>(tlen, 1));
+}
+__device__ __forceinline__ uint32_t target_bin(uint32_t t, uint32_t mbin) { return __umulhi(t, mbin); }
+
 __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uint64_t step,
                                                    const uint32_t *__restrict__ words,
                                                    const uint16_t *__restrict__ vmask,
                                                    const uint64_t *__restrict__ off, uint64_t base, uint32_t weight,
                                                    const uint32_t *__restrict__ starts,
                                                    const uint32_t *__restrict__ positions,
-                                                   uint64_t *__restrict__ hits) {
+                                                   uint32_t mbin, uint64_t *__restrict__ hits,
+                                                   uint32_t *__restrict__ bincnt) {
+    __shared__ uint32_t lbin[kBins];
     __shared__ uint32_t lbeg[kExpWaves][64][kMaxV1];
     __shared__ uint32_t lcum[kExpWaves][64][kMaxV1];
     __shared__ uint64_t lo[kExpWaves][64];
@@ -172,6 +184,7 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
     const uint64_t wend = min(wb + 64, w1);
     const uint64_t w = wb + lane;
     const bool active = wb < w1;
+    if (threadIdx.x < kBins) lbin[threadIdx.x] = 0;
     uint64_t oend = 0;
     if (active) {
         oend = off[wend] - base;
@@ -199,19 +212,57 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
         lnb[wv][lane] = nb;
     }
     __syncthreads();
-    if (!active) return;
-    for (uint64_t j = lo[wv][0] + lane; j < oend; j += 64) {
-        // owning window: the last lane L with lo[L] <= j
-        int L = 0;
+    if (active) {
+        uint32_t nbin[kBins] = {};  // this wavefront's hits per bin (uniform)
+        for (uint64_t j = lo[wv][0] + lane; j < oend; j += 64) {
+            // owning window: the last lane L with lo[L] <= j
+            int L = 0;
 #pragma unroll
-        for (int s = 32; s > 0; s >>= 1)
-            if (lo[wv][L + s] <= j) L += s;
-        const uint32_t r = (uint32_t)(j - lo[wv][L]);
-        const uint32_t nb = lnb[wv][L];
-        uint32_t i = 0;
-        while (i + 1 < nb && lcum[wv][L][i + 1] <= r) i++;
-        const uint32_t p = lbeg[wv][L][i] + (r - lcum[wv][L][i]);
-        hits[j] = (((wb + (uint64_t)L) * step) << 32) | positions[p];
+            for (int s = 32; s > 0; s >>= 1)
+                if (lo[wv][L + s] <= j) L += s;
+            const uint32_t r = (uint32_t)(j - lo[wv][L]);
+            const uint32_t nb = lnb[wv][L];
+            uint32_t i = 0;
+            while (i + 1 < nb && lcum[wv][L][i + 1] <= r) i++;
+            const uint32_t p = lbeg[wv][L][i] + (r - lcum[wv][L][i]);
+            const uint32_t t = positions[p];
+            hits[j] = (((wb + (uint64_t)L) * step) << 32) | t;
+            const uint32_t bin = target_bin(t, mbin);
+#pragma unroll
+            for (uint32_t b = 0; b < kBins; b++) nbin[b] += (uint32_t)__popcll(__ballot(bin == b));
+        }
+        // Lane 0 took part in every iteration (it owns the smallest j);
+        // lanes that left the loop early hold partial counts.
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < kBins; b++) {
+            const uint32_t x = __builtin_amdgcn_readfirstlane(nbin[b]);
+            c = (uint32_t)lane == b ? x : c;
+        }
+        if (lane < (int)kBins && c) atomicAdd(&lbin[lane], c);
+    }
+    __syncthreads();
+    if (threadIdx.x < kBins) bincnt[threadIdx.x * gridDim.x + blockIdx.x] = lbin[threadIdx.x];
+}
+
+// Moves a batch's hits into bin order: block b re-reads the hits of its 256
+// windows and writes each to its bin's range for this block (binoff: the
+// exclusive scan of bincnt, bin-major).  Order within a (block, bin) range is
+// arbitrary; passing hits are sorted by a unique key afterwards.
+__global__ __launch_bounds__(256) void k_bin_hits(uint64_t w0, uint64_t w1, const uint64_t *__restrict__ off,
+                                                  uint64_t base, const uint64_t *__restrict__ hits_in,
+                                                  const uint32_t *__restrict__ binoff, uint32_t mbin,
+                                                  uint64_t *__restrict__ hits_out) {
+    __shared__ uint32_t lc[kBins];
+    const uint64_t wa = w0 + (uint64_t)blockIdx.x * 64 * kExpWaves;
+    const uint64_t wb = min(wa + 64 * kExpWaves, w1);
+    if (threadIdx.x < kBins) lc[threadIdx.x] = binoff[threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint64_t ja = off[wa] - base, jb = off[wb] - base;
+    for (uint64_t j = ja + threadIdx.x; j < jb; j += blockDim.x) {
+        const uint64_t h = hits_in[j];
+        const uint32_t slot = atomicAdd(&lc[target_bin((uint32_t)h, mbin)], 1u);
+        hits_out[slot] = h;
     }
 }
 
@@ -444,7 +495,10 @@ struct XdropArgs {
 struct HitArraySrc {  // (q << 32 | t) records
     const uint64_t *hits;
     uint64_t n, h, stride, cur;
-    __device__ void init(uint64_t gid, uint64_t s) { h = gid; stride = s; }
+    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim) {
+        h = (uint64_t)blk * bdim + tid;
+        stride = (uint64_t)nblk * bdim;
+    }
     __device__ __forceinline__ bool next(uint32_t &t, uint32_t &q) {
         if (h >= n) return false;
         const uint64_t x = hits[h];
@@ -459,7 +513,10 @@ struct HitArraySrc {  // (q << 32 | t) records
 struct PairSrc {  // separate t and q position arrays
     const uint32_t *tpos, *qpos;
     uint64_t n, h, stride, cur;
-    __device__ void init(uint64_t gid, uint64_t s) { h = gid; stride = s; }
+    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim) {
+        h = (uint64_t)blk * bdim + tid;
+        stride = (uint64_t)nblk * bdim;
+    }
     __device__ __forceinline__ bool next(uint32_t &t, uint32_t &q) {
         if (h >= n) return false;
         t = tpos[h];
@@ -470,11 +527,35 @@ struct PairSrc {  // separate t and q position arrays
     }
 };
 
+// Hits of one bin per XCD: block b walks bin b % 8 with the other blocks of
+// its XCD (the grid is a multiple of 8).
+struct BinnedHitSrc {
+    const uint64_t *hits;
+    const uint32_t *binoff;  // bin k starts at binoff[k * nblk_exp]
+    uint32_t nblk_exp;
+    uint64_t n, h, end, stride, cur;
+    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim) {
+        const uint32_t bin = blk % kBins;
+        h = binoff[bin * nblk_exp] + (uint64_t)(blk / kBins) * bdim + tid;
+        end = bin + 1 < kBins ? binoff[(bin + 1) * nblk_exp] : n;
+        stride = (uint64_t)(nblk / kBins) * bdim;
+    }
+    __device__ __forceinline__ bool next(uint32_t &t, uint32_t &q) {
+        if (h >= end) return false;
+        const uint64_t x = hits[h];
+        q = (uint32_t)(x >> 32);
+        t = (uint32_t)x;
+        cur = h;
+        h += stride;
+        return true;
+    }
+};
+
 template <class Src, bool kAppend>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop(
     XdropArgs a, Src src, PassOut po, gxy_hsp *__restrict__ out, uint8_t *__restrict__ found,
     unsigned long long *__restrict__ colsum) {
-    src.init((uint64_t)blockIdx.x * blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
+    src.init(blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
     const int lane = threadIdx.x & 63;
     uint32_t tp = 0, qp = 0, phase = kPhDone;
     uint64_t pt = 0, pq = 0, rem = 0;
@@ -861,29 +942,61 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         if (!hits || !pkeys || !phsp) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)",
                                                      (unsigned long long)H);
         PassOut po{pkeys, phsp, ctr + 1};
+        // transitions <= 1 with the byte layout: hits binned by target eighth
+        // for an XCD-partitioned x-drop
+        const bool binned = fast_var && byte_ok;
+        const uint32_t mbin = bin_multiplier(tlen);
+        uint64_t maxw = 0;
+        for (uint32_t k = 0; k < nb; k++) maxw = std::max(maxw, bounds[k + 1] - bounds[k]);
+        const uint64_t maxblk = grid_for(maxw, 64 * kExpWaves);
+        auto *bincnt = (uint32_t *)scratch(ctx, kSlotBins, kBins * maxblk * 4);
+        auto *binoff = (uint32_t *)scratch(ctx, kSlotBinOff, kBins * maxblk * 4);
+        auto *hits2 = binned ? (uint64_t *)scratch(ctx, kSlotHits2, maxb * 8) : nullptr;
+        if (!bincnt || !binoff || (binned && !hits2))
+            return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)", (unsigned long long)H);
         for (uint32_t k = 0; k < nb; k++) {
             const uint64_t w0 = bounds[k], w1 = bounds[k + 1], nh = boff[k + 1] - boff[k];
             if (nh == 0) continue;
+            const unsigned nblk = grid_for(w1 - w0, 64 * kExpWaves);
             {
                 KernelTimer kt_(ctx, "qexpand");
                 if (fast_var)
-                    k_qexpand_w<<<grid_for(w1 - w0, 64 * kExpWaves), 64 * kExpWaves, 0, ctx->stream>>>(
-                        w0, w1, sp->step, words, vmask, off, boff[k], pat.weight, tab->d_starts, tab->d_positions,
-                        hits);
+                    k_qexpand_w<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(w0, w1, sp->step, words, vmask, off,
+                                                                          boff[k], pat.weight, tab->d_starts,
+                                                                          tab->d_positions, mbin, hits, bincnt);
                 else
                     k_qexpand<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(w0, w1, sp->step, words, off, boff[k],
                                                                               var, tab->d_starts, tab->d_nonempty,
                                                                               tab->d_positions, hits);
             }
             GXY_LAUNCH_CHECK(ctx);
+            if (binned) {
+                size_t tbb = 0;
+                (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tbb, bincnt, binoff, (int)(kBins * nblk), ctx->stream);
+                void *tmpb = scratch(ctx, kSlotTmp, tbb);
+                if (!tmpb) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+                KernelTimer kt_(ctx, "bin");
+                GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmpb, tbb, bincnt, binoff, (int)(kBins * nblk),
+                                                              ctx->stream));
+                k_bin_hits<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(w0, w1, off, boff[k], hits, binoff, mbin, hits2);
+                GXY_LAUNCH_CHECK(ctx);
+            }
             KernelTimer kt_ext(ctx, "extend");
             if (byte_ok) {
                 XdropArgs xa{tab->d_tbytes + kBytePad, qb, tlen, qlen, target->d,
                              make_byte_lut(*scm), sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
                              sp->entropy_enabled, sp->entropy_threshold};
-                HitArraySrc src{hits, nh};
-                const unsigned g = std::min<unsigned>(grid_for(nh, 256), kXdropBlocks);
-                k_xdrop<HitArraySrc, true><<<g, 256, 0, ctx->stream>>>(xa, src, po, nullptr, nullptr, ctr + 2);
+                if (binned) {
+                    // a multiple of kBins blocks, so every XCD walks its own bin
+                    const unsigned g = std::min<unsigned>(
+                        kXdropBlocks, std::max<unsigned>(kBins, (grid_for(nh, 256) + kBins - 1) / kBins * kBins));
+                    BinnedHitSrc src{hits2, binoff, nblk, nh};
+                    k_xdrop<BinnedHitSrc, true><<<g, 256, 0, ctx->stream>>>(xa, src, po, nullptr, nullptr, ctr + 2);
+                } else {
+                    HitArraySrc src{hits, nh};
+                    const unsigned g = std::min<unsigned>(grid_for(nh, 256), kXdropBlocks);
+                    k_xdrop<HitArraySrc, true><<<g, 256, 0, ctx->stream>>>(xa, src, po, nullptr, nullptr, ctr + 2);
+                }
             } else {
                 k_extend<ScorerI32><<<grid_for(nh, 256), 256, 0, ctx->stream>>>(
                     hits, nh, target->d, tlen, query->d, qlen, s32, sp->hsp.x_drop, sp->hsp.hsp_threshold,
