@@ -40,7 +40,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k"])
     ap.add_argument("--target-bp", type=int, default=48_000_000)
-    ap.add_argument("--cpu-sample-bp", type=int, default=4_000_000)
+    ap.add_argument("--cpu-sample-bp", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -105,9 +105,15 @@ def main():
     if dom_name == "extend":
         alg = (8 * st.hits + 0.75 * st.columns + 24 * st.passing) * args.steps
         achieved = alg / (dom_ms / 1e3) / 1e9
+        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+        # runs of this workload (scaled by a 1 GiB copy), when present
+        traffic = None
+        prof = os.path.join(ROOT, "profiles", "k_xdrop_chrom48M.json")
+        if args.workload == "chrom" and os.path.exists(prof):
+            traffic = json.load(open(prof))["traffic_bytes_per_launch"]
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_extend", "avg_launch_ms": round(dom_ms / max(1, dom_launches), 4),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "k_xdrop", "avg_launch_ms": round(dom_ms / max(1, dom_launches), 4),
                 "launches": dom_launches,
                 "algorithmic_bytes_per_launch": round(alg / max(1, dom_launches))}
     elif dom_name == "dp_forward":

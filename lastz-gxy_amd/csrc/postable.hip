@@ -52,19 +52,15 @@ __global__ void k_table_starts(const uint64_t *__restrict__ items, uint64_t nval
     if (i < nvalid) positions[i] = (uint32_t)items[i];
 }
 
-// Bitmap of non-empty words: one thread per 32 words.
+// Bitmap of non-empty words: one thread per word (coalesced reads of
+// `starts`); a wavefront's ballot is two bitmap words.
 __global__ void k_nonempty(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t *__restrict__ bits) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g * 32 >= num_words) return;
-    uint32_t m = 0;
-    const uint64_t w0 = g * 32;
-    uint32_t prev = starts[w0];
-    for (uint32_t k = 0; k < 32 && w0 + k < num_words; k++) {
-        const uint32_t nx = starts[w0 + k + 1];
-        if (nx != prev) m |= 1u << k;
-        prev = nx;
-    }
-    bits[g] = m;
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ne = w < num_words && starts[w + 1] != starts[w];
+    const uint64_t b = __ballot(ne);
+    const int lane = threadIdx.x & 63;
+    const uint64_t g = w >> 5;  // bitmap word of this lane
+    if ((lane & 31) == 0 && g * 32 < num_words) bits[g] = (uint32_t)(lane ? b >> 32 : b);
 }
 
 __global__ void k_prune_counts(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t limit,
@@ -177,7 +173,8 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     int rc;
     {
         KernelTimer kt_(ctx, "table_bytes");
-        k_nonempty<<<grid_for(nbw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words, tab->d_nonempty);
+        k_nonempty<<<grid_for(tab->num_words, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words,
+                                                                            tab->d_nonempty);
         if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_nonempty", e);
         rc = seq_to_bytes(ctx, t, true, tab->d_tbytes);
     }
@@ -258,7 +255,7 @@ int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
     tab->d_starts = ns;
     tab->d_positions = np;
     tab->npos = total;
-    k_nonempty<<<grid_for((nw + 31) / 32, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_nonempty);
+    k_nonempty<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_nonempty);
     GXY_LAUNCH_CHECK(ctx);
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GXY_OK;
