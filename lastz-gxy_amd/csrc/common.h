@@ -327,6 +327,7 @@ struct gxy_postable {
     const void *host_codes = nullptr; // identity of the host target (gxy_search)
     uint64_t host_len = 0;
     uint32_t *d_nonempty = nullptr;   // bit w set when word w has positions
+    uint32_t *d_cmap = nullptr;       // 2 bits per word: min(positions, 3), 16 words per u32
     uint8_t *d_tbytes = nullptr;      // target byte buffer (x-drop layout, sequence at +kBytePad)
     gxy_ctx *pool = nullptr;          // block cache the buffers came from (driver-internal tables)
 };

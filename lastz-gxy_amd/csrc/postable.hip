@@ -63,6 +63,29 @@ __global__ void k_nonempty(const uint32_t *__restrict__ starts, uint64_t num_wor
     if ((lane & 31) == 0 && g * 32 < num_words) bits[g] = (uint32_t)(lane ? b >> 32 : b);
 }
 
+// Count map: min(bucket size, 3) in 2 bits per word, 16 words per u32; one
+// thread per word, a wavefront's two ballots are four map words.
+__global__ void k_cmap(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t *__restrict__ cmap) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = w < num_words ? min(starts[w + 1] - starts[w], 3u) : 0u;
+    const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c >> 1);
+    const int lane = threadIdx.x & 63;
+    if ((lane & 15) == 0 && (w >> 4) * 16 < num_words) {
+        const uint32_t lo = (uint32_t)(b0 >> lane) & 0xFFFFu, hi = (uint32_t)(b1 >> lane) & 0xFFFFu;
+        // interleave: word k of the 16 takes bits 2k (lo) and 2k+1 (hi)
+        uint32_t x = lo, y = hi;
+        x = (x | (x << 8)) & 0x00FF00FFu;
+        x = (x | (x << 4)) & 0x0F0F0F0Fu;
+        x = (x | (x << 2)) & 0x33333333u;
+        x = (x | (x << 1)) & 0x55555555u;
+        y = (y | (y << 8)) & 0x00FF00FFu;
+        y = (y | (y << 4)) & 0x0F0F0F0Fu;
+        y = (y | (y << 2)) & 0x33333333u;
+        y = (y | (y << 1)) & 0x55555555u;
+        cmap[w >> 4] = x | (y << 1);
+    }
+}
+
 __global__ void k_prune_counts(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t limit,
                                uint32_t *__restrict__ keep_cnt) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -165,8 +188,9 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     const uint64_t nbw = (tab->num_words + 31) / 32;
     const uint64_t nbytes = byte_buffer_size(t->len);
     tab->d_nonempty = (uint32_t *)dev_alloc(ctx, nbw * 4, &tab->pool);
+    tab->d_cmap = (uint32_t *)dev_alloc(ctx, ((tab->num_words + 15) / 16) * 4, &tab->pool);
     tab->d_tbytes = (uint8_t *)dev_alloc(ctx, nbytes, &tab->pool);
-    if (!tab->d_nonempty || !tab->d_tbytes) {
+    if (!tab->d_nonempty || !tab->d_cmap || !tab->d_tbytes) {
         gxy_postable_free(tab);
         return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     }
@@ -176,6 +200,8 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         k_nonempty<<<grid_for(tab->num_words, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words,
                                                                             tab->d_nonempty);
         if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_nonempty", e);
+        k_cmap<<<grid_for(tab->num_words, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words, tab->d_cmap);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_cmap", e);
         rc = seq_to_bytes(ctx, t, true, tab->d_tbytes);
     }
     if (rc) {
@@ -257,6 +283,8 @@ int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
     tab->npos = total;
     k_nonempty<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_nonempty);
     GXY_LAUNCH_CHECK(ctx);
+    k_cmap<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_cmap);
+    GXY_LAUNCH_CHECK(ctx);
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GXY_OK;
 }
@@ -312,6 +340,7 @@ void gxy_postable_free(gxy_postable *tab) {
     dev_free(tab->pool, tab->d_starts);
     dev_free(tab->pool, tab->d_positions);
     dev_free(tab->pool, tab->d_nonempty);
+    dev_free(tab->pool, tab->d_cmap);
     dev_free(tab->pool, tab->d_tbytes);
     if (tab->owned_target) gxy_dseq_free(tab->owned_target);
     delete tab;
