@@ -328,6 +328,7 @@ struct gxy_postable {
     uint64_t host_len = 0;
     uint32_t *d_nonempty = nullptr;   // bit w set when word w has positions
     uint32_t *d_cmap = nullptr;       // 2 bits per word: min(positions, 3), 16 words per u32
+    uint32_t *d_spos = nullptr;       // first position of every word (valid when it has one)
     uint8_t *d_tbytes = nullptr;      // target byte buffer (x-drop layout, sequence at +kBytePad)
     gxy_ctx *pool = nullptr;          // block cache the buffers came from (driver-internal tables)
 };

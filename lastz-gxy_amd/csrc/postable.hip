@@ -86,6 +86,16 @@ __global__ void k_cmap(const uint32_t *__restrict__ starts, uint64_t num_words, 
     }
 }
 
+// First position of every non-empty word, so a one-position bucket is read
+// with one load.
+__global__ void k_spos(const uint32_t *__restrict__ starts, const uint32_t *__restrict__ positions,
+                       uint64_t num_words, uint32_t *__restrict__ spos) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= num_words) return;
+    const uint32_t b = starts[w];
+    spos[w] = starts[w + 1] > b ? positions[b] : 0xFFFFFFFFu;
+}
+
 __global__ void k_prune_counts(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t limit,
                                uint32_t *__restrict__ keep_cnt) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,8 +199,9 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     const uint64_t nbytes = byte_buffer_size(t->len);
     tab->d_nonempty = (uint32_t *)dev_alloc(ctx, nbw * 4, &tab->pool);
     tab->d_cmap = (uint32_t *)dev_alloc(ctx, ((tab->num_words + 15) / 16) * 4, &tab->pool);
+    tab->d_spos = (uint32_t *)dev_alloc(ctx, tab->num_words * 4, &tab->pool);
     tab->d_tbytes = (uint8_t *)dev_alloc(ctx, nbytes, &tab->pool);
-    if (!tab->d_nonempty || !tab->d_cmap || !tab->d_tbytes) {
+    if (!tab->d_nonempty || !tab->d_cmap || !tab->d_spos || !tab->d_tbytes) {
         gxy_postable_free(tab);
         return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     }
@@ -202,6 +213,9 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_nonempty", e);
         k_cmap<<<grid_for(tab->num_words, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->num_words, tab->d_cmap);
         if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_cmap", e);
+        k_spos<<<grid_for(tab->num_words, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions,
+                                                                        tab->num_words, tab->d_spos);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_spos", e);
         rc = seq_to_bytes(ctx, t, true, tab->d_tbytes);
     }
     if (rc) {
@@ -285,6 +299,8 @@ int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
     GXY_LAUNCH_CHECK(ctx);
     k_cmap<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_cmap);
     GXY_LAUNCH_CHECK(ctx);
+    k_spos<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, nw, tab->d_spos);
+    GXY_LAUNCH_CHECK(ctx);
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GXY_OK;
 }
@@ -341,6 +357,7 @@ void gxy_postable_free(gxy_postable *tab) {
     dev_free(tab->pool, tab->d_positions);
     dev_free(tab->pool, tab->d_nonempty);
     dev_free(tab->pool, tab->d_cmap);
+    dev_free(tab->pool, tab->d_spos);
     dev_free(tab->pool, tab->d_tbytes);
     if (tab->owned_target) gxy_dseq_free(tab->owned_target);
     delete tab;

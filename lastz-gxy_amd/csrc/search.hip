@@ -93,9 +93,7 @@ __global__ void k_qcount(const Blk *__restrict__ q, uint64_t qlen, DevPattern pa
     if ((threadIdx.x & 63) == 0 && b) spread_add(nvalid, (unsigned long long)__popcll(b));
 }
 
-// transitions <= 1 and weight <= 15: the (at most 16) variant words are held
-// in registers so all bitmap loads issue together; the mask of non-empty
-// variants is kept for k_qexpand_w so it revisits only those.
+// transitions <= 1 and weight <= 15: at most 16 variant words per window.
 constexpr int kMaxV1 = 16;
 
 __device__ __forceinline__ uint64_t variant1(uint64_t idx, uint32_t weight, int i) {
@@ -108,7 +106,7 @@ __device__ __forceinline__ uint64_t variant1(uint64_t idx, uint32_t weight, int 
 __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern pat, uint64_t step,
                            uint64_t nwin, int respect_mask, uint32_t nvar, const uint32_t *__restrict__ starts,
                            const uint32_t *__restrict__ cmap, uint32_t *__restrict__ words,
-                           uint32_t *__restrict__ cnt, uint16_t *__restrict__ vmask,
+                           uint32_t *__restrict__ cnt, uint32_t *__restrict__ vcnt,
                            unsigned long long *__restrict__ nvalid) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = false;
@@ -118,7 +116,7 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
         if (seed_word(q, qlen, pat, w * step, respect_mask, &word)) {
             ok = true;
             // Bucket sizes from the 2-bit count map (min(size, 3), 16 words per
-            // map word).  Variant i >= 1 flips index bit W-i: flips of bits
+            // map word), kept per variant (2 bits each) for k_qexpand_w.  Variant i >= 1 flips index bit W-i: flips of bits
             // 0-3 stay in the word's own map word g, a flip of bit 4+k reads
             // map word g ^ (1 << k) — at most 1 + (W-4) loads, issued back to
             // back.  `starts` is read only for buckets of 3 or more.
@@ -143,7 +141,7 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
                             if (b == 4 + k) cw = (cf[k] >> sh) & 3u;
                         if (b < 4) cw = (c0w >> (sh ^ (2u << b))) & 3u;
                     }
-                    m |= (cw != 0 ? 1u : 0u) << i;
+                    m |= cw << (2 * i);
                     big |= (cw == 3 ? 1u : 0u) << i;
                     c += cw < 3 ? cw : 0u;
                 }
@@ -159,7 +157,7 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
             words[w] = kInvalidWord;
         }
         cnt[w] = c;
-        vmask[w] = (uint16_t)m;
+        vcnt[w] = m;
     }
     const unsigned long long b = __ballot(ok);
     if ((threadIdx.x & 63) == 0 && b) spread_add(nvalid, (unsigned long long)__popcll(b));
@@ -185,10 +183,11 @@ __device__ __forceinline__ uint32_t target_bin(uint32_t t, uint32_t mbin) { retu
 
 __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uint64_t step,
                                                    const uint32_t *__restrict__ words,
-                                                   const uint16_t *__restrict__ vmask,
+                                                   const uint32_t *__restrict__ vcnt,
                                                    const uint64_t *__restrict__ off, uint64_t base, uint32_t weight,
                                                    const uint32_t *__restrict__ starts,
                                                    const uint32_t *__restrict__ positions,
+                                                   const uint32_t *__restrict__ spos,
                                                    uint32_t mbin, uint64_t *__restrict__ hits,
                                                    uint32_t *__restrict__ bincnt) {
     __shared__ uint32_t lbin[kBins];
@@ -196,6 +195,7 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
     __shared__ uint32_t lcum[kExpWaves][64][kMaxV1];
     __shared__ uint64_t lo[kExpWaves][64];
     __shared__ uint32_t lnb[kExpWaves][64];
+    __shared__ uint32_t linl[kExpWaves][64];  // bucket k of the window holds one position, stored in lbeg
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t wb = w0 + ((uint64_t)blockIdx.x * kExpWaves + wv) * 64;
     const uint64_t wend = min(wb + 64, w1);
@@ -206,27 +206,37 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
     if (active) {
         oend = off[wend] - base;
         uint64_t o = oend;
-        uint32_t nb = 0;
+        uint32_t nb = 0, inl = 0;
         if (w < w1) {
             o = off[w] - base;
-            uint32_t m = vmask[w];
+            uint32_t m = vcnt[w];  // 2 bits per variant: min(bucket size, 3)
             if (m) {
                 const uint32_t word = words[w];
                 uint32_t cum = 0;
                 while (m) {
-                    const int i = __ffs(m) - 1;
-                    m &= m - 1;
+                    const int i = (__ffs(m) - 1) >> 1;
+                    const uint32_t cw = (m >> (2 * i)) & 3u;
+                    m &= ~(3u << (2 * i));
                     const uint64_t v = variant1(word, weight, i);
-                    const uint32_t b = starts[v], e = starts[v + 1];
+                    uint32_t b, len;
+                    if (cw == 1) {  // one position: take it directly
+                        b = spos[v];
+                        len = 1;
+                        inl |= 1u << nb;
+                    } else {
+                        b = starts[v];
+                        len = starts[v + 1] - b;
+                    }
                     lbeg[wv][lane][nb] = b;
                     lcum[wv][lane][nb] = cum;
-                    cum += e - b;
+                    cum += len;
                     nb++;
                 }
             }
         }
         lo[wv][lane] = o;
         lnb[wv][lane] = nb;
+        linl[wv][lane] = inl;
     }
     __syncthreads();
     if (active) {
@@ -241,8 +251,12 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
             const uint32_t nb = lnb[wv][L];
             uint32_t i = 0;
             while (i + 1 < nb && lcum[wv][L][i + 1] <= r) i++;
-            const uint32_t p = lbeg[wv][L][i] + (r - lcum[wv][L][i]);
-            const uint32_t t = positions[p];
+            const uint32_t bk = lbeg[wv][L][i];
+            uint32_t t;
+            if ((linl[wv][L] >> i) & 1u)
+                t = bk;
+            else
+                t = positions[bk + (r - lcum[wv][L][i])];
             hits[j] = (((wb + (uint64_t)L) * step) << 32) | t;
             const uint32_t bin = target_bin(t, mbin);
 #pragma unroll
@@ -874,8 +888,8 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     auto *off = (uint64_t *)scratch(ctx, kSlotOffsets, (nwin + 1) * 8);
     auto *ctr = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
     const bool fast_var = var.transitions <= 1 && nvar <= (uint32_t)kMaxV1;
-    auto *vmask = (uint16_t *)scratch(ctx, kSlotVMask, nwin * 2);
-    if (!words || !cnt || !off || !ctr || !vmask) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    auto *vcnt = (uint32_t *)scratch(ctx, kSlotVMask, nwin * 4);  // per-variant bucket sizes, 2 bits each
+    if (!words || !cnt || !off || !ctr || !vcnt) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     auto *spr = (unsigned long long *)scratch(ctx, kSlotSpread, kSpreadBytes);
     if (!spr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     GXY_HIP(ctx, hipMemsetAsync(ctr, 0, 64, ctx->stream));
@@ -885,7 +899,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         if (fast_var)
             k_qcount_v<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin,
                                                                      respect_mask, nvar, tab->d_starts,
-                                                                     tab->d_cmap, words, cnt, vmask, spr);
+                                                                     tab->d_cmap, words, cnt, vcnt, spr);
         else
             k_qcount<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin, respect_mask,
                                                                    var, tab->d_starts, tab->d_nonempty, words, cnt,
@@ -978,9 +992,10 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             {
                 KernelTimer kt_(ctx, "qexpand");
                 if (fast_var)
-                    k_qexpand_w<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(w0, w1, sp->step, words, vmask, off,
+                    k_qexpand_w<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(w0, w1, sp->step, words, vcnt, off,
                                                                           boff[k], pat.weight, tab->d_starts,
-                                                                          tab->d_positions, mbin, hits, bincnt);
+                                                                          tab->d_positions, tab->d_spos, mbin, hits,
+                                                                          bincnt);
                 else
                     k_qexpand<<<grid_for(w1 - w0, 256), 256, 0, ctx->stream>>>(w0, w1, sp->step, words, off, boff[k],
                                                                               var, tab->d_starts, tab->d_nonempty,
