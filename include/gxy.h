@@ -142,7 +142,7 @@ typedef struct gxy_search_stats {
     uint64_t lookups;        /* variant-word lookups                         */
     uint64_t hits;           /* seed hits returned by the lookups            */
     uint64_t extended;       /* hits extended (GPU: every hit, speculatively)*/
-    uint64_t columns;        /* x-drop columns evaluated                     */
+    uint64_t columns;        /* x-drop columns evaluated (of every hit)      */
     uint64_t passing;        /* extensions at or above hsp_threshold         */
     uint64_t emitted;        /* HSPs after diagonal admission, before dedup  */
     uint64_t tie_pairs;      /* adjacent equal-(diag,t_start) unequal HSPs   */

@@ -181,6 +181,16 @@ inline uint32_t bin_multiplier(uint64_t tlen) {
 }
 __device__ __forceinline__ uint32_t target_bin(uint32_t t, uint32_t mbin) { return __umulhi(t, mbin); }
 
+// Load-balanced expansion (transitions <= 1): one wavefront per 64
+// consecutive windows.  The wavefront lists the non-empty buckets of its
+// windows in LDS (a wave prefix sum over the per-window bucket counts gives
+// each lane its slots), then splits its output range evenly over its lanes:
+// output slot j finds its bucket by binary search over the buckets' first
+// slots, so hit stores are coalesced and a window with many hits is shared by
+// all lanes.  One-position buckets carry the position itself.  A wavefront
+// whose buckets overflow the LDS list expands lane by lane instead.
+constexpr int kWaveBk = 512;
+
 __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uint64_t step,
                                                    const uint32_t *__restrict__ words,
                                                    const uint32_t *__restrict__ vcnt,
@@ -191,84 +201,102 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
                                                    uint32_t mbin, uint64_t *__restrict__ hits,
                                                    uint32_t *__restrict__ bincnt) {
     __shared__ uint32_t lbin[kBins];
-    __shared__ uint32_t lbeg[kExpWaves][64][kMaxV1];
-    __shared__ uint32_t lcum[kExpWaves][64][kMaxV1];
-    __shared__ uint64_t lo[kExpWaves][64];
-    __shared__ uint32_t lnb[kExpWaves][64];
-    __shared__ uint32_t linl[kExpWaves][64];  // bucket k of the window holds one position, stored in lbeg
+    __shared__ uint32_t kb[kExpWaves][kWaveBk];  // bucket start in `positions`, or its one position
+    __shared__ uint32_t kc[kExpWaves][kWaveBk];  // bucket's first output slot, relative to the wavefront
+    __shared__ uint8_t kl[kExpWaves][kWaveBk];   // owning lane, | 0x80 for a one-position bucket
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t wb = w0 + ((uint64_t)blockIdx.x * kExpWaves + wv) * 64;
     const uint64_t wend = min(wb + 64, w1);
     const uint64_t w = wb + lane;
-    const bool active = wb < w1;
+    const bool active = wb < w1;  // uniform per wavefront
     if (threadIdx.x < kBins) lbin[threadIdx.x] = 0;
-    uint64_t oend = 0;
+    uint64_t o0 = 0, oend = 0, o = 0;
+    uint32_t m = 0, nb = 0;
     if (active) {
+        o0 = off[wb] - base;
         oend = off[wend] - base;
-        uint64_t o = oend;
-        uint32_t nb = 0, inl = 0;
         if (w < w1) {
             o = off[w] - base;
-            uint32_t m = vcnt[w];  // 2 bits per variant: min(bucket size, 3)
-            if (m) {
-                const uint32_t word = words[w];
-                uint32_t cum = 0;
-                while (m) {
-                    const int i = (__ffs(m) - 1) >> 1;
-                    const uint32_t cw = (m >> (2 * i)) & 3u;
-                    m &= ~(3u << (2 * i));
-                    const uint64_t v = variant1(word, weight, i);
-                    uint32_t b, len;
-                    if (cw == 1) {  // one position: take it directly
-                        b = spos[v];
-                        len = 1;
-                        inl |= 1u << nb;
-                    } else {
-                        b = starts[v];
-                        len = starts[v + 1] - b;
-                    }
-                    lbeg[wv][lane][nb] = b;
-                    lcum[wv][lane][nb] = cum;
-                    cum += len;
-                    nb++;
-                }
-            }
+            m = vcnt[w];  // 2 bits per variant: min(bucket size, 3)
+            nb = (uint32_t)__popc((m | (m >> 1)) & 0x55555555u);
         }
-        lo[wv][lane] = o;
-        lnb[wv][lane] = nb;
-        linl[wv][lane] = inl;
+    }
+    // wavefront exclusive prefix of the bucket counts
+    uint32_t x = nb;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    const uint32_t boff = x - nb, total = __shfl(x, 63);
+    const bool fits = total <= (uint32_t)kWaveBk;
+    if (active && fits && m) {
+        const uint32_t word = words[w];
+        uint32_t cum = (uint32_t)(o - o0), k = boff;
+        while (m) {
+            const int i = (__ffs(m) - 1) >> 1;
+            const uint32_t cw = (m >> (2 * i)) & 3u;
+            m &= ~(3u << (2 * i));
+            const uint64_t v = variant1(word, weight, i);
+            uint32_t bk, len;
+            if (cw == 1) {  // one position: take it directly
+                bk = spos[v];
+                len = 1;
+            } else {
+                bk = starts[v];
+                len = starts[v + 1] - bk;
+            }
+            kb[wv][k] = bk;
+            kc[wv][k] = cum;
+            kl[wv][k] = (uint8_t)(lane | (cw == 1 ? 0x80 : 0));
+            cum += len;
+            k++;
+        }
     }
     __syncthreads();
     if (active) {
         uint32_t nbin[kBins] = {};  // this wavefront's hits per bin (uniform)
-        for (uint64_t j = lo[wv][0] + lane; j < oend; j += 64) {
-            // owning window: the last lane L with lo[L] <= j
-            int L = 0;
+        if (fits) {
+            for (uint64_t j = o0 + lane; j < oend; j += 64) {
+                const uint32_t rel = (uint32_t)(j - o0);
+                // the last bucket whose first slot is <= rel
+                uint32_t k = 0;
 #pragma unroll
-            for (int s = 32; s > 0; s >>= 1)
-                if (lo[wv][L + s] <= j) L += s;
-            const uint32_t r = (uint32_t)(j - lo[wv][L]);
-            const uint32_t nb = lnb[wv][L];
-            uint32_t i = 0;
-            while (i + 1 < nb && lcum[wv][L][i + 1] <= r) i++;
-            const uint32_t bk = lbeg[wv][L][i];
-            uint32_t t;
-            if ((linl[wv][L] >> i) & 1u)
-                t = bk;
-            else
-                t = positions[bk + (r - lcum[wv][L][i])];
-            hits[j] = (((wb + (uint64_t)L) * step) << 32) | t;
-            const uint32_t bin = target_bin(t, mbin);
+                for (uint32_t s2 = kWaveBk / 2; s2 > 0; s2 >>= 1)
+                    if (k + s2 < total && kc[wv][k + s2] <= rel) k += s2;
+                const uint32_t bk = kb[wv][k], L = kl[wv][k];
+                uint32_t t;
+                if (L & 0x80u)
+                    t = bk;
+                else
+                    t = positions[bk + (rel - kc[wv][k])];
+                hits[j] = (((wb + (uint64_t)(L & 63u)) * step) << 32) | t;
+                const uint32_t bin = target_bin(t, mbin);
 #pragma unroll
-            for (uint32_t b = 0; b < kBins; b++) nbin[b] += (uint32_t)__popcll(__ballot(bin == b));
+                for (uint32_t b = 0; b < kBins; b++) nbin[b] += (uint32_t)__popcll(__ballot(bin == b));
+            }
+        } else if (m) {  // overflow: this lane writes its own window's hits
+            const uint32_t word = words[w];
+            uint64_t j = o;
+            while (m) {
+                const int i = (__ffs(m) - 1) >> 1;
+                m &= ~(3u << (2 * i));
+                const uint64_t v = variant1(word, weight, i);
+                const uint32_t b0 = starts[v], e0 = starts[v + 1];
+                for (uint32_t p = b0; p < e0; p++) {
+                    const uint32_t t = positions[p];
+                    hits[j++] = ((w * step) << 32) | t;
+                    atomicAdd(&lbin[target_bin(t, mbin)], 1u);
+                }
+            }
         }
-        // Lane 0 took part in every iteration (it owns the smallest j);
-        // lanes that left the loop early hold partial counts.
+        // Lane 0 took part in every iteration of the balanced loop (it owns
+        // the smallest j); lanes that left early hold partial counts.
         uint32_t c = 0;
 #pragma unroll
         for (uint32_t b = 0; b < kBins; b++) {
-            const uint32_t x = __builtin_amdgcn_readfirstlane(nbin[b]);
-            c = (uint32_t)lane == b ? x : c;
+            const uint32_t xb = __builtin_amdgcn_readfirstlane(nbin[b]);
+            c = (uint32_t)lane == b ? xb : c;
         }
         if (lane < (int)kBins && c) atomicAdd(&lbin[lane], c);
     }

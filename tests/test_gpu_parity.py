@@ -190,6 +190,25 @@ def test_search_matches_oracle(ctx, spec, step, trans, xdrop, thr, entropy):
     assert gh == oh
 
 
+def test_search_dense_table(ctx):
+    """A dense table (match8 on 300 kbp unmasked: ~4.6 positions per word, ~9
+    non-empty variant buckets per window) overflows the expansion kernel's
+    per-wavefront bucket list, so those wavefronts expand lane by lane."""
+    say("search dense match8")
+    rng = np.random.default_rng(808)
+    t = rand_seq(rng, 300000, 0.001, 0.0)
+    c, v, m = t.unpack()
+    qc, qv, qm = synth.mutate(c[50000:150000], v[50000:150000], m[50000:150000], rng, 0.1, 0.01)
+    q = T.PackedSeq.from_arrays(qc, qv, qm)
+    pat = T.pattern_from_spec("match8")
+    sp = T.SearchParams(1, T.HspParams(910, 2000), 1, 0, 2000)
+    gh, gst = gxy.search(gxy.PosTable.build(t, pat, 1, ctx), t, q, HOX, sp)
+    oh, ost = O.search(O.PosTable(t, pat, 1), t, q, HOX, sp)
+    assert gst.hits == ost.hits and gst.hits > 20 * gst.query_windows
+    assert gst.emitted == ost.emitted and len(gh) > 0
+    assert gh == oh
+
+
 def test_search_device_query_and_rc(ctx):
     rng = np.random.default_rng(77)
     t, q = homologous_pair(rng, 100000)
