@@ -107,19 +107,21 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
                            uint64_t nwin, int respect_mask, uint32_t nvar, const uint32_t *__restrict__ starts,
                            const uint32_t *__restrict__ cmap, uint32_t *__restrict__ words,
                            uint32_t *__restrict__ cnt, uint32_t *__restrict__ vcnt,
-                           unsigned long long *__restrict__ nvalid) {
+                           uint64_t *__restrict__ bsum, unsigned long long *__restrict__ nvalid) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = false;
+    uint32_t c = 0;
     if (w < nwin) {
         uint64_t word;
-        uint32_t c = 0, m = 0;
+        uint32_t m = 0;
         if (seed_word(q, qlen, pat, w * step, respect_mask, &word)) {
             ok = true;
             // Bucket sizes from the 2-bit count map (min(size, 3), 16 words per
-            // map word), kept per variant (2 bits each) for k_qexpand_w.  Variant i >= 1 flips index bit W-i: flips of bits
-            // 0-3 stay in the word's own map word g, a flip of bit 4+k reads
-            // map word g ^ (1 << k) — at most 1 + (W-4) loads, issued back to
-            // back.  `starts` is read only for buckets of 3 or more.
+            // map word), kept per variant (2 bits each) for k_qexpand_w.
+            // Variant i >= 1 flips index bit W-i: flips of bits 0-3 stay in
+            // the word's own map word g, a flip of bit 4+k reads map word
+            // g ^ (1 << k) — at most 1 + (W-4) loads, issued back to back.
+            // `starts` is read only for buckets of 3 or more.
             const uint32_t W = pat.weight;
             const uint32_t g = (uint32_t)(word >> 4), sh = 2u * (uint32_t)(word & 15);
             const uint32_t c0w = cmap[g];
@@ -161,6 +163,15 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
     }
     const unsigned long long b = __ballot(ok);
     if ((threadIdx.x & 63) == 0 && b) spread_add(nvalid, (unsigned long long)__popcll(b));
+    // hits of this block of 256 windows (k_qexpand_w rebuilds the per-window
+    // offsets inside the block)
+    uint64_t x = c;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    __shared__ uint64_t ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
 // Load-balanced expansion (transitions <= 1): one wavefront per 64
@@ -182,7 +193,10 @@ inline uint32_t bin_multiplier(uint64_t tlen) {
 __device__ __forceinline__ uint32_t target_bin(uint32_t t, uint32_t mbin) { return __umulhi(t, mbin); }
 
 // Load-balanced expansion (transitions <= 1): one wavefront per 64
-// consecutive windows.  The wavefront lists the non-empty buckets of its
+// consecutive windows, four per block of 256 windows — the blocks of
+// k_qcount_v, whose scanned hit totals `boff` give each block its first
+// output slot; inside the block the window offsets are prefix sums of the
+// windows' counts.  The wavefront lists the non-empty buckets of its
 // windows in LDS (a wave prefix sum over the per-window bucket counts gives
 // each lane its slots), then splits its output range evenly over its lanes:
 // output slot j finds its bucket by binary search over the buckets' first
@@ -194,7 +208,8 @@ constexpr int kWaveBk = 512;
 __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uint64_t step,
                                                    const uint32_t *__restrict__ words,
                                                    const uint32_t *__restrict__ vcnt,
-                                                   const uint64_t *__restrict__ off, uint64_t base, uint32_t weight,
+                                                   const uint32_t *__restrict__ cnt,
+                                                   const uint64_t *__restrict__ boff, uint64_t base, uint32_t weight,
                                                    const uint32_t *__restrict__ starts,
                                                    const uint32_t *__restrict__ positions,
                                                    const uint32_t *__restrict__ spos,
@@ -204,35 +219,38 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
     __shared__ uint32_t kb[kExpWaves][kWaveBk];  // bucket start in `positions`, or its one position
     __shared__ uint32_t kc[kExpWaves][kWaveBk];  // bucket's first output slot, relative to the wavefront
     __shared__ uint8_t kl[kExpWaves][kWaveBk];   // owning lane, | 0x80 for a one-position bucket
+    __shared__ uint64_t wsum[kExpWaves];         // hits per wavefront
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t wb = w0 + ((uint64_t)blockIdx.x * kExpWaves + wv) * 64;
-    const uint64_t wend = min(wb + 64, w1);
     const uint64_t w = wb + lane;
     const bool active = wb < w1;  // uniform per wavefront
     if (threadIdx.x < kBins) lbin[threadIdx.x] = 0;
-    uint64_t o0 = 0, oend = 0, o = 0;
-    uint32_t m = 0, nb = 0;
-    if (active) {
-        o0 = off[wb] - base;
-        oend = off[wend] - base;
-        if (w < w1) {
-            o = off[w] - base;
-            m = vcnt[w];  // 2 bits per variant: min(bucket size, 3)
-            nb = (uint32_t)__popc((m | (m >> 1)) & 0x55555555u);
-        }
+    uint32_t m = 0, nb = 0, c = 0;
+    if (active && w < w1) {
+        c = cnt[w];
+        m = vcnt[w];  // 2 bits per variant: min(bucket size, 3)
+        nb = (uint32_t)__popc((m | (m >> 1)) & 0x55555555u);
     }
-    // wavefront exclusive prefix of the bucket counts
+    // wavefront prefix sums of the hit and bucket counts
+    uint64_t xc = c;
     uint32_t x = nb;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t yc = __shfl_up(xc, d);
         const uint32_t y = __shfl_up(x, d);
-        if (lane >= d) x += y;
+        if (lane >= d) {
+            xc += yc;
+            x += y;
+        }
     }
-    const uint32_t boff = x - nb, total = __shfl(x, 63);
+    const uint64_t wtot = __shfl(xc, 63);
+    const uint32_t kfirst = x - nb, total = __shfl(x, 63);
+    if (lane == 0) wsum[wv] = wtot;
     const bool fits = total <= (uint32_t)kWaveBk;
+    const uint64_t orel = xc - c;  // this window's first slot, relative to the wavefront
     if (active && fits && m) {
         const uint32_t word = words[w];
-        uint32_t cum = (uint32_t)(o - o0), k = boff;
+        uint32_t cum = (uint32_t)orel, k = kfirst;
         while (m) {
             const int i = (__ffs(m) - 1) >> 1;
             const uint32_t cw = (m >> (2 * i)) & 3u;
@@ -255,6 +273,9 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
     }
     __syncthreads();
     if (active) {
+        uint64_t o0 = boff[w0 / (64 * kExpWaves) + blockIdx.x] - base;
+        for (int v = 0; v < wv; v++) o0 += wsum[v];
+        const uint64_t oend = o0 + wtot, o = o0 + orel;
         uint32_t nbin[kBins] = {};  // this wavefront's hits per bin (uniform)
         if (fits) {
             for (uint64_t j = o0 + lane; j < oend; j += 64) {
@@ -305,19 +326,18 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
 }
 
 // Moves a batch's hits into bin order: block b re-reads the hits of its 256
-// windows and writes each to its bin's range for this block (binoff: the
+// windows (range from the block offsets) and writes each to its bin's range
+// for this block (binoff: the
 // exclusive scan of bincnt, bin-major).  Order within a (block, bin) range is
 // arbitrary; passing hits are sorted by a unique key afterwards.
-__global__ __launch_bounds__(256) void k_bin_hits(uint64_t w0, uint64_t w1, const uint64_t *__restrict__ off,
+__global__ __launch_bounds__(256) void k_bin_hits(uint64_t blk0, const uint64_t *__restrict__ boff,
                                                   uint64_t base, const uint64_t *__restrict__ hits_in,
                                                   const uint32_t *__restrict__ binoff, uint32_t mbin,
                                                   uint64_t *__restrict__ hits_out) {
     __shared__ uint32_t lc[kBins];
-    const uint64_t wa = w0 + (uint64_t)blockIdx.x * 64 * kExpWaves;
-    const uint64_t wb = min(wa + 64 * kExpWaves, w1);
     if (threadIdx.x < kBins) lc[threadIdx.x] = binoff[threadIdx.x * gridDim.x + blockIdx.x];
     __syncthreads();
-    const uint64_t ja = off[wa] - base, jb = off[wb] - base;
+    const uint64_t ja = boff[blk0 + blockIdx.x] - base, jb = boff[blk0 + blockIdx.x + 1] - base;
     for (uint64_t j = ja + threadIdx.x; j < jb; j += blockDim.x) {
         const uint64_t h = hits_in[j];
         const uint32_t slot = atomicAdd(&lc[target_bin((uint32_t)h, mbin)], 1u);
@@ -922,12 +942,19 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     if (!spr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     GXY_HIP(ctx, hipMemsetAsync(ctr, 0, 64, ctx->stream));
     GXY_HIP(ctx, hipMemsetAsync(spr, 0, kSpreadBytes, ctx->stream));
+    // Hit offsets: per 256-window block on the fast path (k_qexpand_w
+    // rebuilds the window offsets inside a block), per window otherwise.
+    // `off` then has noff + 1 entries and a unit is unit_win windows.
+    const uint64_t nblk_w = grid_for(nwin, 256);
+    const uint64_t noff = fast_var ? nblk_w : nwin, unit_win = fast_var ? 256 : 1;
+    auto *bsum = fast_var ? (uint64_t *)scratch(ctx, kSlotMisc3, nblk_w * 8) : nullptr;
+    if (fast_var && !bsum) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     {
         KernelTimer kt_(ctx, "qcount");
         if (fast_var)
-            k_qcount_v<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin,
-                                                                     respect_mask, nvar, tab->d_starts,
-                                                                     tab->d_cmap, words, cnt, vcnt, spr);
+            k_qcount_v<<<(unsigned)nblk_w, 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin,
+                                                                  respect_mask, nvar, tab->d_starts, tab->d_cmap,
+                                                                  words, cnt, vcnt, bsum, spr);
         else
             k_qcount<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin, respect_mask,
                                                                    var, tab->d_starts, tab->d_nonempty, words, cnt,
@@ -936,17 +963,23 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     GXY_LAUNCH_CHECK(ctx);
     size_t tmp_bytes = 0;
     hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> cnt64(cnt, ToU64());
-    (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, cnt64, off + 1, (int)nwin, ctx->stream);
+    if (fast_var)
+        (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, bsum, off + 1, (int)noff, ctx->stream);
+    else
+        (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, cnt64, off + 1, (int)noff, ctx->stream);
     void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
     if (!tmp) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     GXY_HIP(ctx, hipMemsetAsync(off, 0, 8, ctx->stream));
     {
         KernelTimer kt_(ctx, "qscan");
-        GXY_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, cnt64, off + 1, (int)nwin, ctx->stream));
+        if (fast_var)
+            GXY_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, bsum, off + 1, (int)noff, ctx->stream));
+        else
+            GXY_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, cnt64, off + 1, (int)noff, ctx->stream));
     }
     uint64_t H = 0;
     unsigned long long nvalid = 0;
-    GXY_HIP(ctx, hipMemcpyAsync(&H, off + nwin, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GXY_HIP(ctx, hipMemcpyAsync(&H, off + noff, 8, hipMemcpyDeviceToHost, ctx->stream));
     {
         uint64_t nv = 0;
         int rc = spread_sum(ctx, spr, &nv);
@@ -976,14 +1009,14 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         // Hits are expanded and extended in batches of at most kCap.
         const uint64_t kCap = 1ull << 27;
         const uint32_t nb = (uint32_t)std::max<uint64_t>(1, (H + kCap / 2 - 1) / (kCap / 2));
-        std::vector<uint64_t> bounds(nb + 1);
+        std::vector<uint64_t> bounds(nb + 1);  // in units of unit_win windows
         if (nb == 1) {
             bounds[0] = 0;
-            bounds[1] = nwin;
+            bounds[1] = noff;
         } else {
             auto *db = (uint64_t *)scratch(ctx, kSlotMisc0, (nb + 1) * 8);
             if (!db) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-            k_find_bounds<<<grid_for(nb + 1, 64), 64, 0, ctx->stream>>>(off, nwin, H, nb, db);
+            k_find_bounds<<<grid_for(nb + 1, 64), 64, 0, ctx->stream>>>(off, noff, H, nb, db);
             GXY_LAUNCH_CHECK(ctx);
             GXY_HIP(ctx, hipMemcpyAsync(bounds.data(), db, (nb + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
             GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1006,7 +1039,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         const bool binned = fast_var && byte_ok;
         const uint32_t mbin = bin_multiplier(tlen);
         uint64_t maxw = 0;
-        for (uint32_t k = 0; k < nb; k++) maxw = std::max(maxw, bounds[k + 1] - bounds[k]);
+        for (uint32_t k = 0; k < nb; k++) maxw = std::max(maxw, (bounds[k + 1] - bounds[k]) * unit_win);
         const uint64_t maxblk = grid_for(maxw, 64 * kExpWaves);
         auto *bincnt = (uint32_t *)scratch(ctx, kSlotBins, kBins * maxblk * 4);
         auto *binoff = (uint32_t *)scratch(ctx, kSlotBinOff, kBins * maxblk * 4);
@@ -1014,13 +1047,14 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         if (!bincnt || !binoff || (binned && !hits2))
             return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)", (unsigned long long)H);
         for (uint32_t k = 0; k < nb; k++) {
-            const uint64_t w0 = bounds[k], w1 = bounds[k + 1], nh = boff[k + 1] - boff[k];
+            const uint64_t w0 = bounds[k] * unit_win, w1 = std::min(bounds[k + 1] * unit_win, nwin);
+            const uint64_t nh = boff[k + 1] - boff[k];
             if (nh == 0) continue;
             const unsigned nblk = grid_for(w1 - w0, 64 * kExpWaves);
             {
                 KernelTimer kt_(ctx, "qexpand");
                 if (fast_var)
-                    k_qexpand_w<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(w0, w1, sp->step, words, vcnt, off,
+                    k_qexpand_w<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(w0, w1, sp->step, words, vcnt, cnt, off,
                                                                           boff[k], pat.weight, tab->d_starts,
                                                                           tab->d_positions, tab->d_spos, mbin, hits,
                                                                           bincnt);
@@ -1038,7 +1072,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
                 KernelTimer kt_(ctx, "bin");
                 GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmpb, tbb, bincnt, binoff, (int)(kBins * nblk),
                                                               ctx->stream));
-                k_bin_hits<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(w0, w1, off, boff[k], hits, binoff, mbin, hits2);
+                k_bin_hits<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(bounds[k], off, boff[k], hits, binoff, mbin, hits2);
                 GXY_LAUNCH_CHECK(ctx);
             }
             KernelTimer kt_ext(ctx, "extend");
