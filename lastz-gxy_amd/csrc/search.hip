@@ -574,7 +574,7 @@ struct XdropArgs {
 struct HitArraySrc {  // (q << 32 | t) records
     const uint64_t *hits;
     uint64_t n, h, stride, cur;
-    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim) {
+    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim, uint64_t *) {
         h = (uint64_t)blk * bdim + tid;
         stride = (uint64_t)nblk * bdim;
     }
@@ -592,7 +592,7 @@ struct HitArraySrc {  // (q << 32 | t) records
 struct PairSrc {  // separate t and q position arrays
     const uint32_t *tpos, *qpos;
     uint64_t n, h, stride, cur;
-    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim) {
+    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim, uint64_t *) {
         h = (uint64_t)blk * bdim + tid;
         stride = (uint64_t)nblk * bdim;
     }
@@ -607,25 +607,39 @@ struct PairSrc {  // separate t and q position arrays
 };
 
 // Hits of one bin per XCD: block b walks bin b % 8 with the other blocks of
-// its XCD (the grid is a multiple of 8).
+// its XCD (the grid is a multiple of 8).  The bin is split into one
+// contiguous segment per wavefront, and the wavefront hands out its segment
+// in order: the lanes asking for a hit in one call take the next consecutive
+// hits (a ballot ranks them; the head lives in LDS).  So the hits a
+// wavefront holds at any time are neighbours in query order and its query
+// reads share cache lines.
 struct BinnedHitSrc {
     const uint64_t *hits;
     const uint32_t *binoff;  // bin k starts at binoff[k * nblk_exp]
     uint32_t nblk_exp;
-    uint64_t n, h, end, stride, cur;
-    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim) {
+    uint64_t n, end, cur;
+    volatile uint64_t *head;  // this wavefront's next hit (LDS)
+    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim, uint64_t *lds) {
         const uint32_t bin = blk % kBins;
-        h = binoff[bin * nblk_exp] + (uint64_t)(blk / kBins) * bdim + tid;
-        end = bin + 1 < kBins ? binoff[(bin + 1) * nblk_exp] : n;
-        stride = (uint64_t)(nblk / kBins) * bdim;
+        const uint64_t lo = binoff[bin * nblk_exp], hi = bin + 1 < kBins ? binoff[(bin + 1) * nblk_exp] : n;
+        const uint64_t wpb = bdim / 64, wi = (blk / kBins) * wpb + tid / 64, nw = (nblk / kBins) * wpb;
+        end = lo + (hi - lo) * (wi + 1) / nw;
+        head = lds;
+        if ((tid & 63) == 0) *head = lo + (hi - lo) * wi / nw;
     }
+    // Every lane that needs a hit must make this call together.
     __device__ __forceinline__ bool next(uint32_t &t, uint32_t &q) {
+        const uint64_t asking = __ballot(1);
+        const int lane = threadIdx.x & 63;
+        const uint32_t rank = (uint32_t)__popcll(asking & ((1ull << lane) - 1ull));
+        const uint64_t h0 = *head;
+        if (rank == 0) *head = h0 + (uint64_t)__popcll(asking);
+        const uint64_t h = h0 + rank;
         if (h >= end) return false;
         const uint64_t x = hits[h];
         q = (uint32_t)(x >> 32);
         t = (uint32_t)x;
         cur = h;
-        h += stride;
         return true;
     }
 };
@@ -634,7 +648,8 @@ template <class Src, bool kAppend>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop(
     XdropArgs a, Src src, PassOut po, gxy_hsp *__restrict__ out, uint8_t *__restrict__ found,
     unsigned long long *__restrict__ colsum) {
-    src.init(blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
+    __shared__ uint64_t src_lds[4];  // one word per wavefront for the hit source
+    src.init(blockIdx.x, gridDim.x, threadIdx.x, blockDim.x, &src_lds[threadIdx.x >> 6]);
     const int lane = threadIdx.x & 63;
     uint32_t tp = 0, qp = 0, phase = kPhDone;
     uint64_t pt = 0, pq = 0, rem = 0;
