@@ -35,6 +35,7 @@ struct Blk {
 static_assert(sizeof(Blk) == 16, "Blk must be 16 bytes");
 
 constexpr uint32_t kMaxWeight = 14;          // dense 4^w table; 4^14+1 starts = 1 GiB
+constexpr uint32_t kNcntMaxWeight = 13;      // neighbourhood counts up to 4^13 words (256 MiB)
 constexpr int32_t kNegInf = INT32_MIN / 4;   // src/gapped_extend.rs:44
 constexpr uint32_t kInvalidWord = 0xFFFFFFFFu;
 
@@ -329,6 +330,10 @@ struct gxy_postable {
     uint32_t *d_nonempty = nullptr;   // bit w set when word w has positions
     uint32_t *d_cmap = nullptr;       // 2 bits per word: min(positions, 3), 16 words per u32
     uint32_t *d_spos = nullptr;       // first position of every word (valid when it has one)
+    // Neighbourhood counts (weight <= kNcntMaxWeight): for every word, the
+    // 2-bit sizes of its 1+W single-transition variants (variant i in bits
+    // 2i..2i+1, same order as the search), so counting a window is one load.
+    uint32_t *d_ncnt = nullptr;
     uint8_t *d_tbytes = nullptr;      // target byte buffer (x-drop layout, sequence at +kBytePad)
     gxy_ctx *pool = nullptr;          // block cache the buffers came from (driver-internal tables)
 };

@@ -96,6 +96,23 @@ __global__ void k_spos(const uint32_t *__restrict__ starts, const uint32_t *__re
     spos[w] = starts[w + 1] > b ? positions[b] : 0xFFFFFFFFu;
 }
 
+// Neighbourhood counts: variant 0 is the word, variant i >= 1 flips index bit
+// W-i (a transition at care position i-1); each takes min(size, 3) from the
+// count map.  Consecutive words read neighbouring map words, so the map stays
+// in cache.
+__global__ void k_ncnt(const uint32_t *__restrict__ cmap, uint64_t num_words, uint32_t weight,
+                       uint32_t *__restrict__ ncnt) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= num_words) return;
+    uint32_t r = 0;
+    for (uint32_t i = 0; i <= weight; i++) {
+        const uint64_t v = i == 0 ? w : (w ^ (1ull << (weight - i)));
+        const uint32_t c = (cmap[v >> 4] >> (2 * (uint32_t)(v & 15))) & 3u;
+        r |= c << (2 * i);
+    }
+    ncnt[w] = r;
+}
+
 __global__ void k_prune_counts(const uint32_t *__restrict__ starts, uint64_t num_words, uint32_t limit,
                                uint32_t *__restrict__ keep_cnt) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,8 +217,10 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     tab->d_nonempty = (uint32_t *)dev_alloc(ctx, nbw * 4, &tab->pool);
     tab->d_cmap = (uint32_t *)dev_alloc(ctx, ((tab->num_words + 15) / 16) * 4, &tab->pool);
     tab->d_spos = (uint32_t *)dev_alloc(ctx, tab->num_words * 4, &tab->pool);
+    if (pat.weight <= kNcntMaxWeight) tab->d_ncnt = (uint32_t *)dev_alloc(ctx, tab->num_words * 4, &tab->pool);
     tab->d_tbytes = (uint8_t *)dev_alloc(ctx, nbytes, &tab->pool);
-    if (!tab->d_nonempty || !tab->d_cmap || !tab->d_spos || !tab->d_tbytes) {
+    if (!tab->d_nonempty || !tab->d_cmap || !tab->d_spos || !tab->d_tbytes ||
+        (pat.weight <= kNcntMaxWeight && !tab->d_ncnt)) {
         gxy_postable_free(tab);
         return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     }
@@ -216,6 +235,11 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         k_spos<<<grid_for(tab->num_words, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions,
                                                                         tab->num_words, tab->d_spos);
         if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_spos", e);
+        if (tab->d_ncnt) {
+            k_ncnt<<<grid_for(tab->num_words, 256), 256, 0, ctx->stream>>>(tab->d_cmap, tab->num_words, pat.weight,
+                                                                            tab->d_ncnt);
+            if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_ncnt", e);
+        }
         rc = seq_to_bytes(ctx, t, true, tab->d_tbytes);
     }
     if (rc) {
@@ -301,6 +325,10 @@ int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
     GXY_LAUNCH_CHECK(ctx);
     k_spos<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, nw, tab->d_spos);
     GXY_LAUNCH_CHECK(ctx);
+    if (tab->d_ncnt) {
+        k_ncnt<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_cmap, nw, tab->pattern.weight, tab->d_ncnt);
+        GXY_LAUNCH_CHECK(ctx);
+    }
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GXY_OK;
 }
@@ -358,6 +386,7 @@ void gxy_postable_free(gxy_postable *tab) {
     dev_free(tab->pool, tab->d_nonempty);
     dev_free(tab->pool, tab->d_cmap);
     dev_free(tab->pool, tab->d_spos);
+    dev_free(tab->pool, tab->d_ncnt);
     dev_free(tab->pool, tab->d_tbytes);
     if (tab->owned_target) gxy_dseq_free(tab->owned_target);
     delete tab;

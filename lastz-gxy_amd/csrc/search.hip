@@ -105,8 +105,8 @@ __device__ __forceinline__ uint64_t variant1(uint64_t idx, uint32_t weight, int 
 
 __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern pat, uint64_t step,
                            uint64_t nwin, int respect_mask, uint32_t nvar, const uint32_t *__restrict__ starts,
-                           const uint32_t *__restrict__ cmap, uint32_t *__restrict__ words,
-                           uint32_t *__restrict__ cnt, uint32_t *__restrict__ vcnt,
+                           const uint32_t *__restrict__ cmap, const uint32_t *__restrict__ ncnt,
+                           uint32_t *__restrict__ words, uint32_t *__restrict__ cnt, uint32_t *__restrict__ vcnt,
                            uint64_t *__restrict__ bsum, unsigned long long *__restrict__ nvalid) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = false;
@@ -116,41 +116,47 @@ __global__ void k_qcount_v(const Blk *__restrict__ q, uint64_t qlen, DevPattern 
         uint32_t m = 0;
         if (seed_word(q, qlen, pat, w * step, respect_mask, &word)) {
             ok = true;
-            // Bucket sizes from the 2-bit count map (min(size, 3), 16 words per
-            // map word), kept per variant (2 bits each) for k_qexpand_w.
-            // Variant i >= 1 flips index bit W-i: flips of bits 0-3 stay in
-            // the word's own map word g, a flip of bit 4+k reads map word
-            // g ^ (1 << k) — at most 1 + (W-4) loads, issued back to back.
-            // `starts` is read only for buckets of 3 or more.
-            const uint32_t W = pat.weight;
-            const uint32_t g = (uint32_t)(word >> 4), sh = 2u * (uint32_t)(word & 15);
-            const uint32_t c0w = cmap[g];
-            uint32_t cf[kMaxWeight - 4];
+            // m: min(bucket size, 3) of every variant, 2 bits each (variant i
+            // in bits 2i..2i+1), kept for k_qexpand_w.
+            if (ncnt) {
+                // one load of the word's neighbourhood counts
+                m = ncnt[word] & (nvar >= 16 ? 0xFFFFFFFFu : ((1u << (2 * nvar)) - 1u));
+            } else {
+                // Count-map probes.  Variant i >= 1 flips index bit W-i: flips
+                // of bits 0-3 stay in the word's own map word g, a flip of bit
+                // 4+k reads map word g ^ (1 << k) — at most 1 + (W-4) loads,
+                // issued back to back.
+                const uint32_t W = pat.weight;
+                const uint32_t g = (uint32_t)(word >> 4), sh = 2u * (uint32_t)(word & 15);
+                const uint32_t c0w = cmap[g];
+                uint32_t cf[kMaxWeight - 4];
 #pragma unroll
-            for (uint32_t k = 0; k < kMaxWeight - 4; k++) cf[k] = 4 + k < W ? cmap[g ^ (1u << k)] : 0u;
-            uint32_t big = 0;  // variants whose bucket holds 3 or more
+                for (uint32_t k = 0; k < kMaxWeight - 4; k++) cf[k] = 4 + k < W ? cmap[g ^ (1u << k)] : 0u;
 #pragma unroll
-            for (uint32_t i = 0; i < (uint32_t)kMaxV1; i++) {
-                if (i < nvar) {
-                    uint32_t cw;
-                    if (i == 0) {
-                        cw = (c0w >> sh) & 3u;
-                    } else {
-                        const uint32_t b = W - i;
-                        cw = 0;
+                for (uint32_t i = 0; i < (uint32_t)kMaxV1; i++) {
+                    if (i < nvar) {
+                        uint32_t cw;
+                        if (i == 0) {
+                            cw = (c0w >> sh) & 3u;
+                        } else {
+                            const uint32_t b = W - i;
+                            cw = 0;
 #pragma unroll
-                        for (uint32_t k = 0; k < kMaxWeight - 4; k++)
-                            if (b == 4 + k) cw = (cf[k] >> sh) & 3u;
-                        if (b < 4) cw = (c0w >> (sh ^ (2u << b))) & 3u;
+                            for (uint32_t k = 0; k < kMaxWeight - 4; k++)
+                                if (b == 4 + k) cw = (cf[k] >> sh) & 3u;
+                            if (b < 4) cw = (c0w >> (sh ^ (2u << b))) & 3u;
+                        }
+                        m |= cw << (2 * i);
                     }
-                    m |= cw << (2 * i);
-                    big |= (cw == 3 ? 1u : 0u) << i;
-                    c += cw < 3 ? cw : 0u;
                 }
             }
-            while (big) {
-                const int i = __ffs(big) - 1;
-                big &= big - 1;
+            // sizes 1 and 2 are exact; a 3 means "3 or more": read `starts`
+            const uint32_t lo = m & 0x55555555u, hi = (m >> 1) & 0x55555555u, big = lo & hi;
+            c = (uint32_t)__popc(lo & ~big) + 2u * (uint32_t)__popc(hi & ~big);
+            uint32_t bb = big;
+            while (bb) {
+                const int i = (__ffs(bb) - 1) >> 1;
+                bb &= bb - 1;
                 const uint64_t v = variant1(word, pat.weight, i);
                 c += starts[v + 1] - starts[v];
             }
@@ -969,7 +975,7 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         if (fast_var)
             k_qcount_v<<<(unsigned)nblk_w, 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin,
                                                                   respect_mask, nvar, tab->d_starts, tab->d_cmap,
-                                                                  words, cnt, vcnt, bsum, spr);
+                                                                  tab->d_ncnt, words, cnt, vcnt, bsum, spr);
         else
             k_qcount<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(query->d, qlen, pat, sp->step, nwin, respect_mask,
                                                                    var, tab->d_starts, tab->d_nonempty, words, cnt,

@@ -209,6 +209,24 @@ def test_search_dense_table(ctx):
     assert gh == oh
 
 
+def test_search_weight14(ctx):
+    """Seed weight 14: the table carries no neighbourhood counts (they stop at
+    weight 13), so counting probes the 2-bit count map per variant."""
+    say("search weight 14")
+    rng = np.random.default_rng(1414)
+    t = rand_seq(rng, 200000, 0.001, 0.0)
+    c, v, m = t.unpack()
+    qc, qv, qm = synth.mutate(c[20000:120000], v[20000:120000], m[20000:120000], rng, 0.05, 0.005)
+    q = T.PackedSeq.from_arrays(qc, qv, qm)
+    pat = T.pattern_from_spec("11101101100110111011")  # 14 care positions
+    sp = T.SearchParams(1, T.HspParams(910, 3000), 1, 0, 3000)
+    gh, gst = gxy.search(gxy.PosTable.build(t, pat, 1, ctx), t, q, HOX, sp)
+    oh, ost = O.search(O.PosTable(t, pat, 1), t, q, HOX, sp)
+    assert gst.query_windows == ost.query_windows and gst.hits == ost.hits
+    assert gst.emitted == ost.emitted and len(gh) > 0
+    assert gh == oh
+
+
 def test_search_device_query_and_rc(ctx):
     rng = np.random.default_rng(77)
     t, q = homologous_pair(rng, 100000)
