@@ -23,6 +23,7 @@
 // src/driver.rs:270) and groups of one (target, query, strand) key are merged
 // in ascending chunk order (rayon push order, src/driver.rs:361-372).
 #include <algorithm>
+#include <atomic>
 #include <charconv>
 #include <condition_variable>
 #include <deque>
@@ -331,7 +332,8 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
 
     std::vector<std::unique_ptr<Group>> groups;
     std::vector<gxy_edit_run> run_store;  // scripts of gapped records
-    HostJobs jobs(2);                     // joined before groups are destroyed
+    std::atomic<uint64_t> ties{0};        // tie pairs counted by the host jobs
+    HostJobs jobs(2);                     // joined before groups and ties are destroyed
 
     clk.lap(kTrSetup);
     for (size_t c = 0; c < chunks.size(); c++) {
@@ -354,7 +356,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
                 const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
                 clk.lap(kTrRecords);
-                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st, qb);
+                // Ungapped: Vec::dedup runs on a host worker with the records.
+                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st, qb,
+                                     cfg->gapped_enabled != 0);
                 if (rc) return rc;
                 clk = PhaseClock(ctx);
                 rs.hits += st.hits;
@@ -368,8 +372,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 if (!cfg->gapped_enabled) {
                     // primary-region filter (src/driver.rs:250-256) and
                     // build_ungapped_record (src/driver.rs:437-468), off the device path
-                    jobs.submit([&g, hsps = std::move(hsps), lo = ch.primary_start_local, hi = ch.primary_end_local,
-                                 gs = ch.global_start] {
+                    jobs.submit([&g, &ties, hsps = std::move(hsps), lo = ch.primary_start_local,
+                                 hi = ch.primary_end_local, gs = ch.global_start]() mutable {
+                        ties += dedup_hsps(&hsps);
                         g.recs.reserve(hsps.size());
                         for (const auto &h : hsps)
                             if (h.t_start >= lo && h.t_start < hi)
@@ -455,6 +460,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         }
     }
     jobs.wait_all();
+    rs.tie_pairs += ties.load();
     for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
 
     clk.lap(kTrRecords);

@@ -21,10 +21,15 @@ int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf);
 
 // Seed search on device sequences.  `qbytes` may hold the query's byte copy
 // (seq_to_bytes buffer + kBytePad) when the caller searches the same query
-// repeatedly; otherwise it is built per call.
+// repeatedly; otherwise it is built per call.  With dedup false the result is
+// the sorted emitted list before Vec::dedup (the caller runs dedup_hsps and
+// counts the ties).
 int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
                     const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
-                    gxy_search_stats *stats, const uint8_t *qbytes = nullptr);
+                    gxy_search_stats *stats, const uint8_t *qbytes = nullptr, bool dedup = true);
+
+// Vec::dedup of a search's emitted HSPs (in place); returns the tie count.
+uint64_t dedup_hsps(std::vector<gxy_hsp> *v);
 
 struct GappedItem {
     const gxy_dseq *t;

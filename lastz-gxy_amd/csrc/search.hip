@@ -934,7 +934,7 @@ namespace gxy {
 // mirrors the driver's --nomasking (masks cleared before seeding).
 int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query, const gxy_scoring *scm,
                     const gxy_search_params *sp, int respect_mask, std::vector<gxy_hsp> *result,
-                    gxy_search_stats *stats, const uint8_t *qbytes) {
+                    gxy_search_stats *stats, const uint8_t *qbytes, bool dedup) {
     if (!tab || !query || !scm || !sp || !result) return set_err(ctx, GXY_EINVAL, "null argument");
     if (sp->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
     const gxy_dseq *target = tab->target;
@@ -1181,28 +1181,37 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             k_gather_hsp<<<grid_for(nemit, 256), 256, 0, ctx->stream>>>(i2, phsp, nemit, outd);
             GXY_LAUNCH_CHECK(ctx);
         }
-        std::vector<gxy_hsp> v(nemit);
-        GXY_HIP(ctx, hipMemcpyAsync(v.data(), outd, nemit * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream));
+        result->resize(nemit);
+        GXY_HIP(ctx,
+                hipMemcpyAsync(result->data(), outd, nemit * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         clk.lap(kTrSearchCopy);
-        // Vec::dedup (src/seed_search.rs:123) and the tie count.
-        result->reserve(nemit);
-        for (uint64_t k = 0; k < nemit; k++) {
-            const gxy_hsp &h = v[k];
-            if (!result->empty()) {
-                const gxy_hsp &b = result->back();
-                const bool same_key = (int64_t)h.t_start - h.q_start == (int64_t)b.t_start - b.q_start &&
-                                      h.t_start == b.t_start;
-                const bool equal = same_key && h.length == b.length && h.score == b.score;
-                if (equal) continue;
-                if (same_key) st.tie_pairs++;
-            }
-            result->push_back(h);
-        }
+        if (dedup) st.tie_pairs = dedup_hsps(result);
         clk.lap(kTrSearchDedup);
     }
     if (stats) *stats = st;
     return GXY_OK;
+}
+
+// Vec::dedup (src/seed_search.rs:123) on HSPs sorted by (diagonal, t_start),
+// in place; returns the number of adjacent pairs with equal sort keys that
+// survive (ties whose order the reference leaves to its unstable sort).
+uint64_t dedup_hsps(std::vector<gxy_hsp> *v) {
+    uint64_t ties = 0;
+    size_t n = 0;
+    for (size_t k = 0; k < v->size(); k++) {
+        const gxy_hsp h = (*v)[k];
+        if (n > 0) {
+            const gxy_hsp &b = (*v)[n - 1];
+            const bool same_key =
+                (int64_t)h.t_start - h.q_start == (int64_t)b.t_start - b.q_start && h.t_start == b.t_start;
+            if (same_key && h.length == b.length && h.score == b.score) continue;
+            if (same_key) ties++;
+        }
+        (*v)[n++] = h;
+    }
+    v->resize(n);
+    return ties;
 }
 
 }  // namespace gxy
