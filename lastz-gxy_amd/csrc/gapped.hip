@@ -1,23 +1,27 @@
 // gapped.hip — anchor selection and banded 3-state affine y-drop DP
 // (src/anchor.rs:25-69, src/gapped_extend.rs:78-463).
 //
-// DP kernel: one wavefront per side-problem, rows swept in order.  Pruning in
-// the reference is row-synchronous (the alive band of row i depends on the
-// best score after the whole row, src/gapped_extend.rs:388-412), so the exact
-// parallel form is row-parallel: the 64 lanes own contiguous segments of the
-// row's window [jl, jh] (<= 1208 cells with HOXD70), the previous row lives in
-// LDS, X and M come straight from it, and the in-row Y recurrence
+// DP kernel: one block of four wavefronts (256 lanes) per side-problem, rows
+// swept in order.  Pruning in the reference is row-synchronous (the alive band
+// of row i depends on the best score after the whole row,
+// src/gapped_extend.rs:388-412), so the exact parallel form is row-parallel:
+// the 256 lanes own contiguous segments of <= 5 cells of the row's window
+// [jl, jh] (<= 1208 cells with HOXD70), the previous row lives in LDS, X and M
+// come straight from it, and the in-row Y recurrence
 //     Y[j] = max(M[j-1] - (O+E), Y[j-1] - E)
 // becomes, with Z[j] = Y[j] + (j-jl)*E (offsets relative to the row window
 // start jl, so they stay below 1280*E), an inclusive prefix max
 //     Z[j] = max(Z[j-1], M[j-1] - (O+E) + (j-jl)*E),  Z[j0-1] = NEG + (j0-1-jl)*E,
-// computed lane-locally and carried across lanes with a shuffle scan.  max and
-// constant shifts commute exactly on int32 (no value comes near overflow), so
-// every M/X/Y value — including the NEG-derived ones in dead cells — equals
-// the reference's, and so do the from-states, best cell and alive band.
+// computed lane-locally, carried across lanes with a shuffle scan and across
+// wavefronts through LDS.  max and constant shifts commute exactly on int32
+// (no value comes near overflow), so every M/X/Y value — including the
+// NEG-derived ones in dead cells — equals the reference's, and so do the
+// from-states, best cell and alive band.  Four wavefronts per problem keep the
+// per-row latency short (the sweep is sequential in rows) and give the SIMDs
+// enough wavefronts to hide it.
 //
 // Traceback: each row stores 4 bits per cell (M-from: 2 bits, X-from: 1 bit,
-// Y-from: 1 bit) in a per-lane 16-byte slot, plus the row window (jl, jh).  A
+// Y-from: 1 bit), one 32-bit word per lane, plus the row window (jl, jh).  A
 // second kernel walks back from the best cell, one lane per problem, and emits
 // run-length edit runs tip-to-root.
 #include <algorithm>
@@ -28,10 +32,11 @@ using namespace gxy;
 
 namespace {
 
-constexpr int kMaxW = 1280;             // widest row window held in LDS
-constexpr int kCMax = kMaxW / 64;       // cells per lane
-constexpr int kSlotBytes = 16;          // per-lane traceback bytes per row
-constexpr int kRowBytes = 64 * kSlotBytes;
+constexpr int kMaxW = 1280;                 // widest row window held in LDS
+constexpr int kDpThreads = 256;             // lanes per side-problem
+constexpr int kDpWaves = kDpThreads / 64;
+constexpr int kCMax = kMaxW / kDpThreads;   // cells per lane
+constexpr int kRowBytes = kDpThreads * 4;   // traceback bytes per row: one word per lane
 
 enum : uint32_t { kStM = 0, kStX = 1, kStY = 2 };
 
@@ -57,20 +62,33 @@ struct DpRes {
 
 __device__ __forceinline__ int32_t shfl_up_i32(int32_t v, int d) { return __shfl_up(v, d); }
 
+// One side-problem per block of kDpThreads lanes (four wavefronts); lane t
+// owns the contiguous cells [jl + t*C, jl + (t+1)*C) of the row window,
+// C = ceil(W / kDpThreads) <= 5.  Per row: read the previous row's values of
+// the lane's cells (and the two cells before them) from LDS; compute X and M;
+// recompute M of the cell just before the segment (the previous lane's last
+// cell) so the Y scan needs no neighbour exchange; Y by the max-plus prefix
+// scan (lane-local, wavefront shuffle scan, carry across wavefronts through
+// LDS); row best and alive band reduced through LDS; the new row overwrites
+// the old one in LDS.  Three barriers per row.
 template <class S>
-__global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ probs, uint32_t nprob, S sc,
-                                                   int32_t oe, int32_t e, int32_t ydrop,
-                                                   uint8_t *__restrict__ tb, uint2 *__restrict__ rows,
-                                                   DpRes *__restrict__ res) {
-    __shared__ int32_t lds[2][3][kMaxW];
+__global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restrict__ probs, uint32_t nprob, S sc,
+                                                           int32_t oe, int32_t e, int32_t ydrop,
+                                                           uint8_t *__restrict__ tb, uint2 *__restrict__ rows,
+                                                           DpRes *__restrict__ res) {
+    __shared__ int32_t lM[kMaxW], lX[kMaxW], lY[kMaxW];  // the previous row, indexed by j - pjl
+    __shared__ int32_t wtot[kDpWaves];                    // Y scan: wavefront totals
+    __shared__ int32_t wmaxv[kDpWaves];                   // row best per wavefront
+    __shared__ uint32_t wmaxj[kDpWaves], wmaxst[kDpWaves];
+    __shared__ uint32_t wlo[kDpWaves], whi[kDpWaves];     // alive band per wavefront
     const uint32_t p = blockIdx.x;
-    if (p >= nprob) return;
-    const int lane = threadIdx.x;
+    if (p >= nprob) return;  // uniform per block
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const DpProb pr = probs[p];
     const uint32_t n = pr.n, m = pr.m;
     DpRes out = {0, 0, 0, kStM, 0, 0, 0};
     if (n == 0 || m == 0) {  // src/gapped_extend.rs:188-195
-        if (lane == 0) res[p] = out;
+        if (t == 0) res[p] = out;
         return;
     }
     // band radius / width (src/gapped_extend.rs:214-233)
@@ -88,7 +106,6 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
     uint32_t bi = 0, bj = 0, bst = kStM;
     uint32_t lo = 0, hi = 0;
     uint32_t pjl = 0, pW = 0;  // previous row window (pW == 0: none)
-    int cur = 0;
     uint64_t cells = 0;
     uint32_t rows_done = 0;
     uint8_t *tbp = tb + pr.tb_off;
@@ -103,21 +120,21 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
         const uint32_t jl = lo >= br ? lo - br : 0;
         const uint32_t jh = (uint32_t)min(min((uint64_t)hi + br, (uint64_t)m), (uint64_t)jl + bw - 1);
         const uint32_t W = jh - jl + 1;
-        const uint32_t C = (W + 63) / 64;
-        const uint32_t c0 = (uint32_t)lane * C;
+        const uint32_t C = (W + kDpThreads - 1) / kDpThreads;
+        const uint32_t c0 = (uint32_t)t * C;
         const uint32_t cnt = c0 < W ? min(C, W - c0) : 0;
         const uint32_t js = jl + c0;  // first cell of this lane
         cells += W - ((i == 0 && jl == 0) ? 1 : 0);
 
-        // target base of row i (T[i-1]); uniform across the wave
+        // target base of row i (T[i-1]); uniform
         uint32_t tc = 0, tv = 0;
         if (i >= 1) {
             const uint64_t tp = pr.t0 + (int64_t)pr.tdir * (int64_t)(i - 1);
             tc = base_code(pr.T, tp);
             tv = base_valid(pr.T, tp);
         }
-        // query bases Q[j-1] for this lane's cells
-        const uint32_t bq = (js >= 1 ? js : 1) - 1;  // Q index of the first needed base
+        // query bases Q[j-1] for j in [js-1, js+cnt): window from bq
+        const uint32_t bq = js >= 2 ? js - 2 : 0;
         Win qw = {0, 0, 0};
         if (cnt > 0 && bq < m) {
             if (pr.qdir > 0) {
@@ -136,27 +153,46 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
                 }
             }
         }
-        const int32_t *pM = lds[cur ^ 1][0];
-        const int32_t *pX = lds[cur ^ 1][1];
-        const int32_t *pY = lds[cur ^ 1][2];
         auto prev = [&](uint32_t j, int32_t &vm, int32_t &vx, int32_t &vy) {
             const uint32_t r = j - pjl;  // wraps when j < pjl
             if (i >= 1 && j >= pjl && r < pW) {
-                vm = pM[r];
-                vx = pX[r];
-                vy = pY[r];
+                vm = lM[r];
+                vx = lX[r];
+                vy = lY[r];
             } else {
                 vm = vx = vy = kNegInf;
             }
         };
+        // M of cell j from the previous row's diagonal (qm, qx, qy) = row i-1
+        // at j-1 (src/gapped_extend.rs:339-363)
+        auto mcell = [&](uint32_t j, int32_t qm, int32_t qx, int32_t qy, uint32_t *arg) -> int32_t {
+            if (i >= 1 && j >= 1) {
+                int32_t d;
+                if (qm >= qx && qm >= qy) { d = qm; *arg = kStM; }
+                else if (qx >= qy) { d = qx; *arg = kStX; }
+                else { d = qy; *arg = kStY; }
+                const uint32_t qi = j - 1 - bq;
+                const uint32_t qc = (uint32_t)(qw.codes >> (2 * qi)) & 3u;
+                const uint32_t qvb = (qw.valid >> qi) & 1u;
+                return d + sc(tc, qc, tv & qvb);
+            }
+            *arg = kStM;
+            return (i == 0 && j == 0) ? 0 : kNegInf;  // the seed cell, src/gapped_extend.rs:245
+        };
 
-        // ---- X and M (depend on the previous row only) -------------------
+        // ---- previous row, X and M --------------------------------------------
         int32_t Mv[kCMax], Xv[kCMax], Yv[kCMax];
         uint32_t nib[kCMax];
-        int32_t qm, qx, qy;  // previous row at j-1
-        prev(js - 1, qm, qx, qy);
-        if (js == 0) qm = qx = qy = kNegInf;
-        int32_t lastM = kNegInf;
+        // M at js-1: the previous lane's last cell (NEG before the window)
+        int32_t firstPrevM = kNegInf;
+        int32_t qm = kNegInf, qx = kNegInf, qy = kNegInf;  // row i-1 at j-1 for the next cell
+        if (cnt > 0 && t > 0) {
+            int32_t dm, dx, dy;
+            prev(js - 2, dm, dx, dy);
+            uint32_t arg;
+            firstPrevM = mcell(js - 1, dm, dx, dy, &arg);
+            prev(js - 1, qm, qx, qy);
+        }
 #pragma unroll
         for (int k = 0; k < kCMax; k++) {
             Mv[k] = kNegInf;
@@ -173,32 +209,18 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
                     Xv[k] = fm >= fx ? fm : fx;
                     nb |= (fm >= fx ? 0u : 1u) << 2;
                 }
-                if (i >= 1 && j >= 1) {  // M (src/gapped_extend.rs:339-363)
-                    int32_t d;
-                    uint32_t arg;
-                    if (qm >= qx && qm >= qy) { d = qm; arg = kStM; }
-                    else if (qx >= qy) { d = qx; arg = kStX; }
-                    else { d = qy; arg = kStY; }
-                    const uint32_t qi = j - 1 - bq;
-                    const uint32_t qc = (uint32_t)(qw.codes >> (2 * qi)) & 3u;
-                    const uint32_t qvb = (qw.valid >> qi) & 1u;
-                    Mv[k] = d + sc(tc, qc, tv & qvb);
-                    nb |= arg;
-                } else if (i == 0 && j == 0) {
-                    Mv[k] = 0;  // the seed cell, src/gapped_extend.rs:245
-                }
+                uint32_t arg;
+                Mv[k] = mcell(j, qm, qx, qy, &arg);
+                if (i >= 1 && j >= 1) nb |= arg;
                 nib[k] = nb;
-                lastM = Mv[k];
                 qm = am;
                 qx = ax;
                 qy = ay;
             }
         }
 
-        // ---- Y as a max-plus prefix scan ----------------------------------
+        // ---- Y as a max-plus prefix scan ----------------------------------------
         const uint32_t j0s = jl >= 1 ? jl : 1;
-        const int32_t neighM = shfl_up_i32(lastM, 1);  // M at js-1 (previous lane's last cell)
-        const int32_t firstPrevM = (lane == 0) ? kNegInf : neighM;
         int32_t tot = INT32_MIN;
         {
             int32_t pm = firstPrevM;
@@ -211,25 +233,25 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
                 }
             }
         }
-        // exclusive max-scan of lane totals
-        int32_t incl = tot;
+        int32_t incl = tot;  // inclusive max-scan within the wavefront
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const int32_t v = shfl_up_i32(incl, d);
             if (lane >= d) incl = max(incl, v);
         }
+        if (lane == 63) wtot[wv] = incl;
+        __syncthreads();  // (1) wavefront totals of the Y scan
         int32_t carry = shfl_up_i32(incl, 1);
+        if (lane == 0) carry = INT32_MIN;
+        for (int v = 0; v < wv; v++) carry = max(carry, wtot[v]);
         const int32_t zinit = kNegInf + ((int32_t)j0s - 1 - (int32_t)jl) * e;
-        carry = lane == 0 ? zinit : max(carry, zinit);
-        int32_t lastY = kNegInf;
+        carry = max(carry, zinit);
         {
             int32_t z = carry;
             int32_t pm = firstPrevM;
-            // Y at js-1 comes from the previous lane (or is out of window)
+            // Y at js-1 from the carry: Y[js-1] = Z[js-1] - (js-1-jl)*e
             int32_t py = kNegInf;
-            // previous lane's last Y is only known after its scan: recompute
-            // from the carry: Y[js-1] = Z[js-1] - (js-1)*e, Z[js-1] = carry
-            if (lane > 0 && js - 1 >= j0s) py = carry - ((int32_t)(js - 1 - jl)) * e;
+            if (t > 0 && js - 1 >= j0s) py = carry - ((int32_t)(js - 1 - jl)) * e;
 #pragma unroll
             for (int k = 0; k < kCMax; k++) {
                 if ((uint32_t)k < cnt) {
@@ -245,16 +267,13 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
                         py = kNegInf;
                     }
                     pm = Mv[k];
-                    lastY = Yv[k];
                 }
             }
         }
-        (void)lastY;
 
-        // ---- row best, alive band ------------------------------------------
+        // ---- row best (first strict maximum in j order) ---------------------
         int32_t rmax = INT32_MIN;
         uint32_t rj = 0, rst = kStM;
-        uint32_t alo = 0xFFFFFFFFu, ahi = 0;
         int32_t cb[kCMax];
 #pragma unroll
         for (int k = 0; k < kCMax; k++) {
@@ -268,19 +287,37 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
         int32_t wmax = rmax;
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) wmax = max(wmax, __shfl_xor(wmax, d));
-        const unsigned long long hasmax = __ballot(cnt > 0 && rmax == wmax);
-        const int src = __ffsll((long long)hasmax) - 1;
-        const uint32_t wj = __shfl(rj, src), wst = __shfl(rst, src);
-        if (wmax > best_score) {  // src/gapped_extend.rs:365-380
-            best_score = wmax;
+        {
+            const unsigned long long hasmax = __ballot(cnt > 0 && rmax == wmax);
+            const int src = hasmax ? __ffsll((long long)hasmax) - 1 : 0;
+            const uint32_t wj = __shfl(rj, src), wst = __shfl(rst, src);
+            if (lane == 0) {
+                wmaxv[wv] = hasmax ? wmax : INT32_MIN;
+                wmaxj[wv] = wj;
+                wmaxst[wv] = wst;
+            }
+        }
+        __syncthreads();  // (2) row best per wavefront
+        int32_t gmax = INT32_MIN;
+        uint32_t gj = 0, gst = kStM;
+#pragma unroll
+        for (int v = 0; v < kDpWaves; v++) {
+            if (wmaxv[v] > gmax) {  // strict: the earliest wavefront holding the maximum
+                gmax = wmaxv[v];
+                gj = wmaxj[v];
+                gst = wmaxst[v];
+            }
+        }
+        if (gmax > best_score) {  // src/gapped_extend.rs:365-380
+            best_score = gmax;
             bi = i;
-            bj = wj;
-            bst = wst;
+            bj = gj;
+            bst = gst;
         }
         const int32_t thr = sat_sub(best_score, ydrop);
-        int32_t *cM = lds[cur][0];
-        int32_t *cX = lds[cur][1];
-        int32_t *cY = lds[cur][2];
+
+        // ---- alive band, the new row into LDS, traceback ---------------------
+        uint32_t alo = 0xFFFFFFFFu, ahi = 0;
 #pragma unroll
         for (int k = 0; k < kCMax; k++) {
             if ((uint32_t)k < cnt) {
@@ -291,37 +328,46 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
                 } else {
                     Mv[k] = Xv[k] = Yv[k] = kNegInf;
                 }
-                cM[c0 + k] = Mv[k];
-                cX[c0 + k] = Xv[k];
-                cY[c0 + k] = Yv[k];
+                lM[c0 + k] = Mv[k];
+                lX[c0 + k] = Xv[k];
+                lY[c0 + k] = Yv[k];
             }
         }
-        // traceback nibbles: 8 cells per 32-bit word, one 16-byte slot per lane
-        uint4 slot = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < kCMax; k++) {
-            const uint32_t v = nib[k] << (4 * (k & 7));
-            if (k < 8) slot.x |= v;
-            else if (k < 16) slot.y |= v;
-            else if (k < 24) slot.z |= v;
-            else slot.w |= v;
+        {
+            const unsigned long long anyalive = __ballot(alo != 0xFFFFFFFFu);
+            const int fl = anyalive ? __ffsll((long long)anyalive) - 1 : 0;
+            const int ll = anyalive ? 63 - __clzll((long long)anyalive) : 0;
+            const uint32_t wl = __shfl(alo, fl), wh = __shfl(ahi, ll);
+            if (lane == 0) {
+                wlo[wv] = anyalive ? wl : 0xFFFFFFFFu;
+                whi[wv] = wh;
+            }
         }
-        *(uint4 *)(tbp + (uint64_t)i * kRowBytes + lane * kSlotBytes) = slot;
-        if (lane == 0) rowp[i] = make_uint2(jl, jh);
+        // traceback nibbles: the lane's cells in one 32-bit word
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < kCMax; k++) word |= nib[k] << (4 * k);
+        *(uint32_t *)(tbp + (uint64_t)i * kRowBytes + (uint64_t)t * 4) = word;
+        if (t == 0) rowp[i] = make_uint2(jl, jh);
         rows_done = i + 1;
-
-        const unsigned long long anyalive = __ballot(alo != 0xFFFFFFFFu);
-        if (!anyalive) break;  // whole row dead (src/gapped_extend.rs:407-409)
-        const int fl = __ffsll((long long)anyalive) - 1;
-        const int ll = 63 - __clzll((long long)anyalive);
-        lo = __shfl(alo, fl);
-        hi = __shfl(ahi, ll);
+        __syncthreads();  // (3) alive band; the new row is in LDS
+        uint32_t nlo = 0xFFFFFFFFu, nhi = 0;
+        bool alive = false;
+#pragma unroll
+        for (int v = 0; v < kDpWaves; v++) {
+            if (wlo[v] != 0xFFFFFFFFu) {
+                if (!alive) nlo = wlo[v];
+                nhi = whi[v];
+                alive = true;
+            }
+        }
+        if (!alive) break;  // whole row dead (src/gapped_extend.rs:407-409)
+        lo = nlo;
+        hi = nhi;
         pjl = jl;
         pW = W;
-        cur ^= 1;
-        __syncthreads();
     }
-    if (lane == 0) {
+    if (t == 0) {
         out.score = best_score;
         out.bi = bi;
         out.bj = bj;
@@ -333,8 +379,6 @@ __global__ __launch_bounds__(64) void k_dp_forward(const DpProb *__restrict__ pr
     }
 }
 
-// Traceback (src/gapped_extend.rs:426-453): one lane per problem, runs
-// written tip-to-root into runs[run_off[p] ...].
 __global__ void k_dp_traceback(const DpProb *__restrict__ probs, const DpRes *__restrict__ res, uint32_t nprob,
                                const uint8_t *__restrict__ tb, const uint2 *__restrict__ rows,
                                const uint64_t *__restrict__ run_off, gxy_edit_run *__restrict__ runs,
@@ -358,10 +402,10 @@ __global__ void k_dp_traceback(const DpProb *__restrict__ probs, const DpRes *__
             break;
         const uint2 win = rowp[i];
         if (j < win.x || j > win.y) break;
-        const uint32_t W = win.y - win.x + 1, C = (W + 63) / 64;
+        const uint32_t W = win.y - win.x + 1, C = (W + kDpThreads - 1) / kDpThreads;
         const uint32_t idx = j - win.x, ln = idx / C, k = idx % C;
-        const uint32_t word = *(const uint32_t *)(tbp + (uint64_t)i * kRowBytes + ln * kSlotBytes + (k / 8) * 4);
-        const uint32_t nb = (word >> (4 * (k & 7))) & 0xFu;
+        const uint32_t word = *(const uint32_t *)(tbp + (uint64_t)i * kRowBytes + ln * 4);
+        const uint32_t nb = (word >> (4 * k)) & 0xFu;
         uint32_t op, next;
         if (st == kStM) {
             op = GXY_OP_MATCH;
@@ -535,6 +579,10 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             pending.push_back(x);
         }
     }
+    // longest sides first, so the long sweeps start early instead of forming
+    // the tail of a pass
+    std::stable_sort(pending.begin(), pending.end(),
+                     [&](uint64_t a, uint64_t b) { return probs[a].n > probs[b].n; });
     uint64_t cells = 0;
     uint64_t cap = std::max<uint64_t>(1024, kRowBudget / std::max<uint64_t>(1, pending.size()));
     while (!pending.empty()) {
@@ -569,10 +617,10 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         {
         KernelTimer kt_(ctx, "dp_forward");
         if (scoring_is_i8(sc))
-            k_dp_forward<ScorerI8><<<(unsigned)npp, 64, 0, ctx->stream>>>(dprob, (uint32_t)npp, make_scorer_i8(sc),
+            k_dp_forward<ScorerI8><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(dprob, (uint32_t)npp, make_scorer_i8(sc),
                                                                           oe, e, gp.y_drop, dtb, drows, dres);
         else
-            k_dp_forward<ScorerI32><<<(unsigned)npp, 64, 0, ctx->stream>>>(dprob, (uint32_t)npp,
+            k_dp_forward<ScorerI32><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(dprob, (uint32_t)npp,
                                                                            make_scorer_i32(sc), oe, e, gp.y_drop,
                                                                            dtb, drows, dres);
         }

@@ -16,6 +16,8 @@
   segments, LogNormal median 5 kbp, half reverse-complemented, inserted at
   random points until the query is 1.05x), then 15 % substitutions and 2 %
   indel events.  The 48 Mbp default is the chromosome-scale HSP-only workload.
+* `dp_anchors` — the batched gapped-DP workload (anchored pairs of log-normal
+  extension length at 70-95 % identity).
 """
 from __future__ import annotations
 
@@ -50,8 +52,10 @@ def lcg_dna(n: int, seed: int) -> bytes:
     return out.tobytes()
 
 
-def mutate(code, valid, mask, rng, sub: float, indel: float):
-    """Return mutated (code, valid, mask) arrays (see module docstring)."""
+def mutate(code, valid, mask, rng, sub: float, indel: float, with_map: bool = False):
+    """Return mutated (code, valid, mask) arrays (see module docstring); with
+    `with_map`, also (kept, qpos): whether each source base survives and its
+    output position."""
     n = len(code)
     u = rng.random(n)
     is_indel = u < indel
@@ -78,7 +82,36 @@ def mutate(code, valid, mask, rng, sub: float, indel: float):
     oc = np.where(is_src, new[idx], rnd).astype(np.uint8)
     ov = np.where(is_src, valid[idx], True)
     om = np.where(is_src, mask[idx], False)
+    if with_map:
+        return oc, ov, om, keep, starts
     return oc, ov, om
+
+
+def dp_anchors(n: int, seed: int = 3, median: float = 2000.0, sigma: float = 1.0, lo: int = 100,
+               hi: int = 50_000):
+    """Batched gapped-DP workload: n anchored pairs.  Per anchor, extension
+    length L ~ LogNormal(median, sigma) clipped to [lo, hi] and identity
+    I ~ U[0.70, 0.95]; target = 2L+1 uniform ACGT; query = target mutated at
+    0.9(1-I) substitutions and 0.1(1-I)/3 indel events (Geometric mean 3);
+    anchor = the target's centre base and its query position (re-drawn while
+    the centre is deleted).  Returns [(target, query, t_pos, q_pos)]."""
+    rng = np.random.default_rng(seed)
+    Ls = np.clip(np.round(median * np.exp(sigma * rng.standard_normal(n))), lo, hi).astype(np.int64)
+    ids = rng.uniform(0.70, 0.95, n)
+    out = []
+    for L, ident in zip(Ls, ids):
+        L = int(L)
+        code = rng.integers(0, 4, 2 * L + 1).astype(np.uint8)
+        valid = np.ones(2 * L + 1, bool)
+        mask = np.zeros(2 * L + 1, bool)
+        while True:
+            qc, qv, qm, keep, qpos = mutate(code, valid, mask, rng, 0.9 * (1 - ident), 0.1 * (1 - ident) / 3,
+                                            with_map=True)
+            if keep[L]:
+                break
+        out.append((PackedSeq.from_arrays(code, valid, mask), PackedSeq.from_arrays(qc, qv, qm), L,
+                    int(qpos[L])))
+    return out
 
 
 def pair_30k(seed: int = 1, n: int = 30000):
