@@ -10,6 +10,13 @@ in HBM before the timed region; one step is a full driver::run over them
 
 `--workload pair30k`: the 30 kbp pair, gapped, MAF (latency-bound).
 
+`--workload dp`: batched gapped DP alone — anchored pairs with extension
+lengths LogNormal(median 2 kbp, sigma 1) clipped to 100 bp-50 kbp at 70-95 %
+identity, HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000, every pair in one
+gxy_gapped_extend_pairs_dev call; `value` is DP GCUPS (cells / s).  The default
+chromosome line carries the same measurement for --dp-anchors pairs in its
+`batched_dp` section.
+
 Prints one JSON line (rank 0, N = 1).  `roofline` is for the kernel with the
 largest device time, measured with HIP events on the library's stream over the
 timed steps; `cpu_baseline` is the oracle (the C++ restatement, AVX2 x-drop,
@@ -27,10 +34,123 @@ sys.path.insert(0, os.path.join(ROOT, "lastz-gxy_amd"))
 
 METRIC = "end-to-end query Mbp/s + HSP/DP GCUPS; p50/p99 serve latency vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+# int32 VALU: 256 CUs x 64 lanes x 2.4 GHz (the 157.3 TFLOP/s fp32 figure
+# counts packed pairs and FMA as 2 each)
+VALU_I32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+DP_OPS_PER_CELL = 11  # X: 2 sub + max; Y: 2 sub + max; M: add + 2 max; cell best: 2 max
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3):
+    """Batched gapped DP over n_anchors synthetic anchored pairs.  Returns a
+    dict (GCUPS, kernel times, VALU roofline, oracle baseline on a stratified
+    sample whose alignments are compared with the GPU's)."""
+    import ctypes as C
+    import numpy as np
+    import gxy
+    import gxy_types as T
+    import synth
+
+    t0 = time.time()
+    code, qcode, toff, qoff, tpos, qpos = synth.dp_anchor_arrays(n_anchors, seed, packed=False)
+    log(f"dp: generated {n_anchors} pairs ({len(code)} + {len(qcode)} bp) in {time.time() - t0:.1f}s")
+    ctx = gxy.default_context()
+    t0 = time.time()
+    dcat_t = gxy.DeviceSeq.upload(synth._packed(code), ctx)
+    dcat_q = gxy.DeviceSeq.upload(synth._packed(qcode), ctx)
+    dts = [dcat_t.slice(int(toff[k]), int(toff[k + 1])) for k in range(n_anchors)]
+    dqs = [dcat_q.slice(int(qoff[k]), int(qoff[k + 1])) for k in range(n_anchors)]
+    del dcat_t, dcat_q
+    ctx.synchronize()
+    log(f"dp: resident as {2 * n_anchors} device sequences in {time.time() - t0:.1f}s")
+    hox = T.scoring_hoxd70()
+    gp = T.GappedParams(9400, 3000)
+    L = gxy.lib()
+    tv = (C.c_void_p * n_anchors)(*[t._h for t in dts])
+    qv = (C.c_void_p * n_anchors)(*[q._h for q in dqs])
+    arr = (T.Anchor * n_anchors)()
+    for k in range(n_anchors):
+        arr[k].t_pos, arr[k].q_pos = int(tpos[k]), int(qpos[k])
+
+    def once():
+        sp = C.POINTER(T.AlignmentSet)()
+        ctx.check(L.gxy_gapped_extend_pairs_dev(ctx._h, C.addressof(tv), C.addressof(qv), C.addressof(arr),
+                                                n_anchors, C.byref(hox), C.byref(gp), C.byref(sp)),
+                  "gxy_gapped_extend_pairs_dev")
+        cells = sp.contents.cells
+        found = int(np.ctypeslib.as_array(sp.contents.found, (n_anchors,)).astype(bool).sum())
+        L.gxy_alignment_set_free(sp)
+        return cells, found
+
+    for _ in range(warmup):
+        once()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        cells, found = once()
+    ctx.synchronize()
+    el = time.perf_counter() - t1
+    ctx.set_timing(False)
+    kt = {nm: ctx.kernel_time(nm) for nm in ("dp_forward", "dp_traceback")}
+    gcups = cells * steps / el / 1e9
+    fwd_ms, fwd_n = kt["dp_forward"]
+    in_kernel = cells * steps / (fwd_ms / 1e3) / 1e9 if fwd_ms else None
+    peak_gcups = VALU_I32_PEAK_TOPS * 1e3 / DP_OPS_PER_CELL
+    out = {
+        "workload": f"{n_anchors} anchored pairs, extension LogNormal(median 2000, sigma 1) clipped 100-50000 bp, "
+                    f"identity U[0.70, 0.95], HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000",
+        "anchors": n_anchors, "cells_per_step": cells, "found": found, "steps": steps,
+        "ms_per_step": round(1000 * el / steps, 2), "dp_gcups": round(gcups, 3),
+        "kernel_ms_per_step": {k: round(v[0] / steps, 3) for k, v in kt.items() if v[1]},
+        "roofline": {"bound": "valu", "kernel": "k_dp_forward",
+                     "achieved": round(in_kernel * DP_OPS_PER_CELL / 1e3, 3) if in_kernel else None,
+                     "peak": round(VALU_I32_PEAK_TOPS, 2), "unit": "T int32 ops/s",
+                     "frac": round(in_kernel / peak_gcups, 5) if in_kernel else None,
+                     "gcups_in_kernel": round(in_kernel, 2) if in_kernel else None,
+                     "peak_gcups": round(peak_gcups, 1), "ops_per_cell": DP_OPS_PER_CELL,
+                     "launches": fwd_n, "traffic": None},
+    }
+    log(f"dp: {cells} cells/step, {found}/{n_anchors} found, {gcups:.2f} GCUPS end to end, kernels {kt}")
+
+    if cpu_anchors:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        from concurrent.futures import ThreadPoolExecutor
+        # stratified: equal counts from each decile of extension length
+        order = np.argsort(tpos, kind="stable")
+        per = max(1, cpu_anchors // 10)
+        pick = []
+        for b in range(10):
+            seg = order[b * n_anchors // 10:(b + 1) * n_anchors // 10]
+            if len(seg):
+                pick.extend(seg[np.linspace(0, len(seg) - 1, min(per, len(seg))).astype(int)].tolist())
+        hp = [(synth._packed(code[toff[k]:toff[k + 1]]), synth._packed(qcode[qoff[k]:qoff[k + 1]]),
+               int(tpos[k]), int(qpos[k])) for k in pick]
+        nthreads = os.cpu_count() or 1
+
+        def one(p):
+            return O.gapped_extend((p[2], p[3]), p[0], p[1], hox, gp)
+
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(nthreads) as ex:
+            ores = list(ex.map(one, hp))
+        cpu_s = time.perf_counter() - t1
+        ocells = sum(c for _, c in ores)
+        gres, gcells = gxy.gapped_extend_pairs_dev([dts[k] for k in pick], [dqs[k] for k in pick],
+                                                   [(p[2], p[3]) for p in hp], hox, gp, ctx)
+        same = gres == [r for r, _ in ores] and gcells == ocells
+        out["cpu_baseline"] = {
+            "value": round(ocells / cpu_s / 1e9, 4), "unit": "GCUPS", "cores": nthreads, "kind": "port",
+            "sample": f"oracle (lazy-row DP) on {len(pick)} of the pairs, {per} from each decile of "
+                      f"extension length: {ocells} cells in {cpu_s:.2f} s",
+            "gpu_output_identical_on_sample": bool(same)}
+        log(f"dp: cpu baseline {ocells / cpu_s / 1e9:.3f} GCUPS on {len(pick)} pairs, identical: {same}")
+    return out
 
 
 def main():
@@ -38,7 +158,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k"])
+    ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k", "dp"])
+    ap.add_argument("--dp-anchors", type=int, default=50_000)
+    ap.add_argument("--dp-cpu-anchors", type=int, default=400)
+    ap.add_argument("--no-dp", action="store_true", help="omit the batched_dp section of the chrom line")
     ap.add_argument("--target-bp", type=int, default=48_000_000)
     ap.add_argument("--cpu-sample-bp", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -52,6 +175,18 @@ def main():
     import gxy
     import gxy_types as T
     import synth
+
+    if args.workload == "dp":
+        d = dp_leg(args.dp_anchors, args.steps, args.warmup, 0 if args.no_cpu else args.dp_cpu_anchors)
+        out = {"metric": METRIC, "value": d["dp_gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": d["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+               "config": {"workload": d["workload"], "anchors": d["anchors"]},
+               "roofline": d["roofline"], "cpu_baseline": d.get("cpu_baseline"),
+               "cells_per_step": d["cells_per_step"], "found": d["found"],
+               "kernel_ms_per_step": d["kernel_ms_per_step"]}
+        print(json.dumps(out), flush=True)
+        return
 
     t0 = time.time()
     if args.workload == "chrom":
@@ -166,6 +301,11 @@ def main():
                "gpu_output_identical_on_sample": bool(same)}
         log(f"cpu baseline {cpu_s:.2f}s on {len(sample)} bp, GPU output identical: {same}")
 
+    batched_dp = None
+    if args.workload == "chrom" and not args.no_dp:
+        del dt, dq
+        batched_dp = dp_leg(args.dp_anchors, 2, 1, 0 if args.no_cpu else args.dp_cpu_anchors)
+
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "query-Mbp/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
@@ -176,7 +316,7 @@ def main():
         "pcie_inclusive_query_mbps": round(qbp / 1e6 / pcie_s, 3),
         "kernel_ms_per_step": {n: round(v[0] / args.steps, 3) for n, v in ktime.items() if v[1]},
         "timed_kernels_ms_per_step": round(sum(v[0] for v in ktime.values()) / args.steps, 3),
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "batched_dp": batched_dp,
     }
     print(json.dumps(out), flush=True)
 

@@ -87,31 +87,105 @@ def mutate(code, valid, mask, rng, sub: float, indel: float, with_map: bool = Fa
     return oc, ov, om
 
 
-def dp_anchors(n: int, seed: int = 3, median: float = 2000.0, sigma: float = 1.0, lo: int = 100,
-               hi: int = 50_000):
+def _mutate_segments(code, lens, sub, indel, rng):
+    """Mutate concatenated independent segments (segment r: lens[r] bases,
+    substitution rate sub[r], indel event rate indel[r]) with the model of
+    `mutate` (all bases valid and uppercase), drawing the random numbers only
+    where events happen.  Returns (out_code, out_seg_start, opos) with opos(p)
+    giving the output position of source base p and keep(p) whether it
+    survives, as a pair of arrays over all source bases."""
+    n = len(code)
+    off = np.zeros(len(lens) + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    u = rng.random(n, dtype=np.float32)
+    thr_indel = np.repeat(indel.astype(np.float32), lens)
+    thr_sub = np.repeat((indel + (1 - indel) * sub).astype(np.float32), lens)
+    is_indel = u < thr_indel
+    del thr_indel
+    is_sub = (~is_indel) & (u < thr_sub)
+    del thr_sub, u
+    new = code.copy()
+    sp = np.flatnonzero(is_sub)
+    new[sp] = (code[sp] + rng.integers(1, 4, len(sp)).astype(np.uint8)) % 4
+    ip = np.flatnonzero(is_indel)
+    L = rng.geometric(1.0 / 3.0, len(ip))
+    is_ins = rng.random(len(ip)) < 0.5
+    seg_of = np.searchsorted(off, ip, side="right") - 1
+    dpos, dend = ip[~is_ins], np.minimum(ip[~is_ins] + L[~is_ins], off[seg_of[~is_ins] + 1])
+    cov = np.bincount(dpos, minlength=n + 1)[:n + 1] - np.bincount(dend, minlength=n + 1)[:n + 1]
+    keep = np.cumsum(cov[:n]) == 0
+    del cov
+    ck = np.cumsum(keep)  # kept bases up to and including p
+    ipos, ilen = ip[is_ins], L[is_ins]
+    kept = new[keep]
+    out = np.insert(kept, np.repeat(ck[ipos], ilen), rng.integers(0, 4, int(ilen.sum())).astype(np.uint8))
+    icum = np.concatenate([[0], np.cumsum(ilen)])
+
+    def opos(p):
+        p = np.asarray(p)
+        return (ck[p] - keep[p]) + icum[np.searchsorted(ipos, p, side="left")]
+
+    seg_start = np.append(opos(off[:-1]), len(out))
+    return out, seg_start, opos, keep
+
+
+def dp_anchor_arrays(n: int, seed: int = 3, median: float = 2000.0, sigma: float = 1.0, lo: int = 100,
+                     hi: int = 50_000, packed: bool = True):
     """Batched gapped-DP workload: n anchored pairs.  Per anchor, extension
     length L ~ LogNormal(median, sigma) clipped to [lo, hi] and identity
     I ~ U[0.70, 0.95]; target = 2L+1 uniform ACGT; query = target mutated at
     0.9(1-I) substitutions and 0.1(1-I)/3 indel events (Geometric mean 3);
-    anchor = the target's centre base and its query position (re-drawn while
-    the centre is deleted).  Returns [(target, query, t_pos, q_pos)]."""
+    anchor = the target's centre base and its query position (the mutation is
+    re-drawn while the centre is deleted).
+
+    Returns (target, query, t_off, q_off, t_pos, q_pos): the targets and the
+    queries concatenated into two PackedSeqs, pair k being
+    target[t_off[k]:t_off[k+1]] and query[q_off[k]:q_off[k+1]], anchored at
+    (t_pos[k], q_pos[k]) relative to those slices.  With packed=False the
+    two sequences are returned as 2-bit code arrays (all bases valid and
+    uppercase)."""
     rng = np.random.default_rng(seed)
     Ls = np.clip(np.round(median * np.exp(sigma * rng.standard_normal(n))), lo, hi).astype(np.int64)
     ids = rng.uniform(0.70, 0.95, n)
-    out = []
-    for L, ident in zip(Ls, ids):
-        L = int(L)
-        code = rng.integers(0, 4, 2 * L + 1).astype(np.uint8)
-        valid = np.ones(2 * L + 1, bool)
-        mask = np.zeros(2 * L + 1, bool)
-        while True:
-            qc, qv, qm, keep, qpos = mutate(code, valid, mask, rng, 0.9 * (1 - ident), 0.1 * (1 - ident) / 3,
-                                            with_map=True)
-            if keep[L]:
-                break
-        out.append((PackedSeq.from_arrays(code, valid, mask), PackedSeq.from_arrays(qc, qv, qm), L,
-                    int(qpos[L])))
-    return out
+    tlen = 2 * Ls + 1
+    t_off = np.zeros(n + 1, np.int64)
+    np.cumsum(tlen, out=t_off[1:])
+    code = rng.integers(0, 4, int(t_off[-1])).astype(np.uint8)
+    qparts = [None] * n
+    qpos = np.zeros(n, np.int64)
+    for c0 in range(0, n, 4096):  # chunks of pairs keep the temporaries small
+        todo = np.arange(c0, min(n, c0 + 4096))
+        while len(todo):
+            lens = tlen[todo]
+            src = np.concatenate([code[t_off[k]:t_off[k + 1]] for k in todo])
+            out, qbeg, opos, keep = _mutate_segments(src, lens, 0.9 * (1 - ids[todo]), 0.1 * (1 - ids[todo]) / 3,
+                                                     rng)
+            soff = np.concatenate([[0], np.cumsum(lens)])
+            centre = soff[:-1] + Ls[todo]
+            ok = keep[centre]
+            cq = opos(centre)
+            for r in np.flatnonzero(ok):
+                k = todo[r]
+                qparts[k] = out[qbeg[r]:qbeg[r + 1]]
+                qpos[k] = cq[r] - qbeg[r]
+            todo = todo[~ok]
+    q_off = np.zeros(n + 1, np.int64)
+    np.cumsum([len(x) for x in qparts], out=q_off[1:])
+    qcode = np.concatenate(qparts)
+    if not packed:
+        return code, qcode, t_off, q_off, Ls.copy(), qpos
+    return _packed(code), _packed(qcode), t_off, q_off, Ls.copy(), qpos
+
+
+def _packed(code):
+    return PackedSeq.from_arrays(code, np.ones(len(code), bool), np.zeros(len(code), bool))
+
+
+def dp_anchors(n: int, seed: int = 3, **kw):
+    """dp_anchor_arrays as a list of per-pair (target, query, t_pos, q_pos)."""
+    t, q, toff, qoff, tp, qp = dp_anchor_arrays(n, seed, packed=False, **kw)
+    return [(_packed(t[toff[k]:toff[k + 1]]), _packed(q[qoff[k]:qoff[k + 1]]), int(tp[k]), int(qp[k]))
+            for k in range(n)]
 
 
 def pair_30k(seed: int = 1, n: int = 30000):
