@@ -32,11 +32,14 @@ using namespace gxy;
 
 namespace {
 
-constexpr int kMaxW = 1280;                 // widest row window held in LDS
+constexpr int kMaxW = 1280;                 // widest row window
 constexpr int kDpThreads = 256;             // lanes per side-problem
 constexpr int kDpWaves = kDpThreads / 64;
 constexpr int kCMax = kMaxW / kDpThreads;   // cells per lane
 constexpr int kRowBytes = kDpThreads * 4;   // traceback bytes per row: one word per lane
+constexpr int kRecs = kMaxW + 2;            // previous-row records in LDS: [0] NEG, [1, 1+W) the row, NEG after
+constexpr int kQCache = 4096;               // query columns cached in LDS, one byte each
+constexpr int kQMargin = 512;               // cached columns kept left of the window on a refill
 
 enum : uint32_t { kStM = 0, kStX = 1, kStY = 2 };
 
@@ -60,27 +63,109 @@ struct DpRes {
     uint64_t cells;
 };
 
-__device__ __forceinline__ int32_t shfl_up_i32(int32_t v, int d) { return __shfl_up(v, d); }
+// Inclusive prefix max over the 64 lanes of a wavefront: shifts of 1, 2, 4, 8
+// within each 16-lane row, then the row-15 and row-31 broadcasts.  Called with
+// every lane active.
+__device__ __forceinline__ int32_t wave_scan_max(int32_t v) {
+    v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x111, 0xF, 0xF, false));  // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x112, 0xF, 0xF, false));  // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x114, 0xF, 0xF, false));  // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x118, 0xF, 0xF, false));  // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return v;
+}
+
+// Value of the previous lane (lane 0 gets `fill`).
+__device__ __forceinline__ int32_t wave_shr1(int32_t v, int32_t fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+
+// 32 bases at positions p0, p0+dir, ..., p0+31*dir in slots 0..31; for
+// dir < 0 the slots past position 0 hold garbage.
+__device__ __forceinline__ Win dir_win(const Blk *__restrict__ b, uint64_t p0, int32_t dir) {
+    if (dir > 0) return load_win(b, p0);
+    Win w;
+    if (p0 >= 31) {
+        const Win x = load_win(b, p0 - 31);
+        w.codes = rev2(x.codes);
+        w.valid = __builtin_bitreverse32(x.valid);
+    } else {
+        const Win x = load_win(b, 0);
+        const uint32_t sh = 31 - (uint32_t)p0;
+        w.codes = rev2(x.codes) >> (2 * sh);
+        w.valid = __builtin_bitreverse32(x.valid) >> sh;
+    }
+    w.mask = 0;
+    return w;
+}
+
+// Scores of one target base against a cached query byte qb = 8 * (code |
+// 4 * invalid) (src/gapped_extend.rs:339-340 via pair_score).
+struct RowScoreI8 {
+    uint64_t tab;  // byte b: the score of query byte 8b
+    __device__ __forceinline__ int32_t operator()(uint32_t qb) const { return (int32_t)(int8_t)(uint8_t)(tab >> qb); }
+};
+__device__ __forceinline__ RowScoreI8 row_scores(const ScorerI8 &s, uint32_t tc, uint32_t tv) {
+    const uint32_t lo = (tc & 1u) ? s.r1 : s.r0;
+    const uint32_t hi = (tc & 1u) ? s.r3 : s.r2;
+    const uint32_t row = (tc & 2u) ? hi : lo;
+    const uint32_t amb = (uint32_t)(s.ambig & 0xFF) * 0x01010101u;
+    return {((uint64_t)amb << 32) | (tv ? row : amb)};
+}
+struct RowScoreI32 {
+    int32_t s0, s1, s2, s3, amb;
+    __device__ __forceinline__ int32_t operator()(uint32_t qb) const {
+        const int32_t a = (qb & 8u) ? s1 : s0;
+        const int32_t b = (qb & 8u) ? s3 : s2;
+        const int32_t x = (qb & 16u) ? b : a;
+        return (qb & 32u) ? amb : x;
+    }
+};
+__device__ __forceinline__ RowScoreI32 row_scores(const ScorerI32 &s, uint32_t tc, uint32_t tv) {
+    RowScoreI32 r;
+    r.amb = s.ambig;
+    const int32_t *row = s.s + 4 * tc;  // tc is uniform: no dynamic register indexing per lane
+    int32_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int32_t a = (tc & 1u) ? s.s[4 + q] : s.s[q];
+        const int32_t b = (tc & 1u) ? s.s[12 + q] : s.s[8 + q];
+        v[q] = tv ? ((tc & 2u) ? b : a) : s.ambig;
+    }
+    (void)row;
+    r.s0 = v[0];
+    r.s1 = v[1];
+    r.s2 = v[2];
+    r.s3 = v[3];
+    return r;
+}
 
 // One side-problem per block of kDpThreads lanes (four wavefronts); lane t
-// owns the contiguous cells [jl + t*C, jl + (t+1)*C) of the row window,
-// C = ceil(W / kDpThreads) <= 5.  Per row: read the previous row's values of
-// the lane's cells (and the two cells before them) from LDS; compute X and M;
-// recompute M of the cell just before the segment (the previous lane's last
-// cell) so the Y scan needs no neighbour exchange; Y by the max-plus prefix
-// scan (lane-local, wavefront shuffle scan, carry across wavefronts through
-// LDS); row best and alive band reduced through LDS; the new row overwrites
-// the old one in LDS.  Three barriers per row.
+// owns the contiguous cells [jl + t*C, jl + t*C + cnt) of the row window,
+// C = ceil(W / kDpThreads) <= 5.  Per row:
+//   * the previous row comes from LDS as one {M, X, Y} record per cell,
+//     indexed by j - jl(prev) + 1 and clamped into [0, kRecs): record 0 and
+//     the min(br, m) + 1 records after the previous row hold NEG, which is
+//     what the reference reads outside the previous row's window;
+//   * query bases come from a 4096-column LDS cache (refilled when the window
+//     leaves it), target bases from a 32-row register window;
+//   * Y by the max-plus prefix scan: each lane's M values feed the next
+//     column, so lane totals need no neighbour; DPP scan inside a wavefront,
+//     wavefront totals across them through LDS;
+//   * row best (first strict maximum in j order) and alive band reduced
+//     through LDS; the new row overwrites the old records.
+// Three barriers per row.
 template <class S>
 __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restrict__ probs, uint32_t nprob, S sc,
                                                            int32_t oe, int32_t e, int32_t ydrop,
                                                            uint8_t *__restrict__ tb, uint2 *__restrict__ rows,
                                                            DpRes *__restrict__ res) {
-    __shared__ int32_t lM[kMaxW], lX[kMaxW], lY[kMaxW];  // the previous row, indexed by j - pjl
-    __shared__ int32_t wtot[kDpWaves];                    // Y scan: wavefront totals
-    __shared__ int32_t wmaxv[kDpWaves];                   // row best per wavefront
-    __shared__ uint32_t wmaxj[kDpWaves], wmaxst[kDpWaves];
-    __shared__ uint32_t wlo[kDpWaves], whi[kDpWaves];     // alive band per wavefront
+    __shared__ int4 rec[kRecs];                       // .x = M, .y = X, .z = Y of the previous row
+    __shared__ __align__(16) uint32_t qcache[kQCache / 4];  // query byte of column qc0 + x at byte x
+    __shared__ int32_t w_tot[kDpWaves], w_alast[kDpWaves];
+    __shared__ int32_t w_max[kDpWaves];
+    __shared__ uint32_t w_j[kDpWaves], w_st[kDpWaves], w_lo[kDpWaves], w_hi[kDpWaves];
     const uint32_t p = blockIdx.x;
     if (p >= nprob) return;  // uniform per block
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -101,15 +186,26 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
     }
     const uint64_t bw64 = min((uint64_t)4 * br + 4, (uint64_t)m + 1);
     const uint32_t bw = (uint32_t)bw64;  // host guarantees bw <= kMaxW
+    // a row window reaches at most min(br, m) columns right of the previous one
+    const uint32_t padr = min(br, m) + 1;
+    const int4 negr = make_int4(kNegInf, kNegInf, kNegInf, 0);
+    for (int x = t; x < kRecs; x += kDpThreads) rec[x] = negr;
+    // Y-from of the window's first cell: its left neighbour is outside the
+    // window, so both candidates are NEG-based (src/gapped_extend.rs:318-334)
+    const uint32_t ybit_lead = (kNegInf - oe >= kNegInf - e) ? 0u : 1u;
+    const int32_t lead_y = max(kNegInf - oe, kNegInf - e);
 
     int32_t best_score = 0;
     uint32_t bi = 0, bj = 0, bst = kStM;
     uint32_t lo = 0, hi = 0;
-    uint32_t pjl = 0, pW = 0;  // previous row window (pW == 0: none)
+    uint32_t pjl = 0;
+    uint32_t qc0 = 0xFFFFFFFFu;  // first cached query column (none yet)
+    Win tw = {0, 0, 0};          // target bases of rows 32g+1 .. 32g+32
     uint64_t cells = 0;
     uint32_t rows_done = 0;
     uint8_t *tbp = tb + pr.tb_off;
     uint2 *rowp = rows + pr.row_off;
+    const uint8_t *qbytes = (const uint8_t *)qcache;
 
     uint32_t truncated = 0;
     for (uint32_t i = 0; i <= n; i++) {
@@ -126,228 +222,226 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
         const uint32_t js = jl + c0;  // first cell of this lane
         cells += W - ((i == 0 && jl == 0) ? 1 : 0);
 
+        // query columns [max(jl,1)-1, jh-1] must be cached (uniform)
+        if (jh >= 1) {
+            const uint32_t need = (jl >= 1 ? jl : 1) - 1;
+            if (qc0 == 0xFFFFFFFFu || need < qc0 || jh - 1 >= qc0 + kQCache) {
+                qc0 = need > (uint32_t)kQMargin ? need - kQMargin : 0;
+                const uint32_t c = qc0 + 16u * (uint32_t)t;
+                if (c < m) {
+                    const Win w = dir_win(pr.Q, pr.qdir > 0 ? pr.q0 + c : pr.q0 - c, pr.qdir);
+                    uint32_t wd[4];
+#pragma unroll
+                    for (int x = 0; x < 4; x++) {
+                        uint32_t v = 0;
+#pragma unroll
+                        for (int y = 0; y < 4; y++) {
+                            const int k = 4 * x + y;
+                            const uint32_t code = (uint32_t)(w.codes >> (2 * k)) & 3u;
+                            const uint32_t inv = ((w.valid >> k) & 1u) ^ 1u;
+                            v |= (8u * (code | (inv << 2))) << (8 * y);
+                        }
+                        wd[x] = v;
+                    }
+                    *(uint4 *)&qcache[4 * t] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+                }
+                __syncthreads();
+            }
+        }
         // target base of row i (T[i-1]); uniform
         uint32_t tc = 0, tv = 0;
         if (i >= 1) {
-            const uint64_t tp = pr.t0 + (int64_t)pr.tdir * (int64_t)(i - 1);
-            tc = base_code(pr.T, tp);
-            tv = base_valid(pr.T, tp);
+            const uint32_t r = (i - 1) & 31u;
+            if (r == 0) tw = dir_win(pr.T, pr.tdir > 0 ? pr.t0 + (i - 1) : pr.t0 - (i - 1), pr.tdir);
+            tc = (uint32_t)(tw.codes >> (2 * r)) & 3u;
+            tv = (tw.valid >> r) & 1u;
         }
-        // query bases Q[j-1] for j in [js-1, js+cnt): window from bq
-        const uint32_t bq = js >= 2 ? js - 2 : 0;
-        Win qw = {0, 0, 0};
-        if (cnt > 0 && bq < m) {
-            if (pr.qdir > 0) {
-                qw = load_win(pr.Q, pr.q0 + bq);
-            } else {
-                const uint64_t pp = pr.q0 - bq;
-                if (pp >= 31) {
-                    const Win x = load_win(pr.Q, pp - 31);
-                    qw.codes = rev2(x.codes);
-                    qw.valid = __builtin_bitreverse32(x.valid);
-                } else {
-                    const Win x = load_win(pr.Q, 0);
-                    const uint32_t sh = 31 - (uint32_t)pp;
-                    qw.codes = rev2(x.codes) >> (2 * sh);
-                    qw.valid = __builtin_bitreverse32(x.valid) >> sh;
-                }
-            }
-        }
-        auto prev = [&](uint32_t j, int32_t &vm, int32_t &vx, int32_t &vy) {
-            const uint32_t r = j - pjl;  // wraps when j < pjl
-            if (i >= 1 && j >= pjl && r < pW) {
-                vm = lM[r];
-                vx = lX[r];
-                vy = lY[r];
-            } else {
-                vm = vx = vy = kNegInf;
-            }
-        };
-        // M of cell j from the previous row's diagonal (qm, qx, qy) = row i-1
-        // at j-1 (src/gapped_extend.rs:339-363)
-        auto mcell = [&](uint32_t j, int32_t qm, int32_t qx, int32_t qy, uint32_t *arg) -> int32_t {
-            if (i >= 1 && j >= 1) {
-                int32_t d;
-                if (qm >= qx && qm >= qy) { d = qm; *arg = kStM; }
-                else if (qx >= qy) { d = qx; *arg = kStX; }
-                else { d = qy; *arg = kStY; }
-                const uint32_t qi = j - 1 - bq;
-                const uint32_t qc = (uint32_t)(qw.codes >> (2 * qi)) & 3u;
-                const uint32_t qvb = (qw.valid >> qi) & 1u;
-                return d + sc(tc, qc, tv & qvb);
-            }
-            *arg = kStM;
-            return (i == 0 && j == 0) ? 0 : kNegInf;  // the seed cell, src/gapped_extend.rs:245
-        };
+        const auto rs = row_scores(sc, tc, tv);
 
-        // ---- previous row, X and M --------------------------------------------
+        // ---- X and M from the previous row ------------------------------------
         int32_t Mv[kCMax], Xv[kCMax], Yv[kCMax];
         uint32_t nib[kCMax];
-        // M at js-1: the previous lane's last cell (NEG before the window)
-        int32_t firstPrevM = kNegInf;
-        int32_t qm = kNegInf, qx = kNegInf, qy = kNegInf;  // row i-1 at j-1 for the next cell
-        if (cnt > 0 && t > 0) {
-            int32_t dm, dx, dy;
-            prev(js - 2, dm, dx, dy);
-            uint32_t arg;
-            firstPrevM = mcell(js - 1, dm, dx, dy, &arg);
-            prev(js - 1, qm, qx, qy);
-        }
+        if (i >= 1) {
+            int4 P[kCMax + 1];  // previous row at j = js - 1 + k
+            const int32_t r0 = (int32_t)js - (int32_t)pjl;  // record index of j = js - 1
 #pragma unroll
-        for (int k = 0; k < kCMax; k++) {
-            Mv[k] = kNegInf;
-            Xv[k] = kNegInf;
-            Yv[k] = kNegInf;
-            nib[k] = 0;
-            if ((uint32_t)k < cnt) {
-                const uint32_t j = js + k;
-                int32_t am, ax, ay;
-                prev(j, am, ax, ay);
-                uint32_t nb = 0;
-                if (i >= 1) {  // X (src/gapped_extend.rs:293-312)
-                    const int32_t fm = am - oe, fx = ax - e;
-                    Xv[k] = fm >= fx ? fm : fx;
-                    nb |= (fm >= fx ? 0u : 1u) << 2;
-                }
-                uint32_t arg;
-                Mv[k] = mcell(j, qm, qx, qy, &arg);
-                if (i >= 1 && j >= 1) nb |= arg;
-                nib[k] = nb;
-                qm = am;
-                qx = ax;
-                qy = ay;
-            }
-        }
-
-        // ---- Y as a max-plus prefix scan ----------------------------------------
-        const uint32_t j0s = jl >= 1 ? jl : 1;
-        int32_t tot = INT32_MIN;
-        {
-            int32_t pm = firstPrevM;
+            for (int k = 0; k <= kCMax; k++)
+                if ((uint32_t)k <= C) P[k] = rec[min(max(r0 + k, 0), kRecs - 1)];
 #pragma unroll
             for (int k = 0; k < kCMax; k++) {
-                if ((uint32_t)k < cnt) {
+                if ((uint32_t)k < C) {
                     const uint32_t j = js + k;
-                    if (j >= j0s) tot = max(tot, pm - oe + (int32_t)(j - jl) * e);
-                    pm = Mv[k];
-                }
-            }
-        }
-        int32_t incl = tot;  // inclusive max-scan within the wavefront
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int32_t v = shfl_up_i32(incl, d);
-            if (lane >= d) incl = max(incl, v);
-        }
-        if (lane == 63) wtot[wv] = incl;
-        __syncthreads();  // (1) wavefront totals of the Y scan
-        int32_t carry = shfl_up_i32(incl, 1);
-        if (lane == 0) carry = INT32_MIN;
-        for (int v = 0; v < wv; v++) carry = max(carry, wtot[v]);
-        const int32_t zinit = kNegInf + ((int32_t)j0s - 1 - (int32_t)jl) * e;
-        carry = max(carry, zinit);
-        {
-            int32_t z = carry;
-            int32_t pm = firstPrevM;
-            // Y at js-1 from the carry: Y[js-1] = Z[js-1] - (js-1-jl)*e
-            int32_t py = kNegInf;
-            if (t > 0 && js - 1 >= j0s) py = carry - ((int32_t)(js - 1 - jl)) * e;
-#pragma unroll
-            for (int k = 0; k < kCMax; k++) {
-                if ((uint32_t)k < cnt) {
-                    const uint32_t j = js + k;
-                    if (j >= j0s) {
-                        z = max(z, pm - oe + (int32_t)(j - jl) * e);
-                        const int32_t y = z - (int32_t)(j - jl) * e;
-                        Yv[k] = y;
-                        // from-state (src/gapped_extend.rs:318-334)
-                        nib[k] |= ((pm - oe >= py - e) ? 0u : 1u) << 3;
-                        py = y;
+                    const uint32_t jq = min(j, jh);  // cells past the window: any cached column
+                    const uint32_t qb = qbytes[jq >= 1 ? jq - 1 - qc0 : 0];
+                    const int4 a = P[k + 1], d = P[k];
+                    // X (src/gapped_extend.rs:293-312)
+                    const int32_t xm = a.x - oe, xx = a.y - e;
+                    Xv[k] = max(xm, xx);
+                    uint32_t nb = (xm >= xx ? 0u : 1u) << 2;
+                    // M (src/gapped_extend.rs:339-363)
+                    const int32_t dmax = max(max(d.x, d.y), d.z);
+                    const uint32_t arg = d.x == dmax ? kStM : (d.y == dmax ? kStX : kStY);
+                    int32_t mv = dmax + rs(qb);
+                    if (k == 0 && js == 0) {  // column 0 has no M
+                        mv = kNegInf;
                     } else {
-                        py = kNegInf;
+                        nb |= arg;
                     }
-                    pm = Mv[k];
+                    Mv[k] = mv;
+                    nib[k] = nb;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kCMax; k++) {
+                Xv[k] = kNegInf;  // row 0: no X; M only at the seed cell (src/gapped_extend.rs:245)
+                Mv[k] = (k == 0 && js == 0) ? 0 : kNegInf;
+                nib[k] = 0;
+            }
+        }
+
+        // ---- Y as a max-plus prefix scan --------------------------------------
+        // Z[j] = Y[j] + (j-jl)*E = max(Z[j-1], a[j]),  a[j] = M[j-1] - (O+E) + (j-jl)*E;
+        // the lane's cell k contributes a[js+k+1].
+        const int32_t cterm = (int32_t)(js + 1 - jl) * e - oe;
+        int32_t av[kCMax];
+        int32_t tot = INT32_MIN, alast = INT32_MIN;
+#pragma unroll
+        for (int k = 0; k < kCMax; k++) {
+            if ((uint32_t)k < C) {
+                av[k] = Mv[k] + cterm + k * e;
+                if ((uint32_t)k < cnt) {
+                    tot = max(tot, av[k]);
+                    alast = av[k];
+                }
+            }
+        }
+        const int32_t incl = wave_scan_max(tot);
+        int32_t excl = wave_shr1(incl, INT32_MIN);
+        int32_t aprev = wave_shr1(alast, INT32_MIN);
+        if (lane == 63) {
+            w_tot[wv] = incl;
+            w_alast[wv] = alast;
+        }
+        __syncthreads();  // (1) wavefront totals of the Y scan
+        for (int v = 0; v < wv; v++) excl = max(excl, w_tot[v]);
+        if (lane == 0 && wv > 0) aprev = w_alast[wv - 1];
+        // Z at the window's first cell: for jl >= 1 both of Y[jl]'s candidates
+        // come from outside the window; for jl = 0, Y[0] is not computed
+        const int32_t zinit = jl >= 1 ? lead_y : kNegInf;
+        {
+            int32_t z = max(excl, zinit);
+#pragma unroll
+            for (int k = 0; k < kCMax; k++) {
+                if ((uint32_t)k < C) {
+                    const uint32_t j = js + k;
+                    uint32_t yb;
+                    if (k == 0) {
+                        yb = t == 0 ? ybit_lead : (z != aprev ? 1u : 0u);
+                    } else {
+                        const int32_t an = av[k - 1];
+                        yb = an < z ? 1u : 0u;
+                        z = max(z, an);
+                    }
+                    int32_t y = z - (int32_t)(j - jl) * e;
+                    if (j == 0) {  // column 0 has no Y
+                        y = kNegInf;
+                        yb = 0;
+                    }
+                    Yv[k] = y;
+                    nib[k] |= yb << 3;
                 }
             }
         }
 
-        // ---- row best (first strict maximum in j order) ---------------------
-        int32_t rmax = INT32_MIN;
-        uint32_t rj = 0, rst = kStM;
+        // ---- row best (first strict maximum in j order) -------------------------
         int32_t cb[kCMax];
+        int32_t rmax = INT32_MIN;
+        int rk = 0;
 #pragma unroll
         for (int k = 0; k < kCMax; k++) {
-            cb[k] = max(max(Mv[k], Xv[k]), Yv[k]);
-            if ((uint32_t)k < cnt && cb[k] > rmax) {
-                rmax = cb[k];
-                rj = js + k;
-                rst = Mv[k] == cb[k] ? kStM : (Xv[k] == cb[k] ? kStX : kStY);
+            if ((uint32_t)k < C) {
+                cb[k] = max(max(Mv[k], Xv[k]), Yv[k]);
+                if ((uint32_t)k < cnt && cb[k] > rmax) {
+                    rmax = cb[k];
+                    rk = k;
+                }
             }
         }
-        int32_t wmax = rmax;
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) wmax = max(wmax, __shfl_xor(wmax, d));
         {
+            const int32_t wmax = __builtin_amdgcn_readlane(wave_scan_max(rmax), 63);
             const unsigned long long hasmax = __ballot(cnt > 0 && rmax == wmax);
-            const int src = hasmax ? __ffsll((long long)hasmax) - 1 : 0;
-            const uint32_t wj = __shfl(rj, src), wst = __shfl(rst, src);
-            if (lane == 0) {
-                wmaxv[wv] = hasmax ? wmax : INT32_MIN;
-                wmaxj[wv] = wj;
-                wmaxst[wv] = wst;
+            if (hasmax) {
+                const int src = __ffsll((long long)hasmax) - 1;
+                if (lane == src) {
+                    uint32_t st = kStY;
+#pragma unroll
+                    for (int k = kCMax - 1; k >= 0; k--) {
+                        if ((uint32_t)k < C && k == rk) st = Mv[k] == rmax ? kStM : (Xv[k] == rmax ? kStX : kStY);
+                    }
+                    w_max[wv] = wmax;
+                    w_j[wv] = js + rk;
+                    w_st[wv] = st;
+                }
+            } else if (lane == 0) {
+                w_max[wv] = INT32_MIN;
             }
         }
         __syncthreads();  // (2) row best per wavefront
-        int32_t gmax = INT32_MIN;
-        uint32_t gj = 0, gst = kStM;
+        {
+            int32_t gmax = INT32_MIN;
+            uint32_t gw = 0;
 #pragma unroll
-        for (int v = 0; v < kDpWaves; v++) {
-            if (wmaxv[v] > gmax) {  // strict: the earliest wavefront holding the maximum
-                gmax = wmaxv[v];
-                gj = wmaxj[v];
-                gst = wmaxst[v];
+            for (int v = 0; v < kDpWaves; v++) {
+                const int32_t x = w_max[v];
+                if (x > gmax) {  // strict: the earliest wavefront holding the maximum
+                    gmax = x;
+                    gw = v;
+                }
             }
-        }
-        if (gmax > best_score) {  // src/gapped_extend.rs:365-380
-            best_score = gmax;
-            bi = i;
-            bj = gj;
-            bst = gst;
+            if (gmax > best_score) {  // src/gapped_extend.rs:365-380
+                best_score = gmax;
+                bi = i;
+                bj = w_j[gw];
+                bst = w_st[gw];
+            }
         }
         const int32_t thr = sat_sub(best_score, ydrop);
 
-        // ---- alive band, the new row into LDS, traceback ---------------------
+        // ---- alive band, the new row into LDS, traceback -----------------------
         uint32_t alo = 0xFFFFFFFFu, ahi = 0;
+        uint32_t word = 0;
 #pragma unroll
         for (int k = 0; k < kCMax; k++) {
-            if ((uint32_t)k < cnt) {
+            if ((uint32_t)k < C && (uint32_t)k < cnt) {
                 const uint32_t j = js + k;
-                if (cb[k] >= thr) {
+                int4 r = make_int4(Mv[k], Xv[k], Yv[k], 0);
+                if (cb[k] >= thr) {  // src/gapped_extend.rs:388-406
                     alo = min(alo, j);
                     ahi = max(ahi, j);
                 } else {
-                    Mv[k] = Xv[k] = Yv[k] = kNegInf;
+                    r = negr;
                 }
-                lM[c0 + k] = Mv[k];
-                lX[c0 + k] = Xv[k];
-                lY[c0 + k] = Yv[k];
+                rec[1 + c0 + k] = r;
+                word |= nib[k] << (4 * k);
             }
+        }
+        {
+            const uint32_t clear_end = min((uint32_t)kRecs, 1 + W + padr);
+            for (uint32_t x = 1 + W + (uint32_t)t; x < clear_end; x += kDpThreads) rec[x] = negr;
         }
         {
             const unsigned long long anyalive = __ballot(alo != 0xFFFFFFFFu);
             const int fl = anyalive ? __ffsll((long long)anyalive) - 1 : 0;
             const int ll = anyalive ? 63 - __clzll((long long)anyalive) : 0;
-            const uint32_t wl = __shfl(alo, fl), wh = __shfl(ahi, ll);
+            const uint32_t wl = __builtin_amdgcn_readlane(alo, fl);
+            const uint32_t wh = __builtin_amdgcn_readlane(ahi, ll);
             if (lane == 0) {
-                wlo[wv] = anyalive ? wl : 0xFFFFFFFFu;
-                whi[wv] = wh;
+                w_lo[wv] = anyalive ? wl : 0xFFFFFFFFu;
+                w_hi[wv] = wh;
             }
         }
-        // traceback nibbles: the lane's cells in one 32-bit word
-        uint32_t word = 0;
-#pragma unroll
-        for (int k = 0; k < kCMax; k++) word |= nib[k] << (4 * k);
-        *(uint32_t *)(tbp + (uint64_t)i * kRowBytes + (uint64_t)t * 4) = word;
+        if (cnt > 0) *(uint32_t *)(tbp + (uint64_t)i * kRowBytes + (uint64_t)t * 4) = word;
         if (t == 0) rowp[i] = make_uint2(jl, jh);
         rows_done = i + 1;
         __syncthreads();  // (3) alive band; the new row is in LDS
@@ -355,9 +449,10 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
         bool alive = false;
 #pragma unroll
         for (int v = 0; v < kDpWaves; v++) {
-            if (wlo[v] != 0xFFFFFFFFu) {
-                if (!alive) nlo = wlo[v];
-                nhi = whi[v];
+            const uint32_t l = w_lo[v];
+            if (l != 0xFFFFFFFFu) {
+                if (!alive) nlo = l;
+                nhi = w_hi[v];
                 alive = true;
             }
         }
@@ -365,7 +460,6 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
         lo = nlo;
         hi = nhi;
         pjl = jl;
-        pW = W;
     }
     if (t == 0) {
         out.score = best_score;
