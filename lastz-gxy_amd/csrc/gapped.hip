@@ -40,6 +40,10 @@ constexpr int kRowBytes = kDpThreads * 4;   // traceback bytes per row: one word
 constexpr int kRecs = kMaxW + 2;            // previous-row records in LDS: [0] NEG, [1, 1+W) the row, NEG after
 constexpr int kQCache = 4096;               // query columns cached in LDS, one byte each
 constexpr int kQMargin = 512;               // cached columns kept left of the window on a refill
+constexpr int kChunkRows = 64;              // traceback rows per pool chunk
+constexpr uint64_t kChunkBytes = (uint64_t)kChunkRows * kRowBytes;
+constexpr uint32_t kNoChunk = 0xFFFFFFFFu;
+constexpr int kTileRows = 8;                // traceback rows staged in LDS by the walk
 
 enum : uint32_t { kStM = 0, kStX = 1, kStY = 2 };
 
@@ -49,18 +53,27 @@ struct DpProb {
     uint64_t t0, q0;   // position of T[0] / Q[0]
     int32_t tdir, qdir;
     uint32_t n, m;
-    uint64_t tb_off;   // byte offset of row 0's traceback
-    uint64_t row_off;  // index of row 0's window record
-    uint32_t rows_cap; // traceback rows reserved; the sweep stops (truncated) there
-    uint32_t pad;
 };
 
 struct DpRes {
     int32_t score;
     uint32_t bi, bj, bstate;
-    uint32_t rows;      // rows swept
-    uint32_t truncated; // 1 when the sweep hit rows_cap before ending
+    uint32_t rows;       // rows swept
+    uint32_t truncated;  // 1 when the traceback pool ran out before the sweep ended
+    uint32_t last_chunk; // pool chunk holding the last swept row
+    uint32_t pad;
     uint64_t cells;
+};
+
+// Traceback store shared by the blocks of one launch: 64-row chunks handed out
+// in order by an atomic counter; each chunk records the one before it in its
+// side's sweep, so a walk from the last row back to row 0 follows `prev`.
+struct DpPool {
+    uint8_t *tb;       // kChunkBytes per chunk: one 32-bit word per lane per row
+    uint2 *win;        // (jl, jh) per row, kChunkRows per chunk
+    uint32_t *prev;    // previous chunk of the same side (kNoChunk for the first)
+    uint32_t *next;    // allocation counter
+    uint32_t nchunks;
 };
 
 // Inclusive prefix max over the 64 lanes of a wavefront: shifts of 1, 2, 4, 8
@@ -157,21 +170,22 @@ __device__ __forceinline__ RowScoreI32 row_scores(const ScorerI32 &s, uint32_t t
 //     through LDS; the new row overwrites the old records.
 // Three barriers per row.
 template <class S>
-__global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restrict__ probs, uint32_t nprob, S sc,
-                                                           int32_t oe, int32_t e, int32_t ydrop,
-                                                           uint8_t *__restrict__ tb, uint2 *__restrict__ rows,
+__global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restrict__ probs,
+                                                           const uint32_t *__restrict__ list, uint32_t nprob, S sc,
+                                                           int32_t oe, int32_t e, int32_t ydrop, DpPool pool,
                                                            DpRes *__restrict__ res) {
     __shared__ int4 rec[kRecs];                       // .x = M, .y = X, .z = Y of the previous row
     __shared__ __align__(16) uint32_t qcache[kQCache / 4];  // query byte of column qc0 + x at byte x
     __shared__ int32_t w_tot[kDpWaves], w_alast[kDpWaves];
     __shared__ int32_t w_max[kDpWaves];
     __shared__ uint32_t w_j[kDpWaves], w_st[kDpWaves], w_lo[kDpWaves], w_hi[kDpWaves];
+    __shared__ uint32_t s_chunk;
     const uint32_t p = blockIdx.x;
     if (p >= nprob) return;  // uniform per block
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const DpProb pr = probs[p];
+    const DpProb pr = probs[list[p]];
     const uint32_t n = pr.n, m = pr.m;
-    DpRes out = {0, 0, 0, kStM, 0, 0, 0};
+    DpRes out = {0, 0, 0, kStM, 0, 0, kNoChunk, 0, 0};
     if (n == 0 || m == 0) {  // src/gapped_extend.rs:188-195
         if (t == 0) res[p] = out;
         return;
@@ -203,15 +217,17 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
     Win tw = {0, 0, 0};          // target bases of rows 32g+1 .. 32g+32
     uint64_t cells = 0;
     uint32_t rows_done = 0;
-    uint8_t *tbp = tb + pr.tb_off;
-    uint2 *rowp = rows + pr.row_off;
+    uint32_t chunk = kNoChunk;
     const uint8_t *qbytes = (const uint8_t *)qcache;
 
     uint32_t truncated = 0;
     for (uint32_t i = 0; i <= n; i++) {
-        if (i >= pr.rows_cap) {
-            truncated = 1;
-            break;
+        // a fresh traceback chunk every kChunkRows rows; read back after barrier (1)
+        const bool newc = (i % kChunkRows) == 0;
+        if (newc && t == 0) {
+            const uint32_t c = atomicAdd(pool.next, 1u);
+            if (c < pool.nchunks) pool.prev[c] = chunk;
+            s_chunk = c;
         }
         const uint32_t jl = lo >= br ? lo - br : 0;
         const uint32_t jh = (uint32_t)min(min((uint64_t)hi + br, (uint64_t)m), (uint64_t)jl + bw - 1);
@@ -323,7 +339,14 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
             w_tot[wv] = incl;
             w_alast[wv] = alast;
         }
-        __syncthreads();  // (1) wavefront totals of the Y scan
+        __syncthreads();  // (1) wavefront totals of the Y scan; the row's chunk
+        if (newc) {
+            chunk = s_chunk;
+            if (chunk >= pool.nchunks) {  // pool exhausted: the side is swept again later
+                truncated = 1;
+                break;
+            }
+        }
         for (int v = 0; v < wv; v++) excl = max(excl, w_tot[v]);
         if (lane == 0 && wv > 0) aprev = w_alast[wv - 1];
         // Z at the window's first cell: for jl >= 1 both of Y[jl]'s candidates
@@ -441,8 +464,9 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
                 w_hi[wv] = wh;
             }
         }
-        if (cnt > 0) *(uint32_t *)(tbp + (uint64_t)i * kRowBytes + (uint64_t)t * 4) = word;
-        if (t == 0) rowp[i] = make_uint2(jl, jh);
+        const uint32_t ir = i % kChunkRows;
+        if (cnt > 0) *(uint32_t *)(pool.tb + chunk * kChunkBytes + ir * kRowBytes + (uint32_t)t * 4) = word;
+        if (t == 0) pool.win[(uint64_t)chunk * kChunkRows + ir] = make_uint2(jl, jh);
         rows_done = i + 1;
         __syncthreads();  // (3) alive band; the new row is in LDS
         uint32_t nlo = 0xFFFFFFFFu, nhi = 0;
@@ -468,38 +492,66 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
         out.bstate = bst;
         out.rows = rows_done;
         out.truncated = truncated;
+        out.last_chunk = chunk;
         out.cells = cells;
         res[p] = out;
     }
 }
 
-__global__ void k_dp_traceback(const DpProb *__restrict__ probs, const DpRes *__restrict__ res, uint32_t nprob,
-                               const uint8_t *__restrict__ tb, const uint2 *__restrict__ rows,
-                               const uint64_t *__restrict__ run_off, gxy_edit_run *__restrict__ runs,
-                               uint32_t *__restrict__ nruns) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+// Walk back from the best cell (src/gapped_extend.rs:415-463), one wavefront
+// per side: the wavefront stages kTileRows traceback rows (whole rows, all
+// lanes' words, plus their windows) in LDS and every lane follows the same
+// path through them, so each step is an LDS read instead of a dependent global
+// load.  Runs are emitted tip-to-root, merged while the op repeats, into
+// run_off[p] (room for bi + bj runs).
+__global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ res, uint32_t nprob, DpPool pool,
+                                                     const uint64_t *__restrict__ run_off,
+                                                     gxy_edit_run *__restrict__ runs, uint32_t *__restrict__ nruns) {
+    __shared__ __align__(16) uint32_t tile[kTileRows][kDpThreads];
+    __shared__ uint2 twin[kTileRows];
+    const uint32_t p = blockIdx.x;
     if (p >= nprob) return;
-    const DpProb pr = probs[p];
+    const int lane = threadIdx.x;
     const DpRes r = res[p];
+    if (r.truncated || (r.bi == 0 && r.bj == 0) || r.rows == 0) {
+        if (lane == 0) nruns[p] = 0;
+        return;
+    }
     uint32_t i = r.bi, j = r.bj, st = r.bstate;
+    // chunk holding row bi
+    uint32_t c = r.last_chunk;
+    for (uint32_t d = (r.rows - 1) / kChunkRows; d > r.bi / kChunkRows; d--) c = pool.prev[c];
     gxy_edit_run *out = runs + run_off[p];
     uint32_t nr = 0;
     uint32_t cur_op = 0xFFFFFFFFu, cur_n = 0;
-    const uint8_t *tbp = tb + pr.tb_off;
-    const uint2 *rowp = rows + pr.row_off;
-    const uint32_t cap = r.bi + r.bj;  // runs reserved for this problem
+    const uint32_t cap = r.bi + r.bj;  // runs reserved for this side
+    int64_t tl = -1;                   // first row of the staged tile
     while (i > 0 || j > 0) {
         // A consistent traceback never leaves the swept cells; these guards
         // stop a walk rather than read outside the traceback store.
         if (i >= r.rows || (st == kStM && (i == 0 || j == 0)) || (st == kStX && i == 0) ||
             (st == kStY && j == 0) || st > kStY || nr >= cap)
             break;
-        const uint2 win = rowp[i];
+        if ((int64_t)i < tl || tl < 0) {
+            const uint32_t ntl = i / kTileRows * kTileRows;
+            // moving down into the previous chunk
+            if (tl >= 0 && (uint32_t)tl % kChunkRows == 0) c = pool.prev[c];
+            tl = ntl;
+            const uint32_t r0 = ntl % kChunkRows;
+            __syncthreads();
+#pragma unroll
+            for (int x = 0; x < kTileRows; x++) {
+                const uint4 w = *(const uint4 *)(pool.tb + c * kChunkBytes + (r0 + x) * kRowBytes + lane * 16);
+                *(uint4 *)&tile[x][lane * 4] = w;
+            }
+            if (lane < kTileRows) twin[lane] = pool.win[(uint64_t)c * kChunkRows + r0 + lane];
+            __syncthreads();
+        }
+        const uint2 win = twin[i - tl];
         if (j < win.x || j > win.y) break;
         const uint32_t W = win.y - win.x + 1, C = (W + kDpThreads - 1) / kDpThreads;
         const uint32_t idx = j - win.x, ln = idx / C, k = idx % C;
-        const uint32_t word = *(const uint32_t *)(tbp + (uint64_t)i * kRowBytes + ln * 4);
-        const uint32_t nb = (word >> (4 * k)) & 0xFu;
+        const uint32_t nb = (tile[i - tl][ln] >> (4 * k)) & 0xFu;
         uint32_t op, next;
         if (st == kStM) {
             op = GXY_OP_MATCH;
@@ -520,15 +572,32 @@ __global__ void k_dp_traceback(const DpProb *__restrict__ probs, const DpRes *__
         } else {
             if (cur_n) {
                 if (nr >= cap) break;
-                out[nr++] = {cur_op, cur_n};
+                if (lane == 0) out[nr] = {cur_op, cur_n};
+                nr++;
             }
             cur_op = op;
             cur_n = 1;
         }
         st = next;
     }
-    if (cur_n && nr < cap) out[nr++] = {cur_op, cur_n};
-    nruns[p] = nr;
+    if (cur_n && nr < cap) {
+        if (lane == 0) out[nr] = {cur_op, cur_n};
+        nr++;
+    }
+    if (lane == 0) nruns[p] = nr;
+}
+
+// Copy each side's runs from its reserved range to a packed range.
+__global__ __launch_bounds__(64) void k_pack_runs(const uint64_t *__restrict__ run_off,
+                                                  const uint64_t *__restrict__ pack_off,
+                                                  const uint32_t *__restrict__ nruns, uint32_t nprob,
+                                                  const gxy_edit_run *__restrict__ src, gxy_edit_run *__restrict__ dst) {
+    const uint32_t p = blockIdx.x;
+    if (p >= nprob) return;
+    const uint32_t nr = nruns[p];
+    const gxy_edit_run *a = src + run_off[p];
+    gxy_edit_run *b = dst + pack_off[p];
+    for (uint32_t x = threadIdx.x; x < nr; x += 64) b[x] = a[x];
 }
 
 // choose_anchor (src/anchor.rs:25-69), one lane per HSP.
@@ -571,6 +640,10 @@ __global__ void k_anchor_cols(const Blk *const *__restrict__ Ts, const Blk *cons
 }
 
 inline unsigned grid_for(uint64_t n, unsigned th) { return (unsigned)((n + th - 1) / th); }
+
+// The DP's per-row score table packs the target row and the ambiguity score
+// as signed bytes; other matrices use the int32 row table.
+bool dp_scores_i8(const gxy_scoring &m) { return scoring_is_i8(m) && m.ambig >= -128 && m.ambig <= 127; }
 
 uint32_t band_width_host(int32_t oe, int32_t e, int32_t ydrop, uint64_t m) {
     uint64_t br;
@@ -656,117 +729,154 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     GXY_LAUNCH_CHECK(ctx);
     std::vector<int32_t> col(n);
     GXY_HIP(ctx, hipMemcpyAsync(col.data(), dcol, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
 
-    // Sweep in passes.  Each pass reserves traceback rows for its problems
-    // from a fixed budget; a side whose sweep outlives its reservation is
-    // re-run in the next pass with a larger one (same result, recomputed).
-    // Completed sides are walked back at once, before the next pass reuses
-    // the traceback store.
-    constexpr uint64_t kRowBudget = 8ull << 20;  // 8 Mi rows = 8 GiB of traceback
     std::vector<DpRes> res(np);
     std::vector<std::vector<gxy_edit_run>> side(np);
-    std::vector<uint64_t> pending;
+    std::vector<uint32_t> pending;
+    uint64_t need_rows = 0;
     for (uint64_t x = 0; x < np; x++) {
         if (probs[x].n == 0 || probs[x].m == 0) {
-            res[x] = DpRes{0, 0, 0, kStM, 0, 0, 0};
+            res[x] = DpRes{0, 0, 0, kStM, 0, 0, kNoChunk, 0, 0};
         } else {
-            pending.push_back(x);
+            pending.push_back((uint32_t)x);
+            need_rows += std::min<uint64_t>((uint64_t)probs[x].n + 1, 1u << 20);
         }
     }
-    // longest sides first, so the long sweeps start early instead of forming
-    // the tail of a pass
-    std::stable_sort(pending.begin(), pending.end(),
-                     [&](uint64_t a, uint64_t b) { return probs[a].n > probs[b].n; });
     uint64_t cells = 0;
-    uint64_t cap = std::max<uint64_t>(1024, kRowBudget / std::max<uint64_t>(1, pending.size()));
-    while (!pending.empty()) {
-        // take as many pending problems as fit the budget at this cap
-        std::vector<DpProb> pass;
-        std::vector<uint64_t> ids;
-        uint64_t rows_used = 0;
-        std::vector<uint64_t> later;
-        for (uint64_t x : pending) {
-            const uint64_t want = std::min<uint64_t>((uint64_t)probs[x].n + 1, cap);
-            if (!ids.empty() && rows_used + want > kRowBudget) {
-                later.push_back(x);
-                continue;
+    if (!pending.empty()) {
+        // longest sides first, so the long sweeps start early instead of
+        // forming the tail of a pass
+        std::stable_sort(pending.begin(), pending.end(),
+                         [&](uint32_t a, uint32_t b) { return probs[a].n > probs[b].n; });
+        // The traceback pool: up to a sixth of the device (at most 48 GiB),
+        // no more than every side could use.
+        size_t free_b = 0, total_b = 0;
+        GXY_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+        const uint64_t cap_b = std::min<uint64_t>(48ull << 30, total_b / 6);
+        uint64_t want = std::min<uint64_t>(cap_b / kChunkBytes,
+                                           (need_rows + kChunkRows - 1) / kChunkRows + pending.size());
+        want = std::max<uint64_t>(want, 16);
+        uint8_t *dtb = nullptr;
+        while (!(dtb = (uint8_t *)scratch(ctx, kSlotDpTb, want * kChunkBytes)) && want > 16) want /= 2;
+        if (!dtb) return set_err(ctx, GXY_ENOMEM, "DP traceback pool allocation failed");
+        const uint64_t nchunks = std::min<uint64_t>(ctx->slot_bytes[kSlotDpTb] / kChunkBytes, 0xFFFFFFF0ull);
+        auto *dwin = (uint8_t *)scratch(ctx, kSlotDpRows, nchunks * (kChunkRows * sizeof(uint2) + 4));
+        auto *dprob = (DpProb *)scratch(ctx, kSlotDpProb, np * sizeof(DpProb));
+        if (!dwin || !dprob) return set_err(ctx, GXY_ENOMEM, "DP scratch allocation failed");
+        DpPool pool;
+        pool.tb = dtb;
+        pool.win = (uint2 *)dwin;
+        pool.prev = (uint32_t *)(dwin + nchunks * kChunkRows * sizeof(uint2));
+        pool.nchunks = (uint32_t)nchunks;
+        GXY_HIP(ctx, hipMemcpyAsync(dprob, probs.data(), np * sizeof(DpProb), hipMemcpyHostToDevice, ctx->stream));
+        // a side's estimated chunks: its full length, at most 1/32 of the pool
+        const uint64_t est_cap = std::max<uint64_t>(1, nchunks / 32);
+        auto est = [&](uint32_t x) {
+            return std::min<uint64_t>(((uint64_t)probs[x].n + kChunkRows) / kChunkRows, est_cap);
+        };
+        bool alone = false;  // run only the first pending side (progress guarantee)
+        while (!pending.empty()) {
+            // take sides in order while their estimates fit the pool
+            uint64_t npp = 0, used = 0;
+            while (npp < pending.size() && npp < (1u << 20) && (npp == 0 || used + est(pending[npp]) <= nchunks)) {
+                used += est(pending[npp]);
+                npp++;
+                if (alone) break;
             }
-            DpProb q = probs[x];
-            q.rows_cap = (uint32_t)want;
-            q.tb_off = rows_used * kRowBytes;
-            q.row_off = rows_used;
-            rows_used += want;
-            pass.push_back(q);
-            ids.push_back(x);
-        }
-        const uint64_t npp = pass.size();
-        auto *dprob = (DpProb *)scratch(ctx, kSlotDpProb, npp * sizeof(DpProb));
-        auto *dres = (DpRes *)scratch(ctx, kSlotDpOut, npp * sizeof(DpRes));
-        auto *dtb = (uint8_t *)scratch(ctx, kSlotDpTb, rows_used * kRowBytes);
-        auto *drows = (uint2 *)scratch(ctx, kSlotDpRows, rows_used * sizeof(uint2));
-        if (!dprob || !dres || !dtb || !drows)
-            return set_err(ctx, GXY_ENOMEM, "DP scratch allocation failed (%llu traceback rows)",
-                           (unsigned long long)rows_used);
-        GXY_HIP(ctx, hipMemcpyAsync(dprob, pass.data(), npp * sizeof(DpProb), hipMemcpyHostToDevice, ctx->stream));
-        {
-        KernelTimer kt_(ctx, "dp_forward");
-        if (scoring_is_i8(sc))
-            k_dp_forward<ScorerI8><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(dprob, (uint32_t)npp, make_scorer_i8(sc),
-                                                                          oe, e, gp.y_drop, dtb, drows, dres);
-        else
-            k_dp_forward<ScorerI32><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(dprob, (uint32_t)npp,
-                                                                           make_scorer_i32(sc), oe, e, gp.y_drop,
-                                                                           dtb, drows, dres);
-        }
-        GXY_LAUNCH_CHECK(ctx);
-        std::vector<DpRes> pr(npp);
-        GXY_HIP(ctx, hipMemcpyAsync(pr.data(), dres, npp * sizeof(DpRes), hipMemcpyDeviceToHost, ctx->stream));
-        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        // walk back every completed side
-        std::vector<uint64_t> roff(npp, 0);
-        uint64_t total_runs = 0;
-        for (uint64_t y = 0; y < npp; y++) {
-            roff[y] = total_runs;
-            if (!pr[y].truncated) total_runs += (uint64_t)pr[y].bi + pr[y].bj;
-            else pr[y].bi = pr[y].bj = 0;  // no walk for truncated sides
-        }
-        std::vector<uint32_t> nr(npp, 0);
-        std::vector<gxy_edit_run> sr(std::max<uint64_t>(1, total_runs));
-        if (total_runs > 0) {
-            auto *droff = (uint64_t *)scratch(ctx, kSlotMisc3, npp * 8);
-            auto *druns = (gxy_edit_run *)scratch(ctx, kSlotDpRuns, total_runs * sizeof(gxy_edit_run));
-            auto *dnr = (uint32_t *)scratch(ctx, kSlotMisc2, npp * 4);
-            if (!droff || !druns || !dnr) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-            GXY_HIP(ctx, hipMemcpyAsync(dres, pr.data(), npp * sizeof(DpRes), hipMemcpyHostToDevice, ctx->stream));
-            GXY_HIP(ctx, hipMemcpyAsync(droff, roff.data(), npp * 8, hipMemcpyHostToDevice, ctx->stream));
+            // per-pass arrays: results | run offsets | packed offsets | run counts | side list | counter
+            const size_t o_res = 0, o_roff = o_res + npp * sizeof(DpRes), o_poff = o_roff + npp * 8,
+                         o_nr = o_poff + npp * 8, o_list = o_nr + npp * 4, o_ctr = (o_list + npp * 4 + 255) & ~255ull;
+            auto *work = (uint8_t *)scratch(ctx, kSlotDpOut, o_ctr + 256);
+            if (!work) return set_err(ctx, GXY_ENOMEM, "DP scratch allocation failed");
+            auto *dres = (DpRes *)(work + o_res);
+            auto *droff = (uint64_t *)(work + o_roff);
+            auto *dpoff = (uint64_t *)(work + o_poff);
+            auto *dnr = (uint32_t *)(work + o_nr);
+            auto *dlist = (uint32_t *)(work + o_list);
+            pool.next = (uint32_t *)(work + o_ctr);
+            GXY_HIP(ctx, hipMemcpyAsync(dlist, pending.data(), npp * 4, hipMemcpyHostToDevice, ctx->stream));
+            GXY_HIP(ctx, hipMemsetAsync(pool.next, 0, 4, ctx->stream));
             {
-                KernelTimer kt_(ctx, "dp_traceback");
-                k_dp_traceback<<<grid_for(npp, 64), 64, 0, ctx->stream>>>(dprob, dres, (uint32_t)npp, dtb, drows, droff,
-                                                                          druns, dnr);
+                KernelTimer kt_(ctx, "dp_forward");
+                if (dp_scores_i8(sc))
+                    k_dp_forward<ScorerI8><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(
+                        dprob, dlist, (uint32_t)npp, make_scorer_i8(sc), oe, e, gp.y_drop, pool, dres);
+                else
+                    k_dp_forward<ScorerI32><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(
+                        dprob, dlist, (uint32_t)npp, make_scorer_i32(sc), oe, e, gp.y_drop, pool, dres);
             }
             GXY_LAUNCH_CHECK(ctx);
-            GXY_HIP(ctx, hipMemcpyAsync(sr.data(), druns, total_runs * sizeof(gxy_edit_run), hipMemcpyDeviceToHost,
-                                        ctx->stream));
-            GXY_HIP(ctx, hipMemcpyAsync(nr.data(), dnr, npp * 4, hipMemcpyDeviceToHost, ctx->stream));
+            std::vector<DpRes> pr(npp);
+            GXY_HIP(ctx, hipMemcpyAsync(pr.data(), dres, npp * sizeof(DpRes), hipMemcpyDeviceToHost, ctx->stream));
             GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        }
-        std::vector<uint64_t> next;
-        GXY_HIP(ctx, hipMemcpyAsync(pr.data(), dres, npp * sizeof(DpRes), hipMemcpyDeviceToHost, ctx->stream));
-        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        for (uint64_t y = 0; y < npp; y++) {
-            const uint64_t x = ids[y];
-            if (pr[y].truncated) {
-                next.push_back(x);
-                continue;
+            // reserve bi + bj runs per completed side and walk them back
+            std::vector<uint64_t> roff(npp, 0);
+            uint64_t total_runs = 0, done = 0;
+            for (uint64_t y = 0; y < npp; y++) {
+                roff[y] = total_runs;
+                if (!pr[y].truncated) {
+                    total_runs += (uint64_t)pr[y].bi + pr[y].bj;
+                    done++;
+                }
             }
-            cells += pr[y].cells;
-            res[x] = pr[y];
-            side[x].assign(sr.begin() + roff[y], sr.begin() + roff[y] + nr[y]);
+            std::vector<uint32_t> nr(npp, 0);
+            std::vector<uint64_t> poff(npp, 0);
+            std::vector<gxy_edit_run> sr;
+            if (total_runs > 0) {
+                auto *druns = (gxy_edit_run *)scratch(ctx, kSlotDpRuns, total_runs * sizeof(gxy_edit_run));
+                if (!druns) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+                GXY_HIP(ctx, hipMemcpyAsync(droff, roff.data(), npp * 8, hipMemcpyHostToDevice, ctx->stream));
+                {
+                    KernelTimer kt_(ctx, "dp_traceback");
+                    k_dp_traceback<<<(unsigned)npp, 64, 0, ctx->stream>>>(dres, (uint32_t)npp, pool, droff, druns,
+                                                                          dnr);
+                }
+                GXY_LAUNCH_CHECK(ctx);
+                GXY_HIP(ctx, hipMemcpyAsync(nr.data(), dnr, npp * 4, hipMemcpyDeviceToHost, ctx->stream));
+                GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                uint64_t packed = 0;
+                for (uint64_t y = 0; y < npp; y++) {
+                    poff[y] = packed;
+                    packed += nr[y];
+                }
+                if (packed > 0) {
+                    auto *dpk = (gxy_edit_run *)scratch(ctx, kSlotMisc3, packed * sizeof(gxy_edit_run));
+                    if (!dpk) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+                    GXY_HIP(ctx, hipMemcpyAsync(dpoff, poff.data(), npp * 8, hipMemcpyHostToDevice, ctx->stream));
+                    k_pack_runs<<<(unsigned)npp, 64, 0, ctx->stream>>>(droff, dpoff, dnr, (uint32_t)npp, druns, dpk);
+                    GXY_LAUNCH_CHECK(ctx);
+                    sr.resize(packed);
+                    GXY_HIP(ctx, hipMemcpyAsync(sr.data(), dpk, packed * sizeof(gxy_edit_run), hipMemcpyDeviceToHost,
+                                                ctx->stream));
+                    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                }
+            }
+            std::vector<uint32_t> next;
+            for (uint64_t y = 0; y < npp; y++) {
+                const uint32_t x = pending[y];
+                if (pr[y].truncated) {
+                    next.push_back(x);
+                    continue;
+                }
+                cells += pr[y].cells;
+                res[x] = pr[y];
+                side[x].assign(sr.begin() + poff[y], sr.begin() + poff[y] + nr[y]);
+            }
+            if (done == 0) {
+                // nothing finished: the pool is shared by too many long sweeps, or
+                // one sweep alone needs more than the pool
+                if (alone)
+                    return set_err(ctx, GXY_ENOMEM,
+                                   "a gapped extension needs more than the %llu-row traceback pool",
+                                   (unsigned long long)(nchunks * kChunkRows));
+                alone = true;
+            } else {
+                alone = false;
+            }
+            for (uint64_t y = npp; y < pending.size(); y++) next.push_back(pending[y]);
+            pending.swap(next);
         }
-        // truncated sides are retried with four times the reservation
-        if (!next.empty()) cap = std::min<uint64_t>(cap * 4, kRowBudget);
-        for (uint64_t x : later) next.push_back(x);
-        pending.swap(next);
     }
     if (cells_out) *cells_out += cells;
 
