@@ -170,7 +170,7 @@ __device__ __forceinline__ RowScoreI32 row_scores(const ScorerI32 &s, uint32_t t
 //     through LDS; the new row overwrites the old records.
 // Three barriers per row.
 template <class S>
-__global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restrict__ probs,
+__global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_dp_forward(const DpProb *__restrict__ probs,
                                                            const uint32_t *__restrict__ list, uint32_t nprob, S sc,
                                                            int32_t oe, int32_t e, int32_t ydrop, DpPool pool,
                                                            DpRes *__restrict__ res) {
@@ -320,16 +320,13 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
         // Z[j] = Y[j] + (j-jl)*E = max(Z[j-1], a[j]),  a[j] = M[j-1] - (O+E) + (j-jl)*E;
         // the lane's cell k contributes a[js+k+1].
         const int32_t cterm = (int32_t)(js + 1 - jl) * e - oe;
-        int32_t av[kCMax];
         int32_t tot = INT32_MIN, alast = INT32_MIN;
 #pragma unroll
         for (int k = 0; k < kCMax; k++) {
-            if ((uint32_t)k < C) {
-                av[k] = Mv[k] + cterm + k * e;
-                if ((uint32_t)k < cnt) {
-                    tot = max(tot, av[k]);
-                    alast = av[k];
-                }
+            if ((uint32_t)k < C && (uint32_t)k < cnt) {
+                const int32_t a = Mv[k] + cterm + k * e;
+                tot = max(tot, a);
+                alast = a;
             }
         }
         const int32_t incl = wave_scan_max(tot);
@@ -362,7 +359,7 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
                     if (k == 0) {
                         yb = t == 0 ? ybit_lead : (z != aprev ? 1u : 0u);
                     } else {
-                        const int32_t an = av[k - 1];
+                        const int32_t an = Mv[k - 1] + cterm + (k - 1) * e;
                         yb = an < z ? 1u : 0u;
                         z = max(z, an);
                     }
@@ -378,15 +375,14 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
         }
 
         // ---- row best (first strict maximum in j order) -------------------------
-        int32_t cb[kCMax];
         int32_t rmax = INT32_MIN;
         int rk = 0;
 #pragma unroll
         for (int k = 0; k < kCMax; k++) {
             if ((uint32_t)k < C) {
-                cb[k] = max(max(Mv[k], Xv[k]), Yv[k]);
-                if ((uint32_t)k < cnt && cb[k] > rmax) {
-                    rmax = cb[k];
+                const int32_t cb = max(max(Mv[k], Xv[k]), Yv[k]);
+                if ((uint32_t)k < cnt && cb > rmax) {
+                    rmax = cb;
                     rk = k;
                 }
             }
@@ -439,7 +435,7 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_forward(const DpProb *__restr
             if ((uint32_t)k < C && (uint32_t)k < cnt) {
                 const uint32_t j = js + k;
                 int4 r = make_int4(Mv[k], Xv[k], Yv[k], 0);
-                if (cb[k] >= thr) {  // src/gapped_extend.rs:388-406
+                if (max(max(Mv[k], Xv[k]), Yv[k]) >= thr) {  // src/gapped_extend.rs:388-406
                     alo = min(alo, j);
                     ahi = max(ahi, j);
                 } else {
