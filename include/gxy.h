@@ -375,6 +375,51 @@ void gxy_record_set_free(gxy_record_set *set);
 /* Free an array returned by the library (gxy_search output, texts). */
 void gxy_free(void *p);
 
+/* ---- Serving: driver::run as a request loop --------------------------------
+ * Replaces the per-query loop of src/driver.rs:219-391 (run() over one query
+ * at a time) for queries that arrive over time.  The server keeps the target
+ * chunks and their position tables resident in HBM, admits waiting queries
+ * each iteration within a query-base cap and an HBM byte budget, searches all
+ * admitted queries of an iteration together (concatenated, separated by runs
+ * of N long enough that neither a seed nor an x-drop walk crosses them) and
+ * gapped-extends their HSPs in shared device batches.  Queries that arrive
+ * while an iteration runs are admitted by the next one.  Each query's records
+ * equal those of gxy_run_dev on that query alone (query_index 0).
+ *
+ * The server owns a context on `device` and a worker thread; submit/wait may
+ * be called from any host threads. */
+typedef struct gxy_server gxy_server;
+
+typedef struct gxy_server_params {
+    uint64_t hbm_budget_bytes; /* device bytes the queries of one iteration may hold (0: 16 GiB) */
+    uint64_t max_batch_bp;     /* query bases admitted per iteration (0: 256 Mbp)                */
+    int32_t co_batch;          /* 0: one query per iteration (the plain per-query driver)          */
+    int32_t reserved;
+} gxy_server_params;
+
+typedef struct gxy_request_stats {
+    double submit_ms;          /* server clock (ms since gxy_server_create) */
+    double admit_ms;
+    double done_ms;
+    uint64_t query_bp;
+    uint64_t iteration;        /* iteration that served the query          */
+    uint64_t batch_queries;    /* queries served by that iteration         */
+    uint64_t batch_bp;
+} gxy_request_stats;
+
+int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_targets,
+                      const gxy_config *cfg, const gxy_server_params *params, gxy_server **out);
+/* Queue a query (copied; the caller's buffers may be reused on return). */
+int gxy_server_submit(gxy_server *server, const gxy_seq *query, uint64_t *ticket);
+/* Block until the ticket's query is served; hands over its records (free with
+ * gxy_record_set_free) and fills `stats` when non-null.  A ticket is waited once. */
+int gxy_server_wait(gxy_server *server, uint64_t ticket, gxy_record_set **out,
+                    gxy_request_stats *stats);
+/* Message of the last failure inside the server (or of the failed request). */
+const char *gxy_server_last_error(gxy_server *server);
+/* Stops the worker after the queued queries are served; frees everything. */
+int gxy_server_destroy(gxy_server *server);
+
 #ifdef __cplusplus
 }
 #endif

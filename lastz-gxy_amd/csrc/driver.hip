@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <atomic>
 #include <charconv>
+#include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <functional>
@@ -32,6 +33,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 
 #include "internal.h"
 
@@ -119,6 +121,220 @@ void finalize_group(Group &g) {
 
 // Host jobs (record building and group sorting) run on worker threads while
 // the calling thread keeps the device busy with the next search.
+struct DevSeqDeleter {
+    void operator()(gxy_dseq *s) const { gxy_dseq_free(s); }
+};
+struct TableDeleter {
+    void operator()(gxy_postable *t) const { gxy_postable_free(t); }
+};
+using DSeq = std::unique_ptr<gxy_dseq, DevSeqDeleter>;
+using Table = std::unique_ptr<gxy_postable, TableDeleter>;
+
+struct Chunk {
+    uint32_t parent;
+    uint32_t global_start;
+    const gxy_dseq *seq;   // the whole target or an owned slice
+    DSeq owned;
+    uint32_t primary_start_local, primary_end_local;
+};
+
+// chunk_target (src/driver.rs:156-194): targets longer than primary + halo
+// become device slices [ps - H, pe + H) with primary region [ps, pe).
+int build_chunks(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets, const gxy_config *cfg,
+                 std::vector<Chunk> *chunks) {
+    for (uint64_t ti = 0; ti < n_targets; ti++) {
+        const uint32_t t_len = (uint32_t)targets[ti]->len;
+        const uint32_t P = cfg->chunk_primary_bp, H = cfg->chunk_halo_bp;
+        if (P == 0 || (uint64_t)t_len <= (uint64_t)P + H) {
+            chunks->push_back({(uint32_t)ti, 0, targets[ti], nullptr, 0, t_len});
+            continue;
+        }
+        uint32_t ps = 0;
+        while (ps < t_len) {
+            const uint32_t pe = (uint32_t)std::min<uint64_t>((uint64_t)ps + P, t_len);
+            const uint32_t lo = ps >= H ? ps - H : 0;
+            const uint32_t hi = (uint32_t)std::min<uint64_t>((uint64_t)pe + H, t_len);
+            gxy_dseq *s;
+            int rc = gxy_dseq_slice(ctx, targets[ti], lo, hi, &s);
+            if (rc) return rc;
+            Chunk c{(uint32_t)ti, lo, s, DSeq(s), ps - lo, pe - lo};
+            chunks->push_back(std::move(c));
+            if (pe == t_len) break;
+            ps = pe;
+        }
+    }
+    return GXY_OK;
+}
+
+// Group merge (src/driver.rs:371-389) into a record set: groups ordered by
+// (target, query, strand, chunk), records in group order.
+int assemble_records(gxy_ctx *ctx, std::vector<std::unique_ptr<Group>> &groups,
+                     const std::vector<gxy_edit_run> &run_store, gxy_record_set **out) {
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const std::unique_ptr<Group> &a, const std::unique_ptr<Group> &b) {
+                         if (a->ti != b->ti) return a->ti < b->ti;
+                         if (a->qi != b->qi) return a->qi < b->qi;
+                         if (a->strand != b->strand) return a->strand < b->strand;
+                         return a->chunk < b->chunk;
+                     });
+    size_t total = 0, runs_ub = 0;
+    for (const auto &g : groups) {
+        total += g->recs.size();
+        for (const auto &r : g->recs) runs_ub += r.n_runs ? r.n_runs : 1;
+    }
+    auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
+    if (!set) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    set->records = (gxy_record *)std::malloc(std::max<size_t>(1, total) * sizeof(gxy_record));
+    set->runs = (gxy_edit_run *)std::malloc(std::max<size_t>(1, runs_ub) * sizeof(gxy_edit_run));
+    if (!set->records || !set->runs) {
+        gxy_record_set_free(set);
+        return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    }
+    size_t nrec = 0, ro = 0;
+    for (const auto &g : groups) {
+        for (const Rec &r : g->recs) {
+            gxy_record &o = set->records[nrec++];
+            o.target_index = r.ti;
+            o.query_index = r.qi;
+            o.strand = r.strand;
+            o.t_start = r.t_start;
+            o.q_start = r.q_start;
+            o.t_span = r.t_span;
+            o.q_span = r.q_span;
+            o.score = r.score;
+            o.reserved = 0;
+            o.runs_offset = ro;
+            if (r.n_runs == 0) {  // build_ungapped_record: one Match run (src/driver.rs:451-452)
+                o.n_runs = 1;
+                set->runs[ro++] = {GXY_OP_MATCH, r.t_span};
+            } else {
+                o.n_runs = r.n_runs;
+                for (uint32_t x = 0; x < r.n_runs; x++) set->runs[ro++] = run_store[r.runs_offset + x];
+            }
+        }
+    }
+    set->n = nrec;
+    set->n_runs = ro;
+    *out = set;
+    return GXY_OK;
+}
+
+// Gapped work of one (chunk, query, strand) group (src/driver.rs:245-300):
+// its primary-region HSPs in score order are taken in turn, an HSP whose seed
+// lies in an earlier record's rectangle is skipped, the rest are anchored and
+// gapped-extended.  The order decides which HSPs are skipped, so records are
+// emitted strictly in that order; extensions run ahead speculatively in
+// batches that double (8 .. 1024), and the batches of all groups go to the
+// device together.
+struct GapGroup {
+    Group *g = nullptr;
+    const gxy_dseq *tseq = nullptr;  // chunk sequence (HSP target coordinates are chunk-local)
+    const gxy_dseq *qseq = nullptr;  // query strand
+    uint32_t global_start = 0;       // chunk offset in its target
+    std::vector<gxy_hsp> kept;
+    std::vector<char> have;
+    std::vector<GappedResult> res;
+    std::vector<std::vector<gxy_edit_run>> scripts;
+    struct Cov {
+        uint32_t t_lo, t_hi, q_lo, q_hi;
+    };
+    std::vector<Cov> covered;
+    size_t pos = 0, batch = 8;
+    bool done = false;
+    // src/driver.rs:288-297 compares chunk-local HSP coordinates with global
+    // record coordinates; kept as written.
+    bool is_covered(const gxy_hsp &h) const {
+        for (const auto &cv : covered)
+            if (h.t_start >= cv.t_lo && h.t_start < cv.t_hi && h.q_start >= cv.q_lo && h.q_start < cv.q_hi)
+                return true;
+        return false;
+    }
+};
+
+// Run the gapped stage of every group to completion, one device batch per
+// round over all groups that still have HSPs to settle.
+int gapped_groups_dev(gxy_ctx *ctx, std::vector<std::unique_ptr<GapGroup>> &groups, const gxy_config *cfg,
+                      std::vector<gxy_edit_run> *run_store, gxy_run_stats *rs) {
+    for (auto &gp : groups) {
+        const size_t K = gp->kept.size();
+        gp->have.assign(K, 0);
+        gp->res.assign(K, GappedResult{false, {}});
+        gp->scripts.assign(K, {});
+        gp->pos = 0;
+        gp->batch = 8;
+        gp->done = K == 0;
+    }
+    std::vector<const gxy_dseq *> ts, qs;
+    std::vector<gxy_hsp> hs;
+    std::vector<std::pair<GapGroup *, size_t>> owner;
+    for (;;) {
+        ts.clear();
+        qs.clear();
+        hs.clear();
+        owner.clear();
+        for (auto &gp : groups) {
+            GapGroup &G = *gp;
+            if (G.done) continue;
+            const size_t K = G.kept.size();
+            // settle HSPs in order while their extensions are known
+            while (G.pos < K) {
+                if (G.is_covered(G.kept[G.pos])) {
+                    G.pos++;
+                    continue;
+                }
+                if (!G.have[G.pos]) break;
+                if (G.res[G.pos].found) {  // build_gapped_record (src/driver.rs:395-434)
+                    const gxy_alignment &a = G.res[G.pos].aln;
+                    const auto &sc = G.scripts[G.pos];
+                    Rec r{G.g->ti, G.g->qi, G.g->strand, a.t_start + G.global_start, a.q_start, a.t_len, a.q_len,
+                          a.score, (uint32_t)sc.size(), run_store->size()};
+                    run_store->insert(run_store->end(), sc.begin(), sc.end());
+                    G.covered.push_back({r.t_start, r.t_start + r.t_span, r.q_start, r.q_start + r.q_span});
+                    G.g->recs.push_back(r);
+                }
+                G.pos++;
+            }
+            if (G.pos >= K) {
+                G.done = true;
+                continue;
+            }
+            size_t taken = 0;
+            for (size_t k = G.pos; k < K && taken < G.batch; k++)
+                if (!G.have[k] && !G.is_covered(G.kept[k])) {
+                    ts.push_back(G.tseq);
+                    qs.push_back(G.qseq);
+                    hs.push_back(G.kept[k]);
+                    owner.push_back({&G, k});
+                    taken++;
+                }
+            G.batch = std::min<size_t>(G.batch * 2, 1024);
+        }
+        if (owner.empty()) break;
+        const size_t nb = owner.size();
+        std::vector<gxy_anchor> anc(nb);
+        int rc = choose_anchor_pairs_dev(ctx, ts.data(), qs.data(), hs.data(), nb, cfg->scoring, cfg->anchor_window,
+                                         anc.data());
+        if (rc) return rc;
+        std::vector<GappedItem> items(nb);
+        for (size_t x = 0; x < nb; x++) items[x] = {ts[x], qs[x], anc[x]};
+        std::vector<GappedResult> br;
+        std::vector<gxy_edit_run> runs;
+        rc = gapped_batch_dev(ctx, items, cfg->scoring, cfg->gapped, &br, &runs, &rs->dp_cells);
+        if (rc) return rc;
+        rs->dp_problems += nb;
+        for (size_t x = 0; x < nb; x++) {
+            GapGroup &G = *owner[x].first;
+            const size_t k = owner[x].second;
+            G.have[k] = 1;
+            G.res[k] = br[x];
+            if (br[x].found)
+                G.scripts[k].assign(runs.begin() + br[x].aln.runs_offset,
+                                    runs.begin() + br[x].aln.runs_offset + br[x].aln.n_runs);
+        }
+    }
+    return GXY_OK;
+}
+
 class HostJobs {
   public:
     explicit HostJobs(unsigned n) {
@@ -172,22 +388,6 @@ class HostJobs {
     bool stop_ = false;
 };
 
-struct DevSeqDeleter {
-    void operator()(gxy_dseq *s) const { gxy_dseq_free(s); }
-};
-struct TableDeleter {
-    void operator()(gxy_postable *t) const { gxy_postable_free(t); }
-};
-using DSeq = std::unique_ptr<gxy_dseq, DevSeqDeleter>;
-using Table = std::unique_ptr<gxy_postable, TableDeleter>;
-
-struct Chunk {
-    uint32_t parent;
-    uint32_t global_start;
-    const gxy_dseq *seq;   // the whole target or an owned slice
-    DSeq owned;
-    uint32_t primary_start_local, primary_end_local;
-};
 
 }  // namespace
 
@@ -300,26 +500,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
 
     // chunk_target (src/driver.rs:156-194)
     std::vector<Chunk> chunks;
-    for (uint64_t ti = 0; ti < n_targets; ti++) {
-        const uint32_t t_len = (uint32_t)targets[ti]->len;
-        const uint32_t P = cfg->chunk_primary_bp, H = cfg->chunk_halo_bp;
-        if (P == 0 || (uint64_t)t_len <= (uint64_t)P + H) {
-            chunks.push_back({(uint32_t)ti, 0, targets[ti], nullptr, 0, t_len});
-            continue;
-        }
-        uint32_t ps = 0;
-        while (ps < t_len) {
-            const uint32_t pe = (uint32_t)std::min<uint64_t>((uint64_t)ps + P, t_len);
-            const uint32_t lo = ps >= H ? ps - H : 0;
-            const uint32_t hi = (uint32_t)std::min<uint64_t>((uint64_t)pe + H, t_len);
-            gxy_dseq *s;
-            int rc = gxy_dseq_slice(ctx, targets[ti], lo, hi, &s);
-            if (rc) return rc;
-            Chunk c{(uint32_t)ti, lo, s, DSeq(s), ps - lo, pe - lo};
-            chunks.push_back(std::move(c));
-            if (pe == t_len) break;
-            ps = pe;
-        }
+    {
+        int rc = build_chunks(ctx, targets, n_targets, cfg, &chunks);
+        if (rc) return rc;
     }
     rs.chunks = chunks.size();
 
@@ -331,6 +514,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     sp.entropy_threshold = cfg->entropy_threshold;
 
     std::vector<std::unique_ptr<Group>> groups;
+    std::vector<std::unique_ptr<GapGroup>> gap_groups;
     std::vector<gxy_edit_run> run_store;  // scripts of gapped records
     std::atomic<uint64_t> ties{0};        // tie pairs counted by the host jobs
     HostJobs jobs(2);                     // joined before groups and ties are destroyed
@@ -383,134 +567,40 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                         finalize_group(g);
                     });
                 } else {
-                    std::vector<gxy_hsp> kept;  // primary-region filter (src/driver.rs:250-256)
-                    kept.reserve(hsps.size());
+                    // primary-region filter (src/driver.rs:250-256); score order (src/driver.rs:270)
+                    auto gg = std::make_unique<GapGroup>();
+                    gg->g = &g;
+                    gg->tseq = ch.seq;
+                    gg->qseq = qs;
+                    gg->global_start = ch.global_start;
+                    gg->kept.reserve(hsps.size());
                     for (const auto &h : hsps)
                         if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
-                            kept.push_back(h);
-                    std::stable_sort(kept.begin(), kept.end(),
+                            gg->kept.push_back(h);
+                    std::stable_sort(gg->kept.begin(), gg->kept.end(),
                                      [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
-                    for (size_t k = 1; k < kept.size(); k++)
-                        if (kept[k].score == kept[k - 1].score) rs.score_ties++;
-                    struct Cov {
-                        uint32_t t_lo, t_hi, q_lo, q_hi;
-                    };
-                    std::vector<Cov> covered;
-                    // src/driver.rs:288-297 compares chunk-local HSP coordinates
-                    // with global record coordinates; kept as written.
-                    auto is_covered = [&](const gxy_hsp &h) {
-                        for (const auto &cv : covered)
-                            if (h.t_start >= cv.t_lo && h.t_start < cv.t_hi && h.q_start >= cv.q_lo &&
-                                h.q_start < cv.q_hi)
-                                return true;
-                        return false;
-                    };
-                    const size_t K = kept.size();
-                    std::vector<char> have(K, 0);
-                    std::vector<GappedResult> res(K);
-                    std::vector<std::vector<gxy_edit_run>> scripts(K);
-                    size_t pos = 0, batch = 8;
-                    while (pos < K) {
-                        while (pos < K) {
-                            if (is_covered(kept[pos])) {
-                                pos++;
-                                continue;
-                            }
-                            if (!have[pos]) break;
-                            if (res[pos].found) {  // build_gapped_record (src/driver.rs:395-434)
-                                const gxy_alignment &a = res[pos].aln;
-                                Rec r{ch.parent, qi, strand, a.t_start + ch.global_start, a.q_start, a.t_len,
-                                      a.q_len, a.score, (uint32_t)scripts[pos].size(), run_store.size()};
-                                run_store.insert(run_store.end(), scripts[pos].begin(), scripts[pos].end());
-                                covered.push_back({r.t_start, r.t_start + r.t_span, r.q_start, r.q_start + r.q_span});
-                                g.recs.push_back(r);
-                            }
-                            pos++;
-                        }
-                        if (pos >= K) break;
-                        std::vector<size_t> cand;
-                        for (size_t k = pos; k < K && cand.size() < batch; k++)
-                            if (!have[k] && !is_covered(kept[k])) cand.push_back(k);
-                        std::vector<gxy_hsp> ch_h(cand.size());
-                        for (size_t x = 0; x < cand.size(); x++) ch_h[x] = kept[cand[x]];
-                        std::vector<gxy_anchor> anc(cand.size());
-                        rc = choose_anchor_dev(ctx, ch.seq, qs, ch_h.data(), ch_h.size(), cfg->scoring,
-                                               cfg->anchor_window, anc.data());
-                        if (rc) return rc;
-                        std::vector<GappedItem> items(cand.size());
-                        for (size_t x = 0; x < cand.size(); x++) items[x] = {ch.seq, qs, anc[x]};
-                        std::vector<GappedResult> br;
-                        std::vector<gxy_edit_run> runs;
-                        rc = gapped_batch_dev(ctx, items, cfg->scoring, cfg->gapped, &br, &runs, &rs.dp_cells);
-                        if (rc) return rc;
-                        rs.dp_problems += cand.size();
-                        for (size_t x = 0; x < cand.size(); x++) {
-                            const size_t k = cand[x];
-                            have[k] = 1;
-                            res[k] = br[x];
-                            if (br[x].found)
-                                scripts[k].assign(runs.begin() + br[x].aln.runs_offset,
-                                                  runs.begin() + br[x].aln.runs_offset + br[x].aln.n_runs);
-                        }
-                        batch = std::min<size_t>(batch * 2, 1024);
-                    }
-                    jobs.submit([&g] { finalize_group(g); });
+                    for (size_t k = 1; k < gg->kept.size(); k++)
+                        if (gg->kept[k].score == gg->kept[k - 1].score) rs.score_ties++;
+                    gap_groups.push_back(std::move(gg));
                 }
             }
         }
+        // the chunk's table is released here; gapped work needs only the sequences
+    }
+    if (!gap_groups.empty()) {
+        int rc = gapped_groups_dev(ctx, gap_groups, cfg, &run_store, &rs);
+        if (rc) return rc;
+        for (auto &gg : gap_groups) jobs.submit([g = gg->g] { finalize_group(*g); });
     }
     jobs.wait_all();
     rs.tie_pairs += ties.load();
     for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
 
     clk.lap(kTrRecords);
-    // group merge (src/driver.rs:371-389)
-    std::stable_sort(groups.begin(), groups.end(),
-                     [](const std::unique_ptr<Group> &a, const std::unique_ptr<Group> &b) {
-                         if (a->ti != b->ti) return a->ti < b->ti;
-                         if (a->qi != b->qi) return a->qi < b->qi;
-                         if (a->strand != b->strand) return a->strand < b->strand;
-                         return a->chunk < b->chunk;
-                     });
-    size_t total = 0, runs_ub = 0;
-    for (const auto &g : groups) {
-        total += g->recs.size();
-        for (const auto &r : g->recs) runs_ub += r.n_runs ? r.n_runs : 1;
+    {
+        int rc = assemble_records(ctx, groups, run_store, out);
+        if (rc) return rc;
     }
-    auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
-    if (!set) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
-    set->records = (gxy_record *)std::malloc(std::max<size_t>(1, total) * sizeof(gxy_record));
-    set->runs = (gxy_edit_run *)std::malloc(std::max<size_t>(1, runs_ub) * sizeof(gxy_edit_run));
-    if (!set->records || !set->runs) {
-        gxy_record_set_free(set);
-        return set_err(ctx, GXY_ENOMEM, "host allocation failed");
-    }
-    size_t nrec = 0, ro = 0;
-    for (const auto &g : groups) {
-        for (const Rec &r : g->recs) {
-            gxy_record &o = set->records[nrec++];
-            o.target_index = r.ti;
-            o.query_index = r.qi;
-            o.strand = r.strand;
-            o.t_start = r.t_start;
-            o.q_start = r.q_start;
-            o.t_span = r.t_span;
-            o.q_span = r.q_span;
-            o.score = r.score;
-            o.reserved = 0;
-            o.runs_offset = ro;
-            if (r.n_runs == 0) {  // build_ungapped_record: one Match run (src/driver.rs:451-452)
-                o.n_runs = 1;
-                set->runs[ro++] = {GXY_OP_MATCH, r.t_span};
-            } else {
-                o.n_runs = r.n_runs;
-                for (uint32_t x = 0; x < r.n_runs; x++) set->runs[ro++] = run_store[r.runs_offset + x];
-            }
-        }
-    }
-    set->n = nrec;
-    set->n_runs = ro;
-    *out = set;
     if (stats) *stats = rs;
     clk.lap(kTrMerge);
     if (ctx->trace) {
@@ -556,6 +646,391 @@ void gxy_record_set_free(gxy_record_set *set) {
     std::free(set);
 }
 
+
+}  // extern "C"
+
+// ---- Serving ----------------------------------------------------------------
+
+namespace {
+
+struct ServeReq {
+    uint64_t id = 0;
+    std::vector<uint8_t> codes, valid, mask;  // the query, PackedSeq layout
+    uint64_t len = 0;
+    double submit_ms = 0, admit_ms = 0, done_ms = 0;
+    uint64_t iteration = 0, batch_queries = 0, batch_bp = 0;
+    gxy_record_set *out = nullptr;
+    int rc = GXY_OK;
+    std::string err;
+    bool done = false;
+};
+
+// Device bytes a query holds while its iteration runs: both strands as 2-bit
+// blocks (0.5 B/base each) and as bytes (1 B/base each), their share of the
+// concatenated copies (the same again), and search scratch that scales with
+// the query (window words and counts, 12 B/window, and hit records, bounded
+// by the batched hit buffers).
+constexpr uint64_t kServeBytesPerBase = 2 * (2 * (0.5 + 1)) + 12;
+
+}  // namespace
+
+struct gxy_server {
+    gxy_ctx *ctx = nullptr;
+    std::vector<const gxy_dseq *> targets;
+    gxy_config cfg;
+    gxy_server_params prm;
+    std::vector<Chunk> chunks;
+    std::vector<Table> tables;
+    std::vector<uint32_t> strands;
+    uint32_t sep = 0;  // N bases between concatenated queries (0: no concatenation)
+    uint32_t align = 32;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServeReq *> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<ServeReq>> reqs;
+    uint64_t next_id = 1, iteration = 0;
+    bool stop = false;
+    std::string err;
+    std::thread worker;
+    std::chrono::steady_clock::time_point t0;
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+// One iteration: the queries of `batch` through search, records and the
+// gapped stage; each query's record set is left in its request.
+int serve_batch(gxy_server *S, std::vector<ServeReq *> &batch) {
+    gxy_ctx *ctx = S->ctx;
+    const gxy_config *cfg = &S->cfg;
+    struct PoolScope {
+        gxy_ctx *c;
+        explicit PoolScope(gxy_ctx *x) : c(x) { c->pool_on = true; }
+        ~PoolScope() { c->pool_on = false; }
+    } pool_scope(ctx);
+    const size_t n = batch.size();
+    const int respect = cfg->respect_masking ? 1 : 0;
+    const bool minus = cfg->strand != GXY_STRAND_PLUS;
+    // queries and their reverse complements (src/driver.rs:221)
+    std::vector<DSeq> dq(n), drc(n);
+    for (size_t i = 0; i < n; i++) {
+        gxy_seq v;
+        v.codes = batch[i]->codes.data();
+        v.valid = batch[i]->valid.data();
+        v.mask = batch[i]->mask.data();
+        v.len = batch[i]->len;
+        gxy_dseq *d;
+        int rc = gxy_dseq_upload(ctx, &v, &d);
+        if (rc) return rc;
+        dq[i].reset(d);
+        if (minus) {
+            rc = gxy_dseq_reverse_complement(ctx, d, &d);
+            if (rc) return rc;
+            drc[i].reset(d);
+        }
+    }
+    // Search inputs: one concatenation per strand (queries at block-aligned
+    // offsets, N between them), or the queries themselves.
+    const bool concat = S->sep > 0 && n > 1;
+    std::vector<uint64_t> off(n + 1, 0);
+    std::vector<DSeq> cat(2);
+    if (concat) {
+        uint64_t o = 0;
+        for (size_t i = 0; i < n; i++) {
+            off[i] = o;
+            o += batch[i]->len + S->sep;
+            o = (o + S->align - 1) / S->align * S->align;
+        }
+        off[n] = o;
+        const uint64_t total = off[n - 1] + batch[n - 1]->len;
+        if (total > 0xFFFFFFF0ull) return set_err(ctx, GXY_EINVAL, "iteration exceeds 2^32 query bases");
+        for (int sidx = 0; sidx < (minus ? 2 : 1); sidx++) {
+            gxy_dseq *c;
+            int rc = dseq_alloc(ctx, total, &c);
+            if (rc) return rc;
+            cat[sidx].reset(c);
+            GXY_HIP(ctx, hipMemsetAsync(c->d, 0, c->nblk * sizeof(Blk), ctx->stream));
+            for (size_t i = 0; i < n; i++) {
+                const gxy_dseq *src = sidx == 0 ? dq[i].get() : drc[i].get();
+                const uint64_t nb = (src->len + 31) / 32;
+                if (nb)
+                    GXY_HIP(ctx, hipMemcpyAsync(c->d + off[i] / 32, src->d, nb * sizeof(Blk),
+                                                hipMemcpyDeviceToDevice, ctx->stream));
+            }
+        }
+    }
+    struct DevBytes {
+        uint8_t *p = nullptr;
+        gxy_ctx *owner = nullptr;
+        ~DevBytes() { dev_free(owner, p); }
+    };
+    const bool byte_ok = scoring_byte_ok(cfg->scoring);
+    gxy_search_params sp;
+    sp.step = cfg->step;
+    sp.hsp = cfg->hsp;
+    sp.transitions = cfg->transitions;
+    sp.entropy_enabled = cfg->entropy_enabled;
+    sp.entropy_threshold = cfg->entropy_threshold;
+
+    std::vector<std::vector<std::unique_ptr<Group>>> rgroups(n);
+    std::vector<std::unique_ptr<GapGroup>> gap_groups;
+    std::vector<gxy_edit_run> run_store;
+    for (uint32_t strand : S->strands) {
+        const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
+        // the search inputs of this strand and their byte copies
+        std::vector<const gxy_dseq *> units;
+        if (concat)
+            units.push_back(cat[si].get());
+        else
+            for (size_t i = 0; i < n; i++) units.push_back(si == 0 ? dq[i].get() : drc[i].get());
+        std::vector<DevBytes> ub(byte_ok ? units.size() : 0);
+        for (size_t u = 0; u < ub.size(); u++) {
+            ub[u].p = (uint8_t *)dev_alloc(ctx, byte_buffer_size(units[u]->len), &ub[u].owner);
+            if (!ub[u].p) return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
+            int rc = seq_to_bytes(ctx, units[u], false, ub[u].p);
+            if (rc) return rc;
+        }
+        for (size_t c = 0; c < S->chunks.size(); c++) {
+            const Chunk &ch = S->chunks[c];
+            std::vector<std::vector<gxy_hsp>> per(n);
+            for (size_t u = 0; u < units.size(); u++) {
+                std::vector<gxy_hsp> hsps;
+                gxy_search_stats st;
+                int rc = search_dev_impl(ctx, S->tables[c].get(), units[u], &cfg->scoring, &sp, respect, &hsps, &st,
+                                         byte_ok ? ub[u].p + kBytePad : nullptr, cfg->gapped_enabled != 0);
+                if (rc) return rc;
+                if (!concat) {
+                    per[u] = std::move(hsps);
+                    continue;
+                }
+                // split by query: HSPs never reach into a separator, so each
+                // lies inside one query's range; order within a query is kept
+                for (const auto &h : hsps) {
+                    const size_t i = (size_t)(std::upper_bound(off.begin(), off.begin() + n, (uint64_t)h.q_start) -
+                                              off.begin()) - 1;
+                    gxy_hsp x = h;
+                    x.q_start = (uint32_t)(h.q_start - off[i]);
+                    if ((uint64_t)x.q_start + x.length > batch[i]->len)
+                        return set_err(ctx, GXY_EDEVICE, "HSP crosses a query separator");
+                    per[i].push_back(x);
+                }
+            }
+            for (size_t i = 0; i < n; i++) {
+                std::vector<gxy_hsp> &hsps = per[i];
+                if (hsps.empty()) continue;
+                rgroups[i].push_back(std::make_unique<Group>(Group{ch.parent, 0, strand, c, {}}));
+                Group &g = *rgroups[i].back();
+                if (!cfg->gapped_enabled) {
+                    // Vec::dedup, primary-region filter (src/driver.rs:250-256) and
+                    // build_ungapped_record (src/driver.rs:437-468)
+                    dedup_hsps(&hsps);
+                    for (const auto &h : hsps)
+                        if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
+                            g.recs.push_back(Rec{g.ti, 0, strand, h.t_start + ch.global_start, h.q_start, h.length,
+                                                 h.length, h.score, 0, 0});
+                    finalize_group(g);
+                } else {
+                    auto gg = std::make_unique<GapGroup>();
+                    gg->g = &g;
+                    gg->tseq = ch.seq;
+                    gg->qseq = si == 0 ? dq[i].get() : drc[i].get();
+                    gg->global_start = ch.global_start;
+                    for (const auto &h : hsps)
+                        if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
+                            gg->kept.push_back(h);
+                    std::stable_sort(gg->kept.begin(), gg->kept.end(),
+                                     [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
+                    gap_groups.push_back(std::move(gg));
+                }
+            }
+        }
+    }
+    if (!gap_groups.empty()) {
+        gxy_run_stats rs;
+        std::memset(&rs, 0, sizeof rs);
+        int rc = gapped_groups_dev(ctx, gap_groups, cfg, &run_store, &rs);
+        if (rc) return rc;
+        for (auto &gg : gap_groups) finalize_group(*gg->g);
+    }
+    for (size_t i = 0; i < n; i++) {
+        int rc = assemble_records(ctx, rgroups[i], run_store, &batch[i]->out);
+        if (rc) return rc;
+    }
+    return GXY_OK;
+}
+
+void serve_loop(gxy_server *S) {
+    (void)hipSetDevice(S->ctx->device);
+    const uint64_t budget = S->prm.hbm_budget_bytes ? S->prm.hbm_budget_bytes : (16ull << 30);
+    const uint64_t max_bp = S->prm.max_batch_bp ? S->prm.max_batch_bp : (256ull << 20);
+    for (;;) {
+        std::vector<ServeReq *> batch;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            if (S->queue.empty()) return;  // stopping, nothing left
+            uint64_t bp = 0, bytes = 0;
+            while (!S->queue.empty()) {
+                ServeReq *r = S->queue.front();
+                const uint64_t rb = r->len * kServeBytesPerBase;
+                if (!batch.empty() && (!S->prm.co_batch || bp + r->len > max_bp || bytes + rb > budget)) break;
+                batch.push_back(r);
+                S->queue.pop_front();
+                bp += r->len;
+                bytes += rb;
+            }
+            S->iteration++;
+            const double t = S->now_ms();
+            for (ServeReq *r : batch) {
+                r->admit_ms = t;
+                r->iteration = S->iteration;
+                r->batch_queries = batch.size();
+                r->batch_bp = bp;
+            }
+        }
+        const int rc = serve_batch(S, batch);
+        std::string msg = rc ? S->ctx->err : std::string();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            const double t = S->now_ms();
+            for (ServeReq *r : batch) {
+                if (rc) {
+                    gxy_record_set_free(r->out);
+                    r->out = nullptr;
+                    r->rc = rc;
+                    r->err = msg;
+                    S->err = msg;
+                }
+                r->done_ms = t;
+                r->done = true;
+            }
+        }
+        S->cv_done.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_targets, const gxy_config *cfg,
+                      const gxy_server_params *params, gxy_server **out) {
+    if (!targets || !n_targets || !cfg || !params || !out) return GXY_EINVAL;
+    for (uint64_t i = 0; i < n_targets; i++)
+        if (!targets[i]) return GXY_EINVAL;
+    if (cfg->chain_enabled || cfg->tweener_enabled || cfg->step < 1) return GXY_EUNSUP;
+    auto S = std::make_unique<gxy_server>();
+    int rc = gxy_ctx_create(device, &S->ctx);
+    if (rc) return rc;
+    S->targets.assign(targets, targets + n_targets);
+    S->cfg = *cfg;
+    S->prm = *params;
+    S->t0 = std::chrono::steady_clock::now();
+    auto fail = [&](int code) {
+        S->chunks.clear();
+        S->tables.clear();
+        gxy_ctx_destroy(S->ctx);
+        S->ctx = nullptr;
+        return code;
+    };
+    if (cfg->strand == GXY_STRAND_PLUS || cfg->strand == GXY_STRAND_BOTH) S->strands.push_back(GXY_STRAND_PLUS);
+    if (cfg->strand == GXY_STRAND_MINUS || cfg->strand == GXY_STRAND_BOTH) S->strands.push_back(GXY_STRAND_MINUS);
+    if (S->strands.empty()) return fail(GXY_EINVAL);
+    rc = build_chunks(S->ctx, targets, n_targets, cfg, &S->chunks);
+    if (rc) return fail(rc);
+    // resident position tables, one per chunk
+    for (const Chunk &ch : S->chunks) {
+        gxy_postable *tp;
+        rc = postable_build_impl(S->ctx, ch.seq, &cfg->pattern, cfg->step, cfg->respect_masking ? 1 : 0, &tp);
+        if (rc) return fail(rc);
+        S->tables.emplace_back(tp);
+        if (cfg->max_word_count > 0) {
+            rc = gxy_postable_prune(S->ctx, tp, cfg->max_word_count);
+            if (rc) return fail(rc);
+        }
+    }
+    // Separator between concatenated queries: an x-drop walk entering N bases
+    // loses |ambig| per column and stops once x_drop is exceeded, and no seed
+    // window contains an N; offsets keep the query windows on the step grid.
+    if (params->co_batch && cfg->scoring.ambig < 0 && cfg->hsp.x_drop >= 0) {
+        const uint64_t walk = (uint64_t)cfg->hsp.x_drop / (uint64_t)(-(int64_t)cfg->scoring.ambig) + 2;
+        S->sep = (uint32_t)std::max<uint64_t>(64, walk);
+        S->align = (uint32_t)(32 * cfg->step / std::__gcd<uint64_t>(32, cfg->step));
+    }
+    S->worker = std::thread(serve_loop, S.get());
+    *out = S.release();
+    return GXY_OK;
+}
+
+int gxy_server_submit(gxy_server *S, const gxy_seq *query, uint64_t *ticket) {
+    if (!S || !query || !ticket || (query->len && (!query->codes || !query->valid || !query->mask)))
+        return GXY_EINVAL;
+    auto r = std::make_unique<ServeReq>();
+    r->len = query->len;
+    r->codes.assign(query->codes, query->codes + (query->len + 3) / 4);
+    r->valid.assign(query->valid, query->valid + (query->len + 7) / 8);
+    r->mask.assign(query->mask, query->mask + (query->len + 7) / 8);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return GXY_EINVAL;
+        r->id = S->next_id++;
+        r->submit_ms = S->now_ms();
+        *ticket = r->id;
+        S->queue.push_back(r.get());
+        S->reqs.emplace(r->id, std::move(r));
+    }
+    S->cv_work.notify_one();
+    return GXY_OK;
+}
+
+int gxy_server_wait(gxy_server *S, uint64_t ticket, gxy_record_set **out, gxy_request_stats *stats) {
+    if (!S || !out) return GXY_EINVAL;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return GXY_EINVAL;
+    ServeReq *r = it->second.get();
+    S->cv_done.wait(lk, [&] { return r->done; });
+    if (stats) {
+        stats->submit_ms = r->submit_ms;
+        stats->admit_ms = r->admit_ms;
+        stats->done_ms = r->done_ms;
+        stats->query_bp = r->len;
+        stats->iteration = r->iteration;
+        stats->batch_queries = r->batch_queries;
+        stats->batch_bp = r->batch_bp;
+    }
+    const int rc = r->rc;
+    if (rc) S->err = r->err;
+    *out = r->out;
+    r->out = nullptr;
+    S->reqs.erase(it);
+    return rc;
+}
+
+const char *gxy_server_last_error(gxy_server *S) {
+    if (!S) return "null server";
+    std::lock_guard<std::mutex> lk(S->mu);
+    return S->err.c_str();
+}
+
+int gxy_server_destroy(gxy_server *S) {
+    if (!S) return GXY_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    for (auto &kv : S->reqs) gxy_record_set_free(kv.second->out);
+    S->reqs.clear();
+    S->tables.clear();
+    S->chunks.clear();
+    gxy_ctx_destroy(S->ctx);
+    delete S;
+    return GXY_OK;
+}
 }  // extern "C"
 
 // ---- writers ---------------------------------------------------------------

@@ -598,10 +598,13 @@ __global__ __launch_bounds__(64) void k_pack_runs(const uint64_t *__restrict__ r
 
 // choose_anchor (src/anchor.rs:25-69), one lane per HSP.
 template <class S>
-__global__ void k_choose_anchor(const gxy_hsp *__restrict__ hsps, uint64_t n, const Blk *__restrict__ T,
-                                const Blk *__restrict__ Q, S sc, uint32_t window, gxy_anchor *__restrict__ out) {
+__global__ void k_choose_anchor(const gxy_hsp *__restrict__ hsps, uint64_t n, const Blk *const *__restrict__ Ts,
+                                const Blk *const *__restrict__ Qs, S sc, uint32_t window,
+                                gxy_anchor *__restrict__ out) {
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
+    const Blk *T = Ts[x];
+    const Blk *Q = Qs[x];
     const gxy_hsp h = hsps[x];
     const uint32_t len = h.length;
     const uint32_t w = min(max(window, 1u), len);
@@ -911,29 +914,41 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     return GXY_OK;
 }
 
-int choose_anchor_dev(gxy_ctx *ctx, const gxy_dseq *t, const gxy_dseq *q, const gxy_hsp *hsps_host, uint64_t n,
-                      const gxy_scoring &sc, uint32_t window, gxy_anchor *out_host) {
+int choose_anchor_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *ts, const gxy_dseq *const *qs,
+                            const gxy_hsp *hsps_host, uint64_t n, const gxy_scoring &sc, uint32_t window,
+                            gxy_anchor *out_host) {
     if (n == 0) return GXY_OK;
+    std::vector<const Blk *> ptrs(2 * n);
     for (uint64_t k = 0; k < n; k++) {
         const gxy_hsp &h = hsps_host[k];
         if (h.length < 1) return set_err(ctx, GXY_EINVAL, "empty HSP cannot be anchored");  // src/anchor.rs:33
-        if ((uint64_t)h.t_start + h.length > t->len || (uint64_t)h.q_start + h.length > q->len)
+        if ((uint64_t)h.t_start + h.length > ts[k]->len || (uint64_t)h.q_start + h.length > qs[k]->len)
             return set_err(ctx, GXY_EINVAL, "HSP outside its sequences");
+        ptrs[k] = ts[k]->d;
+        ptrs[n + k] = qs[k]->d;
     }
     auto *dh = (gxy_hsp *)scratch(ctx, kSlotMisc2, n * sizeof(gxy_hsp));
     auto *da = (gxy_anchor *)scratch(ctx, kSlotMisc3, n * sizeof(gxy_anchor));
-    if (!dh || !da) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+    auto *dp = (const Blk **)scratch(ctx, kSlotMisc1, 2 * n * sizeof(void *));
+    if (!dh || !da || !dp) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     GXY_HIP(ctx, hipMemcpyAsync(dh, hsps_host, n * sizeof(gxy_hsp), hipMemcpyHostToDevice, ctx->stream));
+    GXY_HIP(ctx, hipMemcpyAsync(dp, ptrs.data(), 2 * n * sizeof(void *), hipMemcpyHostToDevice, ctx->stream));
     if (scoring_is_i8(sc))
-        k_choose_anchor<ScorerI8><<<grid_for(n, 128), 128, 0, ctx->stream>>>(dh, n, t->d, q->d, make_scorer_i8(sc),
+        k_choose_anchor<ScorerI8><<<grid_for(n, 128), 128, 0, ctx->stream>>>(dh, n, dp, dp + n, make_scorer_i8(sc),
                                                                              window, da);
     else
-        k_choose_anchor<ScorerI32><<<grid_for(n, 128), 128, 0, ctx->stream>>>(dh, n, t->d, q->d,
+        k_choose_anchor<ScorerI32><<<grid_for(n, 128), 128, 0, ctx->stream>>>(dh, n, dp, dp + n,
                                                                               make_scorer_i32(sc), window, da);
     GXY_LAUNCH_CHECK(ctx);
     GXY_HIP(ctx, hipMemcpyAsync(out_host, da, n * sizeof(gxy_anchor), hipMemcpyDeviceToHost, ctx->stream));
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GXY_OK;
+}
+
+int choose_anchor_dev(gxy_ctx *ctx, const gxy_dseq *t, const gxy_dseq *q, const gxy_hsp *hsps_host, uint64_t n,
+                      const gxy_scoring &sc, uint32_t window, gxy_anchor *out_host) {
+    std::vector<const gxy_dseq *> ts(n, t), qs(n, q);
+    return choose_anchor_pairs_dev(ctx, ts.data(), qs.data(), hsps_host, n, sc, window, out_host);
 }
 
 }  // namespace gxy

@@ -17,6 +17,9 @@ int postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern 
 // Byte-per-base copy of `s` (x-drop layout with target or query byte codes,
 // see common.h) into a buffer of byte_buffer_size(s->len) bytes; the sequence
 // starts at buf + kBytePad.
+// A device sequence of `len` bases (blocks uninitialized), owned by the
+// caller (gxy_dseq_free); from the block cache while ctx->pool_on.
+int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out);
 int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf);
 
 // Seed search on device sequences.  `qbytes` may hold the query's byte copy
@@ -46,6 +49,10 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                      const gxy_gapped_params &gp, std::vector<GappedResult> *results,
                      std::vector<gxy_edit_run> *runs, uint64_t *cells_out);
 
+// choose_anchor over n HSPs, HSP k on the pair (ts[k], qs[k]); one launch.
+int choose_anchor_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *ts, const gxy_dseq *const *qs,
+                            const gxy_hsp *hsps_host, uint64_t n, const gxy_scoring &sc, uint32_t window,
+                            gxy_anchor *out_host);
 int choose_anchor_dev(gxy_ctx *ctx, const gxy_dseq *t, const gxy_dseq *q, const gxy_hsp *hsps_host, uint64_t n,
                       const gxy_scoring &sc, uint32_t window, gxy_anchor *out_host);
 

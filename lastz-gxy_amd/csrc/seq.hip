@@ -283,6 +283,11 @@ __global__ void k_to_bytes(const Blk *__restrict__ src, uint64_t len, uint64_t n
     buf[i] = f;
 }
 
+
+}  // namespace
+
+namespace gxy {
+
 int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
     auto *s = new (std::nothrow) gxy_dseq();
     if (!s) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
@@ -300,7 +305,7 @@ int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
     return GXY_OK;
 }
 
-}  // namespace
+}  // namespace gxy
 
 int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf) {
     const uint64_t ndw = (byte_buffer_size(s->len) + 3) / 4;

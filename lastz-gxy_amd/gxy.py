@@ -81,6 +81,11 @@ _SIGS = {
                                  _P(T.RecordSet), _P(_vp), _P(C.c_uint64)]),
     "gxy_record_set_free": (None, [_P(T.RecordSet)]),
     "gxy_free": (None, [_vp]),
+    "gxy_server_create": (C.c_int, [C.c_int, _vp, C.c_uint64, _P(T.Config), _P(T.ServerParams), _P(_vp)]),
+    "gxy_server_submit": (C.c_int, [_vp, _P(T.Seq), _P(C.c_uint64)]),
+    "gxy_server_wait": (C.c_int, [_vp, C.c_uint64, _P(_P(T.RecordSet)), _P(T.RequestStats)]),
+    "gxy_server_last_error": (C.c_char_p, [_vp]),
+    "gxy_server_destroy": (C.c_int, [_vp]),
 }
 
 _lib = None
@@ -423,6 +428,50 @@ def run_dev(targets, queries, cfg: T.Config, ctx: Context | None = None):
                                 len(queries), C.byref(cfg), C.byref(out), C.byref(st)),
               "gxy_run_dev")
     return Records(out), st
+
+
+class Server:
+    """A serving loop over resident targets (gxy_server): submit PackedSeq
+    queries from any thread, wait for each one's records."""
+
+    def __init__(self, targets, cfg: T.Config, co_batch: bool = True, hbm_budget_bytes: int = 0,
+                 max_batch_bp: int = 0, device: int = 0):
+        self._targets = list(targets)  # DeviceSeqs must outlive the server
+        tv = (_vp * len(self._targets))(*[t._h for t in self._targets])
+        prm = T.ServerParams(hbm_budget_bytes, max_batch_bp, 1 if co_batch else 0, 0)
+        h = _vp()
+        rc = lib().gxy_server_create(device, C.addressof(tv), len(self._targets), C.byref(cfg), C.byref(prm),
+                                     C.byref(h))
+        if rc != 0:
+            raise GxyError(f"gxy_server_create failed ({rc})")
+        self._h = h
+
+    def submit(self, query: T.PackedSeq) -> int:
+        t = C.c_uint64()
+        v = query.view()
+        rc = lib().gxy_server_submit(self._h, C.byref(v), C.byref(t))
+        if rc != 0:
+            raise GxyError(f"gxy_server_submit failed ({rc})")
+        return t.value
+
+    def wait(self, ticket: int):
+        """(Records, RequestStats) of a submitted query."""
+        out = _P(T.RecordSet)()
+        st = T.RequestStats()
+        rc = lib().gxy_server_wait(self._h, ticket, C.byref(out), C.byref(st))
+        if rc != 0:
+            msg = lib().gxy_server_last_error(self._h)
+            raise GxyError(f"gxy_server_wait failed ({rc}): {msg.decode() if msg else ''}")
+        return Records(out), st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gxy_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
 
 
 def _format(fn, targets, queries, recs: Records) -> bytes:

@@ -290,3 +290,16 @@ def record_tuples(rs: RecordSet):
         out.append((r.target_index, r.query_index, r.strand, r.t_start, r.q_start, r.t_span,
                     r.q_span, r.score, cig))
     return out
+
+
+class ServerParams(C.Structure):
+    """gxy_server_params."""
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_bp", C.c_uint64),
+                ("co_batch", C.c_int32), ("reserved", C.c_int32)]
+
+
+class RequestStats(C.Structure):
+    """gxy_request_stats (server clock, ms)."""
+    _fields_ = [("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double),
+                ("query_bp", C.c_uint64), ("iteration", C.c_uint64), ("batch_queries", C.c_uint64),
+                ("batch_bp", C.c_uint64)]

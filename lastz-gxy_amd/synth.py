@@ -271,3 +271,53 @@ def chromosome_pair(n: int = 48_000_000, seed: int = 7, masked: float = 0.5, nfr
     qm0 = np.concatenate(pieces_m)
     qc, qv, qm = mutate(qc0, qv0, qm0, rng, sub, indel)
     return target, PackedSeq.from_arrays(qc, qv, qm)
+
+
+def _masked_blocks(rng, n: int, masked: float):
+    """Soft-mask flags: alternating unmasked / masked blocks with lengths
+    LogNormal(median 300, sigma 1), masked blocks scaled to the share `masked`."""
+    runs = _lognormal_runs(rng, 300.0, 1.0, 2 * n)
+    scale_m = masked / (1 - masked)
+    ends = np.cumsum(np.where(np.arange(len(runs)) % 2 == 1,
+                              np.maximum(1, (runs * scale_m).astype(np.int64)), runs))
+    ends = ends[ends < n]
+    flags = np.zeros(n + 1, np.int8)
+    flags[ends] = 1
+    return np.cumsum(flags[:n]) % 2 == 1
+
+
+def serve_workload(n_target: int = 100_000_000, n_req: int = 1000, seed: int = 11, masked: float = 0.4,
+                   lo: int = 1000, hi: int = 5_000_000, decoy: float = 0.2, rate: float = 1.0):
+    """Serving workload: a uniform target with `masked` of its bases in
+    soft-masked blocks, and n_req queries with lengths log-uniform in [lo, hi]:
+    a share `decoy` of them uniform random, the rest excerpts of the target at
+    a random start on a random strand, mutated at identity I ~ U[0.75, 0.95]
+    (substitutions 0.9(1-I), indel events 0.1(1-I)/3, Geometric mean 3).
+    Arrival times are a Poisson process of `rate` queries per time unit.
+
+    Returns (target PackedSeq, [(query PackedSeq, kind, arrival)]), kind being
+    "excerpt+", "excerpt-" or "decoy"."""
+    rng = np.random.default_rng(seed)
+    code = rng.integers(0, 4, n_target).astype(np.uint8)
+    mask = _masked_blocks(rng, n_target, masked) if masked > 0 else np.zeros(n_target, bool)
+    target = PackedSeq.from_arrays(code, np.ones(n_target, bool), mask)
+    lens = np.exp(rng.uniform(np.log(lo), np.log(hi), n_req)).astype(np.int64)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for k in range(n_req):
+        L = int(min(lens[k], n_target))
+        if rng.random() < decoy:
+            q = rng.integers(0, 4, L).astype(np.uint8)
+            kind = "decoy"
+        else:
+            st = int(rng.integers(0, n_target - L + 1))
+            src = code[st:st + L]
+            minus = bool(rng.random() < 0.5)
+            if minus:
+                src = (3 - src[::-1]).astype(np.uint8)
+            ident = rng.uniform(0.75, 0.95)
+            q, _, _, _ = _mutate_segments(src, np.array([L]), np.array([0.9 * (1 - ident)]),
+                                          np.array([0.1 * (1 - ident) / 3]), rng)
+            kind = "excerpt-" if minus else "excerpt+"
+        reqs.append((_packed(q), kind, float(arrivals[k])))
+    return target, reqs

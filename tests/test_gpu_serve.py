@@ -1,0 +1,96 @@
+"""Serving loop (gxy_server) on the GPU: every query's records, served in a mix
+of concurrently submitted queries, equal gxy_run_dev on that query alone, in
+the co-batched and the plain (one query per iteration) modes, gapped and
+ungapped, with chunked targets and both strands."""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import gxy
+import gxy_types as T
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return gxy.default_context()
+
+
+@pytest.fixture(scope="module")
+def workload():
+    # 3 Mbp target in 1 Mbp chunks (halo 50 kbp), 40 % soft-masked; 14
+    # queries of 1-60 kbp, excerpts on both strands and decoys, plus an
+    # empty-hit query and a query shorter than the seed span
+    target, reqs = synth.serve_workload(n_target=3_000_000, n_req=14, seed=5, lo=1000, hi=60_000)
+    queries = [q for q, _, _ in reqs]
+    queries.append(T.PackedSeq.from_ascii("ACGTACGTAC"))
+    queries.append(T.PackedSeq.from_ascii("N" * 500 + "ACGT" * 100))
+    return target, queries
+
+
+def cfg_for(gapped):
+    cfg = T.config_default()
+    cfg.gapped_enabled = 1 if gapped else 0
+    cfg.chunk_primary_bp = 1_000_000
+    cfg.chunk_halo_bp = 50_000
+    return cfg
+
+
+@pytest.mark.parametrize("gapped,co_batch", [(True, True), (True, False), (False, True)])
+def test_served_equals_alone(ctx, workload, gapped, co_batch):
+    target, queries = workload
+    cfg = cfg_for(gapped)
+    dt = gxy.DeviceSeq.upload(target, ctx)
+    alone = []
+    for k, q in enumerate(queries):
+        say("alone", k, len(q))
+        recs, _ = gxy.run_dev([dt], [gxy.DeviceSeq.upload(q, ctx)], cfg, ctx)
+        alone.append(recs.tuples())
+    say("serve gapped", gapped, "co_batch", co_batch)
+    srv = gxy.Server([dt], cfg, co_batch=co_batch)
+    tickets = [None] * len(queries)
+
+    def submit(idx):
+        for k in idx:
+            tickets[k] = srv.submit(queries[k])
+
+    # submit from two threads at once
+    th = [threading.Thread(target=submit, args=(range(s, len(queries), 2),)) for s in (0, 1)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    iters = set()
+    for k in range(len(queries)):
+        recs, st = srv.wait(tickets[k])
+        assert st.query_bp == len(queries[k])
+        assert st.done_ms >= st.admit_ms >= st.submit_ms
+        iters.add(st.iteration)
+        assert recs.tuples() == alone[k], f"query {k} differs served vs alone"
+    srv.close()
+    if not co_batch:
+        assert len(iters) == len(queries)  # one query per iteration
+    assert sum(len(a) for a in alone) > 0
+
+
+def test_serve_maf_identical(ctx, workload):
+    target, queries = workload
+    cfg = cfg_for(True)
+    dt = gxy.DeviceSeq.upload(target, ctx)
+    srv = gxy.Server([dt], cfg, co_batch=True, max_batch_bp=50_000)  # several iterations
+    tickets = [srv.submit(q) for q in queries[:6]]
+    for k, tk in enumerate(tickets):
+        recs, st = srv.wait(tk)
+        alone, _ = gxy.run_dev([dt], [gxy.DeviceSeq.upload(queries[k], ctx)], cfg, ctx)
+        assert (gxy.format_maf([("t", target)], [("q%d" % k, queries[k])], recs)
+                == gxy.format_maf([("t", target)], [("q%d" % k, queries[k])], alone))
+    srv.close()
