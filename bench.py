@@ -10,6 +10,16 @@ in HBM before the timed region; one step is a full driver::run over them
 
 `--workload pair30k`: the 30 kbp pair, gapped, MAF (latency-bound).
 
+`--workload serve`: the serving loop (gxy_server) on a resident uniform target
+with 40 % soft-masked blocks; queries of log-uniform length, 80 % mutated
+excerpts (identity 75-95 %, either strand) and 20 % random decoys, full seed ->
+HSP -> gapped per query, arriving as a Poisson stream.  The plain per-query
+driver's capacity is measured first (all queries submitted at once, one query
+at a time); arrivals then run at fractions of it (--serve-loads) through the
+co-batched server and through the plain one.  `value` is the co-batched
+server's throughput (query Mbp/s) at the highest load; each run reports p50 /
+p99 latency (arrival -> records).
+
 `--workload dp`: batched gapped DP alone — anchored pairs with extension
 lengths LogNormal(median 2 kbp, sigma 1) clipped to 100 bp-50 kbp at 70-95 %
 identity, HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000, every pair in one
@@ -153,12 +163,102 @@ def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3):
     return out
 
 
+def serve_leg(args):
+    """The serving workload; returns the bench line."""
+    import threading
+    import numpy as np
+    import gxy
+    import gxy_types as T
+    import synth
+
+    t0 = time.time()
+    target, reqs = synth.serve_workload(n_target=args.serve_target_bp, n_req=args.serve_requests, seed=11,
+                                        lo=1000, hi=args.serve_max_len)
+    queries = [q for q, _, _ in reqs]
+    lens = np.array([len(q) for q in queries], np.float64)
+    log(f"serve: target {len(target)} bp, {len(queries)} queries, {int(lens.sum())} bp "
+        f"(generated in {time.time() - t0:.1f}s)")
+    ctx = gxy.default_context()
+    dt = gxy.DeviceSeq.upload(target, ctx)
+    ctx.synchronize()
+    cfg = T.config_default()
+
+    def run(co_batch, arrivals):
+        srv = gxy.Server([dt], cfg, co_batch=co_batch)
+        tickets = [None] * len(queries)
+        start = time.perf_counter()
+
+        def submitter():
+            for k, q in enumerate(queries):
+                if arrivals is not None:
+                    dt_s = arrivals[k] - (time.perf_counter() - start)
+                    if dt_s > 0:
+                        time.sleep(dt_s)
+                tickets[k] = srv.submit(q)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        lat, sub, done, nrec = [], [], [], 0
+        for k in range(len(queries)):
+            recs, st = srv.wait(tickets[k])
+            nrec += len(recs)
+            lat.append(st.done_ms - st.submit_ms)
+            sub.append(st.submit_ms)
+            done.append(st.done_ms)
+        srv.close()
+        span_s = (max(done) - min(sub)) / 1e3
+        lat = np.array(lat)
+        return {"throughput_mbps": round(lens.sum() / 1e6 / span_s, 4), "span_s": round(span_s, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
+                "mean_ms": round(float(lat.mean()), 1), "records": nrec}
+
+    # capacity of the plain per-query driver: everything queued at once
+    cap = run(False, None)
+    log(f"serve: plain capacity {cap}")
+    rate_cap = cap["throughput_mbps"] * 1e6 / lens.mean()  # queries per second
+    runs = {"plain_closed_loop": cap}
+    scap = run(True, None)
+    log(f"serve: served capacity {scap}")
+    runs["served_closed_loop"] = scap
+    rng = np.random.default_rng(12)
+    gaps = rng.exponential(1.0, len(queries))
+    best = None
+    for f in args.serve_loads:
+        lam = f * rate_cap
+        arrivals = np.cumsum(gaps) / lam
+        for co in (True, False):
+            r = run(co, arrivals)
+            r["load"] = f
+            r["qps"] = round(lam, 4)
+            key = f"{'served' if co else 'plain'}_load{f:g}"
+            runs[key] = r
+            log(f"serve: {key} {r}")
+            if co:
+                best = r
+    wl = {"workload": f"serving: {len(target)} bp uniform target, 40% soft-masked; {len(queries)} queries, "
+                      f"length log-uniform 1 kbp-{args.serve_max_len} bp, 80% excerpts (identity 75-95%, "
+                      f"both strands) / 20% decoys, full seed->HSP->gapped, Poisson arrivals",
+          "target_bp": len(target), "queries": len(queries), "query_bp": int(lens.sum())}
+    return {"metric": METRIC, "value": best["throughput_mbps"] if best else None, "unit": "query-Mbp/s",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(best["span_s"] * 1e3, 1) if best else None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic", "config": wl, "p50_ms": best["p50_ms"] if best else None,
+            "p99_ms": best["p99_ms"] if best else None,
+            "capacity_mbps": {"served": scap["throughput_mbps"], "plain": cap["throughput_mbps"]},
+            "runs": runs, "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k", "dp"])
+    ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k", "dp", "serve"])
+    ap.add_argument("--serve-target-bp", type=int, default=100_000_000)
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-max-len", type=int, default=1_000_000)
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.8])
     ap.add_argument("--dp-anchors", type=int, default=50_000)
     ap.add_argument("--dp-cpu-anchors", type=int, default=400)
     ap.add_argument("--no-dp", action="store_true", help="omit the batched_dp section of the chrom line")
@@ -175,6 +275,10 @@ def main():
     import gxy
     import gxy_types as T
     import synth
+
+    if args.workload == "serve":
+        print(json.dumps(serve_leg(args)), flush=True)
+        return
 
     if args.workload == "dp":
         d = dp_leg(args.dp_anchors, args.steps, args.warmup, 0 if args.no_cpu else args.dp_cpu_anchors)

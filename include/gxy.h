@@ -382,9 +382,12 @@ void gxy_free(void *p);
  * each iteration within a query-base cap and an HBM byte budget, searches all
  * admitted queries of an iteration together (concatenated, separated by runs
  * of N long enough that neither a seed nor an x-drop walk crosses them) and
- * gapped-extends their HSPs in shared device batches.  Queries that arrive
- * while an iteration runs are admitted by the next one.  Each query's records
- * equal those of gxy_run_dev on that query alone (query_index 0).
+ * hands their HSPs to gapped-stage workers, each on its own device stream,
+ * which take the waiting queries of one length bin (< 10 kbp, < 100 kbp,
+ * < 1 Mbp, longer) and extend their HSPs in shared device batches.
+ * Queries that arrive while an iteration runs are admitted by the next one;
+ * a query completes as soon as its own extensions are settled.  Each query's
+ * records equal those of gxy_run_dev on that query alone (query_index 0).
  *
  * The server owns a context on `device` and a worker thread; submit/wait may
  * be called from any host threads. */
@@ -394,7 +397,7 @@ typedef struct gxy_server_params {
     uint64_t hbm_budget_bytes; /* device bytes the queries of one iteration may hold (0: 16 GiB) */
     uint64_t max_batch_bp;     /* query bases admitted per iteration (0: 256 Mbp)                */
     int32_t co_batch;          /* 0: one query per iteration (the plain per-query driver)          */
-    int32_t reserved;
+    int32_t dp_workers;        /* gapped-stage threads, one device stream each (0: 4)              */
 } gxy_server_params;
 
 typedef struct gxy_request_stats {

@@ -363,6 +363,7 @@ struct gxy_ctx {
     // hipMalloc/hipFree, so repeated runs of the same shapes allocate nothing.
     // Reuse is ordered by the context's single stream.
     bool pool_on = false;
+    uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(48 GiB, device / 6))
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_used;
     // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.

@@ -16,7 +16,9 @@
 // host — skip if covered by what has been emitted so far, else emit the
 // precomputed result — until an HSP without a result is reached.  K doubles
 // each round, so a unit with few surviving HSPs costs few DPs and one with
-// many costs few rounds.
+// many costs few rounds; an HSP within a few thousand diagonals of one already
+// in the batch waits for that extension, since a long alignment covers most
+// HSPs along it.  All units' batches go to the device together.
 //
 // Canonical order where the reference's is unspecified: equal scores keep
 // (diagonal, t_start) order (sort_unstable_by_key(Reverse(score)),
@@ -224,8 +226,11 @@ int assemble_records(gxy_ctx *ctx, std::vector<std::unique_ptr<Group>> &groups,
 // lies in an earlier record's rectangle is skipped, the rest are anchored and
 // gapped-extended.  The order decides which HSPs are skipped, so records are
 // emitted strictly in that order; extensions run ahead speculatively in
-// batches that double (8 .. 1024), and the batches of all groups go to the
-// device together.
+// batches that double (8 .. 1024), skipping HSPs near the diagonal of one
+// already in the batch, and the batches of all groups go to the device
+// together.
+constexpr int64_t kSpecDiag = 4096;
+
 struct GapGroup {
     Group *g = nullptr;
     const gxy_dseq *tseq = nullptr;  // chunk sequence (HSP target coordinates are chunk-local)
@@ -254,7 +259,8 @@ struct GapGroup {
 // Run the gapped stage of every group to completion, one device batch per
 // round over all groups that still have HSPs to settle.
 int gapped_groups_dev(gxy_ctx *ctx, std::vector<std::unique_ptr<GapGroup>> &groups, const gxy_config *cfg,
-                      std::vector<gxy_edit_run> *run_store, gxy_run_stats *rs) {
+                      std::vector<gxy_edit_run> *run_store, gxy_run_stats *rs,
+                      const std::function<void()> &after_round = {}) {
     for (auto &gp : groups) {
         const size_t K = gp->kept.size();
         gp->have.assign(K, 0);
@@ -298,17 +304,28 @@ int gapped_groups_dev(gxy_ctx *ctx, std::vector<std::unique_ptr<GapGroup>> &grou
                 G.done = true;
                 continue;
             }
-            size_t taken = 0;
-            for (size_t k = G.pos; k < K && taken < G.batch; k++)
-                if (!G.have[k] && !G.is_covered(G.kept[k])) {
-                    ts.push_back(G.tseq);
-                    qs.push_back(G.qseq);
-                    hs.push_back(G.kept[k]);
-                    owner.push_back({&G, k});
-                    taken++;
+            // Speculate on HSPs that an alignment picked this round is
+            // unlikely to cover: a candidate within kSpecDiag diagonals of one
+            // already picked waits for that alignment (the next one to settle
+            // is always picked).  Only which extensions run ahead changes.
+            const size_t first = owner.size();
+            for (size_t k = G.pos; k < K && owner.size() - first < G.batch; k++) {
+                if (G.have[k] || G.is_covered(G.kept[k])) continue;
+                const int64_t dk = (int64_t)G.kept[k].t_start - (int64_t)G.kept[k].q_start;
+                bool near = false;
+                for (size_t x = first; x < owner.size() && !near; x++) {
+                    const int64_t dx = (int64_t)hs[x].t_start - (int64_t)hs[x].q_start;
+                    near = (dk > dx ? dk - dx : dx - dk) <= kSpecDiag;
                 }
+                if (near) continue;
+                ts.push_back(G.tseq);
+                qs.push_back(G.qseq);
+                hs.push_back(G.kept[k]);
+                owner.push_back({&G, k});
+            }
             G.batch = std::min<size_t>(G.batch * 2, 1024);
         }
+        if (after_round) after_round();
         if (owner.empty()) break;
         const size_t nb = owner.size();
         std::vector<gxy_anchor> anc(nb);
@@ -663,19 +680,24 @@ struct ServeReq {
     int rc = GXY_OK;
     std::string err;
     bool done = false;
+    // between search and completion
+    DSeq dq, drc;  // query strands (freed by the search thread: its block cache)
+    std::vector<std::unique_ptr<Group>> groups;
+    std::vector<std::unique_ptr<GapGroup>> gaps;
 };
 
-// Device bytes a query holds while its iteration runs: both strands as 2-bit
+// Device bytes a query holds while it is searched: both strands as 2-bit
 // blocks (0.5 B/base each) and as bytes (1 B/base each), their share of the
 // concatenated copies (the same again), and search scratch that scales with
-// the query (window words and counts, 12 B/window, and hit records, bounded
-// by the batched hit buffers).
+// the query (window words and counts, 12 B/window; hit records are bounded by
+// the batched hit buffers).
 constexpr uint64_t kServeBytesPerBase = 2 * (2 * (0.5 + 1)) + 12;
 
 }  // namespace
 
 struct gxy_server {
-    gxy_ctx *ctx = nullptr;
+    gxy_ctx *ctx = nullptr;               // search stage
+    std::vector<gxy_ctx *> dp_ctx;        // gapped stage, one per worker
     std::vector<const gxy_dseq *> targets;
     gxy_config cfg;
     gxy_server_params prm;
@@ -685,13 +707,16 @@ struct gxy_server {
     uint32_t sep = 0;  // N bases between concatenated queries (0: no concatenation)
     uint32_t align = 32;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<ServeReq *> queue;
+    std::condition_variable cv_work, cv_dp, cv_done;
+    std::deque<ServeReq *> queue;   // waiting for search
+    std::deque<ServeReq *> dpq;     // waiting for the gapped stage
+    std::vector<DSeq> tofree;       // sequences of completed queries
     std::unordered_map<uint64_t, std::unique_ptr<ServeReq>> reqs;
     uint64_t next_id = 1, iteration = 0;
-    bool stop = false;
+    bool stop = false, search_done = false;
     std::string err;
     std::thread worker;
+    std::vector<std::thread> dp_workers;
     std::chrono::steady_clock::time_point t0;
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -700,21 +725,42 @@ struct gxy_server {
 
 namespace {
 
-// One iteration: the queries of `batch` through search, records and the
-// gapped stage; each query's record set is left in its request.
-int serve_batch(gxy_server *S, std::vector<ServeReq *> &batch) {
+// Records of a query whose groups are all settled; call with S->mu free.
+void complete(gxy_server *S, gxy_ctx *ctx, ServeReq *r, const std::vector<gxy_edit_run> &run_store, int rc) {
+    if (!rc) {
+        for (auto &gg : r->gaps) finalize_group(*gg->g);
+        rc = assemble_records(ctx, r->groups, run_store, &r->out);
+    }
+    std::string msg = rc ? ctx->err : std::string();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (rc) {
+            gxy_record_set_free(r->out);
+            r->out = nullptr;
+            r->rc = rc;
+            r->err = msg;
+            S->err = msg;
+        }
+        r->gaps.clear();
+        r->groups.clear();
+        S->tofree.push_back(std::move(r->dq));
+        S->tofree.push_back(std::move(r->drc));
+        r->done_ms = S->now_ms();
+        r->done = true;
+    }
+    S->cv_done.notify_all();
+    S->cv_work.notify_one();
+}
+
+// Search stage of one iteration: the queries of `batch` through search and
+// the ungapped records or the gapped groups of each query.
+int serve_search(gxy_server *S, std::vector<ServeReq *> &batch) {
     gxy_ctx *ctx = S->ctx;
     const gxy_config *cfg = &S->cfg;
-    struct PoolScope {
-        gxy_ctx *c;
-        explicit PoolScope(gxy_ctx *x) : c(x) { c->pool_on = true; }
-        ~PoolScope() { c->pool_on = false; }
-    } pool_scope(ctx);
     const size_t n = batch.size();
     const int respect = cfg->respect_masking ? 1 : 0;
     const bool minus = cfg->strand != GXY_STRAND_PLUS;
     // queries and their reverse complements (src/driver.rs:221)
-    std::vector<DSeq> dq(n), drc(n);
     for (size_t i = 0; i < n; i++) {
         gxy_seq v;
         v.codes = batch[i]->codes.data();
@@ -724,11 +770,11 @@ int serve_batch(gxy_server *S, std::vector<ServeReq *> &batch) {
         gxy_dseq *d;
         int rc = gxy_dseq_upload(ctx, &v, &d);
         if (rc) return rc;
-        dq[i].reset(d);
+        batch[i]->dq.reset(d);
         if (minus) {
             rc = gxy_dseq_reverse_complement(ctx, d, &d);
             if (rc) return rc;
-            drc[i].reset(d);
+            batch[i]->drc.reset(d);
         }
     }
     // Search inputs: one concatenation per strand (queries at block-aligned
@@ -753,7 +799,7 @@ int serve_batch(gxy_server *S, std::vector<ServeReq *> &batch) {
             cat[sidx].reset(c);
             GXY_HIP(ctx, hipMemsetAsync(c->d, 0, c->nblk * sizeof(Blk), ctx->stream));
             for (size_t i = 0; i < n; i++) {
-                const gxy_dseq *src = sidx == 0 ? dq[i].get() : drc[i].get();
+                const gxy_dseq *src = sidx == 0 ? batch[i]->dq.get() : batch[i]->drc.get();
                 const uint64_t nb = (src->len + 31) / 32;
                 if (nb)
                     GXY_HIP(ctx, hipMemcpyAsync(c->d + off[i] / 32, src->d, nb * sizeof(Blk),
@@ -774,17 +820,13 @@ int serve_batch(gxy_server *S, std::vector<ServeReq *> &batch) {
     sp.entropy_enabled = cfg->entropy_enabled;
     sp.entropy_threshold = cfg->entropy_threshold;
 
-    std::vector<std::vector<std::unique_ptr<Group>>> rgroups(n);
-    std::vector<std::unique_ptr<GapGroup>> gap_groups;
-    std::vector<gxy_edit_run> run_store;
     for (uint32_t strand : S->strands) {
         const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
-        // the search inputs of this strand and their byte copies
         std::vector<const gxy_dseq *> units;
         if (concat)
             units.push_back(cat[si].get());
         else
-            for (size_t i = 0; i < n; i++) units.push_back(si == 0 ? dq[i].get() : drc[i].get());
+            for (size_t i = 0; i < n; i++) units.push_back(si == 0 ? batch[i]->dq.get() : batch[i]->drc.get());
         std::vector<DevBytes> ub(byte_ok ? units.size() : 0);
         for (size_t u = 0; u < ub.size(); u++) {
             ub[u].p = (uint8_t *)dev_alloc(ctx, byte_buffer_size(units[u]->len), &ub[u].owner);
@@ -820,8 +862,9 @@ int serve_batch(gxy_server *S, std::vector<ServeReq *> &batch) {
             for (size_t i = 0; i < n; i++) {
                 std::vector<gxy_hsp> &hsps = per[i];
                 if (hsps.empty()) continue;
-                rgroups[i].push_back(std::make_unique<Group>(Group{ch.parent, 0, strand, c, {}}));
-                Group &g = *rgroups[i].back();
+                ServeReq *r = batch[i];
+                r->groups.push_back(std::make_unique<Group>(Group{ch.parent, 0, strand, c, {}}));
+                Group &g = *r->groups.back();
                 if (!cfg->gapped_enabled) {
                     // Vec::dedup, primary-region filter (src/driver.rs:250-256) and
                     // build_ungapped_record (src/driver.rs:437-468)
@@ -835,42 +878,37 @@ int serve_batch(gxy_server *S, std::vector<ServeReq *> &batch) {
                     auto gg = std::make_unique<GapGroup>();
                     gg->g = &g;
                     gg->tseq = ch.seq;
-                    gg->qseq = si == 0 ? dq[i].get() : drc[i].get();
+                    gg->qseq = si == 0 ? r->dq.get() : r->drc.get();
                     gg->global_start = ch.global_start;
                     for (const auto &h : hsps)
                         if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
                             gg->kept.push_back(h);
                     std::stable_sort(gg->kept.begin(), gg->kept.end(),
                                      [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
-                    gap_groups.push_back(std::move(gg));
+                    r->gaps.push_back(std::move(gg));
                 }
             }
         }
     }
-    if (!gap_groups.empty()) {
-        gxy_run_stats rs;
-        std::memset(&rs, 0, sizeof rs);
-        int rc = gapped_groups_dev(ctx, gap_groups, cfg, &run_store, &rs);
-        if (rc) return rc;
-        for (auto &gg : gap_groups) finalize_group(*gg->g);
-    }
-    for (size_t i = 0; i < n; i++) {
-        int rc = assemble_records(ctx, rgroups[i], run_store, &batch[i]->out);
-        if (rc) return rc;
-    }
     return GXY_OK;
 }
 
-void serve_loop(gxy_server *S) {
+void search_loop(gxy_server *S) {
     (void)hipSetDevice(S->ctx->device);
     const uint64_t budget = S->prm.hbm_budget_bytes ? S->prm.hbm_budget_bytes : (16ull << 30);
     const uint64_t max_bp = S->prm.max_batch_bp ? S->prm.max_batch_bp : (256ull << 20);
+    S->ctx->pool_on = true;  // this thread's sequences come from its block cache
     for (;;) {
         std::vector<ServeReq *> batch;
+        std::vector<DSeq> drop;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
-            if (S->queue.empty()) return;  // stopping, nothing left
+            S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty() || !S->tofree.empty(); });
+            drop.swap(S->tofree);
+            if (S->queue.empty() && S->stop) {
+                S->search_done = true;
+                break;
+            }
             uint64_t bp = 0, bytes = 0;
             while (!S->queue.empty()) {
                 ServeReq *r = S->queue.front();
@@ -881,7 +919,7 @@ void serve_loop(gxy_server *S) {
                 bp += r->len;
                 bytes += rb;
             }
-            S->iteration++;
+            if (!batch.empty()) S->iteration++;
             const double t = S->now_ms();
             for (ServeReq *r : batch) {
                 r->admit_ms = t;
@@ -890,24 +928,98 @@ void serve_loop(gxy_server *S) {
                 r->batch_bp = bp;
             }
         }
-        const int rc = serve_batch(S, batch);
-        std::string msg = rc ? S->ctx->err : std::string();
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            const double t = S->now_ms();
-            for (ServeReq *r : batch) {
-                if (rc) {
-                    gxy_record_set_free(r->out);
-                    r->out = nullptr;
-                    r->rc = rc;
-                    r->err = msg;
-                    S->err = msg;
-                }
-                r->done_ms = t;
-                r->done = true;
+        drop.clear();  // completed queries' sequences back to the block cache
+        if (batch.empty()) continue;
+        const int rc = serve_search(S, batch);
+        std::vector<ServeReq *> to_dp;
+        for (ServeReq *r : batch) {
+            if (rc || r->gaps.empty()) {
+                complete(S, S->ctx, r, {}, rc);
+            } else if (!S->prm.co_batch) {
+                // the plain driver: this query's gapped stage before the next query
+                std::vector<gxy_edit_run> run_store;
+                gxy_run_stats rs;
+                std::memset(&rs, 0, sizeof rs);
+                const int rc2 = gapped_groups_dev(S->ctx, r->gaps, &S->cfg, &run_store, &rs);
+                complete(S, S->ctx, r, run_store, rc2);
+            } else {
+                to_dp.push_back(r);
             }
         }
-        S->cv_done.notify_all();
+        if (!to_dp.empty()) {
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                for (ServeReq *r : to_dp) S->dpq.push_back(r);
+            }
+            S->cv_dp.notify_one();
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->tofree.clear();
+    }
+    S->cv_dp.notify_all();
+    S->ctx->pool_on = false;
+}
+
+// Gapped stage: take the waiting queries of one length bin and extend their
+// HSPs in shared rounds on this worker's stream; a query completes when its
+// own groups are settled.
+void dp_loop(gxy_server *S, gxy_ctx *ctx) {
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+        std::vector<ServeReq *> mine;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_dp.wait(lk, [&] { return !S->dpq.empty() || S->search_done; });
+            if (S->dpq.empty()) return;  // search finished and nothing left
+            if (S->prm.co_batch) {
+                // length-binned: the oldest query's bin, so a round (which lasts
+                // as long as its longest extension) holds queries of similar size
+                auto bin = [](uint64_t len) { return len < 10000 ? 0 : len < 100000 ? 1 : len < 1000000 ? 2 : 3; };
+                const int b = bin(S->dpq.front()->len);
+                for (auto it = S->dpq.begin(); it != S->dpq.end();) {
+                    if (bin((*it)->len) == b) {
+                        mine.push_back(*it);
+                        it = S->dpq.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+            } else {
+                mine.push_back(S->dpq.front());
+                S->dpq.pop_front();
+            }
+        }
+        std::vector<std::unique_ptr<GapGroup>> all;
+        std::vector<std::pair<ServeReq *, size_t>> span;  // [first, count) of each query's groups in `all`
+        for (ServeReq *r : mine) {
+            span.push_back({r, r->gaps.size()});
+            for (auto &gg : r->gaps) all.push_back(std::move(gg));
+            r->gaps.clear();
+        }
+        std::vector<gxy_edit_run> run_store;
+        std::vector<char> fin(mine.size(), 0);
+        auto settle = [&](int rc) {
+            size_t o = 0;
+            for (size_t x = 0; x < mine.size(); x++) {
+                const size_t cnt = span[x].second;
+                bool done = true;
+                for (size_t y = 0; y < cnt && done; y++) done = all[o + y]->done;
+                if (!fin[x] && (done || rc)) {
+                    // settled groups stay in `all` (done, never touched again)
+                    fin[x] = 1;
+                    if (!rc)
+                        for (size_t y = 0; y < cnt; y++) finalize_group(*all[o + y]->g);
+                    complete(S, ctx, mine[x], run_store, rc);
+                }
+                o += cnt;
+            }
+        };
+        gxy_run_stats rs;
+        std::memset(&rs, 0, sizeof rs);
+        const int rc = gapped_groups_dev(ctx, all, &S->cfg, &run_store, &rs, [&] { settle(0); });
+        settle(rc);
     }
 }
 
@@ -931,6 +1043,7 @@ int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_tar
     auto fail = [&](int code) {
         S->chunks.clear();
         S->tables.clear();
+        for (gxy_ctx *c : S->dp_ctx) gxy_ctx_destroy(c);
         gxy_ctx_destroy(S->ctx);
         S->ctx = nullptr;
         return code;
@@ -951,6 +1064,7 @@ int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_tar
             if (rc) return fail(rc);
         }
     }
+    GXY_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
     // Separator between concatenated queries: an x-drop walk entering N bases
     // loses |ambig| per column and stops once x_drop is exceeded, and no seed
     // window contains an N; offsets keep the query windows on the step grid.
@@ -959,7 +1073,19 @@ int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_tar
         S->sep = (uint32_t)std::max<uint64_t>(64, walk);
         S->align = (uint32_t)(32 * cfg->step / std::__gcd<uint64_t>(32, cfg->step));
     }
-    S->worker = std::thread(serve_loop, S.get());
+    const int nw = cfg->gapped_enabled && params->co_batch ? (params->dp_workers > 0 ? params->dp_workers : 4) : 0;
+    size_t free_b = 0, total_b = 0;
+    GXY_HIP(S->ctx, hipMemGetInfo(&free_b, &total_b));
+    for (int w = 0; w < nw; w++) {
+        gxy_ctx *c;
+        rc = gxy_ctx_create(device, &c);
+        if (rc) return fail(rc);
+        // the workers share the traceback budget of one context
+        c->dp_pool_cap = std::min<uint64_t>(48ull << 30, total_b / 6) / (uint64_t)nw;
+        S->dp_ctx.push_back(c);
+    }
+    S->worker = std::thread(search_loop, S.get());
+    for (gxy_ctx *c : S->dp_ctx) S->dp_workers.emplace_back(dp_loop, S.get(), c);
     *out = S.release();
     return GXY_OK;
 }
@@ -1023,10 +1149,15 @@ int gxy_server_destroy(gxy_server *S) {
     }
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    S->cv_dp.notify_all();
+    for (auto &t : S->dp_workers)
+        if (t.joinable()) t.join();
     for (auto &kv : S->reqs) gxy_record_set_free(kv.second->out);
     S->reqs.clear();
+    S->tofree.clear();
     S->tables.clear();
     S->chunks.clear();
+    for (gxy_ctx *c : S->dp_ctx) gxy_ctx_destroy(c);
     gxy_ctx_destroy(S->ctx);
     delete S;
     return GXY_OK;

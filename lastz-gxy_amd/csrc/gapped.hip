@@ -752,7 +752,7 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         // no more than every side could use.
         size_t free_b = 0, total_b = 0;
         GXY_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-        const uint64_t cap_b = std::min<uint64_t>(48ull << 30, total_b / 6);
+        const uint64_t cap_b = ctx->dp_pool_cap ? ctx->dp_pool_cap : std::min<uint64_t>(48ull << 30, total_b / 6);
         uint64_t want = std::min<uint64_t>(cap_b / kChunkBytes,
                                            (need_rows + kChunkRows - 1) / kChunkRows + pending.size());
         want = std::max<uint64_t>(want, 16);

@@ -435,10 +435,10 @@ class Server:
     queries from any thread, wait for each one's records."""
 
     def __init__(self, targets, cfg: T.Config, co_batch: bool = True, hbm_budget_bytes: int = 0,
-                 max_batch_bp: int = 0, device: int = 0):
+                 max_batch_bp: int = 0, dp_workers: int = 0, device: int = 0):
         self._targets = list(targets)  # DeviceSeqs must outlive the server
         tv = (_vp * len(self._targets))(*[t._h for t in self._targets])
-        prm = T.ServerParams(hbm_budget_bytes, max_batch_bp, 1 if co_batch else 0, 0)
+        prm = T.ServerParams(hbm_budget_bytes, max_batch_bp, 1 if co_batch else 0, dp_workers)
         h = _vp()
         rc = lib().gxy_server_create(device, C.addressof(tv), len(self._targets), C.byref(cfg), C.byref(prm),
                                      C.byref(h))

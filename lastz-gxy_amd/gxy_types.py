@@ -295,7 +295,7 @@ def record_tuples(rs: RecordSet):
 class ServerParams(C.Structure):
     """gxy_server_params."""
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_bp", C.c_uint64),
-                ("co_batch", C.c_int32), ("reserved", C.c_int32)]
+                ("co_batch", C.c_int32), ("dp_workers", C.c_int32)]
 
 
 class RequestStats(C.Structure):
