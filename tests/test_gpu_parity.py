@@ -335,6 +335,37 @@ def test_gapped_pairs_dev(ctx):
         assert g[k] == o, k
 
 
+def test_gapped_pairs_dev_large_pass(ctx):
+    """320 sides in one pass (several blocks per CU, longest first) equal the
+    oracle, and the same pairs a few at a time give the same alignments."""
+    rng = np.random.default_rng(23)
+    ts, qs, anc, host = [], [], [], []
+    for k in range(160):
+        L = int(rng.integers(30, 700))
+        t = rand_seq(rng, 2 * L + 1, 0.01, 0.0)
+        c, v, m = t.unpack()
+        qc, qv, qm = synth.mutate(c, v, m, rng, float(rng.uniform(0.05, 0.3)), 0.01)
+        q = T.PackedSeq.from_arrays(qc, qv, qm)
+        a = (L, min(L, len(q) - 1))
+        ts.append(gxy.DeviceSeq.upload(t, ctx))
+        qs.append(gxy.DeviceSeq.upload(q, ctx))
+        anc.append(a)
+        host.append((t, q))
+    gp = T.GappedParams(9400, 3000)
+    say("gapped 160 pairs in one pass")
+    g, gcells = gxy.gapped_extend_pairs_dev(ts, qs, anc, HOX, gp, ctx)
+    ocells = 0
+    for k in range(len(anc)):
+        o, c = O.gapped_extend(anc[k], host[k][0], host[k][1], HOX, gp)
+        ocells += c
+        assert g[k] == o, k
+    assert gcells == ocells
+    say("gapped the same pairs 8 at a time")
+    for s0 in range(0, 40, 8):
+        g2, _ = gxy.gapped_extend_pairs_dev(ts[s0:s0 + 8], qs[s0:s0 + 8], anc[s0:s0 + 8], HOX, gp, ctx)
+        assert g2 == g[s0:s0 + 8]
+
+
 # ---- driver ----------------------------------------------------------------------------
 
 def _cfg(**kw):
