@@ -286,8 +286,15 @@ typedef struct gxy_named_seq {
 #define GXY_STRAND_MINUS 1u
 #define GXY_STRAND_BOTH 2u
 
-/* driver::Config (src/driver.rs:50-89).  chain_enabled / tweener_enabled are
- * outside this library's path: run() rejects them with GXY_EUNSUP. */
+/* best_chain(&[Hsp]) -> Vec<Hsp> (src/chain.rs:19-69): the highest-scoring
+ * chain strictly increasing on both axes, ascending; `out` holds n HSPs.
+ * Host-side (O(n log n)); needs no context. */
+int gxy_best_chain(const gxy_hsp *hsps, uint64_t n, gxy_hsp *out, uint64_t *n_out);
+
+/* driver::Config (src/driver.rs:50-89).  tweener_enabled selects
+ * Config::tweener = Some(TweenerConfig { pattern: inner_pattern, min_gap:
+ * inner_min_gap, max_gap: inner_max_gap, relax_factor: inner_relax })
+ * (src/tweener.rs:25-52, --inner / --inner-seed in src/cli.rs:68-80). */
 typedef struct gxy_config {
     gxy_seed_pattern pattern;
     gxy_scoring scoring;
@@ -306,6 +313,10 @@ typedef struct gxy_config {
     uint32_t chunk_halo_bp;
     int32_t entropy_enabled;
     int32_t entropy_threshold;
+    gxy_seed_pattern inner_pattern; /* default match12 */
+    uint32_t inner_min_gap;         /* default 100     */
+    uint32_t inner_max_gap;         /* default 100000  */
+    int32_t inner_relax;            /* default 2       */
 } gxy_config;
 
 /* Record (src/output/mod.rs:17-31) without the owned strings: names and base
@@ -361,6 +372,14 @@ int gxy_run(gxy_ctx *ctx, const gxy_named_seq *targets, uint64_t n_targets,
 int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets,
                 const gxy_dseq *const *queries, uint64_t n_queries, const gxy_config *cfg,
                 gxy_record_set **out, gxy_run_stats *stats);
+/* tweener::interpolate (src/tweener.rs:59-185) over the records of one
+ * (target, query, strand) group: `target` is the whole target, `query` the
+ * group's query strand, cfg's inner_* fields the TweenerConfig (chain_enabled,
+ * transitions, entropy and max_word_count apply as in the reference).  The
+ * output is sorted by (t_start, q_start) and deduplicated; free with
+ * gxy_record_set_free. */
+int gxy_interpolate(gxy_ctx *ctx, const gxy_record_set *records, const gxy_dseq *target,
+                    const gxy_dseq *query, const gxy_config *cfg, gxy_record_set **out);
 /* Render records as MAF (src/output/maf.rs:33-74, header as src/main.rs:42)
  * or PAF (src/output/paf.rs:23-89).  `*text` is NUL-terminated; free with
  * gxy_free. */

@@ -229,6 +229,228 @@ int assemble_records(gxy_ctx *ctx, std::vector<std::unique_ptr<Group>> &groups,
 // batches that double (8 .. 1024), skipping HSPs near the diagonal of one
 // already in the batch, and the batches of all groups go to the device
 // together.
+// best_chain (src/chain.rs:19-69): the highest-scoring chain of HSPs strictly
+// increasing on both axes (h_j ends before h_i starts on t and on q).  The
+// reference's O(n^2) scan keeps, for each HSP in (t_start, q_start) order, the
+// first earlier HSP with the strictly highest chain score; the same choice is a
+// prefix-maximum query — the maximum of (chain score, -order) over HSPs whose
+// t_end <= t_start(i), keyed by q_end <= q_start(i) — answered here by a
+// Fenwick tree over q_end while HSPs are inserted in t_end order: O(n log n).
+// Zero-length HSPs (t_end == t_start) fall back to the scan, which alone
+// restricts predecessors to earlier positions in the order.
+std::vector<gxy_hsp> best_chain(const std::vector<gxy_hsp> &hsps) {
+    const size_t n = hsps.size();
+    if (n == 0) return {};
+    std::vector<gxy_hsp> h = hsps;
+    std::stable_sort(h.begin(), h.end(), [](const gxy_hsp &a, const gxy_hsp &b) {
+        return a.t_start != b.t_start ? a.t_start < b.t_start : a.q_start < b.q_start;
+    });
+    std::vector<int64_t> best(n), parent(n, -1);
+    bool zero = false;
+    for (const auto &x : h) zero |= x.length == 0;
+    if (zero || n < 64) {
+        for (size_t i = 0; i < n; i++) {
+            int64_t bps = 0, bpi = -1;
+            for (size_t j = 0; j < i; j++)
+                if ((uint64_t)h[j].t_start + h[j].length <= h[i].t_start &&
+                    (uint64_t)h[j].q_start + h[j].length <= h[i].q_start && best[j] > bps) {
+                    bps = best[j];
+                    bpi = (int64_t)j;
+                }
+            best[i] = bps + h[i].score;
+            parent[i] = bpi;
+        }
+    } else {
+        // q_end ranks for the Fenwick tree
+        std::vector<uint64_t> qe(n);
+        for (size_t i = 0; i < n; i++) qe[i] = (uint64_t)h[i].q_start + h[i].length;
+        std::vector<uint64_t> keys = qe;
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        const size_t K = keys.size();
+        // node value: (chain score, -order) as a pair compared lexicographically
+        std::vector<std::pair<int64_t, int64_t>> fw(K + 1, {INT64_MIN, 0});
+        auto upd = [&](size_t pos, std::pair<int64_t, int64_t> v) {
+            for (size_t x = pos + 1; x <= K; x += x & (0 - x)) fw[x] = std::max(fw[x], v);
+        };
+        auto qry = [&](size_t cnt) {  // max over the first cnt keys
+            std::pair<int64_t, int64_t> r{INT64_MIN, 0};
+            for (size_t x = cnt; x > 0; x -= x & (0 - x)) r = std::max(r, fw[x]);
+            return r;
+        };
+        std::vector<size_t> by_end(n);
+        for (size_t i = 0; i < n; i++) by_end[i] = i;
+        std::stable_sort(by_end.begin(), by_end.end(), [&](size_t a, size_t b) {
+            return (uint64_t)h[a].t_start + h[a].length < (uint64_t)h[b].t_start + h[b].length;
+        });
+        size_t ins = 0;
+        for (size_t i = 0; i < n; i++) {
+            // every HSP ending at or before t_start(i) precedes i in the order
+            while (ins < n && (uint64_t)h[by_end[ins]].t_start + h[by_end[ins]].length <= h[i].t_start) {
+                const size_t j = by_end[ins++];
+                if (best[j] > 0)  // only strictly positive chains ever beat the start value 0
+                    upd((size_t)(std::lower_bound(keys.begin(), keys.end(), qe[j]) - keys.begin()),
+                        {best[j], -(int64_t)j});
+            }
+            const size_t cnt = (size_t)(std::upper_bound(keys.begin(), keys.end(), (uint64_t)h[i].q_start) -
+                                        keys.begin());
+            const auto r = qry(cnt);
+            const bool has = r.first != INT64_MIN;
+            best[i] = (has ? r.first : 0) + h[i].score;
+            parent[i] = has ? -r.second : -1;
+        }
+    }
+    size_t end = 0;
+    for (size_t i = 1; i < n; i++)
+        if (best[i] > best[end]) end = i;
+    std::vector<gxy_hsp> out;
+    for (int64_t c = (int64_t)end; c >= 0; c = parent[(size_t)c]) out.push_back(h[(size_t)c]);
+    std::reverse(out.begin(), out.end());
+    return out;
+}
+
+// The tweener (src/tweener.rs:59-185) for many (target, query, strand) groups:
+// every qualifying gap between neighbouring records of every group is searched
+// again with the inner pattern (its own small table, step 1, relaxed
+// thresholds), its HSPs chained when chaining is on, and all of them — no
+// covered skip here — anchored and gapped-extended in one device batch.  Each
+// group's records are then stably sorted by (t_start, q_start) and
+// deduplicated.  `target` is the whole target, `query` the group's strand.
+struct TweenGroup {
+    Group *g;
+    const gxy_dseq *target;
+    const gxy_dseq *query;
+};
+
+int tweener_groups_dev(gxy_ctx *ctx, std::vector<TweenGroup> &groups, const gxy_config *cfg,
+                       std::vector<gxy_edit_run> *run_store, gxy_run_stats *rs) {
+    const int32_t relax = std::max(cfg->inner_relax, 1);
+    gxy_search_params sp;
+    sp.step = 1;
+    sp.hsp.x_drop = cfg->hsp.x_drop;
+    sp.hsp.hsp_threshold = std::max(cfg->hsp.hsp_threshold / relax, 100);
+    sp.transitions = cfg->transitions;
+    sp.entropy_enabled = cfg->entropy_enabled;
+    sp.entropy_threshold = cfg->entropy_threshold;
+    gxy_gapped_params gp;
+    gp.y_drop = cfg->gapped.y_drop;
+    gp.gapped_threshold = std::max(cfg->gapped.gapped_threshold / relax, 100);
+    const int respect = cfg->respect_masking ? 1 : 0;
+    const bool byte_ok = scoring_byte_ok(cfg->scoring);
+    auto by_tq = [](const Rec &a, const Rec &b) {
+        return a.t_start != b.t_start ? a.t_start < b.t_start : a.q_start < b.q_start;
+    };
+    struct Win {
+        size_t group;
+        uint32_t tg0, qg0;
+        DSeq t, q;
+        size_t first, count;  // its HSPs in the batch
+    };
+    std::vector<Win> wins;
+    std::vector<const gxy_dseq *> ts, qs;
+    std::vector<gxy_hsp> hs;
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        std::vector<Rec> &recs = groups[gi].g->recs;
+        if (recs.size() < 2) continue;
+        std::stable_sort(recs.begin(), recs.end(), by_tq);
+        for (size_t k = 0; k + 1 < recs.size(); k++) {
+            const Rec &a = recs[k], &b = recs[k + 1];
+            const uint32_t tg0 = a.t_start + a.t_span, tg1 = b.t_start;
+            const uint32_t qg0 = a.q_start + a.q_span, qg1 = b.q_start;
+            if (tg1 <= tg0 || qg1 <= qg0) continue;
+            const uint32_t tgap = tg1 - tg0, qgap = qg1 - qg0;
+            if (tgap < cfg->inner_min_gap || qgap < cfg->inner_min_gap) continue;
+            if (tgap > cfg->inner_max_gap || qgap > cfg->inner_max_gap) continue;
+            gxy_dseq *t, *q;
+            int rc = gxy_dseq_slice(ctx, groups[gi].target, tg0, tg1, &t);
+            if (rc) return rc;
+            DSeq dt(t);
+            rc = gxy_dseq_slice(ctx, groups[gi].query, qg0, qg1, &q);
+            if (rc) return rc;
+            DSeq dq(q);
+            std::vector<gxy_hsp> hsps;
+            {
+                gxy_postable *tp;
+                rc = postable_build_impl(ctx, t, &cfg->inner_pattern, 1, respect, &tp);
+                if (rc) return rc;
+                Table table(tp);
+                if (cfg->max_word_count > 0) {
+                    rc = gxy_postable_prune(ctx, tp, cfg->max_word_count);
+                    if (rc) return rc;
+                }
+                uint8_t *qb = nullptr;
+                gxy_ctx *qb_owner = nullptr;
+                if (byte_ok) {
+                    qb = (uint8_t *)dev_alloc(ctx, byte_buffer_size(q->len), &qb_owner);
+                    if (!qb) return set_err(ctx, GXY_ENOMEM, "query byte copy: allocation failed");
+                    rc = seq_to_bytes(ctx, q, false, qb);
+                }
+                gxy_search_stats st;
+                if (!rc)
+                    rc = search_dev_impl(ctx, tp, q, &cfg->scoring, &sp, respect, &hsps, &st,
+                                         qb ? qb + kBytePad : nullptr, true);
+                dev_free(qb_owner, qb);
+                if (rc) return rc;
+                rs->hits += st.hits;
+                rs->columns += st.columns;
+            }
+            if (hsps.empty()) continue;
+            if (cfg->chain_enabled) hsps = best_chain(hsps);
+            Win w{gi, tg0, qg0, std::move(dt), std::move(dq), hs.size(), hsps.size()};
+            for (const auto &h : hsps) {
+                ts.push_back(w.t.get());
+                qs.push_back(w.q.get());
+                hs.push_back(h);
+            }
+            wins.push_back(std::move(w));
+        }
+    }
+    if (!hs.empty()) {
+        const size_t nb = hs.size();
+        std::vector<gxy_anchor> anc(nb);
+        int rc = choose_anchor_pairs_dev(ctx, ts.data(), qs.data(), hs.data(), nb, cfg->scoring, cfg->anchor_window,
+                                         anc.data());
+        if (rc) return rc;
+        std::vector<GappedItem> items(nb);
+        for (size_t x = 0; x < nb; x++) items[x] = {ts[x], qs[x], anc[x]};
+        std::vector<GappedResult> br;
+        std::vector<gxy_edit_run> runs;
+        rc = gapped_batch_dev(ctx, items, cfg->scoring, gp, &br, &runs, &rs->dp_cells);
+        if (rc) return rc;
+        rs->dp_problems += nb;
+        // extras appended per group in window order, then HSP order
+        for (const Win &w : wins) {
+            Group &g = *groups[w.group].g;
+            for (size_t x = w.first; x < w.first + w.count; x++) {
+                if (!br[x].found) continue;
+                const gxy_alignment &a = br[x].aln;
+                Rec r{g.ti, g.qi, g.strand, w.tg0 + a.t_start, w.qg0 + a.q_start, a.t_len, a.q_len, a.score,
+                      a.n_runs, run_store->size()};
+                run_store->insert(run_store->end(), runs.begin() + a.runs_offset,
+                                  runs.begin() + a.runs_offset + a.n_runs);
+                g.recs.push_back(r);
+            }
+        }
+    }
+    for (auto &tg : groups) {
+        std::vector<Rec> &recs = tg.g->recs;
+        if (recs.size() < 2) continue;
+        std::stable_sort(recs.begin(), recs.end(), by_tq);
+        std::vector<Rec> out;
+        for (const Rec &r : recs) {  // dedup_by on (t_start, q_start, t_span, q_span)
+            if (!out.empty()) {
+                const Rec &p = out.back();
+                if (p.t_start == r.t_start && p.q_start == r.q_start && p.t_span == r.t_span &&
+                    p.q_span == r.q_span)
+                    continue;
+            }
+            out.push_back(r);
+        }
+        recs.swap(out);
+    }
+    return GXY_OK;
+}
+
 constexpr int64_t kSpecDiag = 4096;
 
 struct GapGroup {
@@ -429,6 +651,47 @@ void gxy_config_default(gxy_config *c) {
     c->anchor_window = 31;
     c->transitions = 1;
     c->chunk_halo_bp = 50000;
+    gxy_pattern_parse("match12", &c->inner_pattern);  // TweenerConfig::default (src/tweener.rs:44-52)
+    c->inner_min_gap = 100;
+    c->inner_max_gap = 100000;
+    c->inner_relax = 2;
+}
+
+int gxy_best_chain(const gxy_hsp *hsps, uint64_t n, gxy_hsp *out, uint64_t *n_out) {
+    if ((!hsps && n) || (!out && n) || !n_out) return GXY_EINVAL;
+    const std::vector<gxy_hsp> c = best_chain(std::vector<gxy_hsp>(hsps, hsps + n));
+    std::copy(c.begin(), c.end(), out);
+    *n_out = c.size();
+    return GXY_OK;
+}
+
+int gxy_interpolate(gxy_ctx *ctx, const gxy_record_set *records, const gxy_dseq *target, const gxy_dseq *query,
+                    const gxy_config *cfg, gxy_record_set **out) {
+    if (!ctx) return GXY_EINVAL;
+    if (!records || !target || !query || !cfg || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+    GXY_HIP(ctx, hipSetDevice(ctx->device));
+    Group g{0, 0, 0, 0, {}};
+    std::vector<gxy_edit_run> run_store;
+    for (uint64_t k = 0; k < records->n; k++) {
+        const gxy_record &r = records->records[k];
+        if (r.runs_offset + r.n_runs > records->n_runs) return set_err(ctx, GXY_EINVAL, "record runs out of range");
+        if ((uint64_t)r.t_start + r.t_span > target->len || (uint64_t)r.q_start + r.q_span > query->len)
+            return set_err(ctx, GXY_EINVAL, "record outside its sequences");
+        g.ti = r.target_index;
+        g.qi = r.query_index;
+        g.strand = r.strand;
+        g.recs.push_back(Rec{r.target_index, r.query_index, r.strand, r.t_start, r.q_start, r.t_span, r.q_span,
+                             r.score, r.n_runs, run_store.size()});
+        run_store.insert(run_store.end(), records->runs + r.runs_offset, records->runs + r.runs_offset + r.n_runs);
+    }
+    std::vector<TweenGroup> tg{{&g, target, query}};
+    gxy_run_stats rs;
+    std::memset(&rs, 0, sizeof rs);
+    int rc = tweener_groups_dev(ctx, tg, cfg, &run_store, &rs);
+    if (rc) return rc;
+    std::vector<std::unique_ptr<Group>> groups;
+    groups.push_back(std::make_unique<Group>(std::move(g)));
+    return assemble_records(ctx, groups, run_store, out);
 }
 
 int gxy_pattern_parse(const char *spec, gxy_seed_pattern *out) {
@@ -464,8 +727,6 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         if (!targets[i]) return set_err(ctx, GXY_EINVAL, "null target %llu", (unsigned long long)i);
     for (uint64_t i = 0; i < n_queries; i++)
         if (!queries[i]) return set_err(ctx, GXY_EINVAL, "null query %llu", (unsigned long long)i);
-    if (cfg->chain_enabled || cfg->tweener_enabled)
-        return set_err(ctx, GXY_EUNSUP, "chaining and the tweener are not on this library's path");
     if (cfg->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
     GXY_HIP(ctx, hipSetDevice(ctx->device));
     PhaseClock clk(ctx);
@@ -532,6 +793,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
 
     std::vector<std::unique_ptr<Group>> groups;
     std::vector<std::unique_ptr<GapGroup>> gap_groups;
+    std::vector<TweenGroup> tween_groups;
     std::vector<gxy_edit_run> run_store;  // scripts of gapped records
     std::atomic<uint64_t> ties{0};        // tie pairs counted by the host jobs
     HostJobs jobs(2);                     // joined before groups and ties are destroyed
@@ -570,18 +832,25 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 if (hsps.empty()) continue;
                 groups.push_back(std::make_unique<Group>(Group{ch.parent, qi, strand, c, {}}));
                 Group &g = *groups.back();
+                if (cfg->tweener_enabled) tween_groups.push_back({&g, targets[ch.parent], qs});
                 if (!cfg->gapped_enabled) {
-                    // primary-region filter (src/driver.rs:250-256) and
-                    // build_ungapped_record (src/driver.rs:437-468), off the device path
+                    // primary-region filter (src/driver.rs:250-256), best_chain
+                    // (src/driver.rs:258-260) and build_ungapped_record
+                    // (src/driver.rs:437-468), off the device path
                     jobs.submit([&g, &ties, hsps = std::move(hsps), lo = ch.primary_start_local,
-                                 hi = ch.primary_end_local, gs = ch.global_start]() mutable {
+                                 hi = ch.primary_end_local, gs = ch.global_start, chain = cfg->chain_enabled != 0,
+                                 tween = cfg->tweener_enabled != 0]() mutable {
                         ties += dedup_hsps(&hsps);
-                        g.recs.reserve(hsps.size());
+                        std::vector<gxy_hsp> kept;
+                        kept.reserve(hsps.size());
                         for (const auto &h : hsps)
-                            if (h.t_start >= lo && h.t_start < hi)
-                                g.recs.push_back(Rec{g.ti, g.qi, g.strand, h.t_start + gs, h.q_start, h.length,
-                                                     h.length, h.score, 0, 0});
-                        finalize_group(g);
+                            if (h.t_start >= lo && h.t_start < hi) kept.push_back(h);
+                        if (chain) kept = best_chain(kept);
+                        g.recs.reserve(kept.size());
+                        for (const auto &h : kept)
+                            g.recs.push_back(Rec{g.ti, g.qi, g.strand, h.t_start + gs, h.q_start, h.length,
+                                                 h.length, h.score, 0, 0});
+                        if (!tween) finalize_group(g);  // the tweener runs first otherwise
                     });
                 } else {
                     // primary-region filter (src/driver.rs:250-256); score order (src/driver.rs:270)
@@ -594,6 +863,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                     for (const auto &h : hsps)
                         if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
                             gg->kept.push_back(h);
+                    if (cfg->chain_enabled) gg->kept = best_chain(gg->kept);  // src/driver.rs:258-260
                     std::stable_sort(gg->kept.begin(), gg->kept.end(),
                                      [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
                     for (size_t k = 1; k < gg->kept.size(); k++)
@@ -607,9 +877,16 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     if (!gap_groups.empty()) {
         int rc = gapped_groups_dev(ctx, gap_groups, cfg, &run_store, &rs);
         if (rc) return rc;
-        for (auto &gg : gap_groups) jobs.submit([g = gg->g] { finalize_group(*g); });
+        if (!cfg->tweener_enabled)
+            for (auto &gg : gap_groups) jobs.submit([g = gg->g] { finalize_group(*g); });
     }
     jobs.wait_all();
+    if (cfg->tweener_enabled && !tween_groups.empty()) {
+        // tweener over the whole target (src/driver.rs:340-356), then the group sort
+        int rc = tweener_groups_dev(ctx, tween_groups, cfg, &run_store, &rs);
+        if (rc) return rc;
+        for (auto &tg : tween_groups) finalize_group(*tg.g);
+    }
     rs.tie_pairs += ties.load();
     for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
 
@@ -684,6 +961,7 @@ struct ServeReq {
     DSeq dq, drc;  // query strands (freed by the search thread: its block cache)
     std::vector<std::unique_ptr<Group>> groups;
     std::vector<std::unique_ptr<GapGroup>> gaps;
+    std::vector<TweenGroup> tweens;  // when the tweener is on
 };
 
 // Device bytes a query holds while it is searched: both strands as 2-bit
@@ -726,9 +1004,14 @@ struct gxy_server {
 namespace {
 
 // Records of a query whose groups are all settled; call with S->mu free.
-void complete(gxy_server *S, gxy_ctx *ctx, ServeReq *r, const std::vector<gxy_edit_run> &run_store, int rc) {
+void complete(gxy_server *S, gxy_ctx *ctx, ServeReq *r, std::vector<gxy_edit_run> &run_store, int rc) {
+    if (!rc && S->cfg.tweener_enabled && !r->tweens.empty()) {
+        gxy_run_stats rs;
+        std::memset(&rs, 0, sizeof rs);
+        rc = tweener_groups_dev(ctx, r->tweens, &S->cfg, &run_store, &rs);
+    }
     if (!rc) {
-        for (auto &gg : r->gaps) finalize_group(*gg->g);
+        for (auto &g : r->groups) finalize_group(*g);  // idempotent for groups already sorted
         rc = assemble_records(ctx, r->groups, run_store, &r->out);
     }
     std::string msg = rc ? ctx->err : std::string();
@@ -742,6 +1025,7 @@ void complete(gxy_server *S, gxy_ctx *ctx, ServeReq *r, const std::vector<gxy_ed
             S->err = msg;
         }
         r->gaps.clear();
+        r->tweens.clear();
         r->groups.clear();
         S->tofree.push_back(std::move(r->dq));
         S->tofree.push_back(std::move(r->drc));
@@ -865,24 +1149,31 @@ int serve_search(gxy_server *S, std::vector<ServeReq *> &batch) {
                 ServeReq *r = batch[i];
                 r->groups.push_back(std::make_unique<Group>(Group{ch.parent, 0, strand, c, {}}));
                 Group &g = *r->groups.back();
+                const gxy_dseq *qstrand = si == 0 ? r->dq.get() : r->drc.get();
+                if (cfg->tweener_enabled) r->tweens.push_back({&g, S->targets[ch.parent], qstrand});
                 if (!cfg->gapped_enabled) {
-                    // Vec::dedup, primary-region filter (src/driver.rs:250-256) and
-                    // build_ungapped_record (src/driver.rs:437-468)
+                    // Vec::dedup, primary-region filter (src/driver.rs:250-256),
+                    // best_chain (src/driver.rs:258-260) and build_ungapped_record
+                    // (src/driver.rs:437-468); the group is sorted at completion
                     dedup_hsps(&hsps);
+                    std::vector<gxy_hsp> kept;
                     for (const auto &h : hsps)
                         if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
-                            g.recs.push_back(Rec{g.ti, 0, strand, h.t_start + ch.global_start, h.q_start, h.length,
-                                                 h.length, h.score, 0, 0});
-                    finalize_group(g);
+                            kept.push_back(h);
+                    if (cfg->chain_enabled) kept = best_chain(kept);
+                    for (const auto &h : kept)
+                        g.recs.push_back(Rec{g.ti, 0, strand, h.t_start + ch.global_start, h.q_start, h.length,
+                                             h.length, h.score, 0, 0});
                 } else {
                     auto gg = std::make_unique<GapGroup>();
                     gg->g = &g;
                     gg->tseq = ch.seq;
-                    gg->qseq = si == 0 ? r->dq.get() : r->drc.get();
+                    gg->qseq = qstrand;
                     gg->global_start = ch.global_start;
                     for (const auto &h : hsps)
                         if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
                             gg->kept.push_back(h);
+                    if (cfg->chain_enabled) gg->kept = best_chain(gg->kept);
                     std::stable_sort(gg->kept.begin(), gg->kept.end(),
                                      [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
                     r->gaps.push_back(std::move(gg));
@@ -934,7 +1225,8 @@ void search_loop(gxy_server *S) {
         std::vector<ServeReq *> to_dp;
         for (ServeReq *r : batch) {
             if (rc || r->gaps.empty()) {
-                complete(S, S->ctx, r, {}, rc);
+                std::vector<gxy_edit_run> run_store;
+                complete(S, S->ctx, r, run_store, rc);
             } else if (!S->prm.co_batch) {
                 // the plain driver: this query's gapped stage before the next query
                 std::vector<gxy_edit_run> run_store;
@@ -1009,8 +1301,6 @@ void dp_loop(gxy_server *S, gxy_ctx *ctx) {
                 if (!fin[x] && (done || rc)) {
                     // settled groups stay in `all` (done, never touched again)
                     fin[x] = 1;
-                    if (!rc)
-                        for (size_t y = 0; y < cnt; y++) finalize_group(*all[o + y]->g);
                     complete(S, ctx, mine[x], run_store, rc);
                 }
                 o += cnt;
@@ -1032,7 +1322,7 @@ int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_tar
     if (!targets || !n_targets || !cfg || !params || !out) return GXY_EINVAL;
     for (uint64_t i = 0; i < n_targets; i++)
         if (!targets[i]) return GXY_EINVAL;
-    if (cfg->chain_enabled || cfg->tweener_enabled || cfg->step < 1) return GXY_EUNSUP;
+    if (cfg->step < 1) return GXY_EINVAL;
     auto S = std::make_unique<gxy_server>();
     int rc = gxy_ctx_create(device, &S->ctx);
     if (rc) return rc;

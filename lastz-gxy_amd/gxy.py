@@ -81,6 +81,8 @@ _SIGS = {
                                  _P(T.RecordSet), _P(_vp), _P(C.c_uint64)]),
     "gxy_record_set_free": (None, [_P(T.RecordSet)]),
     "gxy_free": (None, [_vp]),
+    "gxy_best_chain": (C.c_int, [_vp, C.c_uint64, _vp, _P(C.c_uint64)]),
+    "gxy_interpolate": (C.c_int, [_vp, _P(T.RecordSet), _vp, _vp, _P(T.Config), _P(_P(T.RecordSet))]),
     "gxy_server_create": (C.c_int, [C.c_int, _vp, C.c_uint64, _P(T.Config), _P(T.ServerParams), _P(_vp)]),
     "gxy_server_submit": (C.c_int, [_vp, _P(T.Seq), _P(C.c_uint64)]),
     "gxy_server_wait": (C.c_int, [_vp, C.c_uint64, _P(_P(T.RecordSet)), _P(T.RequestStats)]),
@@ -428,6 +430,38 @@ def run_dev(targets, queries, cfg: T.Config, ctx: Context | None = None):
                                 len(queries), C.byref(cfg), C.byref(out), C.byref(st)),
               "gxy_run_dev")
     return Records(out), st
+
+
+def best_chain(hsps):
+    """best_chain (src/chain.rs:19); hsps are (t_start, q_start, length, score)."""
+    n = len(hsps)
+    arr = (T.Hsp * max(1, n))(*[T.Hsp(*h) for h in hsps])
+    out = (T.Hsp * max(1, n))()
+    m = C.c_uint64()
+    rc = lib().gxy_best_chain(C.addressof(arr), n, C.addressof(out), C.byref(m))
+    if rc != 0:
+        raise GxyError(f"gxy_best_chain failed ({rc})")
+    return [(out[k].t_start, out[k].q_start, out[k].length, out[k].score) for k in range(m.value)]
+
+
+def interpolate(records, target, query, cfg: T.Config, ctx=None):
+    """tweener::interpolate (src/tweener.rs:59) over one group's records given as
+    (ti, qi, strand, t_start, q_start, t_span, q_span, score, [(op, n), ...]);
+    target/query are DeviceSeqs (whole target, the group's query strand)."""
+    ctx = ctx or default_context()
+    n = len(records)
+    recs = (T.Record * max(1, n))()
+    runs = []
+    for k, r in enumerate(records):
+        ti, qi, strand, ts, qs, tsp, qsp, score, script = r
+        recs[k] = T.Record(ti, qi, strand, ts, qs, tsp, qsp, score, len(script), 0, len(runs))
+        runs.extend(script)
+    rarr = (T.EditRun * max(1, len(runs)))(*[T.EditRun(op, m) for op, m in runs])
+    rs = T.RecordSet(n, C.cast(recs, _P(T.Record)), len(runs), C.cast(rarr, _P(T.EditRun)))
+    out = _P(T.RecordSet)()
+    ctx.check(lib().gxy_interpolate(ctx._h, C.byref(rs), target._h, query._h, C.byref(cfg), C.byref(out)),
+              "gxy_interpolate")
+    return Records(out).tuples()
 
 
 class Server:

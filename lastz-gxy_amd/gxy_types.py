@@ -95,7 +95,8 @@ class Config(C.Structure):
                 ("tweener_enabled", C.c_int32), ("anchor_window", C.c_uint32),
                 ("transitions", C.c_uint32), ("chunk_primary_bp", C.c_uint32),
                 ("chunk_halo_bp", C.c_uint32), ("entropy_enabled", C.c_int32),
-                ("entropy_threshold", C.c_int32)]
+                ("entropy_threshold", C.c_int32), ("inner_pattern", SeedPatternC),
+                ("inner_min_gap", C.c_uint32), ("inner_max_gap", C.c_uint32), ("inner_relax", C.c_int32)]
 
 
 class Record(C.Structure):
@@ -178,6 +179,10 @@ def config_default() -> Config:
     c.chunk_halo_bp = 50000
     c.entropy_enabled = 0
     c.entropy_threshold = 0
+    c.inner_pattern = pattern_from_spec("match12")  # TweenerConfig::default (src/tweener.rs:44-52)
+    c.inner_min_gap = 100
+    c.inner_max_gap = 100_000
+    c.inner_relax = 2
     return c
 
 

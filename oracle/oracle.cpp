@@ -820,6 +820,125 @@ void prune_table(gxo_postable *t, uint32_t limit) {  // src/pos_table.rs:113-136
 constexpr uint32_t kMaxWeight = 14;  // 4^14 + 1 starts = 1 GiB; the reference's
                                      // dense table is infeasible beyond this.
 
+// best_chain (src/chain.rs:19-69): the highest-scoring chain of HSPs strictly
+// increasing on both axes, by a weighted LIS over (t_start, q_start) order; a
+// predecessor must score strictly more than the best so far (the first one
+// found wins ties), the terminal is the first maximum.  Output ascending.
+std::vector<gxy_hsp> best_chain(const std::vector<gxy_hsp> &hsps) {
+    if (hsps.empty()) return {};
+    std::vector<gxy_hsp> sorted = hsps;
+    std::stable_sort(sorted.begin(), sorted.end(), [](const gxy_hsp &a, const gxy_hsp &b) {
+        return a.t_start != b.t_start ? a.t_start < b.t_start : a.q_start < b.q_start;
+    });
+    const size_t n = sorted.size();
+    std::vector<int64_t> best(n);
+    std::vector<int64_t> parent(n, -1);
+    for (size_t i = 0; i < n; i++) best[i] = sorted[i].score;
+    for (size_t i = 1; i < n; i++) {
+        const gxy_hsp &hi = sorted[i];
+        int64_t bps = 0, bpi = -1;
+        for (size_t j = 0; j < i; j++) {
+            const gxy_hsp &hj = sorted[j];
+            if ((uint64_t)hj.t_start + hj.length <= hi.t_start && (uint64_t)hj.q_start + hj.length <= hi.q_start &&
+                best[j] > bps) {
+                bps = best[j];
+                bpi = (int64_t)j;
+            }
+        }
+        best[i] = bps + hi.score;
+        parent[i] = bpi;
+    }
+    size_t end = 0;
+    for (size_t i = 1; i < n; i++)
+        if (best[i] > best[end]) end = i;
+    std::vector<gxy_hsp> out;
+    for (int64_t c = (int64_t)end; c >= 0; c = parent[(size_t)c]) out.push_back(sorted[(size_t)c]);
+    std::reverse(out.begin(), out.end());
+    return out;
+}
+
+// tweener::interpolate (src/tweener.rs:59-185) over one (target, query,
+// strand) group's records: each gap between neighbours in (t_start, q_start)
+// order whose target and query spans both lie in [min_gap, max_gap] is
+// searched again with the inner pattern (step 1) and relaxed thresholds; the
+// HSPs found (chained when chaining is on) are all anchored and gapped-
+// extended inside the gap, added, and the records re-sorted and deduplicated.
+// `target` is the whole target, `query` the group's query strand.
+void interpolate(std::vector<Rec> &recs, const gxy_seq &target, const gxy_seq &query, uint32_t ti, uint32_t qi,
+                 uint32_t strand, const gxy_config *cfg, bool simd, bool lazy, uint64_t *dp, uint64_t *cells) {
+    if (recs.size() < 2) return;
+    auto by_tq = [](const Rec &a, const Rec &b) {
+        return a.t_start != b.t_start ? a.t_start < b.t_start : a.q_start < b.q_start;
+    };
+    std::stable_sort(recs.begin(), recs.end(), by_tq);
+    const int32_t relax = std::max(cfg->inner_relax, 1);
+    gxy_search_params sp;
+    sp.step = 1;
+    sp.hsp.x_drop = cfg->hsp.x_drop;
+    sp.hsp.hsp_threshold = std::max(cfg->hsp.hsp_threshold / relax, 100);
+    sp.transitions = cfg->transitions;
+    sp.entropy_enabled = cfg->entropy_enabled;
+    sp.entropy_threshold = cfg->entropy_threshold;
+    gxy_gapped_params gp;
+    gp.y_drop = cfg->gapped.y_drop;
+    gp.gapped_threshold = std::max(cfg->gapped.gapped_threshold / relax, 100);
+    Pattern pat;
+    if (!make_pattern(&cfg->inner_pattern, &pat)) return;
+    std::vector<Rec> extras;
+    for (size_t k = 0; k + 1 < recs.size(); k++) {
+        const Rec &a = recs[k], &b = recs[k + 1];
+        const uint32_t tg0 = a.t_start + a.t_span, tg1 = b.t_start;
+        const uint32_t qg0 = a.q_start + a.q_span, qg1 = b.q_start;
+        if (tg1 <= tg0 || qg1 <= qg0) continue;
+        const uint32_t tgap = tg1 - tg0, qgap = qg1 - qg0;
+        if (tgap < cfg->inner_min_gap || qgap < cfg->inner_min_gap) continue;
+        if (tgap > cfg->inner_max_gap || qgap > cfg->inner_max_gap) continue;
+        const OSeq ts = slice_to_new(target, tg0, tg1), qs = slice_to_new(query, qg0, qg1);
+        const gxy_seq tv = ts.view(), qv = qs.view();
+        gxo_postable *table = build_table(tv, pat, 1);
+        if (cfg->max_word_count > 0) prune_table(table, cfg->max_word_count);
+        gxy_search_stats st;
+        std::memset(&st, 0, sizeof st);
+        std::vector<gxy_hsp> hsps = search(*table, tv, qv, cfg->scoring, sp, simd, &st);
+        delete table;
+        if (hsps.empty()) continue;
+        if (cfg->chain_enabled) hsps = best_chain(hsps);
+        for (const auto &h : hsps) {
+            const gxy_anchor an = choose_anchor(h, tv, qv, cfg->scoring, cfg->anchor_window);
+            gxy_alignment aln;
+            Script sc;
+            (*dp)++;
+            if (!gapped_extend(an, tv, qv, cfg->scoring, gp, lazy, &aln, &sc, cells)) continue;
+            Rec r;
+            r.ti = ti;
+            r.qi = qi;
+            r.strand = strand;
+            r.t_start = tg0 + aln.t_start;
+            r.q_start = qg0 + aln.q_start;
+            r.t_span = aln.t_len;
+            r.q_span = aln.q_len;
+            r.score = aln.score;
+            r.script = std::move(sc);
+            extras.push_back(std::move(r));
+        }
+    }
+    for (auto &r : extras) recs.push_back(std::move(r));
+    std::stable_sort(recs.begin(), recs.end(), by_tq);
+    // dedup_by on (t_start, q_start, t_span, q_span): the first of a run stays
+    std::vector<Rec> out;
+    for (auto &r : recs) {
+        if (!out.empty()) {
+            const Rec &p = out.back();
+            if (p.t_start == r.t_start && p.q_start == r.q_start && p.t_span == r.t_span && p.q_span == r.q_span)
+                continue;
+        }
+        out.push_back(std::move(r));
+    }
+    recs.swap(out);
+}
+
+gxy_record_set *record_set_of(const std::vector<Rec> &all);
+
 }  // namespace
 
 // ================================ C exports =====================================
@@ -946,12 +1065,46 @@ int gxo_passes_entropy_gate(const gxy_seq *seq, uint32_t t_start, uint32_t t_end
     return seq && passes_entropy_gate(*seq, t_start, t_end, score, threshold) ? 1 : 0;
 }
 
+int gxo_best_chain(const gxy_hsp *hsps, uint64_t n, gxy_hsp *out, uint64_t *n_out) {
+    if ((!hsps && n) || !n_out || (!out && n)) return GXY_EINVAL;
+    const std::vector<gxy_hsp> c = best_chain(std::vector<gxy_hsp>(hsps, hsps + n));
+    std::copy(c.begin(), c.end(), out);
+    *n_out = c.size();
+    return GXY_OK;
+}
+
+int gxo_interpolate(const gxy_record_set *in, const gxy_seq *target, const gxy_seq *query, const gxy_config *cfg,
+                    gxy_record_set **out) {
+    if (!in || !target || !query || !cfg || !out) return GXY_EINVAL;
+    std::vector<Rec> recs;
+    uint32_t ti = 0, qi = 0, strand = 0;
+    for (uint64_t k = 0; k < in->n; k++) {
+        const gxy_record &g = in->records[k];
+        Rec r;
+        r.ti = ti = g.target_index;
+        r.qi = qi = g.query_index;
+        r.strand = strand = g.strand;
+        r.t_start = g.t_start;
+        r.q_start = g.q_start;
+        r.t_span = g.t_span;
+        r.q_span = g.q_span;
+        r.score = g.score;
+        for (uint32_t x = 0; x < g.n_runs; x++) r.script.push(in->runs[g.runs_offset + x].op, in->runs[g.runs_offset + x].n);
+        recs.push_back(std::move(r));
+    }
+    uint64_t dp = 0, cells = 0;
+    interpolate(recs, *target, *query, ti, qi, strand, cfg, true, true, &dp, &cells);
+    *out = record_set_of(recs);
+    return *out ? GXY_OK : GXY_ENOMEM;
+}
+
 int gxo_run(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,
             uint64_t n_queries, const gxy_config *cfg, int threads, int lazy, int simd,
             gxy_record_set **out, gxy_run_stats *stats) {
     if ((!targets && n_targets) || (!queries && n_queries) || !cfg || !out) return GXY_EINVAL;
-    if (cfg->chain_enabled || cfg->tweener_enabled) return GXY_EUNSUP;
-    Pattern pat;
+    Pattern pat, ipat;
+    if (cfg->tweener_enabled && (!make_pattern(&cfg->inner_pattern, &ipat) || ipat.weight() > kMaxWeight))
+        return GXY_EINVAL;
     if (!make_pattern(&cfg->pattern, &pat) || cfg->step < 1) return GXY_EINVAL;
     if (pat.weight() > kMaxWeight) return GXY_ERANGE;
     gxy_run_stats rs;
@@ -1041,6 +1194,7 @@ int gxo_run(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_se
                 for (const auto &h : hsps)
                     if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
                         kept.push_back(h);
+                if (cfg->chain_enabled) kept = best_chain(kept);  // src/driver.rs:258-260
                 if (cfg->gapped_enabled) {
                     std::stable_sort(kept.begin(), kept.end(),
                                      [](const gxy_hsp &a, const gxy_hsp &b) {
@@ -1101,6 +1255,10 @@ int gxo_run(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_se
                         u.recs.push_back(std::move(r));
                     }
                 }
+                // tweener over the whole target (src/driver.rs:340-356)
+                if (cfg->tweener_enabled)
+                    interpolate(u.recs, tv[ch.parent], qseq, ch.parent, u.qi, u.strand, cfg, simd != 0, lazy != 0,
+                                &u.dp, &u.cells);
             }
         };
         std::vector<std::thread> pool;
@@ -1155,15 +1313,32 @@ int gxo_run(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_se
     }
     for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += qv[qi].len * strands.size();
 
-    auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
+    gxy_record_set *set = record_set_of(all);
     if (!set) return GXY_ENOMEM;
+    *out = set;
+    if (stats) *stats = rs;
+    return GXY_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+gxy_record_set *record_set_of(const std::vector<Rec> &all) {
+    auto *set = (gxy_record_set *)std::calloc(1, sizeof(gxy_record_set));
+    if (!set) return nullptr;
     size_t nruns = 0;
     for (auto &r : all) nruns += r.script.runs.size();
     set->n = all.size();
     set->records = (gxy_record *)std::calloc(std::max<size_t>(1, all.size()), sizeof(gxy_record));
     set->n_runs = nruns;
     set->runs = (gxy_edit_run *)std::calloc(std::max<size_t>(1, nruns), sizeof(gxy_edit_run));
-    if (!set->records || !set->runs) return GXY_ENOMEM;
+    if (!set->records || !set->runs) {
+        std::free(set->records);
+        std::free(set->runs);
+        std::free(set);
+        return nullptr;
+    }
     size_t ro = 0;
     for (size_t k = 0; k < all.size(); k++) {
         gxy_record &o = set->records[k];
@@ -1180,14 +1355,8 @@ int gxo_run(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_se
         o.runs_offset = ro;
         for (auto &x : r.script.runs) set->runs[ro++] = x;
     }
-    *out = set;
-    if (stats) *stats = rs;
-    return GXY_OK;
+    return set;
 }
-
-}  // extern "C"
-
-namespace {
 
 // Shared by the MAF and PAF writers: ASCII of each target and of each query
 // strand, produced on demand.

@@ -74,6 +74,13 @@ double gxo_shannon_entropy(const gxy_seq *seq, uint64_t start, uint64_t end);
 int gxo_passes_entropy_gate(const gxy_seq *seq, uint32_t t_start, uint32_t t_end,
                             int32_t score, int32_t threshold);
 
+/* best_chain (src/chain.rs:19-69); `out` has room for n HSPs. */
+int gxo_best_chain(const gxy_hsp *hsps, uint64_t n, gxy_hsp *out, uint64_t *n_out);
+/* tweener::interpolate (src/tweener.rs:59-185) over the records of one
+ * (target, query, strand) group; `target` whole, `query` the group's strand;
+ * cfg's inner_* fields are the TweenerConfig.  Free with gxo_record_set_free. */
+int gxo_interpolate(const gxy_record_set *in, const gxy_seq *target, const gxy_seq *query,
+                    const gxy_config *cfg, gxy_record_set **out);
 /* driver::run (src/driver.rs:199-392) over `threads` worker threads (0 = all
  * cores).  Chunk groups are merged in ascending chunk order. */
 int gxo_run(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,

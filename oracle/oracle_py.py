@@ -58,6 +58,9 @@ def _load():
                                      _P(T.RecordSet), _P(C.c_void_p), _P(C.c_uint64)]),
         "gxo_record_set_free": (None, [_P(T.RecordSet)]),
         "gxo_free": (None, [vp]),
+        "gxo_best_chain": (C.c_int, [vp, C.c_uint64, vp, _P(C.c_uint64)]),
+        "gxo_interpolate": (C.c_int, [_P(T.RecordSet), _P(T.Seq), _P(T.Seq), _P(T.Config),
+                                      _P(_P(T.RecordSet))]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -210,6 +213,34 @@ def run(targets, queries, cfg: T.Config, threads=0, lazy=True, simd=True):
            "gxo_run")
     del tk, qk
     return Records(out), st
+
+
+def best_chain(hsps):
+    """best_chain (src/chain.rs:19); hsps are (t_start, q_start, length, score)."""
+    n = len(hsps)
+    arr = (T.Hsp * max(1, n))(*[T.Hsp(*h) for h in hsps])
+    out = (T.Hsp * max(1, n))()
+    m = C.c_uint64()
+    _check(lib().gxo_best_chain(C.addressof(arr), n, C.addressof(out), C.byref(m)), "gxo_best_chain")
+    return [(out[k].t_start, out[k].q_start, out[k].length, out[k].score) for k in range(m.value)]
+
+
+def interpolate(records, target, query, cfg: T.Config):
+    """tweener::interpolate (src/tweener.rs:59) over one group's records, given as
+    (ti, qi, strand, t_start, q_start, t_span, q_span, score, [(op, n), ...])."""
+    n = len(records)
+    recs = (T.Record * max(1, n))()
+    runs = []
+    for k, r in enumerate(records):
+        ti, qi, strand, ts, qs, tsp, qsp, score, script = r
+        recs[k] = T.Record(ti, qi, strand, ts, qs, tsp, qsp, score, len(script), 0, len(runs))
+        runs.extend(script)
+    rarr = (T.EditRun * max(1, len(runs)))(*[T.EditRun(op, m) for op, m in runs])
+    rs = T.RecordSet(n, C.cast(recs, _P(T.Record)), len(runs), C.cast(rarr, _P(T.EditRun)))
+    out = _P(T.RecordSet)()
+    _check(lib().gxo_interpolate(C.byref(rs), C.byref(target.view()), C.byref(query.view()), C.byref(cfg),
+                                 C.byref(out)), "gxo_interpolate")
+    return Records(out).tuples()
 
 
 def _format(fn, targets, queries, recs: Records) -> bytes:

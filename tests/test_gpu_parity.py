@@ -425,3 +425,58 @@ def test_run_chunked_gapped_nomasking_identical(ctx):
         g, _ = gxy.run(targets, queries, cfg, ctx)
         o, _ = O.run(targets, queries, cfg)
         assert gxy.format_maf(targets, queries, g) == O.format_maf(targets, queries, o)
+
+
+# ---- chain and tweener --------------------------------------------------------------
+
+def test_interpolate_known_answer(ctx):  # src/tweener.rs:254-330
+    blk = "GATTACACATGGCATGTCGA"
+    t = PS(blk + "N" * 20 + "AGCTAGCTAGCTAGC" + "N" * 20 + blk)
+    recs = [(0, 0, 0, 0, 0, 20, 20, 1900, [(0, 20)]), (0, 0, 0, 75, 75, 20, 20, 1900, [(0, 20)])]
+    cfg = T.config_default()
+    cfg.pattern = T.pattern_from_spec("match19")
+    cfg.chain_enabled = 1
+    cfg.hsp.hsp_threshold = 5000
+    cfg.gapped.gapped_threshold = 5000
+    cfg.inner_pattern = T.pattern_from_spec("match12")
+    cfg.inner_min_gap, cfg.inner_max_gap, cfg.inner_relax = 10, 10_000, 100
+    dt = gxy.DeviceSeq.upload(t, ctx)
+    g = gxy.interpolate(recs, dt, dt, cfg, ctx)
+    assert g == O.interpolate(recs, t, t, cfg)
+    assert len(g) == 3 and g[1][3] == 40 and g[1][8] == "15M"
+
+
+def test_interpolate_random_matches_oracle(ctx):
+    rng = np.random.default_rng(31)
+    t, q = homologous_pair(rng, 40000, 0.18, 0.01, 0.0)
+    cfg = T.config_default()
+    cfg.strand = T.STRAND_PLUS
+    cfg.chain_enabled = 1
+    cfg.inner_relax = 4
+    dt, dq = gxy.DeviceSeq.upload(t, ctx), gxy.DeviceSeq.upload(q, ctx)
+    # records: the chained gapped run's, which leaves gaps between alignments
+    recs = O.run([("t", t)], [("q", q)], cfg, threads=4)[0].tuples()
+    def script(c):
+        import re
+        return [("MID".index(op), int(n)) for n, op in re.findall(r"(\d+)([MID])", c)]
+    grp = [r[:8] + (script(r[8]),) for r in recs]
+    say("interpolate", len(grp), "records")
+    assert gxy.interpolate(grp, dt, dq, cfg, ctx) == O.interpolate(grp, t, q, cfg)
+
+
+@pytest.mark.parametrize("gapped,chunk", [(True, 0), (True, 20_000), (False, 0)])
+def test_run_chain_tweener_matches_oracle(ctx, gapped, chunk):
+    rng = np.random.default_rng(33 + chunk)
+    t, q = homologous_pair(rng, 60000, 0.2, 0.015, 0.1)
+    cfg = _cfg(gapped_enabled=1 if gapped else 0, chain_enabled=1, tweener_enabled=1, inner_relax=4,
+               chunk_primary_bp=chunk, chunk_halo_bp=5000)
+    say("run chain+tweener gapped", gapped, "chunk", chunk)
+    grec, _ = gxy.run([("t", t)], [("q", q)], cfg, ctx)
+    orec, _ = O.run([("t", t)], [("q", q)], cfg, threads=4)
+    assert grec.tuples() == orec.tuples()
+    assert gxy.format_maf([("t", t)], [("q", q)], grec) == O.format_maf([("t", t)], [("q", q)], orec)
+    cfg.tweener_enabled = 0
+    g2, _ = gxy.run([("t", t)], [("q", q)], cfg, ctx)
+    o2, _ = O.run([("t", t)], [("q", q)], cfg, threads=4)
+    assert g2.tuples() == o2.tuples()
+

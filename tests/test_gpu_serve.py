@@ -37,18 +37,23 @@ def workload():
     return target, queries
 
 
-def cfg_for(gapped):
+def cfg_for(gapped, chain=False):
     cfg = T.config_default()
     cfg.gapped_enabled = 1 if gapped else 0
     cfg.chunk_primary_bp = 1_000_000
     cfg.chunk_halo_bp = 50_000
+    if chain:  # --chain --inner (the tweener relaxed further so that gaps get searched)
+        cfg.chain_enabled = 1
+        cfg.tweener_enabled = 1
+        cfg.inner_relax = 6
     return cfg
 
 
-@pytest.mark.parametrize("gapped,co_batch", [(True, True), (True, False), (False, True)])
-def test_served_equals_alone(ctx, workload, gapped, co_batch):
+@pytest.mark.parametrize("gapped,co_batch,chain", [(True, True, False), (True, False, False), (False, True, False),
+                                                   (True, True, True)])
+def test_served_equals_alone(ctx, workload, gapped, co_batch, chain):
     target, queries = workload
-    cfg = cfg_for(gapped)
+    cfg = cfg_for(gapped, chain)
     dt = gxy.DeviceSeq.upload(target, ctx)
     alone = []
     for k, q in enumerate(queries):

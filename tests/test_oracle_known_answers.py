@@ -442,3 +442,98 @@ def test_run_thread_count_invariant():
     assert O.format_maf(targets, queries, a) == O.format_maf(targets, queries, b)
     c, _ = O.run(targets, queries, cfg, threads=4, simd=False, lazy=False)
     assert O.format_maf(targets, queries, a) == O.format_maf(targets, queries, c)
+
+
+# ---- chain.rs ------------------------------------------------------------------------
+
+def test_best_chain_known_answers():  # src/chain.rs:79-137
+    assert O.best_chain([]) == []
+    h = (10, 20, 30, 1000)
+    assert O.best_chain([h]) == [h]
+    a, b, c = (0, 0, 20, 500), (40, 40, 20, 500), (80, 10, 20, 700)
+    assert O.best_chain([a, b, c]) == [a, b]
+    a, b, c, d = (0, 0, 10, 100), (20, 20, 10, 100), (40, 40, 10, 100), (60, 60, 10, 500)
+    assert O.best_chain([a, b, c, d]) == [a, b, c, d]
+    a, b = (0, 50, 10, 1000), (20, 0, 10, 800)
+    assert O.best_chain([a, b]) == [a]
+    a, b = (10, 10, 5, 100), (0, 0, 5, 100)
+    assert O.best_chain([a, b]) == [b, a]
+
+
+def test_best_chain_total_is_brute_force_maximum():
+    """The chain is strictly increasing on both axes and its total score is the
+    maximum over every subset that is (brute force over all subsets)."""
+    rng = np.random.default_rng(41)
+    for _ in range(200):
+        n = int(rng.integers(1, 10))
+        hs = [(int(rng.integers(0, 200)), int(rng.integers(0, 200)), int(rng.integers(1, 40)),
+               int(rng.integers(1, 1000))) for _ in range(n)]
+        ch = O.best_chain(hs)
+        for x, y in zip(ch, ch[1:]):
+            assert x[0] + x[2] <= y[0] and x[1] + x[2] <= y[1]
+        best = 0
+        for m in range(1, 1 << n):
+            sub = sorted((hs[k] for k in range(n) if m >> k & 1), key=lambda h: (h[0], h[1]))
+            if all(x[0] + x[2] <= y[0] and x[1] + x[2] <= y[1] for x, y in zip(sub, sub[1:])):
+                best = max(best, sum(h[3] for h in sub))
+        assert sum(h[3] for h in ch) == best
+
+
+# ---- tweener.rs ----------------------------------------------------------------------
+
+def _tween_cfg():  # minimal_driver_cfg (src/tweener.rs:214-232)
+    c = T.config_default()
+    c.pattern = solid(19)
+    c.strand = T.STRAND_PLUS
+    c.chain_enabled = 1
+    return c
+
+
+def test_tweener_known_answers():  # src/tweener.rs:234-383
+    one = PS("A")
+    cfg = _tween_cfg()
+    assert O.interpolate([], one, one, cfg) == []  # no_gap_means_passthrough
+    # finds_missed_middle_alignment: two 20 bp records around a 15 bp block
+    blk = "GATTACACATGGCATGTCGA"
+    t = PS(blk + "N" * 20 + "AGCTAGCTAGCTAGC" + "N" * 20 + blk)
+    recs = [(0, 0, 0, 0, 0, 20, 20, 1900, [(0, 20)]), (0, 0, 0, 75, 75, 20, 20, 1900, [(0, 20)])]
+    cfg.hsp.hsp_threshold = 5000
+    cfg.gapped.gapped_threshold = 5000
+    cfg.inner_pattern = solid(12)
+    cfg.inner_min_gap, cfg.inner_max_gap, cfg.inner_relax = 10, 10_000, 100
+    out = O.interpolate(recs, t, t, cfg)
+    mid = [r for r in out if r[3] >= 20 and r[3] + r[5] <= 75 and r[5] >= 15]
+    assert mid, out
+    # the block is an exact match: its gapped score is the HOXD70 diagonal sum
+    assert any(r[7] == diag_sum("AGCTAGCTAGCTAGC") and r[5] == 15 and r[8] == "15M" for r in mid), mid
+    assert len(out) == 3
+    # respects_max_gap: an 89 bp gap with max_gap 5 is skipped
+    cfg2 = _tween_cfg()
+    cfg2.inner_max_gap = 5
+    r1 = (0, 0, 0, 0, 0, 10, 10, 0, [])
+    r2 = (0, 0, 0, 99, 99, 1, 1, 0, [])
+    assert len(O.interpolate([r1, r2], one, one, cfg2)) == 2
+
+
+def test_run_chain_and_tweener_known_answers():
+    """driver::run with chaining and the tweener on the fixture-like pair: the
+    records are a superset of the chained run's, every added record lies in a
+    gap between chained records, and chaining alone never adds records."""
+    rng = np.random.default_rng(5)
+    n = 20000
+    tc = rng.integers(0, 4, n)
+    t = T.PackedSeq.from_arrays(tc, np.ones(n, bool), np.zeros(n, bool))
+    qc = tc.copy()
+    flip = rng.random(n) < 0.12
+    qc[flip] = (qc[flip] + 1) % 4
+    q = T.PackedSeq.from_arrays(qc, np.ones(n, bool), np.zeros(n, bool))
+    base = T.config_default()
+    base.strand = T.STRAND_PLUS
+    plain, _ = O.run([("t", t)], [("q", q)], base, threads=2)
+    base.chain_enabled = 1
+    chained, _ = O.run([("t", t)], [("q", q)], base, threads=2)
+    base.tweener_enabled = 1
+    base.inner_relax = 4
+    tween, _ = O.run([("t", t)], [("q", q)], base, threads=2)
+    P, Cd, Tw = plain.tuples(), chained.tuples(), tween.tuples()
+    assert len(Cd) <= len(P) and set(Cd) <= set(Tw)
