@@ -20,6 +20,13 @@ co-batched server and through the plain one.  `value` is the co-batched
 server's throughput (query Mbp/s) at the highest load; each run reports p50 /
 p99 latency (arrival -> records).
 
+`--workload contigs`: many-contig full sensitivity — a uniform unmasked target
+against contigs of log-uniform length, excerpts at 70-90 % identity with ~3 %
+of bases in indels, 12of19 with --chain and --inner (match12 tweener), gapped,
+PAF out; the batched driver (one gxy_run_dev over all contigs) and the server
+(all contigs submitted at once), each query-Mbp/s; oracle baseline and PAF
+parity on a sample of contigs.
+
 `--workload dp`: batched gapped DP alone — anchored pairs with extension
 lengths LogNormal(median 2 kbp, sigma 1) clipped to 100 bp-50 kbp at 70-95 %
 identity, HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000, every pair in one
@@ -249,12 +256,113 @@ def serve_leg(args):
             "runs": runs, "roofline": None, "cpu_baseline": None}
 
 
+def contigs_leg(args):
+    """The many-contig workload; returns the bench line."""
+    import numpy as np
+    import gxy
+    import gxy_types as T
+    import synth
+
+    t0 = time.time()
+    target, contigs = synth.contig_workload(n_target=args.contig_target_bp, n_contigs=args.contigs,
+                                            hi=args.contig_max_len)
+    qbp = sum(len(q) for q in contigs)
+    log(f"contigs: target {len(target)} bp, {len(contigs)} contigs, {qbp} bp (generated in {time.time() - t0:.1f}s)")
+    ctx = gxy.default_context()
+    dt = gxy.DeviceSeq.upload(target, ctx)
+    dqs = [gxy.DeviceSeq.upload(q, ctx) for q in contigs]
+    ctx.synchronize()
+    cfg = T.config_default()
+    cfg.chain_enabled = 1
+    cfg.tweener_enabled = 1
+    names_t = [("target", target)]
+    names_q = [(f"contig{k}", q) for k, q in enumerate(contigs)]
+
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t1 = time.perf_counter()
+    recs, st = gxy.run_dev([dt], dqs, cfg, ctx)
+    ctx.synchronize()
+    batched_s = time.perf_counter() - t1
+    ctx.set_timing(False)
+    t1 = time.perf_counter()
+    paf = gxy.format_paf(names_t, names_q, recs)
+    paf_s = time.perf_counter() - t1
+    log(f"contigs: batched {batched_s:.2f}s, {len(recs)} records, PAF {len(paf)} bytes in {paf_s:.2f}s")
+    kt = {nm: ctx.kernel_time(nm) for nm in ("extend", "qexpand", "qcount", "dp_forward", "dp_traceback")}
+
+    srv = gxy.Server([dt], cfg)
+    t1 = time.perf_counter()
+    tickets = [srv.submit(q) for q in contigs]
+    lat, nrec = [], 0
+    for tk in tickets:
+        r, sst = srv.wait(tk)
+        nrec += len(r)
+        lat.append(sst.done_ms - sst.submit_ms)
+    served_s = time.perf_counter() - t1
+    srv.close()
+    lat = np.array(lat)
+    log(f"contigs: served {served_s:.2f}s, {nrec} records")
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        # sample: contigs at the 20th-80th length percentiles, up to ~2 Mbp in all
+        order = np.argsort([len(q) for q in contigs])
+        pick, tot = [], 0
+        for k in order[len(order) // 5:]:
+            if tot + len(contigs[k]) > args.contig_cpu_bp:
+                break
+            pick.append(int(k))
+            tot += len(contigs[k])
+        sample = [names_q[k] for k in pick]
+        nthreads = os.cpu_count() or 1
+        # fixed cost: the target's table (a 1 kbp query adds almost no search work)
+        t1 = time.perf_counter()
+        O.run(names_t, [("tiny", contigs[int(order[0])].slice(0, min(1000, len(contigs[int(order[0])]))))], cfg,
+              threads=nthreads)
+        build_s = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        orec, _ = O.run(names_t, sample, cfg, threads=nthreads)
+        cpu_s = time.perf_counter() - t1
+        grec, _ = gxy.run_dev([dt], [dqs[k] for k in pick], cfg, ctx)
+        same = gxy.format_paf(names_t, sample, grec) == O.format_paf(names_t, sample, orec)
+        per_bp = max(cpu_s - build_s, 1e-9) / tot
+        full_s = build_s + per_bp * qbp
+        cpu = {"value": round(qbp / 1e6 / full_s, 4), "unit": "query-Mbp/s", "cores": nthreads, "kind": "port",
+               "sample": f"oracle (lazy-row DP, chain + tweener) on {len(pick)} contigs ({tot} bp, the 20th "
+                         f"percentile of length upward) against the same target: {cpu_s:.2f} s, of which "
+                         f"{build_s:.2f} s the table; value = all contigs extrapolated ({full_s:.1f} s)",
+               "sample_query_mbps": round(tot / 1e6 / cpu_s, 4),
+               "gpu_output_identical_on_sample": bool(same)}
+        log(f"contigs: cpu {cpu}")
+    wl = {"workload": f"many contigs: {len(target)} bp uniform target vs {len(contigs)} contigs, length log-uniform "
+                      f"1 kbp-{args.contig_max_len} bp, identity 70-90%, ~3% of bases in indels; 12of19, --chain, "
+                      f"--inner (match12), gapped, PAF",
+          "target_bp": len(target), "contigs": len(contigs), "query_bp": qbp}
+    return {"metric": METRIC, "value": round(qbp / 1e6 / batched_s, 4), "unit": "query-Mbp/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": round(batched_s * 1e3, 1), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic", "config": wl,
+            "records": len(recs), "paf_bytes": len(paf), "paf_format_s": round(paf_s, 3),
+            "dp_cells": st.dp_cells, "dp_gcups": round(st.dp_cells / 1e9 / batched_s, 3),
+            "kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]},
+            "served": {"query_mbps": round(qbp / 1e6 / served_s, 4), "wall_s": round(served_s, 2),
+                       "records": nrec, "p50_ms": round(float(np.percentile(lat, 50)), 1),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 1)},
+            "roofline": None, "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k", "dp", "serve"])
+    ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k", "dp", "serve", "contigs"])
+    ap.add_argument("--contig-target-bp", type=int, default=200_000_000)
+    ap.add_argument("--contigs", type=int, default=200)
+    ap.add_argument("--contig-max-len", type=int, default=1_000_000)
+    ap.add_argument("--contig-cpu-bp", type=int, default=1_500_000)
     ap.add_argument("--serve-target-bp", type=int, default=100_000_000)
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-max-len", type=int, default=1_000_000)
@@ -278,6 +386,9 @@ def main():
 
     if args.workload == "serve":
         print(json.dumps(serve_leg(args)), flush=True)
+        return
+    if args.workload == "contigs":
+        print(json.dumps(contigs_leg(args)), flush=True)
         return
 
     if args.workload == "dp":

@@ -155,8 +155,9 @@ __device__ __forceinline__ RowScoreI32 row_scores(const ScorerI32 &s, uint32_t t
 }
 
 // One side-problem per block of kDpThreads lanes (four wavefronts); lane t
-// owns the contiguous cells [jl + t*C, jl + t*C + cnt) of the row window,
-// C = ceil(W / kDpThreads) <= 5.  Per row:
+// owns the contiguous cells [jl + 5t, jl + 5t + cnt) of the row window
+// (C = kCMax = 5 cells per lane in every row, so the row loop has no uniform
+// branches on the window width).  Per row:
 //   * the previous row comes from LDS as one {M, X, Y} record per cell,
 //     indexed by j - jl(prev) + 1 and clamped into [0, kRecs): record 0 and
 //     the min(br, m) + 1 records after the previous row hold NEG, which is
@@ -232,7 +233,9 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
         const uint32_t jl = lo >= br ? lo - br : 0;
         const uint32_t jh = (uint32_t)min(min((uint64_t)hi + br, (uint64_t)m), (uint64_t)jl + bw - 1);
         const uint32_t W = jh - jl + 1;
-        const uint32_t C = (W + kDpThreads - 1) / kDpThreads;
+        // every lane takes kCMax cells (lanes past the window compute padding
+        // cells that are masked out): no per-row uniform branches on C
+        constexpr uint32_t C = kCMax;
         const uint32_t c0 = (uint32_t)t * C;
         const uint32_t cnt = c0 < W ? min(C, W - c0) : 0;
         const uint32_t js = jl + c0;  // first cell of this lane
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
             const int32_t r0 = (int32_t)js - (int32_t)pjl;  // record index of j = js - 1
 #pragma unroll
             for (int k = 0; k <= kCMax; k++)
-                if ((uint32_t)k <= C) P[k] = rec[min(max(r0 + k, 0), kRecs - 1)];
+                P[k] = rec[min(max(r0 + k, 0), kRecs - 1)];
 #pragma unroll
             for (int k = 0; k < kCMax; k++) {
                 if ((uint32_t)k < C) {
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
         }
         const uint2 win = twin[i - tl];
         if (j < win.x || j > win.y) break;
-        const uint32_t W = win.y - win.x + 1, C = (W + kDpThreads - 1) / kDpThreads;
+        const uint32_t C = kCMax;  // cells per lane in every row
         const uint32_t idx = j - win.x, ln = idx / C, k = idx % C;
         const uint32_t nb = (tile[i - tl][ln] >> (4 * k)) & 0xFu;
         uint32_t op, next;

@@ -321,3 +321,28 @@ def serve_workload(n_target: int = 100_000_000, n_req: int = 1000, seed: int = 1
             kind = "excerpt-" if minus else "excerpt+"
         reqs.append((_packed(q), kind, float(arrivals[k])))
     return target, reqs
+
+
+def contig_workload(n_target: int = 200_000_000, n_contigs: int = 10_000, seed: int = 13, lo: int = 1000,
+                    hi: int = 1_000_000):
+    """Many-contig workload: a uniform unmasked target and n_contigs queries of
+    length log-uniform in [lo, hi], each an excerpt of the target at a random
+    start on a random strand, mutated at identity I ~ U[0.70, 0.90] with
+    indel events at 0.01 per base (Geometric mean 3: ~3 % of bases) and
+    substitutions at 1 - I.  Returns (target, [query PackedSeq])."""
+    rng = np.random.default_rng(seed)
+    code = rng.integers(0, 4, n_target).astype(np.uint8)
+    target = _packed(code)
+    lens = np.exp(rng.uniform(np.log(lo), np.log(hi), n_contigs)).astype(np.int64)
+    out = []
+    for k in range(n_contigs):
+        L = int(min(lens[k], n_target))
+        st = int(rng.integers(0, n_target - L + 1))
+        src = code[st:st + L]
+        if rng.random() < 0.5:
+            src = (3 - src[::-1]).astype(np.uint8)
+        ident = rng.uniform(0.70, 0.90)
+        q, _, _, _ = _mutate_segments(src, np.array([L]), np.array([1 - ident]), np.array([0.01]), rng)
+        out.append(_packed(q))
+    return target, out
+
