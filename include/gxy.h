@@ -286,6 +286,21 @@ typedef struct gxy_named_seq {
 #define GXY_STRAND_MINUS 1u
 #define GXY_STRAND_BOTH 2u
 
+/* Sequences loaded straight to the device layout, with their names. */
+typedef struct gxy_dseq_list {
+    uint64_t n;
+    char **names;      /* NUL-terminated */
+    gxy_dseq **seqs;
+} gxy_dseq_list;
+/* load_2bit / parse_2bit (src/bit2.rs:73-164) over the bytes of a .2bit file:
+ * either byte order, N blocks invalid, mask blocks soft-masked. */
+int gxy_load_2bit(gxy_ctx *ctx, const uint8_t *bytes, uint64_t len, gxy_dseq_list **out);
+/* load_fasta / parse_fasta (src/sequences.rs:338-377) over FASTA text. */
+int gxy_load_fasta(gxy_ctx *ctx, const char *text, uint64_t len, gxy_dseq_list **out);
+/* Frees the names and the device sequences.  A caller that keeps the
+ * sequences frees each name with gxy_free, sets n to 0, then frees the list. */
+void gxy_dseq_list_free(gxy_dseq_list *list);
+
 /* best_chain(&[Hsp]) -> Vec<Hsp> (src/chain.rs:19-69): the highest-scoring
  * chain strictly increasing on both axes, ascending; `out` holds n HSPs.
  * Host-side (O(n log n)); needs no context. */

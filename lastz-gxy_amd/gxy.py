@@ -81,6 +81,9 @@ _SIGS = {
                                  _P(T.RecordSet), _P(_vp), _P(C.c_uint64)]),
     "gxy_record_set_free": (None, [_P(T.RecordSet)]),
     "gxy_free": (None, [_vp]),
+    "gxy_load_2bit": (C.c_int, [_vp, C.c_char_p, C.c_uint64, _P(_P(T.DSeqList))]),
+    "gxy_load_fasta": (C.c_int, [_vp, C.c_char_p, C.c_uint64, _P(_P(T.DSeqList))]),
+    "gxy_dseq_list_free": (None, [_P(T.DSeqList)]),
     "gxy_best_chain": (C.c_int, [_vp, C.c_uint64, _vp, _P(C.c_uint64)]),
     "gxy_interpolate": (C.c_int, [_vp, _P(T.RecordSet), _vp, _vp, _P(T.Config), _P(_P(T.RecordSet))]),
     "gxy_server_create": (C.c_int, [C.c_int, _vp, C.c_uint64, _P(T.Config), _P(T.ServerParams), _P(_vp)]),
@@ -430,6 +433,36 @@ def run_dev(targets, queries, cfg: T.Config, ctx: Context | None = None):
                                 len(queries), C.byref(cfg), C.byref(out), C.byref(st)),
               "gxy_run_dev")
     return Records(out), st
+
+
+def _take_list(lp):
+    """[(name, DeviceSeq)] from a gxy_dseq_list; the DeviceSeqs own the sequences."""
+    L = lib()
+    lst = lp.contents
+    out = []
+    for k in range(lst.n):
+        name = C.string_at(lst.names[k]).decode()
+        L.gxy_free(lst.names[k])
+        out.append((name, DeviceSeq(C.c_void_p(lst.seqs[k]), default_context())))
+    lst.n = 0
+    L.gxy_dseq_list_free(lp)
+    return out
+
+
+def load_2bit(data: bytes, ctx=None):
+    """load_2bit (src/bit2.rs:73) over a .2bit file's bytes: [(name, DeviceSeq)]."""
+    ctx = ctx or default_context()
+    lp = _P(T.DSeqList)()
+    ctx.check(lib().gxy_load_2bit(ctx._h, data, len(data), C.byref(lp)), "gxy_load_2bit")
+    return _take_list(lp)
+
+
+def load_fasta(text: bytes, ctx=None):
+    """load_fasta (src/sequences.rs:338) over FASTA text: [(name, DeviceSeq)]."""
+    ctx = ctx or default_context()
+    lp = _P(T.DSeqList)()
+    ctx.check(lib().gxy_load_fasta(ctx._h, text, len(text), C.byref(lp)), "gxy_load_fasta")
+    return _take_list(lp)
 
 
 def best_chain(hsps):

@@ -308,3 +308,9 @@ class RequestStats(C.Structure):
     _fields_ = [("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double),
                 ("query_bp", C.c_uint64), ("iteration", C.c_uint64), ("batch_queries", C.c_uint64),
                 ("batch_bp", C.c_uint64)]
+
+
+class DSeqList(C.Structure):
+    """gxy_dseq_list."""
+    _fields_ = [("n", C.c_uint64), ("names", C.POINTER(C.c_void_p)), ("seqs", C.POINTER(C.c_void_p))]
+

@@ -261,3 +261,86 @@ def format_maf(targets, queries, recs: Records) -> bytes:
 
 def format_paf(targets, queries, recs: Records) -> bytes:
     return _format(lib().gxo_format_paf, targets, queries, recs)
+
+
+# ---- input formats (pure Python restatements; small inputs) ---------------------
+
+def parse_2bit(data: bytes):
+    """parse_2bit (src/bit2.rs:78-164): [(name, ASCII bytes)] with N overlays and
+    lowercase mask overlays, either byte order."""
+    import struct
+    if len(data) < 4:
+        raise ValueError("truncated 2bit file")
+    magic = struct.unpack("<I", data[:4])[0]
+    if magic == 0x1A412743:
+        fmt = "<I"
+    elif magic == 0x4327411A:
+        fmt = ">I"
+    else:
+        raise ValueError(f"not a 2bit file (magic = 0x{magic:08x})")
+    pos = 4
+
+    def u32():
+        nonlocal pos
+        v = struct.unpack(fmt, data[pos:pos + 4])[0]
+        pos += 4
+        return v
+
+    if u32() != 0:
+        raise ValueError("unsupported 2bit version")
+    count = u32()
+    u32()
+    index = []
+    for _ in range(count):
+        ns = data[pos]
+        pos += 1
+        name = data[pos:pos + ns].decode()
+        pos += ns
+        index.append((name, u32()))
+    out = []
+    for name, off in index:
+        pos = off
+        dna = u32()
+        nn = u32()
+        nst = [u32() for _ in range(nn)]
+        nsz = [u32() for _ in range(nn)]
+        nm = u32()
+        mst = [u32() for _ in range(nm)]
+        msz = [u32() for _ in range(nm)]
+        u32()
+        packed = data[pos:pos + (dna + 3) // 4]
+        asc = bytearray(b"TCAG"[(packed[i >> 2] >> (6 - (i & 3) * 2)) & 3] for i in range(dna))
+        for s_, z in zip(nst, nsz):
+            for i in range(s_, min(s_ + z, dna)):
+                asc[i] = ord("N")
+        for s_, z in zip(mst, msz):
+            for i in range(s_, min(s_ + z, dna)):
+                if asc[i] != ord("N"):
+                    asc[i] = ord(chr(asc[i]).lower())
+        out.append((name, bytes(asc)))
+    return out
+
+
+def parse_fasta(text: bytes):
+    """parse_fasta (src/sequences.rs:338-377): [(name, ASCII bytes)]; sequence
+    lines before the first header stay with the first record."""
+    recs, cur, name = [], bytearray(), None
+    for line in text.split(b"\n"):
+        if line.endswith(b"\r"):
+            line = line[:-1]
+        if not line:
+            continue
+        if line.startswith(b">"):
+            if name is not None:
+                recs.append((name, bytes(cur)))
+                cur = bytearray()
+            toks = line[1:].split()
+            if not toks:
+                raise ValueError("empty FASTA header")
+            name = toks[0].decode()
+        else:
+            cur.extend(b for b in line if b not in b" \t\r\n\x0c")
+    if name is not None:
+        recs.append((name, bytes(cur)))
+    return recs
+

@@ -480,3 +480,56 @@ def test_run_chain_tweener_matches_oracle(ctx, gapped, chunk):
     o2, _ = O.run([("t", t)], [("q", q)], cfg, threads=4)
     assert g2.tuples() == o2.tuples()
 
+
+# ---- input formats --------------------------------------------------------------
+
+def _rand_ascii(rng, n):
+    """Random bases with N and lowercase, plus runs of N and of soft-masking."""
+    a = rng.choice(np.frombuffer(b"ACGTacgtNn", np.uint8), n,
+                   p=[0.2, 0.2, 0.2, 0.2, 0.04, 0.04, 0.04, 0.04, 0.03, 0.01]).astype(np.uint8)
+    for _ in range(max(1, n // 5000)):
+        s0 = int(rng.integers(0, max(1, n)))
+        a[s0:s0 + int(rng.integers(1, 400))] = ord("N")
+        s1 = int(rng.integers(0, max(1, n)))
+        seg = a[s1:s1 + int(rng.integers(1, 900))]
+        seg[(seg >= 65) & (seg <= 90)] += 32  # a view: lowercases in place
+    return a.tobytes()
+
+
+@pytest.mark.parametrize("big_endian", [False, True])
+def test_load_2bit_matches_parse(ctx, big_endian):
+    import sys as _s
+    _s.path.insert(0, __import__("os").path.dirname(__file__))
+    from twobit import build_2bit
+    rng = np.random.default_rng(17 + big_endian)
+    seqs = [(f"s{k}", _rand_ascii(rng, int(n))) for k, n in enumerate([1, 3, 4, 5, 31, 32, 33, 64, 1000, 100003])]
+    data = build_2bit(seqs, big_endian=big_endian)
+    got = gxy.load_2bit(data, ctx)
+    ref = O.parse_2bit(data)
+    assert [n for n, _ in got] == [n for n, _ in ref]
+    for (n, d), (_, a) in zip(got, ref):
+        h = d.download()
+        e = T.PackedSeq.from_ascii(a)
+        assert h.len == e.len and np.array_equal(h.codes, e.codes) and np.array_equal(h.valid, e.valid) \
+            and np.array_equal(h.mask, e.mask), n
+
+
+def test_load_fasta_matches_parse(ctx):
+    rng = np.random.default_rng(19)
+    parts = [b"AC\n"]  # before the first header: joins the first record
+    for k in range(6):
+        s = _rand_ascii(rng, int(rng.integers(0, 70000)))
+        w = int(rng.choice([60, 61, 80, 1000]))
+        lines = b"\n".join(s[i:i + w] for i in range(0, len(s), w))
+        nl = b"\r\n" if k % 2 else b"\n"
+        parts.append(b">seq%d some description\n" % k + lines.replace(b"\n", nl) + nl + (b"\n" if k == 3 else b""))
+    text = b"".join(parts)
+    got = gxy.load_fasta(text, ctx)
+    ref = O.parse_fasta(text)
+    assert [n for n, _ in got] == [n for n, _ in ref]
+    for (n, d), (_, a) in zip(got, ref):
+        h = d.download()
+        e = T.PackedSeq.from_ascii(a)
+        assert h.len == e.len and np.array_equal(h.codes, e.codes) and np.array_equal(h.valid, e.valid) \
+            and np.array_equal(h.mask, e.mask), n
+

@@ -537,3 +537,36 @@ def test_run_chain_and_tweener_known_answers():
     tween, _ = O.run([("t", t)], [("q", q)], base, threads=2)
     P, Cd, Tw = plain.tuples(), chained.tuples(), tween.tuples()
     assert len(Cd) <= len(P) and set(Cd) <= set(Tw)
+
+
+# ---- bit2.rs / sequences.rs parse_fasta ---------------------------------------------
+
+from twobit import build_2bit  # noqa: E402
+
+
+def test_parse_2bit_known_answers():  # src/bit2.rs:318-420
+    with pytest.raises(ValueError):
+        O.parse_2bit(bytes(32))  # header_magic_is_detected
+    import struct
+    be = (struct.pack(">I", 0x1A412743) + struct.pack(">I", 0) + struct.pack(">I", 1) + struct.pack(">I", 0)
+          + bytes([4]) + b"chr1")
+    be += struct.pack(">I", len(be) + 4) + struct.pack(">I", 8) + struct.pack(">I", 0) * 3 + bytes([0x9C, 0x9C])
+    assert O.parse_2bit(be) == [("chr1", b"ACGTACGT")]  # big_endian_round_trip
+    assert O.parse_2bit(build_2bit([("chr1", b"ACGTACGTACGT")])) == [("chr1", b"ACGTACGTACGT")]
+    multi = [("chrA", b"ACGT"), ("chrB", b"TTTT"), ("chrC", b"GATTACACATG")]
+    assert O.parse_2bit(build_2bit(multi)) == multi
+    assert O.parse_2bit(build_2bit([("chr1", b"ACGTNNNNACGT")])) == [("chr1", b"ACGTNNNNACGT")]
+    assert O.parse_2bit(build_2bit([("chr1", b"ACGTacgtACGT")])) == [("chr1", b"ACGTacgtACGT")]
+    # overlapping_n_and_mask: lowercase n is an N (invalid), t stays masked
+    (n, s), = O.parse_2bit(build_2bit([("chr1", b"ACntACGT")]))
+    p = T.PackedSeq.from_ascii(s)
+    c, v, m = p.unpack()
+    assert not v[2] and v[3] and m[3]
+    assert O.parse_2bit(build_2bit([("x", b"ACGTA")])) == [("x", b"ACGTA")]  # non_multiple_of_four
+
+
+def test_parse_fasta_known_answers():  # src/sequences.rs:338-377 and its tests
+    assert O.parse_fasta(b">chr1 desc\nACGT\nacgt\n\n>chr2\r\nNN NN\r\n") == [("chr1", b"ACGTacgt"), ("chr2", b"NNNN")]
+    assert O.parse_fasta(b"AC\n>r\nGT\n") == [("r", b"ACGT")]  # data before the first header
+    with pytest.raises(ValueError):
+        O.parse_fasta(b">\nACGT\n")
