@@ -497,17 +497,51 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
     }
 }
 
+// One lane's words of kTileRows consecutive traceback rows.  Named members
+// (not an array) keep them in registers while they are carried round the
+// walk loop.
+static_assert(kTileRows == 8, "TbRows holds eight rows");
+struct TbRows {
+    uint4 r0, r1, r2, r3, r4, r5, r6, r7;
+};
+
+__device__ __forceinline__ TbRows tb_load(const uint8_t *p) {
+    TbRows t;
+    t.r0 = *(const uint4 *)(p + 0 * kRowBytes);
+    t.r1 = *(const uint4 *)(p + 1 * kRowBytes);
+    t.r2 = *(const uint4 *)(p + 2 * kRowBytes);
+    t.r3 = *(const uint4 *)(p + 3 * kRowBytes);
+    t.r4 = *(const uint4 *)(p + 4 * kRowBytes);
+    t.r5 = *(const uint4 *)(p + 5 * kRowBytes);
+    t.r6 = *(const uint4 *)(p + 6 * kRowBytes);
+    t.r7 = *(const uint4 *)(p + 7 * kRowBytes);
+    return t;
+}
+
+__device__ __forceinline__ void tb_store(uint32_t (*tile)[kDpThreads], int lane, const TbRows &t) {
+    *(uint4 *)&tile[0][lane * 4] = t.r0;
+    *(uint4 *)&tile[1][lane * 4] = t.r1;
+    *(uint4 *)&tile[2][lane * 4] = t.r2;
+    *(uint4 *)&tile[3][lane * 4] = t.r3;
+    *(uint4 *)&tile[4][lane * 4] = t.r4;
+    *(uint4 *)&tile[5][lane * 4] = t.r5;
+    *(uint4 *)&tile[6][lane * 4] = t.r6;
+    *(uint4 *)&tile[7][lane * 4] = t.r7;
+}
+
 // Walk back from the best cell (src/gapped_extend.rs:415-463), one wavefront
 // per side: the wavefront stages kTileRows traceback rows (whole rows, all
-// lanes' words, plus their windows) in LDS and every lane follows the same
-// path through them, so each step is an LDS read instead of a dependent global
-// load.  Runs are emitted tip-to-root, merged while the op repeats, into
-// run_off[p] (room for bi + bj runs).
+// lanes' words, plus their windows) in LDS, and each iteration takes a whole
+// run of one op at once (its lanes read the run's next 64 steps in parallel
+// and a ballot finds where it ends), so a run costs one pair of LDS reads
+// instead of one dependent read per step.  Runs are emitted tip-to-root,
+// merged while the op repeats, into run_off[p] (room for bi + bj runs).
 __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ res, uint32_t nprob, DpPool pool,
                                                      const uint64_t *__restrict__ run_off,
                                                      gxy_edit_run *__restrict__ runs, uint32_t *__restrict__ nruns) {
     __shared__ __align__(16) uint32_t tile[kTileRows][kDpThreads];
     __shared__ uint2 twin[kTileRows];
+    __shared__ uint32_t tlink;  // chunk below the staged tile's chunk
     const uint32_t p = blockIdx.x;
     if (p >= nprob) return;
     const int lane = threadIdx.x;
@@ -524,7 +558,17 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
     uint32_t nr = 0;
     uint32_t cur_op = 0xFFFFFFFFu, cur_n = 0;
     const uint32_t cap = r.bi + r.bj;  // runs reserved for this side
+    // The walk moves down at most one row a step, so tiles are visited in
+    // order: the tile below the staged one is loaded into registers (pt, pw)
+    // while the walk crosses the staged one, hiding the global round trip.
+    // Lane kTileRows carries the link to the chunk below the loaded tile's.
     int64_t tl = -1;                   // first row of the staged tile
+    int64_t ptl = -1;                  // first row of the loaded tile
+    uint32_t pc = c;                   // chunk of the loaded tile
+    uint32_t cbelow = 0;               // chunk below c
+    TbRows pt;
+    uint2 pw = {0u, 0u};
+    uint32_t plink = 0;
     while (i > 0 || j > 0) {
         // A consistent traceback never leaves the swept cells; these guards
         // stop a walk rather than read outside the traceback store.
@@ -533,51 +577,94 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
             break;
         if ((int64_t)i < tl || tl < 0) {
             const uint32_t ntl = i / kTileRows * kTileRows;
-            // moving down into the previous chunk
-            if (tl >= 0 && (uint32_t)tl % kChunkRows == 0) c = pool.prev[c];
+            if ((int64_t)ntl != ptl) {
+                // first tile of the walk: load it now
+                const uint32_t r0 = ntl % kChunkRows;
+                pc = c;
+                pt = tb_load(pool.tb + (uint64_t)pc * kChunkBytes + r0 * kRowBytes + lane * 16);
+                if (lane < kTileRows) pw = pool.win[(uint64_t)pc * kChunkRows + r0 + lane];
+                if (lane == kTileRows) plink = pool.prev[pc];
+            }
+            __syncthreads();
+            tb_store(tile, lane, pt);
+            if (lane < kTileRows) twin[lane] = pw;
+            if (lane == kTileRows) tlink = plink;
+            __syncthreads();
+            c = pc;
+            cbelow = tlink;
             tl = ntl;
-            const uint32_t r0 = ntl % kChunkRows;
-            __syncthreads();
-#pragma unroll
-            for (int x = 0; x < kTileRows; x++) {
-                const uint4 w = *(const uint4 *)(pool.tb + c * kChunkBytes + (r0 + x) * kRowBytes + lane * 16);
-                *(uint4 *)&tile[x][lane * 4] = w;
+            if (ntl >= kTileRows) {
+                ptl = ntl - kTileRows;
+                pc = (ntl % kChunkRows == 0) ? cbelow : c;
+                const uint32_t r0 = (uint32_t)ptl % kChunkRows;
+                pt = tb_load(pool.tb + (uint64_t)pc * kChunkBytes + r0 * kRowBytes + lane * 16);
+                if (lane < kTileRows) pw = pool.win[(uint64_t)pc * kChunkRows + r0 + lane];
+                if (lane == kTileRows) plink = pool.prev[pc];
+            } else {
+                ptl = -1;
             }
-            if (lane < kTileRows) twin[lane] = pool.win[(uint64_t)c * kChunkRows + r0 + lane];
-            __syncthreads();
         }
-        const uint2 win = twin[i - tl];
-        if (j < win.x || j > win.y) break;
-        const uint32_t C = kCMax;  // cells per lane in every row
-        const uint32_t idx = j - win.x, ln = idx / C, k = idx % C;
-        const uint32_t nb = (tile[i - tl][ln] >> (4 * k)) & 0xFu;
-        uint32_t op, next;
-        if (st == kStM) {
-            op = GXY_OP_MATCH;
-            next = nb & 3u;
-            i--;
-            j--;
-        } else if (st == kStX) {
-            op = GXY_OP_DELETE;
-            next = ((nb >> 2) & 1u) ? kStX : kStM;
-            i--;
-        } else {
-            op = GXY_OP_INSERT;
-            next = ((nb >> 3) & 1u) ? kStY : kStM;
-            j--;
-        }
-        if (op == cur_op) {
-            cur_n++;
-        } else {
-            if (cur_n) {
-                if (nr >= cap) break;
-                if (lane == 0) out[nr] = {cur_op, cur_n};
-                nr++;
+        // Lane k takes step k of a run of the current op from (i, j): the
+        // cell (i - k, j - k) for M, (i - k, j) for X, (i, j - k) for Y.  A
+        // step is valid where the one-step walk's guards pass (cell off row
+        // or column 0 as the op needs, j inside the row's window) and
+        // continues the run where its from-state is the run's own state.
+        // The first step that does not continue ends the run: it is taken
+        // (one more op, then its from-state) if valid, else the walk stops
+        // before it.  Rows below the staged tile are left to the next tile.
+        const uint32_t dk = (uint32_t)lane;
+        const uint32_t di = st == kStY ? 0u : dk, dj = st == kStX ? 0u : dk;
+        const uint32_t lim = st == kStY ? 64u : min(64u, i - (uint32_t)tl + 1u);  // steps inside the tile
+        bool valid = false;
+        uint32_t next = kStY + 1u;
+        if (dk < lim && dj <= j) {
+            const uint32_t ci = i - di, cj = j - dj;
+            const bool edge_ok = st == kStM ? (ci > 0 && cj > 0) : st == kStX ? ci > 0 : cj > 0;
+            const uint2 win = twin[ci - (uint32_t)tl];
+            if (edge_ok && cj >= win.x && cj <= win.y) {
+                valid = true;
+                const uint32_t C = kCMax;  // cells per lane in every row
+                const uint32_t idx = cj - win.x, ln = idx / C, k = idx % C;
+                const uint32_t nb = (tile[ci - (uint32_t)tl][ln] >> (4 * k)) & 0xFu;
+                next = st == kStM ? (nb & 3u)
+                                  : st == kStX ? (((nb >> 2) & 1u) ? kStX : kStM) : (((nb >> 3) & 1u) ? kStY : kStM);
             }
-            cur_op = op;
-            cur_n = 1;
         }
-        st = next;
+        const uint64_t ends = __ballot(dk < lim && (!valid || next != st));
+        uint32_t n;               // ops of this run taken now
+        bool stop = false;        // the walk ends before step n
+        uint32_t nst = st;
+        if (ends == 0) {
+            n = lim;
+        } else {
+            const uint32_t kf = (uint32_t)__ffsll((long long)ends) - 1u;
+            const bool vf = (__ballot(valid) >> kf) & 1u;
+            if (vf) {
+                n = kf + 1u;
+                nst = (uint32_t)__shfl((int)next, (int)kf);
+            } else {
+                n = kf;
+                stop = true;
+            }
+        }
+        const uint32_t op = st == kStM ? GXY_OP_MATCH : st == kStX ? GXY_OP_DELETE : GXY_OP_INSERT;
+        if (n) {
+            if (op == cur_op) {
+                cur_n += n;
+            } else {
+                if (cur_n) {
+                    if (nr >= cap) break;
+                    if (lane == 0) out[nr] = {cur_op, cur_n};
+                    nr++;
+                }
+                cur_op = op;
+                cur_n = n;
+            }
+            if (st != kStY) i -= n;
+            if (st != kStX) j -= n;
+        }
+        if (stop) break;
+        st = nst;
     }
     if (cur_n && nr < cap) {
         if (lane == 0) out[nr] = {cur_op, cur_n};
