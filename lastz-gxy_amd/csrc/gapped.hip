@@ -833,11 +833,20 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         }
     }
     uint64_t cells = 0;
+    // Rows a side is expected to sweep: an alignment's target span stays near
+    // its query span, and the band dies about y_drop / gap_extend rows past
+    // the last good cell, so a contig against a long target does not count
+    // the whole rest of the target.  Only pass packing uses it: a side that
+    // outruns it is truncated when the pool runs out and swept again.
+    auto est_rows = [&](uint32_t x) {
+        const uint64_t m = probs[x].m;
+        return std::min<uint64_t>((uint64_t)probs[x].n + 1, m + m / 8 + 512);
+    };
     if (!pending.empty()) {
         // longest sides first, so the long sweeps start early instead of
         // forming the tail of a pass
         std::stable_sort(pending.begin(), pending.end(),
-                         [&](uint32_t a, uint32_t b) { return probs[a].n > probs[b].n; });
+                         [&](uint32_t a, uint32_t b) { return est_rows(a) > est_rows(b); });
         // The traceback pool: up to a sixth of the device (at most 48 GiB),
         // no more than every side could use.
         size_t free_b = 0, total_b = 0;
@@ -859,10 +868,10 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         pool.prev = (uint32_t *)(dwin + nchunks * kChunkRows * sizeof(uint2));
         pool.nchunks = (uint32_t)nchunks;
         GXY_HIP(ctx, hipMemcpyAsync(dprob, probs.data(), np * sizeof(DpProb), hipMemcpyHostToDevice, ctx->stream));
-        // a side's estimated chunks: its full length, at most 1/32 of the pool
+        // a side's estimated chunks, at most 1/32 of the pool
         const uint64_t est_cap = std::max<uint64_t>(1, nchunks / 32);
         auto est = [&](uint32_t x) {
-            return std::min<uint64_t>(((uint64_t)probs[x].n + kChunkRows) / kChunkRows, est_cap);
+            return std::min<uint64_t>((est_rows(x) + kChunkRows - 1) / kChunkRows + 1, est_cap);
         };
         bool alone = false;  // run only the first pending side (progress guarantee)
         while (!pending.empty()) {
