@@ -342,7 +342,11 @@ struct gxy_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    // Growable scratch slots (device memory), reused across calls.
+    // Growable scratch slots (device memory), reused across calls.  Where the
+    // device has memory pools they are allocated and freed in stream order on
+    // `stream` (hipMallocAsync / hipFreeAsync), so growing one never waits for
+    // work on other contexts' streams as hipFree would.
+    bool async_alloc = false;
     static constexpr int kSlots = 32;
     void *slot[kSlots] = {};
     size_t slot_bytes[kSlots] = {};

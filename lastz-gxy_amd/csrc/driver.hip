@@ -1259,6 +1259,9 @@ void search_loop(gxy_server *S) {
 // own groups are settled.
 void dp_loop(gxy_server *S, gxy_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
+    // the tweener's sequences and tables come from this worker's block cache:
+    // a hipFree would wait for every stream on the device
+    ctx->pool_on = true;
     for (;;) {
         std::vector<ServeReq *> mine;
         {
@@ -1294,6 +1297,7 @@ void dp_loop(gxy_server *S, gxy_ctx *ctx) {
         std::vector<char> fin(mine.size(), 0);
         auto settle = [&](int rc) {
             size_t o = 0;
+            std::vector<size_t> now;  // queries whose groups all settled since the last call
             for (size_t x = 0; x < mine.size(); x++) {
                 const size_t cnt = span[x].second;
                 bool done = true;
@@ -1301,10 +1305,22 @@ void dp_loop(gxy_server *S, gxy_ctx *ctx) {
                 if (!fin[x] && (done || rc)) {
                     // settled groups stay in `all` (done, never touched again)
                     fin[x] = 1;
-                    complete(S, ctx, mine[x], run_store, rc);
+                    now.push_back(x);
                 }
                 o += cnt;
             }
+            // the tweener of all of them in one call (one batched gapped stage)
+            if (!rc && S->cfg.tweener_enabled) {
+                std::vector<TweenGroup> tw;
+                for (size_t x : now) tw.insert(tw.end(), mine[x]->tweens.begin(), mine[x]->tweens.end());
+                if (!tw.empty()) {
+                    gxy_run_stats trs;
+                    std::memset(&trs, 0, sizeof trs);
+                    rc = tweener_groups_dev(ctx, tw, &S->cfg, &run_store, &trs);
+                }
+                for (size_t x : now) mine[x]->tweens.clear();
+            }
+            for (size_t x : now) complete(S, ctx, mine[x], run_store, rc);
         };
         gxy_run_stats rs;
         std::memset(&rs, 0, sizeof rs);
@@ -1370,8 +1386,10 @@ int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_tar
         gxy_ctx *c;
         rc = gxy_ctx_create(device, &c);
         if (rc) return fail(rc);
-        // the workers share the traceback budget of one context
-        c->dp_pool_cap = std::min<uint64_t>(48ull << 30, total_b / 6) / (uint64_t)nw;
+        // Together the workers' traceback pools may take half of the device
+        // (each at most 48 GiB).  A pool grows only as far as its passes need,
+        // and the longest queries all land in one worker's length bin.
+        c->dp_pool_cap = std::min<uint64_t>(48ull << 30, total_b / (2 * (uint64_t)nw));
         S->dp_ctx.push_back(c);
     }
     S->worker = std::thread(search_loop, S.get());

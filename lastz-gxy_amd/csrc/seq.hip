@@ -31,14 +31,20 @@ void *scratch(gxy_ctx *ctx, int k, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (ctx->slot_bytes[k] >= bytes) return ctx->slot[k];
     if (ctx->slot[k]) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipFree(ctx->slot[k]);
+        if (ctx->async_alloc) {
+            (void)hipFreeAsync(ctx->slot[k], ctx->stream);
+        } else {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(ctx->slot[k]);
+        }
         ctx->slot[k] = nullptr;
         ctx->slot_bytes[k] = 0;
     }
     size_t sz = bytes + bytes / 8;  // headroom so slowly growing inputs do not reallocate
     sz = (sz + 255) & ~(size_t)255;
-    if (hipMalloc(&ctx->slot[k], sz) != hipSuccess) {
+    const hipError_t e =
+        ctx->async_alloc ? hipMallocAsync(&ctx->slot[k], sz, ctx->stream) : hipMalloc(&ctx->slot[k], sz);
+    if (e != hipSuccess) {
         (void)hipGetLastError();
         ctx->slot[k] = nullptr;
         return nullptr;
@@ -354,6 +360,12 @@ int gxy_ctx_create(int device, gxy_ctx **out) {
         delete c;
         return set_err(nullptr, GXY_EDEVICE, "hipStreamCreate failed");
     }
+    int pools = 0;
+    if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) != hipSuccess) {
+        (void)hipGetLastError();
+        pools = 0;
+    }
+    c->async_alloc = pools != 0;
     *out = c;
     return GXY_OK;
 }
@@ -391,7 +403,8 @@ void gxy_ctx_destroy(gxy_ctx *ctx) {
     timing_flush(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < gxy_ctx::kSlots; k++)
-        if (ctx->slot[k]) (void)hipFree(ctx->slot[k]);
+        if (ctx->slot[k]) (void)(ctx->async_alloc ? hipFreeAsync(ctx->slot[k], ctx->stream) : hipFree(ctx->slot[k]));
+    (void)hipStreamSynchronize(ctx->stream);
     pool_trim(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
