@@ -1,0 +1,122 @@
+"""GPU parity on degenerate and ragged inputs, against the oracle.
+
+Queries that are empty, one base, shorter than / exactly / one past the seed
+span, all N, all soft-masked, N runs inside homology, and reverse complements,
+in one run (ragged lengths in one batch), with masking respected and not,
+gapped and ungapped; a target shorter than the seed span; sequences of one
+base through the gapped extension; seeds at and past the sequence ends.
+Everything is compared bit-exactly (records as MAF/PAF bytes).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gxy
+import gxy_types as T
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+HOX = T.scoring_hoxd70()
+PS = T.PackedSeq.from_ascii
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return gxy.default_context()
+
+
+def _acgt(rng, n):
+    return rng.choice(list(b"ACGT"), n).astype(np.uint8).tobytes()
+
+
+def _degenerate_queries(base):
+    return [
+        ("empty", PS(b"")),
+        ("one", PS(b"A")),
+        ("short18", PS(base[100:118])),
+        ("span19", PS(base[200:219])),
+        ("span20", PS(base[300:320])),
+        ("allN", PS(b"N" * 500)),
+        ("lower", PS(base[500:1500].lower())),
+        ("nrun", PS(base[1000:2000] + b"N" * 40 + base[2000:2600])),
+        ("rc", PS(base[400:1400]).reverse_complement()),
+        ("ragged", PS(base[2100:2100 + 333])),
+    ]
+
+
+@pytest.mark.parametrize("respect,gapped", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_run_degenerate_queries(ctx, respect, gapped):
+    say("degenerate queries respect", respect, "gapped", gapped)
+    rng = np.random.default_rng(5)
+    base = _acgt(rng, 3000)
+    targets = [("t", PS(base)), ("tiny", PS(b"ACG"))]
+    queries = _degenerate_queries(base)
+    cfg = T.config_default()
+    cfg.respect_masking = respect
+    cfg.gapped_enabled = gapped
+    cfg.hsp = T.HspParams(910, 1000)
+    cfg.gapped = T.GappedParams(9400, 1000)
+    g, gst = gxy.run(targets, queries, cfg, ctx)
+    o, ost = O.run(targets, queries, cfg)
+    assert gst.hits == ost.hits
+    om = O.format_maf(targets, queries, o)
+    assert len(om) > 0
+    assert gxy.format_maf(targets, queries, g) == om
+    assert gxy.format_paf(targets, queries, g) == O.format_paf(targets, queries, o)
+
+
+def test_run_chunk_larger_than_target(ctx):
+    say("chunk larger than target")
+    rng = np.random.default_rng(6)
+    base = _acgt(rng, 5000)
+    targets = [("t", PS(base))]
+    queries = [("q", PS(base[1000:4000])), ("e", PS(b""))]
+    cfg = T.config_default()
+    cfg.chunk_primary_bp = 1_000_000
+    cfg.chunk_halo_bp = 50_000
+    g, _ = gxy.run(targets, queries, cfg, ctx)
+    o, _ = O.run(targets, queries, cfg)
+    assert gxy.format_maf(targets, queries, g) == O.format_maf(targets, queries, o)
+
+
+def test_search_target_shorter_than_span(ctx):
+    rng = np.random.default_rng(7)
+    t = PS(b"ACGTACGTACGT")
+    q = PS(_acgt(rng, 2000))
+    pat = T.pattern_from_spec("12of19")
+    sp = T.SearchParams(1, T.HspParams(910, 3000), 1, 0, 3000)
+    gh, gst = gxy.search(gxy.PosTable.build(t, pat, 1, ctx), t, q, HOX, sp)
+    oh, ost = O.search(O.PosTable(t, pat, 1), t, q, HOX, sp)
+    assert gh == oh == []
+    assert gst.hits == ost.hits == 0
+
+
+def test_gapped_one_base_sequences(ctx):
+    gp = T.GappedParams(9400, 0)
+    for tb, qb in ((b"A", b"A"), (b"A", b"ACGT"), (b"ACGT", b"T"), (b"N", b"A"), (b"G", b"C")):
+        t, q = PS(tb), PS(qb)
+        for a in ((0, 0), (len(tb) - 1, len(qb) - 1)):
+            r, _ = gxy.gapped_extend(a, t, q, HOX, gp, ctx)
+            o, _ = O.gapped_extend(a, t, q, HOX, gp)
+            assert r == o, (tb, qb, a)
+
+
+def test_extend_hit_at_and_past_the_ends(ctx):
+    rng = np.random.default_rng(8)
+    tb = _acgt(rng, 97)
+    t = PS(tb)
+    q = PS(tb[10:90])
+    p = T.HspParams(910, 0)
+    span = 19
+    tp = np.array([0, 1, 97 - span, 97 - span + 1, 96, 97, 10, 70, 71], dtype=np.int64)
+    qp = np.array([0, 0, 80 - span, 80 - span, 79, 80, 0, 60, 61], dtype=np.int64)
+    g = gxy.extend_hit_batch(t, q, HOX, p, tp, qp, span, ctx)
+    for k in range(len(tp)):
+        assert g[k] == O.extend_hit(t, q, HOX, p, int(tp[k]), int(qp[k]), span, True), k
