@@ -353,6 +353,89 @@ def contigs_leg(args):
             "roofline": None, "cpu_baseline": cpu}
 
 
+def plain_dropin_leg(target, query, cfg, steps, recs):
+    """The plain drop-in on the chromosome pair: the reference driver's loop
+    (src/driver.rs:199-300) with the GPU behind its two calls.  Per target chunk
+    (src/driver.rs:156-194), one host thread each as rayon runs them, each with
+    its own context: PosTable::build on the chunk, then per strand search, each
+    a synchronous library call that uploads its host inputs, launches its
+    stages and downloads its HSPs; the primary-range filter and the shift to
+    global coordinates on the host, then one sort.  The chunk slices and the
+    query's reverse complement are made before the timed region (the
+    reference recomputes the reverse complement per chunk), in the drop-in's
+    favour.  Its HSPs must equal the batched run's records."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    import gxy
+    import gxy_types as T
+    tlen = len(target)
+    prim, halo = cfg.chunk_primary_bp, cfg.chunk_halo_bp
+    bounds = []  # (lo, hi, primary start, primary end), chunk-local primary range
+    if prim == 0 or tlen <= prim + halo:
+        bounds.append((0, tlen, 0, tlen))
+    else:
+        ps = 0
+        while ps < tlen:
+            pe = min(ps + prim, tlen)
+            lo, hi = max(ps - halo, 0), min(pe + halo, tlen)
+            bounds.append((lo, hi, ps - lo, pe - lo))
+            if pe == tlen:
+                break
+            ps = pe
+    chunks = [target.slice(lo, hi) for lo, hi, _, _ in bounds]
+    strands = [(0, query), (1, query.reverse_complement())]
+    sp = T.SearchParams(cfg.step, cfg.hsp, cfg.transitions, cfg.entropy_enabled, cfg.entropy_threshold)
+    ctxs = [gxy.Context(0) for _ in bounds]
+
+    def one(ci):
+        lo, _, pl, pe = bounds[ci]
+        tab = gxy.PosTable.build(chunks[ci], cfg.pattern, cfg.step, ctxs[ci])
+        out = []
+        for strand, qs in strands:
+            h, _ = gxy.search_array(tab, chunks[ci], qs, cfg.scoring, sp)
+            h = h[(h["t_start"] >= pl) & (h["t_start"] < pe)]
+            out.append((strand, lo, h))
+        return out
+
+    def step():
+        with ThreadPoolExecutor(len(bounds)) as ex:
+            parts = [x for r in ex.map(one, range(len(bounds))) for x in r]
+        n = sum(len(h) for _, _, h in parts)
+        rec = np.empty(n, [("strand", np.uint32), ("t_start", np.uint32), ("q_start", np.uint32),
+                           ("length", np.uint32), ("score", np.int32)])
+        o = 0
+        for strand, lo, h in parts:
+            k = len(h)
+            rec["strand"][o:o + k] = strand
+            rec["t_start"][o:o + k] = h["t_start"] + lo
+            rec["q_start"][o:o + k] = h["q_start"]
+            rec["length"][o:o + k] = h["length"]
+            rec["score"][o:o + k] = h["score"]
+            o += k
+        rec.sort(order=["strand", "t_start", "q_start", "length", "score"])
+        return rec
+
+    step()  # warmup: contexts, scratch
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rec = step()
+    dt = (time.perf_counter() - t0) / steps
+    g = np.array([(r[2], r[3], r[4], r[5], r[7]) for r in recs.tuples()],
+                 dtype=[("strand", np.uint32), ("t_start", np.uint32), ("q_start", np.uint32),
+                        ("length", np.uint32), ("score", np.int32)])
+    g.sort(order=["strand", "t_start", "q_start", "length", "score"])
+    same = len(g) == len(rec) and bool(np.array_equal(g, rec))
+    del ctxs
+    return {"query_mbps": round(len(query) / 1e6 / dt, 3), "ms_per_step": round(1000 * dt, 2),
+            "steps": steps, "host_threads": len(bounds), "records": int(len(rec)),
+            "records_identical_to_batched": same,
+            "form": "reference driver loop (chunks on host threads, one context each) over host "
+                    "sequences; per chunk one PosTable::build call and one search call per strand, "
+                    "each uploading its inputs and downloading its HSPs; primary-range filter, "
+                    "coordinate shift and sort on the host"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +456,7 @@ def main():
     ap.add_argument("--target-bp", type=int, default=48_000_000)
     ap.add_argument("--cpu-sample-bp", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="omit the plain drop-in leg of the chrom line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -485,6 +569,14 @@ def main():
     gxy.run([("t", target)], [("q", query)], cfg, ctx)
     pcie_s = time.perf_counter() - t1
 
+    plain = None
+    if args.workload == "chrom" and not args.no_plain:
+        plain = plain_dropin_leg(target, query, cfg, 2, recs)
+        plain["batched_over_plain"] = round(value / plain["query_mbps"], 2)
+        plain["pcie_inclusive_over_plain"] = round(qbp / 1e6 / pcie_s / plain["query_mbps"], 2)
+        log(f"plain drop-in: {plain['query_mbps']} query-Mbp/s ({plain['ms_per_step']} ms/step), "
+            f"batched/plain {plain['batched_over_plain']}x, identical {plain['records_identical_to_batched']}")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -531,7 +623,7 @@ def main():
         "pcie_inclusive_query_mbps": round(qbp / 1e6 / pcie_s, 3),
         "kernel_ms_per_step": {n: round(v[0] / args.steps, 3) for n, v in ktime.items() if v[1]},
         "timed_kernels_ms_per_step": round(sum(v[0] for v in ktime.values()) / args.steps, 3),
-        "roofline": roof, "cpu_baseline": cpu, "batched_dp": batched_dp,
+        "roofline": roof, "cpu_baseline": cpu, "plain_dropin": plain, "batched_dp": batched_dp,
     }
     print(json.dumps(out), flush=True)
 

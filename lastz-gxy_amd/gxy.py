@@ -290,6 +290,28 @@ def search(table: PosTable, target, query, matrix: T.Scoring, params: T.SearchPa
     return res, st
 
 
+HSP_DTYPE = np.dtype([("t_start", np.uint32), ("q_start", np.uint32), ("length", np.uint32),
+                      ("score", np.int32)])
+
+
+def search_array(table: PosTable, target: T.PackedSeq, query: T.PackedSeq, matrix: T.Scoring,
+                 params: T.SearchParams):
+    """search() on host sequences, the HSPs as a numpy record array (HSP_DTYPE)
+    copied straight from the library's buffer."""
+    ctx = table.ctx
+    out = _P(T.Hsp)()
+    n = C.c_uint64()
+    st = T.SearchStats()
+    ctx.check(lib().gxy_search(ctx._h, table._h, C.byref(target.view()), C.byref(query.view()),
+                               C.byref(matrix), C.byref(params), C.byref(out), C.byref(n), C.byref(st)),
+              "gxy_search")
+    res = np.empty(n.value, HSP_DTYPE)
+    if n.value:
+        C.memmove(res.ctypes.data, out, n.value * res.itemsize)
+    lib().gxy_free(out)
+    return res, st
+
+
 def extend_hit_batch(target, query, matrix, params, t_pos, q_pos, seed_len, ctx=None):
     """Batched extend_hit; returns a list of HSP tuples or None."""
     ctx = ctx or default_context()
