@@ -429,6 +429,9 @@ class KernelTimer {
 void timing_flush(gxy_ctx *ctx);
 // Device scratch slot `k` of at least `bytes` bytes (contents undefined).
 void *scratch(gxy_ctx *ctx, int k, size_t bytes);
+// Slot `k` grown to at least `bytes` bytes, its first `keep` bytes kept
+// (copied in stream order when the slot moves).
+void *scratch_grow_keep(gxy_ctx *ctx, int k, size_t keep, size_t bytes);
 int dseq_upload_impl(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out);
 
 // Scratch slot assignment.

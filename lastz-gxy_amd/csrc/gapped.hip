@@ -821,7 +821,10 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
 
     std::vector<DpRes> res(np);
-    std::vector<std::vector<gxy_edit_run>> side(np);
+    // each side's runs: side_runs[side_off[x] .. side_off[x] + res[x]'s count)
+    std::vector<gxy_edit_run> side_runs;
+    std::vector<uint64_t> side_off(np, 0);
+    std::vector<uint32_t> side_n(np, 0);
     std::vector<uint32_t> pending;
     uint64_t need_rows = 0;
     for (uint64_t x = 0; x < np; x++) {
@@ -874,11 +877,15 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             return std::min<uint64_t>((est_rows(x) + kChunkRows - 1) / kChunkRows + 1, est_cap);
         };
         bool alone = false;  // run only the first pending side (progress guarantee)
-        while (!pending.empty()) {
+        // pending[head ..] waits; a pass takes its sides from the head, and the
+        // sides it truncates go back just before the new head (first in line)
+        size_t head = 0;
+        while (head < pending.size()) {
             // take sides in order while their estimates fit the pool
             uint64_t npp = 0, used = 0;
-            while (npp < pending.size() && npp < (1u << 20) && (npp == 0 || used + est(pending[npp]) <= nchunks)) {
-                used += est(pending[npp]);
+            while (head + npp < pending.size() && npp < (1u << 20) &&
+                   (npp == 0 || used + est(pending[head + npp]) <= nchunks)) {
+                used += est(pending[head + npp]);
                 npp++;
                 if (alone) break;
             }
@@ -893,7 +900,7 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             auto *dnr = (uint32_t *)(work + o_nr);
             auto *dlist = (uint32_t *)(work + o_list);
             pool.next = (uint32_t *)(work + o_ctr);
-            GXY_HIP(ctx, hipMemcpyAsync(dlist, pending.data(), npp * 4, hipMemcpyHostToDevice, ctx->stream));
+            GXY_HIP(ctx, hipMemcpyAsync(dlist, pending.data() + head, npp * 4, hipMemcpyHostToDevice, ctx->stream));
             GXY_HIP(ctx, hipMemsetAsync(pool.next, 0, 4, ctx->stream));
             {
                 KernelTimer kt_(ctx, "dp_forward");
@@ -950,16 +957,18 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
                 }
             }
-            std::vector<uint32_t> next;
+            std::vector<uint32_t> again;  // truncated: swept again
             for (uint64_t y = 0; y < npp; y++) {
-                const uint32_t x = pending[y];
+                const uint32_t x = pending[head + y];
                 if (pr[y].truncated) {
-                    next.push_back(x);
+                    again.push_back(x);
                     continue;
                 }
                 cells += pr[y].cells;
                 res[x] = pr[y];
-                side[x].assign(sr.begin() + poff[y], sr.begin() + poff[y] + nr[y]);
+                side_off[x] = side_runs.size();
+                side_n[x] = nr[y];
+                side_runs.insert(side_runs.end(), sr.begin() + poff[y], sr.begin() + poff[y] + nr[y]);
             }
             if (done == 0) {
                 // nothing finished: the pool is shared by too many long sweeps, or
@@ -972,8 +981,8 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             } else {
                 alone = false;
             }
-            for (uint64_t y = npp; y < pending.size(); y++) next.push_back(pending[y]);
-            pending.swap(next);
+            head += npp - again.size();
+            std::copy(again.begin(), again.end(), pending.begin() + head);
         }
     }
     if (cells_out) *cells_out += cells;
@@ -1004,10 +1013,11 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             else
                 runs->push_back({op, cnt});
         };
-        for (const auto &r : side[2 * k + 1]) push(r.op, r.n);
+        const gxy_edit_run *lr = side_runs.data() + side_off[2 * k + 1];
+        for (uint32_t x = 0; x < side_n[2 * k + 1]; x++) push(lr[x].op, lr[x].n);
         push(GXY_OP_MATCH, 1);
-        const auto &rr = side[2 * k];
-        for (size_t x = rr.size(); x-- > 0;) push(rr[x].op, rr[x].n);
+        const gxy_edit_run *rr = side_runs.data() + side_off[2 * k];
+        for (uint32_t x = side_n[2 * k]; x-- > 0;) push(rr[x].op, rr[x].n);
         g.aln.n_runs = (uint32_t)(runs->size() - g.aln.runs_offset);
     }
     return GXY_OK;

@@ -1050,11 +1050,14 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (uint32_t k = 0; k < nb; k++) maxb = std::max(maxb, boff[k + 1] - boff[k]);
         auto *hits = (uint64_t *)scratch(ctx, kSlotHits, maxb * 8);
-        auto *pkeys = (uint64_t *)scratch(ctx, kSlotPassKeys, H * 8);
-        auto *phsp = (gxy_hsp *)scratch(ctx, kSlotPassHsp, H * sizeof(gxy_hsp));
+        // Passing extensions: room for every hit of the batch about to run on
+        // top of those already written (a hit passes at most once), grown
+        // between batches — not for all H hits at once.
+        uint64_t pcap = std::min<uint64_t>(H, maxb);
+        auto *pkeys = (uint64_t *)scratch(ctx, kSlotPassKeys, pcap * 8);
+        auto *phsp = (gxy_hsp *)scratch(ctx, kSlotPassHsp, pcap * sizeof(gxy_hsp));
         if (!hits || !pkeys || !phsp) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)",
                                                      (unsigned long long)H);
-        PassOut po{pkeys, phsp, ctr + 1};
         // transitions <= 1 with the byte layout: hits binned by target eighth
         // for an XCD-partitioned x-drop
         const bool binned = fast_var && byte_ok;
@@ -1067,10 +1070,28 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         auto *hits2 = binned ? (uint64_t *)scratch(ctx, kSlotHits2, maxb * 8) : nullptr;
         if (!bincnt || !binoff || (binned && !hits2))
             return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)", (unsigned long long)H);
+        uint64_t written = 0;  // passing extensions written before this batch
         for (uint32_t k = 0; k < nb; k++) {
             const uint64_t w0 = bounds[k] * unit_win, w1 = std::min(bounds[k + 1] * unit_win, nwin);
             const uint64_t nh = boff[k + 1] - boff[k];
             if (nh == 0) continue;
+            if (k > 0) {
+                unsigned long long c = 0;
+                GXY_HIP(ctx, hipMemcpyAsync(&c, ctr + 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+                GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                written = c;
+            }
+            if (written + nh > pcap) {
+                pcap = std::min<uint64_t>(H, std::max<uint64_t>(written + nh, pcap + pcap / 2));
+                pkeys = (uint64_t *)scratch_grow_keep(ctx, kSlotPassKeys, written * 8, pcap * 8);
+                phsp = pkeys ? (gxy_hsp *)scratch_grow_keep(ctx, kSlotPassHsp, written * sizeof(gxy_hsp),
+                                                            pcap * sizeof(gxy_hsp))
+                             : nullptr;
+                if (!pkeys || !phsp)
+                    return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu passing extensions)",
+                                   (unsigned long long)pcap);
+            }
+            PassOut po{pkeys, phsp, ctr + 1};
             const unsigned nblk = grid_for(w1 - w0, 64 * kExpWaves);
             {
                 KernelTimer kt_(ctx, "qexpand");

@@ -53,6 +53,32 @@ void *scratch(gxy_ctx *ctx, int k, size_t bytes) {
     return ctx->slot[k];
 }
 
+void *scratch_grow_keep(gxy_ctx *ctx, int k, size_t keep, size_t bytes) {
+    if (ctx->slot_bytes[k] >= bytes) return ctx->slot[k];
+    void *old = ctx->slot[k];
+    const size_t old_bytes = ctx->slot_bytes[k];
+    ctx->slot[k] = nullptr;  // keep the old block alive across scratch()
+    ctx->slot_bytes[k] = 0;
+    void *p = scratch(ctx, k, bytes);
+    if (!p) {
+        ctx->slot[k] = old;
+        ctx->slot_bytes[k] = old_bytes;
+        return nullptr;
+    }
+    if (old) {
+        if (keep && hipMemcpyAsync(p, old, std::min(keep, old_bytes), hipMemcpyDeviceToDevice, ctx->stream) !=
+                        hipSuccess)
+            (void)hipGetLastError();
+        if (ctx->async_alloc) {
+            (void)hipFreeAsync(old, ctx->stream);
+        } else {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(old);
+        }
+    }
+    return p;
+}
+
 void *dev_alloc(gxy_ctx *ctx, size_t bytes, gxy_ctx **owner) {
     *owner = ctx->pool_on ? ctx : nullptr;
     return dev_alloc_from(*owner, bytes);
