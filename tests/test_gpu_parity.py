@@ -481,6 +481,28 @@ def test_run_chain_tweener_matches_oracle(ctx, gapped, chunk):
     assert g2.tuples() == o2.tuples()
 
 
+def test_run_queries_across_length_bins(ctx):
+    """Queries of 0.8, 3, 30 and 150 kbp in one run: the gapped stage splits
+    the groups by query length and runs each bin on a context and stream of
+    its own; the records must equal the oracle's."""
+    rng = np.random.default_rng(44)
+    t = rand_seq(rng, 400_000, 0.002, 0.05)
+    c, v, m = t.unpack()
+    queries = []
+    for k, (lo, n) in enumerate([(1000, 3000), (50_000, 30_000), (150_000, 150_000), (10_000, 800)]):
+        qc, qv, qm = synth.mutate(c[lo:lo + n], v[lo:lo + n], m[lo:lo + n], rng, 0.12, 0.01)
+        queries.append((f"q{k}", T.PackedSeq.from_arrays(qc, qv, qm)))
+    targets = [("t", t)]
+    for chain_tw in (0, 1):
+        say("run across length bins, chain+tweener", chain_tw)
+        cfg = _cfg(chain_enabled=chain_tw, tweener_enabled=chain_tw)
+        g, _ = gxy.run(targets, queries, cfg, ctx)
+        o, _ = O.run(targets, queries, cfg, threads=4)
+        assert len(g) > 0
+        assert g.tuples() == o.tuples()
+        assert gxy.format_maf(targets, queries, g) == O.format_maf(targets, queries, o)
+
+
 # ---- input formats --------------------------------------------------------------
 
 def _rand_ascii(rng, n):
