@@ -295,10 +295,13 @@ def contigs_leg(args):
     t1 = time.perf_counter()
     tickets = [srv.submit(q) for q in contigs]
     lat, nrec = [], 0
-    for tk in tickets:
+    step = max(1, len(tickets) // 10)
+    for k, tk in enumerate(tickets):
         r, sst = srv.wait(tk)
         nrec += len(r)
         lat.append(sst.done_ms - sst.submit_ms)
+        if (k + 1) % step == 0:
+            log(f"contigs: served {k + 1}/{len(tickets)} answered, {time.perf_counter() - t1:.1f}s")
     served_s = time.perf_counter() - t1
     srv.close()
     lat = np.array(lat)

@@ -367,7 +367,7 @@ struct gxy_ctx {
     // hipMalloc/hipFree, so repeated runs of the same shapes allocate nothing.
     // Reuse is ordered by the context's single stream.
     bool pool_on = false;
-    uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(48 GiB, device / 6))
+    uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(96 GiB, device / 3))
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_used;
     // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.
@@ -432,6 +432,8 @@ void *scratch(gxy_ctx *ctx, int k, size_t bytes);
 // Slot `k` grown to at least `bytes` bytes, its first `keep` bytes kept
 // (copied in stream order when the slot moves).
 void *scratch_grow_keep(gxy_ctx *ctx, int k, size_t keep, size_t bytes);
+// Frees slot `k` (in stream order) if it holds more than `above` bytes.
+void scratch_release(gxy_ctx *ctx, int k, size_t above);
 int dseq_upload_impl(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out);
 
 // Scratch slot assignment.

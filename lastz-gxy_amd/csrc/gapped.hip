@@ -850,11 +850,12 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         // forming the tail of a pass
         std::stable_sort(pending.begin(), pending.end(),
                          [&](uint32_t a, uint32_t b) { return est_rows(a) > est_rows(b); });
-        // The traceback pool: up to a sixth of the device (at most 48 GiB),
-        // no more than every side could use.
+        // The traceback pool: up to a third of the device (at most 96 GiB),
+        // no more than every side could use.  Its size bounds how many long
+        // sweeps run at once (rows in flight x 1 KiB).
         size_t free_b = 0, total_b = 0;
         GXY_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-        const uint64_t cap_b = ctx->dp_pool_cap ? ctx->dp_pool_cap : std::min<uint64_t>(48ull << 30, total_b / 6);
+        const uint64_t cap_b = ctx->dp_pool_cap ? ctx->dp_pool_cap : std::min<uint64_t>(96ull << 30, total_b / 3);
         uint64_t want = std::min<uint64_t>(cap_b / kChunkBytes,
                                            (need_rows + kChunkRows - 1) / kChunkRows + pending.size());
         want = std::max<uint64_t>(want, 16);

@@ -79,6 +79,18 @@ void *scratch_grow_keep(gxy_ctx *ctx, int k, size_t keep, size_t bytes) {
     return p;
 }
 
+void scratch_release(gxy_ctx *ctx, int k, size_t above) {
+    if (!ctx->slot[k] || ctx->slot_bytes[k] <= above) return;
+    if (ctx->async_alloc) {
+        (void)hipFreeAsync(ctx->slot[k], ctx->stream);
+    } else {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->slot[k]);
+    }
+    ctx->slot[k] = nullptr;
+    ctx->slot_bytes[k] = 0;
+}
+
 void *dev_alloc(gxy_ctx *ctx, size_t bytes, gxy_ctx **owner) {
     *owner = ctx->pool_on ? ctx : nullptr;
     return dev_alloc_from(*owner, bytes);
