@@ -21,9 +21,9 @@
 // enough wavefronts to hide it.
 //
 // Traceback: each row stores 4 bits per cell (M-from: 2 bits, X-from: 1 bit,
-// Y-from: 1 bit), one 32-bit word per lane, plus the row window (jl, jh).  A
-// second kernel walks back from the best cell, one lane per problem, and emits
-// run-length edit runs tip-to-root.
+// Y-from: 1 bit), packed in window order (640 bytes for the widest row), plus
+// the row window (jl, jh).  A second kernel walks back from the best cell, one
+// wavefront per problem, and emits run-length edit runs tip-to-root.
 #include <algorithm>
 
 #include "internal.h"
@@ -36,7 +36,11 @@ constexpr int kMaxW = 1280;                 // widest row window
 constexpr int kDpThreads = 256;             // lanes per side-problem
 constexpr int kDpWaves = kDpThreads / 64;
 constexpr int kCMax = kMaxW / kDpThreads;   // cells per lane
-constexpr int kRowBytes = kDpThreads * 4;   // traceback bytes per row: one word per lane
+// A traceback row packs the 4-bit from-states of its cells in window order:
+// cell idx = j - jl at bits [4 idx, 4 idx + 4) (lane t's five cells are bits
+// [20 t, 20 t + 20)), 256 lanes x 20 bits = 640 bytes.
+constexpr int kRowWords = kDpThreads * kCMax * 4 / 32;
+constexpr int kRowBytes = kRowWords * 4;
 constexpr int kRecs = kMaxW + 2;            // previous-row records in LDS: [0] NEG, [1, 1+W) the row, NEG after
 constexpr int kQCache = 4096;               // query columns cached in LDS, one byte each
 constexpr int kQMargin = 512;               // cached columns kept left of the window on a refill
@@ -44,6 +48,7 @@ constexpr int kChunkRows = 64;              // traceback rows per pool chunk
 constexpr uint64_t kChunkBytes = (uint64_t)kChunkRows * kRowBytes;
 constexpr uint32_t kNoChunk = 0xFFFFFFFFu;
 constexpr int kTileRows = 8;                // traceback rows staged in LDS by the walk
+constexpr int kRowLoads = kRowBytes / 16;    // lanes that load a packed row, 16 bytes each
 
 enum : uint32_t { kStM = 0, kStX = 1, kStY = 2 };
 
@@ -69,7 +74,7 @@ struct DpRes {
 // in order by an atomic counter; each chunk records the one before it in its
 // side's sweep, so a walk from the last row back to row 0 follows `prev`.
 struct DpPool {
-    uint8_t *tb;       // kChunkBytes per chunk: one 32-bit word per lane per row
+    uint8_t *tb;       // kChunkBytes per chunk: kRowBytes of packed from-states per row
     uint2 *win;        // (jl, jh) per row, kChunkRows per chunk
     uint32_t *prev;    // previous chunk of the same side (kNoChunk for the first)
     uint32_t *next;    // allocation counter
@@ -181,6 +186,7 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
     __shared__ int32_t w_max[kDpWaves];
     __shared__ uint32_t w_j[kDpWaves], w_st[kDpWaves], w_lo[kDpWaves], w_hi[kDpWaves];
     __shared__ uint32_t s_chunk;
+    __shared__ uint32_t stage[kRowWords];  // the row's packed from-states, stored after barrier (3)
     const uint32_t p = blockIdx.x;
     if (p >= nprob) return;  // uniform per block
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -205,6 +211,7 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
     const uint32_t padr = min(br, m) + 1;
     const int4 negr = make_int4(kNegInf, kNegInf, kNegInf, 0);
     for (int x = t; x < kRecs; x += kDpThreads) rec[x] = negr;
+    if (t < kRowWords) stage[t] = 0u;
     // Y-from of the window's first cell: its left neighbour is outside the
     // window, so both candidates are NEG-based (src/gapped_extend.rs:318-334)
     const uint32_t ybit_lead = (kNegInf - oe >= kNegInf - e) ? 0u : 1u;
@@ -464,10 +471,18 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
             }
         }
         const uint32_t ir = i % kChunkRows;
-        if (cnt > 0) *(uint32_t *)(pool.tb + chunk * kChunkBytes + ir * kRowBytes + (uint32_t)t * 4) = word;
+        if (cnt > 0) {  // this lane's 20 bits at bit 20 t of the packed row
+            const uint32_t b = 20u * (uint32_t)t, w0 = b >> 5, b0 = b & 31u;
+            atomicOr(&stage[w0], word << b0);
+            if (b0 > 12) atomicOr(&stage[w0 + 1], word >> (32 - b0));
+        }
         if (t == 0) pool.win[(uint64_t)chunk * kChunkRows + ir] = make_uint2(jl, jh);
         rows_done = i + 1;
         __syncthreads();  // (3) alive band; the new row is in LDS
+        if (t < kRowWords) {  // the packed row to its chunk; the next row's bits come after barrier (2)
+            *(uint32_t *)(pool.tb + chunk * kChunkBytes + ir * kRowBytes + (uint32_t)t * 4) = stage[t];
+            stage[t] = 0u;
+        }
         uint32_t nlo = 0xFFFFFFFFu, nhi = 0;
         bool alive = false;
 #pragma unroll
@@ -518,7 +533,7 @@ __device__ __forceinline__ TbRows tb_load(const uint8_t *p) {
     return t;
 }
 
-__device__ __forceinline__ void tb_store(uint32_t (*tile)[kDpThreads], int lane, const TbRows &t) {
+__device__ __forceinline__ void tb_store(uint32_t (*tile)[kRowWords], int lane, const TbRows &t) {
     *(uint4 *)&tile[0][lane * 4] = t.r0;
     *(uint4 *)&tile[1][lane * 4] = t.r1;
     *(uint4 *)&tile[2][lane * 4] = t.r2;
@@ -530,8 +545,8 @@ __device__ __forceinline__ void tb_store(uint32_t (*tile)[kDpThreads], int lane,
 }
 
 // Walk back from the best cell (src/gapped_extend.rs:415-463), one wavefront
-// per side: the wavefront stages kTileRows traceback rows (whole rows, all
-// lanes' words, plus their windows) in LDS, and each iteration takes a whole
+// per side: the wavefront stages kTileRows packed traceback rows (plus their
+// windows) in LDS, and each iteration takes a whole
 // run of one op at once (its lanes read the run's next 64 steps in parallel
 // and a ballot finds where it ends), so a run costs one pair of LDS reads
 // instead of one dependent read per step.  Runs are emitted tip-to-root,
@@ -539,7 +554,7 @@ __device__ __forceinline__ void tb_store(uint32_t (*tile)[kDpThreads], int lane,
 __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ res, uint32_t nprob, DpPool pool,
                                                      const uint64_t *__restrict__ run_off,
                                                      gxy_edit_run *__restrict__ runs, uint32_t *__restrict__ nruns) {
-    __shared__ __align__(16) uint32_t tile[kTileRows][kDpThreads];
+    __shared__ __align__(16) uint32_t tile[kTileRows][kRowWords];
     __shared__ uint2 twin[kTileRows];
     __shared__ uint32_t tlink;  // chunk below the staged tile's chunk
     const uint32_t p = blockIdx.x;
@@ -581,12 +596,12 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
                 // first tile of the walk: load it now
                 const uint32_t r0 = ntl % kChunkRows;
                 pc = c;
-                pt = tb_load(pool.tb + (uint64_t)pc * kChunkBytes + r0 * kRowBytes + lane * 16);
+                if (lane < kRowLoads) pt = tb_load(pool.tb + (uint64_t)pc * kChunkBytes + r0 * kRowBytes + lane * 16);
                 if (lane < kTileRows) pw = pool.win[(uint64_t)pc * kChunkRows + r0 + lane];
                 if (lane == kTileRows) plink = pool.prev[pc];
             }
             __syncthreads();
-            tb_store(tile, lane, pt);
+            if (lane < kRowLoads) tb_store(tile, lane, pt);
             if (lane < kTileRows) twin[lane] = pw;
             if (lane == kTileRows) tlink = plink;
             __syncthreads();
@@ -597,7 +612,7 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
                 ptl = ntl - kTileRows;
                 pc = (ntl % kChunkRows == 0) ? cbelow : c;
                 const uint32_t r0 = (uint32_t)ptl % kChunkRows;
-                pt = tb_load(pool.tb + (uint64_t)pc * kChunkBytes + r0 * kRowBytes + lane * 16);
+                if (lane < kRowLoads) pt = tb_load(pool.tb + (uint64_t)pc * kChunkBytes + r0 * kRowBytes + lane * 16);
                 if (lane < kTileRows) pw = pool.win[(uint64_t)pc * kChunkRows + r0 + lane];
                 if (lane == kTileRows) plink = pool.prev[pc];
             } else {
@@ -623,9 +638,8 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
             const uint2 win = twin[ci - (uint32_t)tl];
             if (edge_ok && cj >= win.x && cj <= win.y) {
                 valid = true;
-                const uint32_t C = kCMax;  // cells per lane in every row
-                const uint32_t idx = cj - win.x, ln = idx / C, k = idx % C;
-                const uint32_t nb = (tile[ci - (uint32_t)tl][ln] >> (4 * k)) & 0xFu;
+                const uint32_t idx = cj - win.x;  // packed: cell idx at bits [4 idx, 4 idx + 4)
+                const uint32_t nb = (tile[ci - (uint32_t)tl][idx >> 3] >> (4 * (idx & 7u))) & 0xFu;
                 next = st == kStM ? (nb & 3u)
                                   : st == kStX ? (((nb >> 2) & 1u) ? kStX : kStM) : (((nb >> 3) & 1u) ? kStY : kStM);
             }
@@ -852,7 +866,7 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                          [&](uint32_t a, uint32_t b) { return est_rows(a) > est_rows(b); });
         // The traceback pool: up to a third of the device (at most 96 GiB),
         // no more than every side could use.  Its size bounds how many long
-        // sweeps run at once (rows in flight x 1 KiB).
+        // sweeps run at once (rows in flight x kRowBytes).
         size_t free_b = 0, total_b = 0;
         GXY_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
         const uint64_t cap_b = ctx->dp_pool_cap ? ctx->dp_pool_cap : std::min<uint64_t>(96ull << 30, total_b / 3);
