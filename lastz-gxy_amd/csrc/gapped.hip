@@ -41,6 +41,8 @@ constexpr int kCMax = kMaxW / kDpThreads;   // cells per lane
 // [20 t, 20 t + 20)), 256 lanes x 20 bits = 640 bytes.
 constexpr int kRowWords = kDpThreads * kCMax * 4 / 32;
 constexpr int kRowBytes = kRowWords * 4;
+static_assert(kRowWords % kDpWaves == 0 && kRowWords / kDpWaves * 32 == 64 * kCMax * 4,
+              "a wavefront's lanes fill whole words of the packed row");
 constexpr int kRecs = kMaxW + 2;            // previous-row records in LDS: [0] NEG, [1, 1+W) the row, NEG after
 constexpr int kQCache = 4096;               // query columns cached in LDS, one byte each
 constexpr int kQMargin = 512;               // cached columns kept left of the window on a refill
@@ -186,7 +188,6 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
     __shared__ int32_t w_max[kDpWaves];
     __shared__ uint32_t w_j[kDpWaves], w_st[kDpWaves], w_lo[kDpWaves], w_hi[kDpWaves];
     __shared__ uint32_t s_chunk;
-    __shared__ uint32_t stage[kRowWords];  // the row's packed from-states, stored after barrier (3)
     const uint32_t p = blockIdx.x;
     if (p >= nprob) return;  // uniform per block
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -211,7 +212,6 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
     const uint32_t padr = min(br, m) + 1;
     const int4 negr = make_int4(kNegInf, kNegInf, kNegInf, 0);
     for (int x = t; x < kRecs; x += kDpThreads) rec[x] = negr;
-    if (t < kRowWords) stage[t] = 0u;
     // Y-from of the window's first cell: its left neighbour is outside the
     // window, so both candidates are NEG-based (src/gapped_extend.rs:318-334)
     const uint32_t ybit_lead = (kNegInf - oe >= kNegInf - e) ? 0u : 1u;
@@ -471,18 +471,24 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
             }
         }
         const uint32_t ir = i % kChunkRows;
-        if (cnt > 0) {  // this lane's 20 bits at bit 20 t of the packed row
-            const uint32_t b = 20u * (uint32_t)t, w0 = b >> 5, b0 = b & 31u;
-            atomicOr(&stage[w0], word << b0);
-            if (b0 > 12) atomicOr(&stage[w0 + 1], word >> (32 - b0));
+        {
+            // The packed row: lane t's 20 bits at bit 20 t, so a wavefront's 64
+            // lanes fill exactly 40 words.  Lane w < 40 builds word w of its
+            // wavefront from the lanes a = 32 w / 20, a + 1 and a + 2 that it
+            // spans (empty lanes contribute 0).
+            const uint32_t a = (32u * (uint32_t)lane) / 20u, o = 32u * (uint32_t)lane - 20u * a;
+            const uint32_t v0 = (uint32_t)__shfl((int)word, (int)min(a, 63u));
+            const uint32_t v1 = (uint32_t)__shfl((int)word, (int)min(a + 1u, 63u));
+            const uint32_t v2 = (uint32_t)__shfl((int)word, (int)min(a + 2u, 63u));
+            uint32_t pw = (v0 >> o) | (v1 << (20u - o));
+            if (o > 8u) pw |= v2 << (40u - o);
+            if (lane < kRowWords / kDpWaves)
+                *(uint32_t *)(pool.tb + chunk * kChunkBytes + ir * kRowBytes +
+                              ((uint32_t)wv * (kRowWords / kDpWaves) + (uint32_t)lane) * 4) = pw;
         }
         if (t == 0) pool.win[(uint64_t)chunk * kChunkRows + ir] = make_uint2(jl, jh);
         rows_done = i + 1;
         __syncthreads();  // (3) alive band; the new row is in LDS
-        if (t < kRowWords) {  // the packed row to its chunk; the next row's bits come after barrier (2)
-            *(uint32_t *)(pool.tb + chunk * kChunkBytes + ir * kRowBytes + (uint32_t)t * 4) = stage[t];
-            stage[t] = 0u;
-        }
         uint32_t nlo = 0xFFFFFFFFu, nhi = 0;
         bool alive = false;
 #pragma unroll
