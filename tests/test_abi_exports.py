@@ -54,3 +54,18 @@ def test_context_without_gpu_fails_cleanly():
     h = ctypes.c_void_p()
     assert gxy.lib().gxy_ctx_create(0, ctypes.byref(h)) != 0
     assert b"device" in gxy.lib().gxy_last_error(None)
+
+
+def test_hsp_record_array_matches_c_layout():
+    """search_array copies the library's gxy_hsp buffer byte for byte into a
+    numpy record array: the two layouts must agree field by field."""
+    import gxy_types as T
+    dt = gxy.HSP_DTYPE
+    assert dt.itemsize == ctypes.sizeof(T.Hsp)
+    for name, ctype in T.Hsp._fields_:
+        assert dt.fields[name][1] == getattr(T.Hsp, name).offset
+        assert dt.fields[name][0].itemsize == ctypes.sizeof(ctype)
+    h = T.Hsp(7, 11, 19, -3000)
+    a = gxy.np.frombuffer(bytes(h), dt)
+    assert (int(a["t_start"][0]), int(a["q_start"][0]), int(a["length"][0]), int(a["score"][0])) == \
+        (7, 11, 19, -3000)
