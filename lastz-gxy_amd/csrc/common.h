@@ -367,6 +367,9 @@ struct gxy_ctx {
     // hipMalloc/hipFree, so repeated runs of the same shapes allocate nothing.
     // Reuse is ordered by the context's single stream.
     bool pool_on = false;
+    // A second context for the driver's other-strand searches (created on
+    // first use, destroyed with this one).
+    gxy_ctx *helper = nullptr;
     uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(96 GiB, device / 3))
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_used;

@@ -797,6 +797,19 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     std::vector<gxy_edit_run> run_store;  // scripts of gapped records
     std::atomic<uint64_t> ties{0};        // tie pairs counted by the host jobs
     HostJobs jobs(2);                     // joined before groups and ties are destroyed
+    // With both strands, a query's minus-strand search runs on a helper
+    // context (its own stream and scratch) from a helper thread while the
+    // plus-strand search runs here, so each search's host round trips overlap
+    // the other's kernels.  The two read the same table and sequences;
+    // their results are taken in strand order, as one after the other.
+    gxy_ctx *helper = nullptr;
+    if (strands.size() == 2) {
+        if (!ctx->helper && gxy_ctx_create(ctx->device, &ctx->helper) == GXY_OK)
+            ctx->helper->pool_on = true;  // its own block cache: nothing it frees waits on other streams
+        helper = ctx->helper;
+        if (helper) helper->timing = ctx->timing;
+    }
+    HostJobs helper_job(helper ? 1 : 0);
 
     clk.lap(kTrSetup);
     for (size_t c = 0; c < chunks.size(); c++) {
@@ -812,18 +825,40 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             if (rc) return rc;
         }
         for (uint32_t qi = 0; qi < n_queries; qi++) {
-            for (uint32_t strand : strands) {
+            std::vector<gxy_hsp> shs[2];
+            gxy_search_stats sst[2];
+            int src[2] = {GXY_OK, GXY_OK};
+            // Ungapped: Vec::dedup runs on a host worker with the records.
+            auto run_search = [&](size_t k, gxy_ctx *c) {
+                const uint32_t strand = strands[k];
                 const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
-                std::vector<gxy_hsp> hsps;
-                gxy_search_stats st;
                 const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
                 const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
-                clk.lap(kTrRecords);
-                // Ungapped: Vec::dedup runs on a host worker with the records.
-                rc = search_dev_impl(ctx, tp, qs, &cfg->scoring, &sp, respect, &hsps, &st, qb,
-                                     cfg->gapped_enabled != 0);
-                if (rc) return rc;
-                clk = PhaseClock(ctx);
+                src[k] = search_dev_impl(c, tp, qs, &cfg->scoring, &sp, respect, &shs[k], &sst[k], qb,
+                                         cfg->gapped_enabled != 0);
+            };
+            clk.lap(kTrRecords);
+            if (helper) {
+                helper_job.submit([&] {
+                    (void)hipSetDevice(ctx->device);
+                    run_search(1, helper);
+                });
+                run_search(0, ctx);
+                helper_job.wait_all();
+                if (!src[0] && src[1]) {
+                    set_err(ctx, src[1], "%s", helper->err.c_str());
+                    return src[1];
+                }
+            } else {
+                for (size_t k = 0; k < strands.size() && !src[0]; k++) run_search(k, ctx);
+            }
+            if (src[0]) return src[0];
+            clk = PhaseClock(ctx);
+            for (size_t k = 0; k < strands.size(); k++) {
+                const uint32_t strand = strands[k];
+                const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
+                std::vector<gxy_hsp> hsps = std::move(shs[k]);
+                const gxy_search_stats &st = sst[k];
                 rs.hits += st.hits;
                 rs.columns += st.columns;
                 rs.hsps += st.emitted;
@@ -886,6 +921,21 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         int rc = tweener_groups_dev(ctx, tween_groups, cfg, &run_store, &rs);
         if (rc) return rc;
         for (auto &tg : tween_groups) finalize_group(*tg.g);
+    }
+    if (helper && helper->timing) {  // the helper's kernel times count as this context's
+        timing_flush(helper);
+        for (const auto &t : helper->totals) {
+            bool found = false;
+            for (auto &u : ctx->totals)
+                if (u.first == t.first) {
+                    u.second.first += t.second.first;
+                    u.second.second += t.second.second;
+                    found = true;
+                    break;
+                }
+            if (!found) ctx->totals.push_back(t);
+        }
+        helper->totals.clear();
     }
     // A traceback pool grown past 48 GiB by this run goes back to the device
     // (the next large run allocates it again), so it does not hold memory that

@@ -436,6 +436,7 @@ int gxy_ctx_reset_timing(gxy_ctx *ctx) {
 
 void gxy_ctx_destroy(gxy_ctx *ctx) {
     if (!ctx) return;
+    if (ctx->helper) gxy_ctx_destroy(ctx->helper);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     timing_flush(ctx);
