@@ -146,6 +146,18 @@ def extend_hit(t, q, sc, params, t_pos, q_pos, seed_len, simd=False):
     return out.astuple() if found.value else None
 
 
+def shannon_entropy(seq: T.PackedSeq, start: int, end: int) -> float:
+    """src/entropy.rs:27-49: bits of A/C/G/T frequency over the window's valid bases."""
+    return lib().gxo_shannon_entropy(C.byref(seq.view()), start, end)
+
+
+def passes_entropy_gate(seq: T.PackedSeq, t_start: int, t_end: int, score: int,
+                        threshold: int) -> bool:
+    """src/entropy.rs:56-66: score * H / 2 >= threshold."""
+    return bool(lib().gxo_passes_entropy_gate(C.byref(seq.view()), t_start, t_end, score,
+                                              threshold))
+
+
 def search(table: PosTable, t, q, sc, sp, simd=True):
     out = _P(T.Hsp)()
     n = C.c_uint64()

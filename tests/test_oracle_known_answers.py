@@ -570,3 +570,53 @@ def test_parse_fasta_known_answers():  # src/sequences.rs:338-377 and its tests
     assert O.parse_fasta(b"AC\n>r\nGT\n") == [("r", b"ACGT")]  # data before the first header
     with pytest.raises(ValueError):
         O.parse_fasta(b">\nACGT\n")
+
+
+# ---- entropy.rs ----------------------------------------------------------------------
+
+def test_shannon_entropy_known_answers():  # src/entropy.rs:72-99
+    assert abs(O.shannon_entropy(PS(b"ACGTACGTACGTACGT"), 0, 16) - 2.0) < 1e-9
+    assert O.shannon_entropy(PS(b"AAAAAAAAAA"), 0, 10) == 0.0
+    assert abs(O.shannon_entropy(PS(b"ATATATATAT"), 0, 10) - 1.0) < 1e-9
+    s = PS(b"NNNNACGTACGT")
+    assert abs(O.shannon_entropy(s, 0, 12) - 2.0) < 1e-9
+    # a window of N only has no valid bases: 0.0 (src/entropy.rs:36-38)
+    assert O.shannon_entropy(s, 0, 4) == 0.0
+    # soft-masked (lowercase) bases are valid and count like uppercase
+    assert abs(O.shannon_entropy(PS(b"acgtACGT"), 0, 8) - 2.0) < 1e-9
+
+
+def test_shannon_entropy_against_scipy():
+    """Random windows (with N, lowercase, and ends past the sequence, which clamp
+    to its length) against scipy's Shannon entropy of the valid-base counts."""
+    from scipy.stats import entropy
+    rng = np.random.default_rng(21)
+    for _ in range(200):
+        n = int(rng.integers(1, 300))
+        w = rng.dirichlet(np.ones(5) * 0.7)
+        raw = rng.choice(list(b"ACGTN"), n, p=w).astype(np.uint8)
+        low = rng.random(n) < 0.3
+        raw[low] = np.char.lower(raw[low].view("S1")).view(np.uint8)
+        seq = bytes(raw)
+        a = int(rng.integers(0, n))
+        b = int(rng.integers(a, n + 40))
+        win = seq[a:min(b, n)].upper()
+        counts = [win.count(c) for c in b"ACGT"]
+        want = entropy(counts, base=2) if sum(counts) else 0.0
+        assert abs(O.shannon_entropy(PS(seq), a, b) - want) < 1e-12, (seq, a, b)
+
+
+def test_entropy_gate_known_answers():  # src/entropy.rs:101-122
+    assert not O.passes_entropy_gate(PS(b"AAAAAAAAAA"), 0, 10, 10_000, 1_000)
+    s = PS(b"ACGTACGTACGT")
+    assert O.passes_entropy_gate(s, 0, 12, 1_000, 1_000)
+    assert not O.passes_entropy_gate(s, 0, 12, 999, 1_000)
+    d = PS(b"ATATATATAT")
+    assert not O.passes_entropy_gate(d, 0, 10, 1_000, 1_000)
+    assert O.passes_entropy_gate(d, 0, 10, 2_000, 1_000)
+    assert not O.passes_entropy_gate(d, 0, 10, 1_999, 1_000)
+    # three letters in equal shares: H = log2(3), so the gate passes from
+    # ceil(2 * threshold / log2(3)) = 1262 at threshold 1000
+    t3 = PS(b"ACGACGACGACG")
+    assert not O.passes_entropy_gate(t3, 0, 12, 1_261, 1_000)
+    assert O.passes_entropy_gate(t3, 0, 12, 1_262, 1_000)
