@@ -4,7 +4,8 @@ Queries that are empty, one base, shorter than / exactly / one past the seed
 span, all N, all soft-masked, N runs inside homology, and reverse complements,
 in one run (ragged lengths in one batch), with masking respected and not,
 gapped and ungapped; a target shorter than the seed span; sequences of one
-base through the gapped extension; seeds at and past the sequence ends.
+base through the gapped extension; seeds at and past the sequence ends; the
+entropy gate over low-complexity blocks (repeats and skewed composition).
 Everything is compared bit-exactly (records as MAF/PAF bytes).
 """
 import sys
@@ -15,6 +16,7 @@ import pytest
 import gxy
 import gxy_types as T
 import oracle_py as O
+import synth
 
 pytestmark = pytest.mark.gpu
 
@@ -120,3 +122,49 @@ def test_extend_hit_at_and_past_the_ends(ctx):
     g = gxy.extend_hit_batch(t, q, HOX, p, tp, qp, span, ctx)
     for k in range(len(tp)):
         assert g[k] == O.extend_hit(t, q, HOX, p, int(tp[k]), int(qp[k]), span, True), k
+
+
+def _low_complexity_pair(rng):
+    """Blocks of (AT)n, (ACG)n, A-rich (0.7/0.1/0.1/0.1) and uniform bases, 4 %
+    mutated, separated by uniform spacers; the query is a 6 % / 0.5 % mutant."""
+    parts = []
+    for k in range(24):
+        n = int(rng.integers(300, 900))
+        kind = k % 4
+        if kind == 0:
+            b = np.frombuffer(b"AT" * n, np.uint8)[:n].copy()
+        elif kind == 1:
+            b = np.frombuffer(b"ACG" * n, np.uint8)[:n].copy()
+        elif kind == 2:
+            b = rng.choice(list(b"ACGT"), n, p=[0.7, 0.1, 0.1, 0.1]).astype(np.uint8)
+        else:
+            b = rng.choice(list(b"ACGT"), n).astype(np.uint8)
+        flip = rng.random(n) < 0.04
+        b[flip] = rng.choice(list(b"ACGT"), int(flip.sum())).astype(np.uint8)
+        parts.append(b)
+        parts.append(rng.choice(list(b"ACGT"), 500).astype(np.uint8))
+    t = PS(np.concatenate(parts).tobytes())
+    c, v, m = t.unpack()
+    qc, qv, qm = synth.mutate(c, v, m, rng, 0.06, 0.005)
+    return t, T.PackedSeq.from_arrays(qc, qv, qm)
+
+
+@pytest.mark.parametrize("thr", [3000, 6000])
+def test_search_entropy_gate_on_low_complexity(ctx, thr):
+    """HSPs whose target intervals span the whole entropy range (0 to 2 bits):
+    the gate rejects thousands of them and the device's choices equal the
+    oracle's, HSP for HSP."""
+    say("entropy gate low complexity thr", thr)
+    rng = np.random.default_rng(31)
+    t, q = _low_complexity_pair(rng)
+    pat = T.pattern_from_spec("12of19")
+    g_tab = gxy.PosTable.build(t, pat, 1, ctx)
+    o_tab = O.PosTable(t, pat, 1)
+    off, _ = O.search(o_tab, t, q, HOX, T.SearchParams(1, T.HspParams(910, 3000), 1, 0, thr))
+    sp = T.SearchParams(1, T.HspParams(910, 3000), 1, 1, thr)
+    gh, gst = gxy.search(g_tab, t, q, HOX, sp)
+    oh, ost = O.search(o_tab, t, q, HOX, sp)
+    assert 0 < len(oh) < len(off) - 1000
+    assert gst.hits == ost.hits
+    assert gst.emitted == ost.emitted
+    assert gh == oh
