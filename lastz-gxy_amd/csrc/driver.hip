@@ -802,6 +802,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
     // plus-strand search runs here, so each search's host round trips overlap
     // the other's kernels.  The two read the same table and sequences;
     // their results are taken in strand order, as one after the other.
+    // postable_build_impl and gxy_postable_prune return with ctx's stream
+    // drained, so the table, the chunk sequences and the query copies are
+    // complete before the helper's stream reads them.
     gxy_ctx *helper = nullptr;
     if (strands.size() == 2) {
         if (!ctx->helper && gxy_ctx_create(ctx->device, &ctx->helper) == GXY_OK)
