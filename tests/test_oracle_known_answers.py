@@ -235,6 +235,38 @@ def test_choose_anchor_known_answers():
     assert O.choose_anchor((0, 0, 4, 0), s, s, HOX, 1000)[0] == 2
 
 
+def test_choose_anchor_against_window_brute_force():
+    """src/anchor.rs:25-69 restated as its definition: every window's score
+    summed afresh, the first window with the strictly highest score, its
+    midpoint.  An asymmetric matrix (target base indexes the row) catches a
+    transposed lookup; a two-letter alphabet with N makes ties common."""
+    rng = np.random.default_rng(17)
+    sc = T.Scoring()
+    sub = rng.integers(-150, 120, (4, 4))
+    for i in range(4):
+        for j in range(4):
+            sc.sub[i][j] = int(sub[i, j])
+    sc.gap_open, sc.gap_extend, sc.ambig = 400, 30, -100
+    for trial in range(400):
+        alpha = list(b"ACN") if trial % 2 else list(b"ACGTN")
+        n = int(rng.integers(1, 120))
+        t = PS(rng.choice(alpha, n).astype(np.uint8).tobytes())
+        q = PS(rng.choice(alpha, n + 7).astype(np.uint8).tobytes())
+        tc, tv, _ = t.unpack()
+        qc, qv, _ = q.unpack()
+        length = int(rng.integers(1, n + 1))
+        ts = int(rng.integers(0, n - length + 1))
+        qs = int(rng.integers(0, n + 7 - length + 1))
+        window = int(rng.choice([0, 1, 2, 3, 8, length, length + 5, 1000]))
+        cols = [int(sub[tc[ts + k], qc[qs + k]]) if tv[ts + k] and qv[qs + k] else sc.ambig
+                for k in range(length)]
+        w = min(max(window, 1), length)
+        sums = [sum(cols[st:st + w]) for st in range(length - w + 1)]
+        best = sums.index(max(sums))
+        want = (ts + best + w // 2, qs + best + w // 2)
+        assert O.choose_anchor((ts, qs, length, 0), t, q, sc, window) == want, trial
+
+
 # ---- gapped_extend.rs -----------------------------------------------------------------
 
 @pytest.mark.parametrize("lazy", [False, True])
