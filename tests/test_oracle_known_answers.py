@@ -154,6 +154,76 @@ def test_search_known_answers():
     assert len({h[0] - h[1] for h in hs}) >= 2
 
 
+def _py_search(t, q, pat, sc, step, transitions, hsp, entropy_thr=None):
+    """src/seed_search.rs:45-125 and src/diag_hash.rs:19-58 restated as plain
+    loops over the oracle's pinned primitives (seed_word, extend_hit, the
+    entropy gate): query windows on the step grid in order, the word and its
+    transition variants (care position i toggles bit 2 << 2*(weight-1-i)),
+    target positions ascending, diagonal admission t >= next[d], the mark at
+    the HSP's end diagonal, then (diagonal, t_start) order without duplicates
+    (ties in full-tuple order, as the oracle canonicalises them)."""
+    span, weight = T.pattern_span_weight(pat)
+    index = {}
+    for p in range(len(t) - span + 1):
+        w = O.seed_word(t, pat, p)
+        if w is not None:
+            index.setdefault(w, []).append(p)
+    shifts = [2 * (weight - 1 - i) for i in range(weight)]
+    nxt, out = {}, []
+    for qp in range(0, len(q) - span + 1, step):
+        w = O.seed_word(q, pat, qp)
+        if w is None:
+            continue
+        vs = [w]
+        if transitions >= 1:
+            vs += [w ^ (2 << s) for s in shifts]
+        if transitions >= 2:
+            vs += [w ^ (2 << shifts[i]) ^ (2 << shifts[j]) for i in range(weight) for j in range(i + 1, weight)]
+        for v in vs:
+            for tp in index.get(v, []):
+                n = nxt.get(tp - qp)
+                if n is not None and tp < n:
+                    continue
+                h = O.extend_hit(t, q, sc, hsp, tp, qp, span, False)
+                if h is None:
+                    continue
+                if entropy_thr is not None and not O.passes_entropy_gate(t, h[0], h[0] + h[2], h[3],
+                                                                         entropy_thr):
+                    continue
+                te, qe = h[0] + h[2], h[1] + h[2]
+                nxt[te - qe] = max(nxt.get(te - qe, te), te)
+                out.append(tuple(h))
+    return sorted(set(out), key=lambda h: (h[0] - h[1], h[0], h))
+
+
+def test_search_against_plain_loop_restatement():
+    """The oracle's search (speculative-free, with DiagHash) equals the plain
+    restatement above over 12 configurations: three seed shapes, steps 1-2,
+    0-2 transitions, x-drop 300/910, thresholds 300/1000, the entropy gate
+    on and off, N and soft-masked bases.  Dropping the admission step from
+    the restatement changes 6 of the 12 results (extensions from covered hits
+    add HSPs); the admission boundary itself (>= against >) leaves these
+    outputs unchanged, so it is pinned by the reference's own tests only."""
+    import synth
+    rng = np.random.default_rng(3)
+    total = 0
+    for trial in range(12):
+        tb = rng.choice(list(b"ACGTacgtN"), 1500, p=[0.22] * 4 + [0.025] * 4 + [0.02]).astype(np.uint8)
+        t = PS(tb.tobytes())
+        c, v, m = t.unpack()
+        qc, qv, qm = synth.mutate(c[200:1300], v[200:1300], m[200:1300], rng, 0.1, 0.01)
+        q = T.PackedSeq.from_arrays(qc, qv, qm)
+        pat = T.pattern_from_spec(["12of19", "1101011", "match8"][trial % 3])
+        step, trans = 1 + trial % 2, trial % 3
+        hsp = T.HspParams(int(rng.choice([300, 910])), int(rng.choice([300, 1000])))
+        ent = 1 if trial % 4 == 3 else 0
+        oh, _ = O.search(O.PosTable(t, pat, 1), t, q, HOX, T.SearchParams(step, hsp, trans, ent, 1200))
+        ph = _py_search(t, q, pat, HOX, step, trans, hsp, 1200 if ent else None)
+        assert [tuple(x) for x in oh] == ph, trial
+        total += len(ph)
+    assert total > 2000
+
+
 # ---- hsp.rs ----------------------------------------------------------------------
 
 @pytest.mark.parametrize("simd", [False, True])
