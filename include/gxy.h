@@ -37,7 +37,7 @@ extern "C" {
 #define GXY_EINVAL (-1)   /* invalid argument (bad size, null pointer, ...)  */
 #define GXY_ENOMEM (-2)   /* host or device allocation failed                */
 #define GXY_EDEVICE (-3)  /* HIP runtime / kernel launch error               */
-#define GXY_EUNSUP (-4)   /* feature outside the hot path (chain, tweener)   */
+#define GXY_EUNSUP (-4)   /* parameters outside what a kernel supports       */
 #define GXY_ERANGE (-5)   /* input exceeds a documented limit                */
 
 /* ---- POD views ---------------------------------------------------------- */
@@ -173,6 +173,20 @@ const char *gxy_version(void);
 int gxy_ctx_set_timing(gxy_ctx *ctx, int enabled);
 int gxy_ctx_kernel_time(gxy_ctx *ctx, const char *name, double *ms, uint64_t *launches);
 int gxy_ctx_reset_timing(gxy_ctx *ctx);
+/* Traceback-pool cap of the gapped extension (no counterpart in the
+ * reference, whose DP keeps every row of a side: src/gapped_extend.rs:233-242).
+ * The library keeps the from-states of the rows being swept in a pool of
+ * 64-row chunks, by default up to a third of the device (at most 96 GiB).
+ * With rows > 0 the pool holds at most max(16, rows / 64) chunks: sides that
+ * find the pool empty stop and are swept again in a later pass, a pass that
+ * finishes nothing is followed by one that runs its first side alone, and a
+ * side that needs more rows than the whole pool fails with GXY_ENOMEM.  Results
+ * do not depend on the cap.  rows = 0 restores the default. */
+int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows);
+/* Gapped-extension passes on the context since it was created: passes run,
+ * sides stopped by a full pool (swept again later) and passes that ran one
+ * side alone. */
+int gxy_ctx_dp_pool_stats(gxy_ctx *ctx, uint64_t *passes, uint64_t *truncated, uint64_t *alone);
 
 /* ---- sequences ---------------------------------------------------------- */
 /* PackedSeq::from_ascii (src/sequences.rs:164-206) into caller buffers sized
@@ -449,10 +463,12 @@ int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_tar
 /* Queue a query (copied; the caller's buffers may be reused on return). */
 int gxy_server_submit(gxy_server *server, const gxy_seq *query, uint64_t *ticket);
 /* Block until the ticket's query is served; hands over its records (free with
- * gxy_record_set_free) and fills `stats` when non-null.  A ticket is waited once. */
+ * gxy_record_set_free) and fills `stats` when non-null.  A ticket is waited on
+ * once: a second wait on it (also from another thread) returns GXY_EINVAL. */
 int gxy_server_wait(gxy_server *server, uint64_t ticket, gxy_record_set **out,
                     gxy_request_stats *stats);
-/* Message of the last failure inside the server (or of the failed request). */
+/* Message of the last failure inside the server (or of the failed request).
+ * The text is a copy owned by the calling thread, valid until its next call. */
 const char *gxy_server_last_error(gxy_server *server);
 /* Stops the worker after the queued queries are served; frees everything. */
 int gxy_server_destroy(gxy_server *server);

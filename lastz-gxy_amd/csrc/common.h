@@ -19,6 +19,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -371,6 +373,9 @@ struct gxy_ctx {
     // first use, destroyed with this one).
     gxy_ctx *helper = nullptr;
     uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(96 GiB, device / 3))
+    uint64_t dp_pool_rows = 0; // traceback pool rows set by the caller (0: no row cap)
+    // gapped passes since creation: passes, sides stopped by a full pool, passes run with one side alone
+    uint64_t dp_passes = 0, dp_truncated = 0, dp_alone = 0;
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_used;
     // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.
@@ -404,6 +409,21 @@ enum TracePhase : int {
 namespace gxy {
 
 int set_err(gxy_ctx *ctx, int code, const char *fmt, ...);
+// Runs the body of a C entry point: a C++ exception (host allocation failure
+// or a library invariant) becomes an error code and message instead of
+// crossing the ABI.
+template <class F>
+int guarded(gxy_ctx *ctx, F &&body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+    } catch (const std::exception &ex) {
+        return set_err(ctx, GXY_EINVAL, "%s", ex.what());
+    } catch (...) {
+        return set_err(ctx, GXY_EINVAL, "unexpected exception");
+    }
+}
 // Device allocation for sequences and tables: from the context's block cache
 // when ctx->pool_on (then *owner = ctx), else hipMalloc (*owner = nullptr).
 // nullptr on failure.

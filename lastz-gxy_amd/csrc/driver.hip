@@ -658,336 +658,346 @@ void gxy_config_default(gxy_config *c) {
 }
 
 int gxy_best_chain(const gxy_hsp *hsps, uint64_t n, gxy_hsp *out, uint64_t *n_out) {
-    if ((!hsps && n) || (!out && n) || !n_out) return GXY_EINVAL;
-    const std::vector<gxy_hsp> c = best_chain(std::vector<gxy_hsp>(hsps, hsps + n));
-    std::copy(c.begin(), c.end(), out);
-    *n_out = c.size();
-    return GXY_OK;
+    return guarded(nullptr, [&]() -> int {
+        if ((!hsps && n) || (!out && n) || !n_out) return GXY_EINVAL;
+        const std::vector<gxy_hsp> c = best_chain(std::vector<gxy_hsp>(hsps, hsps + n));
+        std::copy(c.begin(), c.end(), out);
+        *n_out = c.size();
+        return GXY_OK;
+    });
 }
 
 int gxy_interpolate(gxy_ctx *ctx, const gxy_record_set *records, const gxy_dseq *target, const gxy_dseq *query,
                     const gxy_config *cfg, gxy_record_set **out) {
-    if (!ctx) return GXY_EINVAL;
-    if (!records || !target || !query || !cfg || !out) return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    Group g{0, 0, 0, 0, {}};
-    std::vector<gxy_edit_run> run_store;
-    for (uint64_t k = 0; k < records->n; k++) {
-        const gxy_record &r = records->records[k];
-        if (r.runs_offset + r.n_runs > records->n_runs) return set_err(ctx, GXY_EINVAL, "record runs out of range");
-        if ((uint64_t)r.t_start + r.t_span > target->len || (uint64_t)r.q_start + r.q_span > query->len)
-            return set_err(ctx, GXY_EINVAL, "record outside its sequences");
-        g.ti = r.target_index;
-        g.qi = r.query_index;
-        g.strand = r.strand;
-        g.recs.push_back(Rec{r.target_index, r.query_index, r.strand, r.t_start, r.q_start, r.t_span, r.q_span,
-                             r.score, r.n_runs, run_store.size()});
-        run_store.insert(run_store.end(), records->runs + r.runs_offset, records->runs + r.runs_offset + r.n_runs);
-    }
-    std::vector<TweenGroup> tg{{&g, target, query}};
-    gxy_run_stats rs;
-    std::memset(&rs, 0, sizeof rs);
-    int rc = tweener_groups_dev(ctx, tg, cfg, &run_store, &rs);
-    if (rc) return rc;
-    std::vector<std::unique_ptr<Group>> groups;
-    groups.push_back(std::make_unique<Group>(std::move(g)));
-    return assemble_records(ctx, groups, run_store, out);
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!records || !target || !query || !cfg || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        Group g{0, 0, 0, 0, {}};
+        std::vector<gxy_edit_run> run_store;
+        for (uint64_t k = 0; k < records->n; k++) {
+            const gxy_record &r = records->records[k];
+            if (r.runs_offset + r.n_runs > records->n_runs) return set_err(ctx, GXY_EINVAL, "record runs out of range");
+            if ((uint64_t)r.t_start + r.t_span > target->len || (uint64_t)r.q_start + r.q_span > query->len)
+                return set_err(ctx, GXY_EINVAL, "record outside its sequences");
+            g.ti = r.target_index;
+            g.qi = r.query_index;
+            g.strand = r.strand;
+            g.recs.push_back(Rec{r.target_index, r.query_index, r.strand, r.t_start, r.q_start, r.t_span, r.q_span,
+                                 r.score, r.n_runs, run_store.size()});
+            run_store.insert(run_store.end(), records->runs + r.runs_offset, records->runs + r.runs_offset + r.n_runs);
+        }
+        std::vector<TweenGroup> tg{{&g, target, query}};
+        gxy_run_stats rs;
+        std::memset(&rs, 0, sizeof rs);
+        int rc = tweener_groups_dev(ctx, tg, cfg, &run_store, &rs);
+        if (rc) return rc;
+        std::vector<std::unique_ptr<Group>> groups;
+        groups.push_back(std::make_unique<Group>(std::move(g)));
+        return assemble_records(ctx, groups, run_store, out);
+    });
 }
 
 int gxy_pattern_parse(const char *spec, gxy_seed_pattern *out) {
-    if (!spec || !out) return GXY_EINVAL;
-    std::string s(spec), bits;
-    if (s == "12of19") bits = "1110100110010101111";
-    else if (s == "19of20") bits = "11111111101111111111";
-    else if (s.rfind("match", 0) == 0) {
-        const std::string k = s.substr(5);
-        if (k.empty() || k.find_first_not_of("0123456789") != std::string::npos) return GXY_EINVAL;
-        const unsigned long v = std::stoul(k);
-        if (v == 0 || v > 32) return GXY_EINVAL;
-        bits.assign(v, '1');
-    } else if (!s.empty() && s.find_first_not_of("01") == std::string::npos) {
-        bits = s;
-    } else {
-        return GXY_EINVAL;
-    }
-    if (bits.empty() || bits.size() > 32) return GXY_EINVAL;
-    out->length = (uint32_t)bits.size();
-    out->care = 0;
-    for (size_t k = 0; k < bits.size(); k++)
-        if (bits[k] == '1') out->care |= 1u << k;
-    return GXY_OK;
+    return guarded(nullptr, [&]() -> int {
+        if (!spec || !out) return GXY_EINVAL;
+        std::string s(spec), bits;
+        if (s == "12of19") bits = "1110100110010101111";
+        else if (s == "19of20") bits = "11111111101111111111";
+        else if (s.rfind("match", 0) == 0) {
+            const std::string k = s.substr(5);
+            if (k.empty() || k.find_first_not_of("0123456789") != std::string::npos) return GXY_EINVAL;
+            const unsigned long v = std::stoul(k);
+            if (v == 0 || v > 32) return GXY_EINVAL;
+            bits.assign(v, '1');
+        } else if (!s.empty() && s.find_first_not_of("01") == std::string::npos) {
+            bits = s;
+        } else {
+            return GXY_EINVAL;
+        }
+        if (bits.empty() || bits.size() > 32) return GXY_EINVAL;
+        out->length = (uint32_t)bits.size();
+        out->care = 0;
+        for (size_t k = 0; k < bits.size(); k++)
+            if (bits[k] == '1') out->care |= 1u << k;
+        return GXY_OK;
+    });
 }
 
 int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets, const gxy_dseq *const *queries,
                 uint64_t n_queries, const gxy_config *cfg, gxy_record_set **out, gxy_run_stats *stats) {
-    if (!ctx) return GXY_EINVAL;
-    if ((!targets && n_targets) || (!queries && n_queries) || !cfg || !out)
-        return set_err(ctx, GXY_EINVAL, "null argument");
-    for (uint64_t i = 0; i < n_targets; i++)
-        if (!targets[i]) return set_err(ctx, GXY_EINVAL, "null target %llu", (unsigned long long)i);
-    for (uint64_t i = 0; i < n_queries; i++)
-        if (!queries[i]) return set_err(ctx, GXY_EINVAL, "null query %llu", (unsigned long long)i);
-    if (cfg->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    PhaseClock clk(ctx);
-    // Internal sequences and tables come from the context's block cache.
-    struct PoolScope {
-        gxy_ctx *c;
-        explicit PoolScope(gxy_ctx *x) : c(x) { c->pool_on = true; }
-        ~PoolScope() { c->pool_on = false; }
-    } pool_scope(ctx);
-    gxy_run_stats rs;
-    std::memset(&rs, 0, sizeof rs);
-    const int respect = cfg->respect_masking ? 1 : 0;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if ((!targets && n_targets) || (!queries && n_queries) || !cfg || !out)
+            return set_err(ctx, GXY_EINVAL, "null argument");
+        for (uint64_t i = 0; i < n_targets; i++)
+            if (!targets[i]) return set_err(ctx, GXY_EINVAL, "null target %llu", (unsigned long long)i);
+        for (uint64_t i = 0; i < n_queries; i++)
+            if (!queries[i]) return set_err(ctx, GXY_EINVAL, "null query %llu", (unsigned long long)i);
+        if (cfg->step < 1) return set_err(ctx, GXY_EINVAL, "step must be >= 1");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        PhaseClock clk(ctx);
+        // Internal sequences and tables come from the context's block cache.
+        struct PoolScope {
+            gxy_ctx *c;
+            explicit PoolScope(gxy_ctx *x) : c(x) { c->pool_on = true; }
+            ~PoolScope() { c->pool_on = false; }
+        } pool_scope(ctx);
+        gxy_run_stats rs;
+        std::memset(&rs, 0, sizeof rs);
+        const int respect = cfg->respect_masking ? 1 : 0;
 
-    std::vector<uint32_t> strands;
-    if (cfg->strand == GXY_STRAND_PLUS || cfg->strand == GXY_STRAND_BOTH) strands.push_back(GXY_STRAND_PLUS);
-    if (cfg->strand == GXY_STRAND_MINUS || cfg->strand == GXY_STRAND_BOTH) strands.push_back(GXY_STRAND_MINUS);
-    if (strands.empty()) return set_err(ctx, GXY_EINVAL, "invalid strand selector");
-    // minus strand: one device reverse complement per query (src/driver.rs:221)
-    std::vector<DSeq> drc(n_queries);
-    if (cfg->strand != GXY_STRAND_PLUS)
-        for (uint64_t i = 0; i < n_queries; i++) {
-            gxy_dseq *d;
-            int rc = gxy_dseq_reverse_complement(ctx, queries[i], &d);
-            if (rc) return rc;
-            drc[i].reset(d);
+        std::vector<uint32_t> strands;
+        if (cfg->strand == GXY_STRAND_PLUS || cfg->strand == GXY_STRAND_BOTH) strands.push_back(GXY_STRAND_PLUS);
+        if (cfg->strand == GXY_STRAND_MINUS || cfg->strand == GXY_STRAND_BOTH) strands.push_back(GXY_STRAND_MINUS);
+        if (strands.empty()) return set_err(ctx, GXY_EINVAL, "invalid strand selector");
+        // minus strand: one device reverse complement per query (src/driver.rs:221)
+        std::vector<DSeq> drc(n_queries);
+        if (cfg->strand != GXY_STRAND_PLUS)
+            for (uint64_t i = 0; i < n_queries; i++) {
+                gxy_dseq *d;
+                int rc = gxy_dseq_reverse_complement(ctx, queries[i], &d);
+                if (rc) return rc;
+                drc[i].reset(d);
+            }
+
+        // byte copies of every query strand, shared by all chunk searches
+        struct DevBytes {
+            uint8_t *p = nullptr;
+            gxy_ctx *owner = nullptr;
+            ~DevBytes() { dev_free(owner, p); }
+        };
+        const bool byte_ok = scoring_byte_ok(cfg->scoring);
+        std::vector<DevBytes> qbytes(byte_ok ? 2 * n_queries : 0);
+        if (byte_ok) {
+            for (uint64_t i = 0; i < n_queries; i++) {
+                for (int s = 0; s < 2; s++) {
+                    const gxy_dseq *qs = s == 0 ? queries[i] : drc[i].get();
+                    if (!qs) continue;
+                    DevBytes &b = qbytes[2 * i + s];
+                    b.p = (uint8_t *)dev_alloc(ctx, byte_buffer_size(qs->len), &b.owner);
+                    if (!b.p) return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
+                    int rc = seq_to_bytes(ctx, qs, false, b.p);
+                    if (rc) return rc;
+                }
+            }
         }
 
-    // byte copies of every query strand, shared by all chunk searches
-    struct DevBytes {
-        uint8_t *p = nullptr;
-        gxy_ctx *owner = nullptr;
-        ~DevBytes() { dev_free(owner, p); }
-    };
-    const bool byte_ok = scoring_byte_ok(cfg->scoring);
-    std::vector<DevBytes> qbytes(byte_ok ? 2 * n_queries : 0);
-    if (byte_ok) {
-        for (uint64_t i = 0; i < n_queries; i++) {
-            for (int s = 0; s < 2; s++) {
-                const gxy_dseq *qs = s == 0 ? queries[i] : drc[i].get();
-                if (!qs) continue;
-                DevBytes &b = qbytes[2 * i + s];
-                b.p = (uint8_t *)dev_alloc(ctx, byte_buffer_size(qs->len), &b.owner);
-                if (!b.p) return set_err(ctx, GXY_ENOMEM, "query byte copies: allocation failed");
-                int rc = seq_to_bytes(ctx, qs, false, b.p);
+        // chunk_target (src/driver.rs:156-194)
+        std::vector<Chunk> chunks;
+        {
+            int rc = build_chunks(ctx, targets, n_targets, cfg, &chunks);
+            if (rc) return rc;
+        }
+        rs.chunks = chunks.size();
+
+        gxy_search_params sp;
+        sp.step = cfg->step;
+        sp.hsp = cfg->hsp;
+        sp.transitions = cfg->transitions;
+        sp.entropy_enabled = cfg->entropy_enabled;
+        sp.entropy_threshold = cfg->entropy_threshold;
+
+        std::vector<std::unique_ptr<Group>> groups;
+        std::vector<std::unique_ptr<GapGroup>> gap_groups;
+        std::vector<TweenGroup> tween_groups;
+        std::vector<gxy_edit_run> run_store;  // scripts of gapped records
+        std::atomic<uint64_t> ties{0};        // tie pairs counted by the host jobs
+        HostJobs jobs(2);                     // joined before groups and ties are destroyed
+        // With both strands, a query's minus-strand search runs on a helper
+        // context (its own stream and scratch) from a helper thread while the
+        // plus-strand search runs here, so each search's host round trips overlap
+        // the other's kernels.  The two read the same table and sequences;
+        // their results are taken in strand order, as one after the other.
+        // postable_build_impl and gxy_postable_prune return with ctx's stream
+        // drained, so the table, the chunk sequences and the query copies are
+        // complete before the helper's stream reads them.
+        gxy_ctx *helper = nullptr;
+        if (strands.size() == 2) {
+            if (!ctx->helper && gxy_ctx_create(ctx->device, &ctx->helper) == GXY_OK)
+                ctx->helper->pool_on = true;  // its own block cache: nothing it frees waits on other streams
+            helper = ctx->helper;
+            if (helper) helper->timing = ctx->timing;
+        }
+        HostJobs helper_job(helper ? 1 : 0);
+
+        clk.lap(kTrSetup);
+        for (size_t c = 0; c < chunks.size(); c++) {
+            const Chunk &ch = chunks[c];
+            gxy_postable *tp;
+            clk.lap(kTrRecords);
+            int rc = postable_build_impl(ctx, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
+            clk.lap(kTrTable);
+            if (rc) return rc;
+            Table table(tp);
+            if (cfg->max_word_count > 0) {
+                rc = gxy_postable_prune(ctx, tp, cfg->max_word_count);
                 if (rc) return rc;
             }
+            for (uint32_t qi = 0; qi < n_queries; qi++) {
+                std::vector<gxy_hsp> shs[2];
+                gxy_search_stats sst[2];
+                int src[2] = {GXY_OK, GXY_OK};
+                // Ungapped: Vec::dedup runs on a host worker with the records.
+                auto run_search = [&](size_t k, gxy_ctx *c) {
+                    const uint32_t strand = strands[k];
+                    const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
+                    const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
+                    const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
+                    src[k] = search_dev_impl(c, tp, qs, &cfg->scoring, &sp, respect, &shs[k], &sst[k], qb,
+                                             cfg->gapped_enabled != 0);
+                };
+                clk.lap(kTrRecords);
+                if (helper) {
+                    helper_job.submit([&] {
+                        (void)hipSetDevice(ctx->device);
+                        run_search(1, helper);
+                    });
+                    run_search(0, ctx);
+                    helper_job.wait_all();
+                    if (!src[0] && src[1]) {
+                        set_err(ctx, src[1], "%s", helper->err.c_str());
+                        return src[1];
+                    }
+                } else {
+                    for (size_t k = 0; k < strands.size() && !src[0]; k++) run_search(k, ctx);
+                }
+                if (src[0]) return src[0];
+                clk = PhaseClock(ctx);
+                for (size_t k = 0; k < strands.size(); k++) {
+                    const uint32_t strand = strands[k];
+                    const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
+                    std::vector<gxy_hsp> hsps = std::move(shs[k]);
+                    const gxy_search_stats &st = sst[k];
+                    rs.hits += st.hits;
+                    rs.columns += st.columns;
+                    rs.hsps += st.emitted;
+                    rs.tie_pairs += st.tie_pairs;
+                    rs.passing += st.passing;
+                    if (hsps.empty()) continue;
+                    groups.push_back(std::make_unique<Group>(Group{ch.parent, qi, strand, c, {}}));
+                    Group &g = *groups.back();
+                    if (cfg->tweener_enabled) tween_groups.push_back({&g, targets[ch.parent], qs});
+                    if (!cfg->gapped_enabled) {
+                        // primary-region filter (src/driver.rs:250-256), best_chain
+                        // (src/driver.rs:258-260) and build_ungapped_record
+                        // (src/driver.rs:437-468), off the device path
+                        jobs.submit([&g, &ties, hsps = std::move(hsps), lo = ch.primary_start_local,
+                                     hi = ch.primary_end_local, gs = ch.global_start, chain = cfg->chain_enabled != 0,
+                                     tween = cfg->tweener_enabled != 0]() mutable {
+                            ties += dedup_hsps(&hsps);
+                            std::vector<gxy_hsp> kept;
+                            kept.reserve(hsps.size());
+                            for (const auto &h : hsps)
+                                if (h.t_start >= lo && h.t_start < hi) kept.push_back(h);
+                            if (chain) kept = best_chain(kept);
+                            g.recs.reserve(kept.size());
+                            for (const auto &h : kept)
+                                g.recs.push_back(Rec{g.ti, g.qi, g.strand, h.t_start + gs, h.q_start, h.length,
+                                                     h.length, h.score, 0, 0});
+                            if (!tween) finalize_group(g);  // the tweener runs first otherwise
+                        });
+                    } else {
+                        // primary-region filter (src/driver.rs:250-256); score order (src/driver.rs:270)
+                        auto gg = std::make_unique<GapGroup>();
+                        gg->g = &g;
+                        gg->tseq = ch.seq;
+                        gg->qseq = qs;
+                        gg->global_start = ch.global_start;
+                        gg->kept.reserve(hsps.size());
+                        for (const auto &h : hsps)
+                            if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
+                                gg->kept.push_back(h);
+                        if (cfg->chain_enabled) gg->kept = best_chain(gg->kept);  // src/driver.rs:258-260
+                        std::stable_sort(gg->kept.begin(), gg->kept.end(),
+                                         [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
+                        for (size_t k = 1; k < gg->kept.size(); k++)
+                            if (gg->kept[k].score == gg->kept[k - 1].score) rs.score_ties++;
+                        gap_groups.push_back(std::move(gg));
+                    }
+                }
+            }
+            // the chunk's table is released here; gapped work needs only the sequences
         }
-    }
+        if (!gap_groups.empty()) {
+            int rc = gapped_groups_dev(ctx, gap_groups, cfg, &run_store, &rs);
+            if (rc) return rc;
+            if (!cfg->tweener_enabled)
+                for (auto &gg : gap_groups) jobs.submit([g = gg->g] { finalize_group(*g); });
+        }
+        jobs.wait_all();
+        if (cfg->tweener_enabled && !tween_groups.empty()) {
+            // tweener over the whole target (src/driver.rs:340-356), then the group sort
+            int rc = tweener_groups_dev(ctx, tween_groups, cfg, &run_store, &rs);
+            if (rc) return rc;
+            for (auto &tg : tween_groups) finalize_group(*tg.g);
+        }
+        if (helper && helper->timing) {  // the helper's kernel times count as this context's
+            timing_flush(helper);
+            for (const auto &t : helper->totals) {
+                bool found = false;
+                for (auto &u : ctx->totals)
+                    if (u.first == t.first) {
+                        u.second.first += t.second.first;
+                        u.second.second += t.second.second;
+                        found = true;
+                        break;
+                    }
+                if (!found) ctx->totals.push_back(t);
+            }
+            helper->totals.clear();
+        }
+        // A traceback pool grown past 48 GiB by this run goes back to the device
+        // (the next large run allocates it again), so it does not hold memory that
+        // other contexts on the device need.
+        scratch_release(ctx, kSlotDpTb, 48ull << 30);
+        rs.tie_pairs += ties.load();
+        for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
 
-    // chunk_target (src/driver.rs:156-194)
-    std::vector<Chunk> chunks;
-    {
-        int rc = build_chunks(ctx, targets, n_targets, cfg, &chunks);
-        if (rc) return rc;
-    }
-    rs.chunks = chunks.size();
-
-    gxy_search_params sp;
-    sp.step = cfg->step;
-    sp.hsp = cfg->hsp;
-    sp.transitions = cfg->transitions;
-    sp.entropy_enabled = cfg->entropy_enabled;
-    sp.entropy_threshold = cfg->entropy_threshold;
-
-    std::vector<std::unique_ptr<Group>> groups;
-    std::vector<std::unique_ptr<GapGroup>> gap_groups;
-    std::vector<TweenGroup> tween_groups;
-    std::vector<gxy_edit_run> run_store;  // scripts of gapped records
-    std::atomic<uint64_t> ties{0};        // tie pairs counted by the host jobs
-    HostJobs jobs(2);                     // joined before groups and ties are destroyed
-    // With both strands, a query's minus-strand search runs on a helper
-    // context (its own stream and scratch) from a helper thread while the
-    // plus-strand search runs here, so each search's host round trips overlap
-    // the other's kernels.  The two read the same table and sequences;
-    // their results are taken in strand order, as one after the other.
-    // postable_build_impl and gxy_postable_prune return with ctx's stream
-    // drained, so the table, the chunk sequences and the query copies are
-    // complete before the helper's stream reads them.
-    gxy_ctx *helper = nullptr;
-    if (strands.size() == 2) {
-        if (!ctx->helper && gxy_ctx_create(ctx->device, &ctx->helper) == GXY_OK)
-            ctx->helper->pool_on = true;  // its own block cache: nothing it frees waits on other streams
-        helper = ctx->helper;
-        if (helper) helper->timing = ctx->timing;
-    }
-    HostJobs helper_job(helper ? 1 : 0);
-
-    clk.lap(kTrSetup);
-    for (size_t c = 0; c < chunks.size(); c++) {
-        const Chunk &ch = chunks[c];
-        gxy_postable *tp;
         clk.lap(kTrRecords);
-        int rc = postable_build_impl(ctx, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
-        clk.lap(kTrTable);
-        if (rc) return rc;
-        Table table(tp);
-        if (cfg->max_word_count > 0) {
-            rc = gxy_postable_prune(ctx, tp, cfg->max_word_count);
+        {
+            int rc = assemble_records(ctx, groups, run_store, out);
             if (rc) return rc;
         }
-        for (uint32_t qi = 0; qi < n_queries; qi++) {
-            std::vector<gxy_hsp> shs[2];
-            gxy_search_stats sst[2];
-            int src[2] = {GXY_OK, GXY_OK};
-            // Ungapped: Vec::dedup runs on a host worker with the records.
-            auto run_search = [&](size_t k, gxy_ctx *c) {
-                const uint32_t strand = strands[k];
-                const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
-                const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
-                const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
-                src[k] = search_dev_impl(c, tp, qs, &cfg->scoring, &sp, respect, &shs[k], &sst[k], qb,
-                                         cfg->gapped_enabled != 0);
-            };
-            clk.lap(kTrRecords);
-            if (helper) {
-                helper_job.submit([&] {
-                    (void)hipSetDevice(ctx->device);
-                    run_search(1, helper);
-                });
-                run_search(0, ctx);
-                helper_job.wait_all();
-                if (!src[0] && src[1]) {
-                    set_err(ctx, src[1], "%s", helper->err.c_str());
-                    return src[1];
-                }
-            } else {
-                for (size_t k = 0; k < strands.size() && !src[0]; k++) run_search(k, ctx);
-            }
-            if (src[0]) return src[0];
-            clk = PhaseClock(ctx);
-            for (size_t k = 0; k < strands.size(); k++) {
-                const uint32_t strand = strands[k];
-                const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
-                std::vector<gxy_hsp> hsps = std::move(shs[k]);
-                const gxy_search_stats &st = sst[k];
-                rs.hits += st.hits;
-                rs.columns += st.columns;
-                rs.hsps += st.emitted;
-                rs.tie_pairs += st.tie_pairs;
-                rs.passing += st.passing;
-                if (hsps.empty()) continue;
-                groups.push_back(std::make_unique<Group>(Group{ch.parent, qi, strand, c, {}}));
-                Group &g = *groups.back();
-                if (cfg->tweener_enabled) tween_groups.push_back({&g, targets[ch.parent], qs});
-                if (!cfg->gapped_enabled) {
-                    // primary-region filter (src/driver.rs:250-256), best_chain
-                    // (src/driver.rs:258-260) and build_ungapped_record
-                    // (src/driver.rs:437-468), off the device path
-                    jobs.submit([&g, &ties, hsps = std::move(hsps), lo = ch.primary_start_local,
-                                 hi = ch.primary_end_local, gs = ch.global_start, chain = cfg->chain_enabled != 0,
-                                 tween = cfg->tweener_enabled != 0]() mutable {
-                        ties += dedup_hsps(&hsps);
-                        std::vector<gxy_hsp> kept;
-                        kept.reserve(hsps.size());
-                        for (const auto &h : hsps)
-                            if (h.t_start >= lo && h.t_start < hi) kept.push_back(h);
-                        if (chain) kept = best_chain(kept);
-                        g.recs.reserve(kept.size());
-                        for (const auto &h : kept)
-                            g.recs.push_back(Rec{g.ti, g.qi, g.strand, h.t_start + gs, h.q_start, h.length,
-                                                 h.length, h.score, 0, 0});
-                        if (!tween) finalize_group(g);  // the tweener runs first otherwise
-                    });
-                } else {
-                    // primary-region filter (src/driver.rs:250-256); score order (src/driver.rs:270)
-                    auto gg = std::make_unique<GapGroup>();
-                    gg->g = &g;
-                    gg->tseq = ch.seq;
-                    gg->qseq = qs;
-                    gg->global_start = ch.global_start;
-                    gg->kept.reserve(hsps.size());
-                    for (const auto &h : hsps)
-                        if (h.t_start >= ch.primary_start_local && h.t_start < ch.primary_end_local)
-                            gg->kept.push_back(h);
-                    if (cfg->chain_enabled) gg->kept = best_chain(gg->kept);  // src/driver.rs:258-260
-                    std::stable_sort(gg->kept.begin(), gg->kept.end(),
-                                     [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
-                    for (size_t k = 1; k < gg->kept.size(); k++)
-                        if (gg->kept[k].score == gg->kept[k - 1].score) rs.score_ties++;
-                    gap_groups.push_back(std::move(gg));
-                }
-            }
+        if (stats) *stats = rs;
+        clk.lap(kTrMerge);
+        if (ctx->trace) {
+            static const char *names[] = {"setup", "table", "s.count", "s.extend", "s.admit", "s.copy", "s.dedup",
+                                          "records", "merge", "s.other"};
+            std::fprintf(stderr, "gxy_run phases ms:");
+            for (int k = 0; k < 10; k++) std::fprintf(stderr, " %s %.2f", names[k], ctx->trace_ms[k]);
+            std::fprintf(stderr, "\n");
+            std::memset(ctx->trace_ms, 0, sizeof ctx->trace_ms);
         }
-        // the chunk's table is released here; gapped work needs only the sequences
-    }
-    if (!gap_groups.empty()) {
-        int rc = gapped_groups_dev(ctx, gap_groups, cfg, &run_store, &rs);
-        if (rc) return rc;
-        if (!cfg->tweener_enabled)
-            for (auto &gg : gap_groups) jobs.submit([g = gg->g] { finalize_group(*g); });
-    }
-    jobs.wait_all();
-    if (cfg->tweener_enabled && !tween_groups.empty()) {
-        // tweener over the whole target (src/driver.rs:340-356), then the group sort
-        int rc = tweener_groups_dev(ctx, tween_groups, cfg, &run_store, &rs);
-        if (rc) return rc;
-        for (auto &tg : tween_groups) finalize_group(*tg.g);
-    }
-    if (helper && helper->timing) {  // the helper's kernel times count as this context's
-        timing_flush(helper);
-        for (const auto &t : helper->totals) {
-            bool found = false;
-            for (auto &u : ctx->totals)
-                if (u.first == t.first) {
-                    u.second.first += t.second.first;
-                    u.second.second += t.second.second;
-                    found = true;
-                    break;
-                }
-            if (!found) ctx->totals.push_back(t);
-        }
-        helper->totals.clear();
-    }
-    // A traceback pool grown past 48 GiB by this run goes back to the device
-    // (the next large run allocates it again), so it does not hold memory that
-    // other contexts on the device need.
-    scratch_release(ctx, kSlotDpTb, 48ull << 30);
-    rs.tie_pairs += ties.load();
-    for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
-
-    clk.lap(kTrRecords);
-    {
-        int rc = assemble_records(ctx, groups, run_store, out);
-        if (rc) return rc;
-    }
-    if (stats) *stats = rs;
-    clk.lap(kTrMerge);
-    if (ctx->trace) {
-        static const char *names[] = {"setup", "table", "s.count", "s.extend", "s.admit", "s.copy", "s.dedup",
-                                      "records", "merge", "s.other"};
-        std::fprintf(stderr, "gxy_run phases ms:");
-        for (int k = 0; k < 10; k++) std::fprintf(stderr, " %s %.2f", names[k], ctx->trace_ms[k]);
-        std::fprintf(stderr, "\n");
-        std::memset(ctx->trace_ms, 0, sizeof ctx->trace_ms);
-    }
-    return GXY_OK;
+        return GXY_OK;
+    });
 }
 
 int gxy_run(gxy_ctx *ctx, const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,
             uint64_t n_queries, const gxy_config *cfg, gxy_record_set **out, gxy_run_stats *stats) {
-    if (!ctx) return GXY_EINVAL;
-    if ((!targets && n_targets) || (!queries && n_queries) || !cfg || !out)
-        return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    std::vector<DSeq> dt(n_targets), dq(n_queries);
-    std::vector<const gxy_dseq *> tp(n_targets), qp(n_queries);
-    for (uint64_t i = 0; i < n_targets; i++) {
-        gxy_dseq *d;
-        int rc = dseq_upload_impl(ctx, &targets[i].seq, &d);
-        if (rc) return rc;
-        dt[i].reset(d);
-        tp[i] = d;
-    }
-    for (uint64_t i = 0; i < n_queries; i++) {
-        gxy_dseq *d;
-        int rc = dseq_upload_impl(ctx, &queries[i].seq, &d);
-        if (rc) return rc;
-        dq[i].reset(d);
-        qp[i] = d;
-    }
-    return gxy_run_dev(ctx, tp.data(), n_targets, qp.data(), n_queries, cfg, out, stats);
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if ((!targets && n_targets) || (!queries && n_queries) || !cfg || !out)
+            return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        std::vector<DSeq> dt(n_targets), dq(n_queries);
+        std::vector<const gxy_dseq *> tp(n_targets), qp(n_queries);
+        for (uint64_t i = 0; i < n_targets; i++) {
+            gxy_dseq *d;
+            int rc = dseq_upload_impl(ctx, &targets[i].seq, &d);
+            if (rc) return rc;
+            dt[i].reset(d);
+            tp[i] = d;
+        }
+        for (uint64_t i = 0; i < n_queries; i++) {
+            gxy_dseq *d;
+            int rc = dseq_upload_impl(ctx, &queries[i].seq, &d);
+            if (rc) return rc;
+            dq[i].reset(d);
+            qp[i] = d;
+        }
+        return gxy_run_dev(ctx, tp.data(), n_targets, qp.data(), n_queries, cfg, out, stats);
+    });
 }
 
 void gxy_record_set_free(gxy_record_set *set) {
@@ -1014,6 +1024,7 @@ struct ServeReq {
     int rc = GXY_OK;
     std::string err;
     bool done = false;
+    bool claimed = false;  // a gxy_server_wait has taken this ticket
     // between search and completion
     DSeq dq, drc;  // query strands (freed by the search thread: its block cache)
     std::vector<std::unique_ptr<Group>> groups;
@@ -1392,140 +1403,160 @@ extern "C" {
 
 int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_targets, const gxy_config *cfg,
                       const gxy_server_params *params, gxy_server **out) {
-    if (!targets || !n_targets || !cfg || !params || !out) return GXY_EINVAL;
-    for (uint64_t i = 0; i < n_targets; i++)
-        if (!targets[i]) return GXY_EINVAL;
-    if (cfg->step < 1) return GXY_EINVAL;
-    auto S = std::make_unique<gxy_server>();
-    int rc = gxy_ctx_create(device, &S->ctx);
-    if (rc) return rc;
-    S->targets.assign(targets, targets + n_targets);
-    S->cfg = *cfg;
-    S->prm = *params;
-    S->t0 = std::chrono::steady_clock::now();
-    auto fail = [&](int code) {
-        S->chunks.clear();
-        S->tables.clear();
-        for (gxy_ctx *c : S->dp_ctx) gxy_ctx_destroy(c);
-        gxy_ctx_destroy(S->ctx);
-        S->ctx = nullptr;
-        return code;
-    };
-    if (cfg->strand == GXY_STRAND_PLUS || cfg->strand == GXY_STRAND_BOTH) S->strands.push_back(GXY_STRAND_PLUS);
-    if (cfg->strand == GXY_STRAND_MINUS || cfg->strand == GXY_STRAND_BOTH) S->strands.push_back(GXY_STRAND_MINUS);
-    if (S->strands.empty()) return fail(GXY_EINVAL);
-    rc = build_chunks(S->ctx, targets, n_targets, cfg, &S->chunks);
-    if (rc) return fail(rc);
-    // resident position tables, one per chunk
-    for (const Chunk &ch : S->chunks) {
-        gxy_postable *tp;
-        rc = postable_build_impl(S->ctx, ch.seq, &cfg->pattern, cfg->step, cfg->respect_masking ? 1 : 0, &tp);
+    return guarded(nullptr, [&]() -> int {
+        if (!targets || !n_targets || !cfg || !params || !out) return GXY_EINVAL;
+        for (uint64_t i = 0; i < n_targets; i++)
+            if (!targets[i]) return GXY_EINVAL;
+        if (cfg->step < 1) return GXY_EINVAL;
+        auto S = std::make_unique<gxy_server>();
+        int rc = gxy_ctx_create(device, &S->ctx);
+        if (rc) return rc;
+        S->targets.assign(targets, targets + n_targets);
+        S->cfg = *cfg;
+        S->prm = *params;
+        S->t0 = std::chrono::steady_clock::now();
+        auto fail = [&](int code) {
+            S->chunks.clear();
+            S->tables.clear();
+            for (gxy_ctx *c : S->dp_ctx) gxy_ctx_destroy(c);
+            gxy_ctx_destroy(S->ctx);
+            S->ctx = nullptr;
+            return code;
+        };
+        if (cfg->strand == GXY_STRAND_PLUS || cfg->strand == GXY_STRAND_BOTH) S->strands.push_back(GXY_STRAND_PLUS);
+        if (cfg->strand == GXY_STRAND_MINUS || cfg->strand == GXY_STRAND_BOTH) S->strands.push_back(GXY_STRAND_MINUS);
+        if (S->strands.empty()) return fail(GXY_EINVAL);
+        rc = build_chunks(S->ctx, targets, n_targets, cfg, &S->chunks);
         if (rc) return fail(rc);
-        S->tables.emplace_back(tp);
-        if (cfg->max_word_count > 0) {
-            rc = gxy_postable_prune(S->ctx, tp, cfg->max_word_count);
+        // resident position tables, one per chunk
+        for (const Chunk &ch : S->chunks) {
+            gxy_postable *tp;
+            rc = postable_build_impl(S->ctx, ch.seq, &cfg->pattern, cfg->step, cfg->respect_masking ? 1 : 0, &tp);
             if (rc) return fail(rc);
+            S->tables.emplace_back(tp);
+            if (cfg->max_word_count > 0) {
+                rc = gxy_postable_prune(S->ctx, tp, cfg->max_word_count);
+                if (rc) return fail(rc);
+            }
         }
-    }
-    GXY_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
-    // Separator between concatenated queries: an x-drop walk entering N bases
-    // loses |ambig| per column and stops once x_drop is exceeded, and no seed
-    // window contains an N; offsets keep the query windows on the step grid.
-    if (params->co_batch && cfg->scoring.ambig < 0 && cfg->hsp.x_drop >= 0) {
-        const uint64_t walk = (uint64_t)cfg->hsp.x_drop / (uint64_t)(-(int64_t)cfg->scoring.ambig) + 2;
-        S->sep = (uint32_t)std::max<uint64_t>(64, walk);
-        S->align = (uint32_t)(32 * cfg->step / std::__gcd<uint64_t>(32, cfg->step));
-    }
-    const int nw = cfg->gapped_enabled && params->co_batch ? (params->dp_workers > 0 ? params->dp_workers : 4) : 0;
-    size_t free_b = 0, total_b = 0;
-    GXY_HIP(S->ctx, hipMemGetInfo(&free_b, &total_b));
-    for (int w = 0; w < nw; w++) {
-        gxy_ctx *c;
-        rc = gxy_ctx_create(device, &c);
-        if (rc) return fail(rc);
-        // Together the workers' traceback pools may take half of the device
-        // (each at most 48 GiB).  A pool grows only as far as its passes need,
-        // and the longest queries all land in one worker's length bin.
-        c->dp_pool_cap = std::min<uint64_t>(48ull << 30, total_b / (2 * (uint64_t)nw));
-        S->dp_ctx.push_back(c);
-    }
-    S->worker = std::thread(search_loop, S.get());
-    for (gxy_ctx *c : S->dp_ctx) S->dp_workers.emplace_back(dp_loop, S.get(), c);
-    *out = S.release();
-    return GXY_OK;
+        GXY_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+        // Separator between concatenated queries: an x-drop walk entering N bases
+        // loses |ambig| per column and stops once x_drop is exceeded, and no seed
+        // window contains an N; offsets keep the query windows on the step grid.
+        // A window lies inside one query exactly when its first and last offsets
+        // do, so concatenation needs a pattern whose first and last positions are
+        // care positions: with a don't-care end a window could reach into the
+        // separator with all its care positions inside a query, a window that the
+        // query alone does not have (pos + span > len, src/seeds.rs:133-147).
+        // Other patterns search each admitted query on its own.
+        const uint32_t care_ends = cfg->pattern.length ? 1u | (1u << (cfg->pattern.length - 1)) : 1u;
+        if (params->co_batch && cfg->scoring.ambig < 0 && cfg->hsp.x_drop >= 0 &&
+            (cfg->pattern.care & care_ends) == care_ends) {
+            const uint64_t walk = (uint64_t)cfg->hsp.x_drop / (uint64_t)(-(int64_t)cfg->scoring.ambig) + 2;
+            S->sep = (uint32_t)std::max<uint64_t>(64, walk);
+            S->align = (uint32_t)(32 * cfg->step / std::__gcd<uint64_t>(32, cfg->step));
+        }
+        const int nw = cfg->gapped_enabled && params->co_batch ? (params->dp_workers > 0 ? params->dp_workers : 4) : 0;
+        size_t free_b = 0, total_b = 0;
+        GXY_HIP(S->ctx, hipMemGetInfo(&free_b, &total_b));
+        for (int w = 0; w < nw; w++) {
+            gxy_ctx *c;
+            rc = gxy_ctx_create(device, &c);
+            if (rc) return fail(rc);
+            // Together the workers' traceback pools may take half of the device
+            // (each at most 48 GiB).  A pool grows only as far as its passes need,
+            // and the longest queries all land in one worker's length bin.
+            c->dp_pool_cap = std::min<uint64_t>(48ull << 30, total_b / (2 * (uint64_t)nw));
+            S->dp_ctx.push_back(c);
+        }
+        S->worker = std::thread(search_loop, S.get());
+        for (gxy_ctx *c : S->dp_ctx) S->dp_workers.emplace_back(dp_loop, S.get(), c);
+        *out = S.release();
+        return GXY_OK;
+    });
 }
 
 int gxy_server_submit(gxy_server *S, const gxy_seq *query, uint64_t *ticket) {
-    if (!S || !query || !ticket || (query->len && (!query->codes || !query->valid || !query->mask)))
-        return GXY_EINVAL;
-    auto r = std::make_unique<ServeReq>();
-    r->len = query->len;
-    r->codes.assign(query->codes, query->codes + (query->len + 3) / 4);
-    r->valid.assign(query->valid, query->valid + (query->len + 7) / 8);
-    r->mask.assign(query->mask, query->mask + (query->len + 7) / 8);
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        if (S->stop) return GXY_EINVAL;
-        r->id = S->next_id++;
-        r->submit_ms = S->now_ms();
-        *ticket = r->id;
-        S->queue.push_back(r.get());
-        S->reqs.emplace(r->id, std::move(r));
-    }
-    S->cv_work.notify_one();
-    return GXY_OK;
+    return guarded(nullptr, [&]() -> int {
+        if (!S || !query || !ticket || (query->len && (!query->codes || !query->valid || !query->mask)))
+            return GXY_EINVAL;
+        auto r = std::make_unique<ServeReq>();
+        r->len = query->len;
+        r->codes.assign(query->codes, query->codes + (query->len + 3) / 4);
+        r->valid.assign(query->valid, query->valid + (query->len + 7) / 8);
+        r->mask.assign(query->mask, query->mask + (query->len + 7) / 8);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (S->stop) return GXY_EINVAL;
+            r->id = S->next_id++;
+            r->submit_ms = S->now_ms();
+            *ticket = r->id;
+            S->queue.push_back(r.get());
+            S->reqs.emplace(r->id, std::move(r));
+        }
+        S->cv_work.notify_one();
+        return GXY_OK;
+    });
 }
 
 int gxy_server_wait(gxy_server *S, uint64_t ticket, gxy_record_set **out, gxy_request_stats *stats) {
-    if (!S || !out) return GXY_EINVAL;
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->reqs.find(ticket);
-    if (it == S->reqs.end()) return GXY_EINVAL;
-    ServeReq *r = it->second.get();
-    S->cv_done.wait(lk, [&] { return r->done; });
-    if (stats) {
-        stats->submit_ms = r->submit_ms;
-        stats->admit_ms = r->admit_ms;
-        stats->done_ms = r->done_ms;
-        stats->query_bp = r->len;
-        stats->iteration = r->iteration;
-        stats->batch_queries = r->batch_queries;
-        stats->batch_bp = r->batch_bp;
-    }
-    const int rc = r->rc;
-    if (rc) S->err = r->err;
-    *out = r->out;
-    r->out = nullptr;
-    S->reqs.erase(it);
-    return rc;
+    return guarded(nullptr, [&]() -> int {
+        if (!S || !out) return GXY_EINVAL;
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->reqs.find(ticket);
+        if (it == S->reqs.end() || it->second->claimed) return GXY_EINVAL;  // unknown, or waited on already
+        ServeReq *r = it->second.get();
+        r->claimed = true;
+        S->cv_done.wait(lk, [&] { return r->done; });
+        if (stats) {
+            stats->submit_ms = r->submit_ms;
+            stats->admit_ms = r->admit_ms;
+            stats->done_ms = r->done_ms;
+            stats->query_bp = r->len;
+            stats->iteration = r->iteration;
+            stats->batch_queries = r->batch_queries;
+            stats->batch_bp = r->batch_bp;
+        }
+        const int rc = r->rc;
+        if (rc) S->err = r->err;
+        *out = r->out;
+        r->out = nullptr;
+        S->reqs.erase(it);
+        return rc;
+    });
 }
 
 const char *gxy_server_last_error(gxy_server *S) {
     if (!S) return "null server";
+    // a copy per calling thread: worker threads may replace S->err at any time
+    static thread_local std::string copy;
     std::lock_guard<std::mutex> lk(S->mu);
-    return S->err.c_str();
+    copy = S->err;
+    return copy.c_str();
 }
 
 int gxy_server_destroy(gxy_server *S) {
-    if (!S) return GXY_EINVAL;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->stop = true;
-    }
-    S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
-    S->cv_dp.notify_all();
-    for (auto &t : S->dp_workers)
-        if (t.joinable()) t.join();
-    for (auto &kv : S->reqs) gxy_record_set_free(kv.second->out);
-    S->reqs.clear();
-    S->tofree.clear();
-    S->tables.clear();
-    S->chunks.clear();
-    for (gxy_ctx *c : S->dp_ctx) gxy_ctx_destroy(c);
-    gxy_ctx_destroy(S->ctx);
-    delete S;
-    return GXY_OK;
+    return guarded(nullptr, [&]() -> int {
+        if (!S) return GXY_EINVAL;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->stop = true;
+        }
+        S->cv_work.notify_all();
+        if (S->worker.joinable()) S->worker.join();
+        S->cv_dp.notify_all();
+        for (auto &t : S->dp_workers)
+            if (t.joinable()) t.join();
+        for (auto &kv : S->reqs) gxy_record_set_free(kv.second->out);
+        S->reqs.clear();
+        S->tofree.clear();
+        S->tables.clear();
+        S->chunks.clear();
+        for (gxy_ctx *c : S->dp_ctx) gxy_ctx_destroy(c);
+        gxy_ctx_destroy(S->ctx);
+        delete S;
+        return GXY_OK;
+    });
 }
 }  // extern "C"
 
@@ -1586,6 +1617,16 @@ bool records_ok(uint64_t nt, uint64_t nq, const gxy_record_set *rs, const gxy_na
         if (r.runs_offset + r.n_runs > rs->n_runs) return false;
         if ((uint64_t)r.t_start + r.t_span > ts[r.target_index].seq.len) return false;
         if ((uint64_t)r.q_start + r.q_span > qs[r.query_index].seq.len) return false;
+        // the script must consume exactly the record's spans (EditScript
+        // target_span / query_span, src/edit_script.rs:104-121)
+        uint64_t tsum = 0, qsum = 0;
+        for (uint32_t x = 0; x < r.n_runs; x++) {
+            const gxy_edit_run &run = rs->runs[r.runs_offset + x];
+            if (run.op > GXY_OP_DELETE) return false;
+            tsum += run.op != GXY_OP_INSERT ? run.n : 0;
+            qsum += run.op != GXY_OP_DELETE ? run.n : 0;
+        }
+        if (tsum != r.t_span || qsum != r.q_span) return false;
     }
     return true;
 }
@@ -1605,123 +1646,127 @@ extern "C" {
 
 int gxy_format_maf(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,
                    uint64_t n_queries, const gxy_record_set *records, char **text, uint64_t *len) {
-    if (!records || !text || !len || !records_ok(n_targets, n_queries, records, targets, queries))
-        return GXY_EINVAL;
-    std::string o, ta, qa, tt, qq;
-    if (records->n > 0) o += "##maf version=1 program=lastz-gxy\n# scoring=HOXD70\n";
-    for (uint64_t k = 0; k < records->n; k++) {
-        const gxy_record &r = records->records[k];
-        ascii_slice(targets[r.target_index].seq, false, r.t_start, r.t_span, ta);
-        ascii_slice(queries[r.query_index].seq, r.strand == GXY_STRAND_MINUS, r.q_start, r.q_span, qa);
-        // render_aligned_bases (src/edit_script.rs:140-172)
-        tt.clear();
-        qq.clear();
-        size_t ti = 0, qi = 0;
-        for (uint32_t x = 0; x < r.n_runs; x++) {
-            const gxy_edit_run &run = records->runs[r.runs_offset + x];
-            if (run.op == GXY_OP_MATCH) {
-                tt.append(ta, ti, run.n);
-                qq.append(qa, qi, run.n);
-                ti += run.n;
-                qi += run.n;
-            } else if (run.op == GXY_OP_INSERT) {
-                tt.append(run.n, '-');
-                qq.append(qa, qi, run.n);
-                qi += run.n;
-            } else {
-                tt.append(ta, ti, run.n);
-                qq.append(run.n, '-');
-                ti += run.n;
+    return guarded(nullptr, [&]() -> int {
+        if (!records || !text || !len || !records_ok(n_targets, n_queries, records, targets, queries))
+            return GXY_EINVAL;
+        std::string o, ta, qa, tt, qq;
+        if (records->n > 0) o += "##maf version=1 program=lastz-gxy\n# scoring=HOXD70\n";
+        for (uint64_t k = 0; k < records->n; k++) {
+            const gxy_record &r = records->records[k];
+            ascii_slice(targets[r.target_index].seq, false, r.t_start, r.t_span, ta);
+            ascii_slice(queries[r.query_index].seq, r.strand == GXY_STRAND_MINUS, r.q_start, r.q_span, qa);
+            // render_aligned_bases (src/edit_script.rs:140-172)
+            tt.clear();
+            qq.clear();
+            size_t ti = 0, qi = 0;
+            for (uint32_t x = 0; x < r.n_runs; x++) {
+                const gxy_edit_run &run = records->runs[r.runs_offset + x];
+                if (run.op == GXY_OP_MATCH) {
+                    tt.append(ta, ti, run.n);
+                    qq.append(qa, qi, run.n);
+                    ti += run.n;
+                    qi += run.n;
+                } else if (run.op == GXY_OP_INSERT) {
+                    tt.append(run.n, '-');
+                    qq.append(qa, qi, run.n);
+                    qi += run.n;
+                } else {
+                    tt.append(ta, ti, run.n);
+                    qq.append(run.n, '-');
+                    ti += run.n;
+                }
             }
+            o += "a score=";
+            append_i(o, r.score);
+            o += "\ns ";
+            append_left(o, targets[r.target_index].name, 20);
+            o += ' ';
+            append_right(o, r.t_start, 10);
+            o += ' ';
+            append_right(o, r.t_span, 6);
+            o += " + ";
+            append_right(o, (uint32_t)targets[r.target_index].seq.len, 10);
+            o += ' ';
+            o += tt;
+            o += "\ns ";
+            append_left(o, queries[r.query_index].name, 20);
+            o += ' ';
+            append_right(o, r.q_start, 10);
+            o += ' ';
+            append_right(o, r.q_span, 6);
+            o += r.strand == GXY_STRAND_MINUS ? " - " : " + ";
+            append_right(o, (uint32_t)queries[r.query_index].seq.len, 10);
+            o += ' ';
+            o += qq;
+            o += "\n\n";
         }
-        o += "a score=";
-        append_i(o, r.score);
-        o += "\ns ";
-        append_left(o, targets[r.target_index].name, 20);
-        o += ' ';
-        append_right(o, r.t_start, 10);
-        o += ' ';
-        append_right(o, r.t_span, 6);
-        o += " + ";
-        append_right(o, (uint32_t)targets[r.target_index].seq.len, 10);
-        o += ' ';
-        o += tt;
-        o += "\ns ";
-        append_left(o, queries[r.query_index].name, 20);
-        o += ' ';
-        append_right(o, r.q_start, 10);
-        o += ' ';
-        append_right(o, r.q_span, 6);
-        o += r.strand == GXY_STRAND_MINUS ? " - " : " + ";
-        append_right(o, (uint32_t)queries[r.query_index].seq.len, 10);
-        o += ' ';
-        o += qq;
-        o += "\n\n";
-    }
-    *text = finish(o, len);
-    return *text ? GXY_OK : GXY_ENOMEM;
+        *text = finish(o, len);
+        return *text ? GXY_OK : GXY_ENOMEM;
+    });
 }
 
 int gxy_format_paf(const gxy_named_seq *targets, uint64_t n_targets, const gxy_named_seq *queries,
                    uint64_t n_queries, const gxy_record_set *records, char **text, uint64_t *len) {
-    if (!records || !text || !len || !records_ok(n_targets, n_queries, records, targets, queries))
-        return GXY_EINVAL;
-    std::string o, ta, qa, cig;
-    for (uint64_t k = 0; k < records->n; k++) {
-        const gxy_record &r = records->records[k];
-        ascii_slice(targets[r.target_index].seq, false, r.t_start, r.t_span, ta);
-        ascii_slice(queries[r.query_index].seq, r.strand == GXY_STRAND_MINUS, r.q_start, r.q_span, qa);
-        // score_script (src/output/paf.rs:59-89); positions past a slice are None
-        uint64_t mt = 0, mm = 0, ind = 0;
-        size_t ti = 0, qi = 0;
-        cig.clear();
-        for (uint32_t x = 0; x < r.n_runs; x++) {
-            const gxy_edit_run &run = records->runs[r.runs_offset + x];
-            append_u(cig, run.n);
-            cig += run.op == GXY_OP_MATCH ? 'M' : (run.op == GXY_OP_INSERT ? 'I' : 'D');
-            if (run.op == GXY_OP_MATCH) {
-                for (uint32_t c = 0; c < run.n; c++, ti++, qi++) {
-                    const bool th = ti < ta.size(), qh = qi < qa.size();
-                    const bool eq = (th && qh) ? ta[ti] == qa[qi] : th == qh;
-                    if (eq) mt++; else mm++;
+    return guarded(nullptr, [&]() -> int {
+        if (!records || !text || !len || !records_ok(n_targets, n_queries, records, targets, queries))
+            return GXY_EINVAL;
+        std::string o, ta, qa, cig;
+        for (uint64_t k = 0; k < records->n; k++) {
+            const gxy_record &r = records->records[k];
+            ascii_slice(targets[r.target_index].seq, false, r.t_start, r.t_span, ta);
+            ascii_slice(queries[r.query_index].seq, r.strand == GXY_STRAND_MINUS, r.q_start, r.q_span, qa);
+            // score_script (src/output/paf.rs:59-89); positions past a slice are None
+            uint64_t mt = 0, mm = 0, ind = 0;
+            size_t ti = 0, qi = 0;
+            cig.clear();
+            for (uint32_t x = 0; x < r.n_runs; x++) {
+                const gxy_edit_run &run = records->runs[r.runs_offset + x];
+                append_u(cig, run.n);
+                cig += run.op == GXY_OP_MATCH ? 'M' : (run.op == GXY_OP_INSERT ? 'I' : 'D');
+                if (run.op == GXY_OP_MATCH) {
+                    for (uint32_t c = 0; c < run.n; c++, ti++, qi++) {
+                        const bool th = ti < ta.size(), qh = qi < qa.size();
+                        const bool eq = (th && qh) ? ta[ti] == qa[qi] : th == qh;
+                        if (eq) mt++; else mm++;
+                    }
+                } else if (run.op == GXY_OP_INSERT) {
+                    ind += run.n;
+                    qi += run.n;
+                } else {
+                    ind += run.n;
+                    ti += run.n;
                 }
-            } else if (run.op == GXY_OP_INSERT) {
-                ind += run.n;
-                qi += run.n;
-            } else {
-                ind += run.n;
-                ti += run.n;
             }
+            o += queries[r.query_index].name;
+            o += '\t';
+            append_u(o, (uint32_t)queries[r.query_index].seq.len);
+            o += '\t';
+            append_u(o, r.q_start);
+            o += '\t';
+            append_u(o, (uint64_t)r.q_start + r.q_span);
+            o += r.strand == GXY_STRAND_MINUS ? "\t-\t" : "\t+\t";
+            o += targets[r.target_index].name;
+            o += '\t';
+            append_u(o, (uint32_t)targets[r.target_index].seq.len);
+            o += '\t';
+            append_u(o, r.t_start);
+            o += '\t';
+            append_u(o, (uint64_t)r.t_start + r.t_span);
+            o += '\t';
+            append_u(o, mt);
+            o += '\t';
+            append_u(o, mt + mm + ind);
+            o += "\t60\tNM:i:";
+            append_u(o, mm + ind);
+            o += "\tAS:i:";
+            append_i(o, r.score);
+            o += "\ttp:A:P\tcg:Z:";
+            o += cig;
+            o += '\n';
         }
-        o += queries[r.query_index].name;
-        o += '\t';
-        append_u(o, (uint32_t)queries[r.query_index].seq.len);
-        o += '\t';
-        append_u(o, r.q_start);
-        o += '\t';
-        append_u(o, (uint64_t)r.q_start + r.q_span);
-        o += r.strand == GXY_STRAND_MINUS ? "\t-\t" : "\t+\t";
-        o += targets[r.target_index].name;
-        o += '\t';
-        append_u(o, (uint32_t)targets[r.target_index].seq.len);
-        o += '\t';
-        append_u(o, r.t_start);
-        o += '\t';
-        append_u(o, (uint64_t)r.t_start + r.t_span);
-        o += '\t';
-        append_u(o, mt);
-        o += '\t';
-        append_u(o, mt + mm + ind);
-        o += "\t60\tNM:i:";
-        append_u(o, mm + ind);
-        o += "\tAS:i:";
-        append_i(o, r.score);
-        o += "\ttp:A:P\tcg:Z:";
-        o += cig;
-        o += '\n';
-    }
-    *text = finish(o, len);
-    return *text ? GXY_OK : GXY_ENOMEM;
+        *text = finish(o, len);
+        return *text ? GXY_OK : GXY_ENOMEM;
+    });
 }
 
 }  // extern "C"

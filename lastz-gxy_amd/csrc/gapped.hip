@@ -876,13 +876,17 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         size_t free_b = 0, total_b = 0;
         GXY_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
         const uint64_t cap_b = ctx->dp_pool_cap ? ctx->dp_pool_cap : std::min<uint64_t>(96ull << 30, total_b / 3);
-        uint64_t want = std::min<uint64_t>(cap_b / kChunkBytes,
+        // a row cap set by the caller (gxy_ctx_set_dp_pool_rows) bounds the pool
+        // even where the slot is larger from earlier calls
+        const uint64_t row_cap = ctx->dp_pool_rows ? std::max<uint64_t>(16, ctx->dp_pool_rows / kChunkRows) : ~0ull;
+        uint64_t want = std::min<uint64_t>(std::min<uint64_t>(cap_b / kChunkBytes, row_cap),
                                            (need_rows + kChunkRows - 1) / kChunkRows + pending.size());
         want = std::max<uint64_t>(want, 16);
         uint8_t *dtb = nullptr;
         while (!(dtb = (uint8_t *)scratch(ctx, kSlotDpTb, want * kChunkBytes)) && want > 16) want /= 2;
         if (!dtb) return set_err(ctx, GXY_ENOMEM, "DP traceback pool allocation failed");
-        const uint64_t nchunks = std::min<uint64_t>(ctx->slot_bytes[kSlotDpTb] / kChunkBytes, 0xFFFFFFF0ull);
+        const uint64_t nchunks =
+            std::min<uint64_t>(std::min<uint64_t>(ctx->slot_bytes[kSlotDpTb] / kChunkBytes, row_cap), 0xFFFFFFF0ull);
         auto *dwin = (uint8_t *)scratch(ctx, kSlotDpRows, nchunks * (kChunkRows * sizeof(uint2) + 4));
         auto *dprob = (DpProb *)scratch(ctx, kSlotDpProb, np * sizeof(DpProb));
         if (!dwin || !dprob) return set_err(ctx, GXY_ENOMEM, "DP scratch allocation failed");
@@ -979,10 +983,13 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                 }
             }
             std::vector<uint32_t> again;  // truncated: swept again
+            ctx->dp_passes++;
+            ctx->dp_alone += alone ? 1 : 0;
             for (uint64_t y = 0; y < npp; y++) {
                 const uint32_t x = pending[head + y];
                 if (pr[y].truncated) {
                     again.push_back(x);
+                    ctx->dp_truncated++;
                     continue;
                 }
                 cells += pr[y].cells;
@@ -1114,89 +1121,99 @@ extern "C" {
 
 int gxy_choose_anchor_batch(gxy_ctx *ctx, const gxy_hsp *hsps, uint64_t n, const gxy_seq *target,
                             const gxy_seq *query, const gxy_scoring *scoring, uint32_t window, gxy_anchor *out) {
-    if (!ctx) return GXY_EINVAL;
-    if (!target || !query || !scoring || (n && (!hsps || !out))) return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    if (n == 0) return GXY_OK;
-    gxy_dseq *dt = nullptr, *dq = nullptr;
-    int rc = dseq_upload_impl(ctx, target, &dt);
-    if (!rc) rc = dseq_upload_impl(ctx, query, &dq);
-    if (!rc) rc = choose_anchor_dev(ctx, dt, dq, hsps, n, *scoring, window, out);
-    gxy_dseq_free(dt);
-    gxy_dseq_free(dq);
-    return rc;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!target || !query || !scoring || (n && (!hsps || !out))) return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        if (n == 0) return GXY_OK;
+        gxy_dseq *dt = nullptr, *dq = nullptr;
+        int rc = dseq_upload_impl(ctx, target, &dt);
+        if (!rc) rc = dseq_upload_impl(ctx, query, &dq);
+        if (!rc) rc = choose_anchor_dev(ctx, dt, dq, hsps, n, *scoring, window, out);
+        gxy_dseq_free(dt);
+        gxy_dseq_free(dq);
+        return rc;
+    });
 }
 
 int gxy_choose_anchor(gxy_ctx *ctx, const gxy_hsp *hsp, const gxy_seq *target, const gxy_seq *query,
                       const gxy_scoring *scoring, uint32_t window, gxy_anchor *out) {
-    return gxy_choose_anchor_batch(ctx, hsp, 1, target, query, scoring, window, out);
+    return guarded(ctx, [&]() -> int {
+        return gxy_choose_anchor_batch(ctx, hsp, 1, target, query, scoring, window, out);
+    });
 }
 
 int gxy_gapped_extend_batch(gxy_ctx *ctx, const gxy_anchor *anchors, uint64_t n, const gxy_seq *target,
                             const gxy_seq *query, const gxy_scoring *scoring, const gxy_gapped_params *params,
                             gxy_alignment_set **out) {
-    if (!ctx) return GXY_EINVAL;
-    if (!target || !query || !scoring || !params || !out || (n && !anchors))
-        return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    gxy_dseq *dt = nullptr, *dq = nullptr;
-    int rc = dseq_upload_impl(ctx, target, &dt);
-    if (!rc) rc = dseq_upload_impl(ctx, query, &dq);
-    std::vector<GappedResult> res;
-    std::vector<gxy_edit_run> runs;
-    uint64_t cells = 0;
-    if (!rc) {
-        std::vector<GappedItem> items(n);
-        for (uint64_t k = 0; k < n; k++) items[k] = {dt, dq, anchors[k]};
-        rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
-    }
-    gxy_dseq_free(dt);
-    gxy_dseq_free(dq);
-    if (rc) return rc;
-    return make_set(ctx, res, runs, cells, out);
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!target || !query || !scoring || !params || !out || (n && !anchors))
+            return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        gxy_dseq *dt = nullptr, *dq = nullptr;
+        int rc = dseq_upload_impl(ctx, target, &dt);
+        if (!rc) rc = dseq_upload_impl(ctx, query, &dq);
+        std::vector<GappedResult> res;
+        std::vector<gxy_edit_run> runs;
+        uint64_t cells = 0;
+        if (!rc) {
+            std::vector<GappedItem> items(n);
+            for (uint64_t k = 0; k < n; k++) items[k] = {dt, dq, anchors[k]};
+            rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
+        }
+        gxy_dseq_free(dt);
+        gxy_dseq_free(dq);
+        if (rc) return rc;
+        return make_set(ctx, res, runs, cells, out);
+    });
 }
 
 int gxy_gapped_extend_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, const gxy_dseq *const *queries,
                                 const gxy_anchor *anchors, uint64_t n, const gxy_scoring *scoring,
                                 const gxy_gapped_params *params, gxy_alignment_set **out) {
-    if (!ctx) return GXY_EINVAL;
-    if (!scoring || !params || !out || (n && (!targets || !queries || !anchors)))
-        return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    std::vector<GappedItem> items(n);
-    for (uint64_t k = 0; k < n; k++) {
-        if (!targets[k] || !queries[k]) return set_err(ctx, GXY_EINVAL, "null sequence in pair %llu",
-                                                       (unsigned long long)k);
-        items[k] = {targets[k], queries[k], anchors[k]};
-    }
-    std::vector<GappedResult> res;
-    std::vector<gxy_edit_run> runs;
-    uint64_t cells = 0;
-    int rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
-    if (rc) return rc;
-    return make_set(ctx, res, runs, cells, out);
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!scoring || !params || !out || (n && (!targets || !queries || !anchors)))
+            return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        std::vector<GappedItem> items(n);
+        for (uint64_t k = 0; k < n; k++) {
+            if (!targets[k] || !queries[k]) return set_err(ctx, GXY_EINVAL, "null sequence in pair %llu",
+                                                           (unsigned long long)k);
+            items[k] = {targets[k], queries[k], anchors[k]};
+        }
+        std::vector<GappedResult> res;
+        std::vector<gxy_edit_run> runs;
+        uint64_t cells = 0;
+        int rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
+        if (rc) return rc;
+        return make_set(ctx, res, runs, cells, out);
+    });
 }
 
 int gxy_gapped_extend(gxy_ctx *ctx, const gxy_anchor *anchor, const gxy_seq *target, const gxy_seq *query,
                       const gxy_scoring *scoring, const gxy_gapped_params *params, gxy_alignment *out,
                       gxy_edit_run *runs, uint64_t runs_cap, int *found) {
-    if (!ctx) return GXY_EINVAL;
-    if (!anchor || !out || !found || (runs_cap && !runs)) return set_err(ctx, GXY_EINVAL, "null argument");
-    gxy_alignment_set *s = nullptr;
-    int rc = gxy_gapped_extend_batch(ctx, anchor, 1, target, query, scoring, params, &s);
-    if (rc) return rc;
-    *found = s->found[0];
-    std::memset(out, 0, sizeof *out);
-    if (s->found[0]) {
-        *out = s->alns[0];
-        out->runs_offset = 0;
-        const uint64_t c = std::min<uint64_t>(runs_cap, s->alns[0].n_runs);
-        if (c) std::memcpy(runs, s->runs + s->alns[0].runs_offset, c * sizeof(gxy_edit_run));
-        if (s->alns[0].n_runs > runs_cap) rc = set_err(ctx, GXY_ERANGE, "edit script needs %u runs",
-                                                        s->alns[0].n_runs);
-    }
-    gxy_alignment_set_free(s);
-    return rc;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!anchor || !out || !found || (runs_cap && !runs)) return set_err(ctx, GXY_EINVAL, "null argument");
+        gxy_alignment_set *s = nullptr;
+        int rc = gxy_gapped_extend_batch(ctx, anchor, 1, target, query, scoring, params, &s);
+        if (rc) return rc;
+        *found = s->found[0];
+        std::memset(out, 0, sizeof *out);
+        if (s->found[0]) {
+            *out = s->alns[0];
+            out->runs_offset = 0;
+            const uint64_t c = std::min<uint64_t>(runs_cap, s->alns[0].n_runs);
+            if (c) std::memcpy(runs, s->runs + s->alns[0].runs_offset, c * sizeof(gxy_edit_run));
+            if (s->alns[0].n_runs > runs_cap) rc = set_err(ctx, GXY_ERANGE, "edit script needs %u runs",
+                                                            s->alns[0].n_runs);
+        }
+        gxy_alignment_set_free(s);
+        return rc;
+    });
 }
 
 void gxy_alignment_set_free(gxy_alignment_set *set) {

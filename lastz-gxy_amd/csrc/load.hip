@@ -187,160 +187,168 @@ void gxy_dseq_list_free(gxy_dseq_list *l) {
 }
 
 int gxy_load_2bit(gxy_ctx *ctx, const uint8_t *bytes, uint64_t len, gxy_dseq_list **out) {
-    if (!ctx) return GXY_EINVAL;
-    if ((!bytes && len) || !out) return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    // src/bit2.rs:73-95: magic decides the byte order
-    if (len < 4) return set_err(ctx, GXY_EINVAL, "truncated 2bit file (needed 4 bytes at offset 0)");
-    uint32_t magic;
-    std::memcpy(&magic, bytes, 4);
-    Reader r{bytes, len};
-    if (magic == 0x1A412743u)
-        r.be = false;
-    else if (magic == 0x4327411Au)
-        r.be = true;
-    else
-        return set_err(ctx, GXY_EINVAL, "not a 2bit file (magic = 0x%08x)", magic);
-    r.pos = 4;
-    const uint32_t version = r.u32();
-    if (r.ok && version != 0) return set_err(ctx, GXY_EINVAL, "unsupported 2bit version %u", version);
-    const uint32_t count = r.u32();
-    (void)r.u32();  // reserved
-    if (!r.ok) return set_err(ctx, GXY_EINVAL, "truncated 2bit header");
-    std::vector<std::pair<std::string, uint32_t>> index;  // src/bit2.rs:98-106
-    for (uint32_t k = 0; k < count; k++) {
-        if (r.pos + 1 > len) return set_err(ctx, GXY_EINVAL, "truncated 2bit index");
-        const uint32_t ns = bytes[r.pos++];
-        if (r.pos + ns > len) return set_err(ctx, GXY_EINVAL, "truncated 2bit index");
-        std::string name((const char *)bytes + r.pos, ns);
-        r.pos += ns;
-        const uint32_t off = r.u32();
-        if (!r.ok) return set_err(ctx, GXY_EINVAL, "truncated 2bit index");
-        index.push_back({std::move(name), off});
-    }
-    gxy_dseq_list *list = new_list(count);
-    if (!list) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
-    auto fail = [&](int code) {
-        gxy_dseq_list_free(list);
-        return code;
-    };
-    for (const auto &ix : index) {  // parse_sequence_record (src/bit2.rs:115-164)
-        Reader s{bytes, len};
-        s.be = r.be;
-        s.pos = ix.second;
-        if (s.pos > len) return fail(set_err(ctx, GXY_EINVAL, "2bit record offset past the end"));
-        const uint32_t dna = s.u32();
-        const uint32_t nn = s.u32();
-        std::vector<uint32_t> ns(nn), nz(nn);
-        for (auto &x : ns) x = s.u32();
-        for (auto &x : nz) x = s.u32();
-        const uint32_t nm = s.ok ? s.u32() : 0;
-        std::vector<uint32_t> ms(nm), mz(nm);
-        for (auto &x : ms) x = s.u32();
-        for (auto &x : mz) x = s.u32();
-        (void)s.u32();  // reserved
-        const uint64_t pb = ((uint64_t)dna + 3) / 4;
-        if (!s.ok || s.pos + pb > len) return fail(set_err(ctx, GXY_EINVAL, "truncated 2bit record"));
-        const std::vector<Iv> niv = merged(ns, nz, dna), miv = merged(ms, mz, dna);
-        gxy_dseq *d;
-        int rc = dseq_alloc(ctx, dna, &d);
-        if (rc) return fail(rc);
-        list->names[list->n] = dup((const uint8_t *)ix.first.data(), ix.first.size());
-        list->seqs[list->n] = d;
-        list->n++;
-        const size_t bytes_dev = (size_t)pb + niv.size() * sizeof(Iv) + miv.size() * sizeof(Iv) + 16;
-        auto *stage = (uint8_t *)scratch(ctx, kSlotMisc0, bytes_dev);
-        if (!stage) return fail(set_err(ctx, GXY_ENOMEM, "scratch allocation failed"));
-        const size_t o_n = (pb + 7) & ~(size_t)7, o_m = o_n + niv.size() * sizeof(Iv);
-        if (pb) GXY_HIP(ctx, hipMemcpyAsync(stage, bytes + s.pos, pb, hipMemcpyHostToDevice, ctx->stream));
-        if (!niv.empty())
-            GXY_HIP(ctx, hipMemcpyAsync(stage + o_n, niv.data(), niv.size() * sizeof(Iv), hipMemcpyHostToDevice,
-                                        ctx->stream));
-        if (!miv.empty())
-            GXY_HIP(ctx, hipMemcpyAsync(stage + o_m, miv.data(), miv.size() * sizeof(Iv), hipMemcpyHostToDevice,
-                                        ctx->stream));
-        k_from_2bit<<<grid_for(d->nblk, 256), 256, 0, ctx->stream>>>(stage, dna, (const Iv *)(stage + o_n),
-                                                                     (uint32_t)niv.size(), (const Iv *)(stage + o_m),
-                                                                     (uint32_t)miv.size(), d->d, d->nblk);
-        GXY_LAUNCH_CHECK(ctx);
-        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the staging slot is reused by the next record
-    }
-    *out = list;
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if ((!bytes && len) || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        // src/bit2.rs:73-95: magic decides the byte order
+        if (len < 4) return set_err(ctx, GXY_EINVAL, "truncated 2bit file (needed 4 bytes at offset 0)");
+        uint32_t magic;
+        std::memcpy(&magic, bytes, 4);
+        Reader r{bytes, len};
+        if (magic == 0x1A412743u)
+            r.be = false;
+        else if (magic == 0x4327411Au)
+            r.be = true;
+        else
+            return set_err(ctx, GXY_EINVAL, "not a 2bit file (magic = 0x%08x)", magic);
+        r.pos = 4;
+        const uint32_t version = r.u32();
+        if (r.ok && version != 0) return set_err(ctx, GXY_EINVAL, "unsupported 2bit version %u", version);
+        const uint32_t count = r.u32();
+        (void)r.u32();  // reserved
+        if (!r.ok) return set_err(ctx, GXY_EINVAL, "truncated 2bit header");
+        std::vector<std::pair<std::string, uint32_t>> index;  // src/bit2.rs:98-106
+        for (uint32_t k = 0; k < count; k++) {
+            if (r.pos + 1 > len) return set_err(ctx, GXY_EINVAL, "truncated 2bit index");
+            const uint32_t ns = bytes[r.pos++];
+            if (r.pos + ns > len) return set_err(ctx, GXY_EINVAL, "truncated 2bit index");
+            std::string name((const char *)bytes + r.pos, ns);
+            r.pos += ns;
+            const uint32_t off = r.u32();
+            if (!r.ok) return set_err(ctx, GXY_EINVAL, "truncated 2bit index");
+            index.push_back({std::move(name), off});
+        }
+        gxy_dseq_list *list = new_list(count);
+        if (!list) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+        auto fail = [&](int code) {
+            gxy_dseq_list_free(list);
+            return code;
+        };
+        for (const auto &ix : index) {  // parse_sequence_record (src/bit2.rs:115-164)
+            Reader s{bytes, len};
+            s.be = r.be;
+            s.pos = ix.second;
+            if (s.pos > len) return fail(set_err(ctx, GXY_EINVAL, "2bit record offset past the end"));
+            const uint32_t dna = s.u32();
+            const uint32_t nn = s.u32();
+            // block counts are checked against the bytes left before anything is
+            // sized from them (8 bytes per block: start and size)
+            if (!s.ok || nn > (len - s.pos) / 8) return fail(set_err(ctx, GXY_EINVAL, "truncated 2bit record"));
+            std::vector<uint32_t> ns(nn), nz(nn);
+            for (auto &x : ns) x = s.u32();
+            for (auto &x : nz) x = s.u32();
+            const uint32_t nm = s.ok ? s.u32() : 0;
+            if (!s.ok || nm > (len - s.pos) / 8) return fail(set_err(ctx, GXY_EINVAL, "truncated 2bit record"));
+            std::vector<uint32_t> ms(nm), mz(nm);
+            for (auto &x : ms) x = s.u32();
+            for (auto &x : mz) x = s.u32();
+            (void)s.u32();  // reserved
+            const uint64_t pb = ((uint64_t)dna + 3) / 4;
+            if (!s.ok || s.pos + pb > len) return fail(set_err(ctx, GXY_EINVAL, "truncated 2bit record"));
+            const std::vector<Iv> niv = merged(ns, nz, dna), miv = merged(ms, mz, dna);
+            gxy_dseq *d;
+            int rc = dseq_alloc(ctx, dna, &d);
+            if (rc) return fail(rc);
+            list->names[list->n] = dup((const uint8_t *)ix.first.data(), ix.first.size());
+            list->seqs[list->n] = d;
+            list->n++;
+            const size_t bytes_dev = (size_t)pb + niv.size() * sizeof(Iv) + miv.size() * sizeof(Iv) + 16;
+            auto *stage = (uint8_t *)scratch(ctx, kSlotMisc0, bytes_dev);
+            if (!stage) return fail(set_err(ctx, GXY_ENOMEM, "scratch allocation failed"));
+            const size_t o_n = (pb + 7) & ~(size_t)7, o_m = o_n + niv.size() * sizeof(Iv);
+            if (pb) GXY_HIP(ctx, hipMemcpyAsync(stage, bytes + s.pos, pb, hipMemcpyHostToDevice, ctx->stream));
+            if (!niv.empty())
+                GXY_HIP(ctx, hipMemcpyAsync(stage + o_n, niv.data(), niv.size() * sizeof(Iv), hipMemcpyHostToDevice,
+                                            ctx->stream));
+            if (!miv.empty())
+                GXY_HIP(ctx, hipMemcpyAsync(stage + o_m, miv.data(), miv.size() * sizeof(Iv), hipMemcpyHostToDevice,
+                                            ctx->stream));
+            k_from_2bit<<<grid_for(d->nblk, 256), 256, 0, ctx->stream>>>(stage, dna, (const Iv *)(stage + o_n),
+                                                                         (uint32_t)niv.size(), (const Iv *)(stage + o_m),
+                                                                         (uint32_t)miv.size(), d->d, d->nblk);
+            GXY_LAUNCH_CHECK(ctx);
+            GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the staging slot is reused by the next record
+        }
+        *out = list;
+        return GXY_OK;
+    });
 }
 
 int gxy_load_fasta(gxy_ctx *ctx, const char *text, uint64_t len, gxy_dseq_list **out) {
-    if (!ctx) return GXY_EINVAL;
-    if ((!text && len) || !out) return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    // parse_fasta (src/sequences.rs:338-377): lines split at '\n' (a '\r'
-    // before it dropped), empty lines skipped, '>' starts a record named by
-    // its first whitespace-delimited token, other lines add their
-    // non-whitespace bytes; bytes before the first header stay with the
-    // first record.
-    struct RecT {
-        std::string name;
-        std::string seq;
-    };
-    std::vector<RecT> recs;
-    std::string cur;
-    bool named = false;
-    std::string cur_name;
-    auto isws = [](unsigned char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == 0x0C; };
-    uint64_t p = 0;
-    uint64_t lineno = 0;
-    while (p < len) {
-        uint64_t e = p;
-        while (e < len && text[e] != '\n') e++;
-        uint64_t le = e;
-        if (le > p && text[le - 1] == '\r') le--;
-        lineno++;
-        if (le > p) {
-            if (text[p] == '>') {
-                if (named) {
-                    recs.push_back({std::move(cur_name), std::move(cur)});
-                    cur.clear();
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if ((!text && len) || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        // parse_fasta (src/sequences.rs:338-377): lines split at '\n' (a '\r'
+        // before it dropped), empty lines skipped, '>' starts a record named by
+        // its first whitespace-delimited token, other lines add their
+        // non-whitespace bytes; bytes before the first header stay with the
+        // first record.
+        struct RecT {
+            std::string name;
+            std::string seq;
+        };
+        std::vector<RecT> recs;
+        std::string cur;
+        bool named = false;
+        std::string cur_name;
+        auto isws = [](unsigned char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == 0x0C; };
+        uint64_t p = 0;
+        uint64_t lineno = 0;
+        while (p < len) {
+            uint64_t e = p;
+            while (e < len && text[e] != '\n') e++;
+            uint64_t le = e;
+            if (le > p && text[le - 1] == '\r') le--;
+            lineno++;
+            if (le > p) {
+                if (text[p] == '>') {
+                    if (named) {
+                        recs.push_back({std::move(cur_name), std::move(cur)});
+                        cur.clear();
+                    }
+                    uint64_t a = p + 1;
+                    while (a < le && isws((unsigned char)text[a])) a++;
+                    uint64_t b = a;
+                    while (b < le && !isws((unsigned char)text[b])) b++;
+                    if (a == b) return set_err(ctx, GXY_EINVAL, "empty FASTA header on line %llu", (unsigned long long)lineno);
+                    cur_name.assign(text + a, b - a);
+                    named = true;
+                } else {
+                    for (uint64_t k = p; k < le; k++)
+                        if (!isws((unsigned char)text[k])) cur.push_back(text[k]);
                 }
-                uint64_t a = p + 1;
-                while (a < le && isws((unsigned char)text[a])) a++;
-                uint64_t b = a;
-                while (b < le && !isws((unsigned char)text[b])) b++;
-                if (a == b) return set_err(ctx, GXY_EINVAL, "empty FASTA header on line %llu", (unsigned long long)lineno);
-                cur_name.assign(text + a, b - a);
-                named = true;
-            } else {
-                for (uint64_t k = p; k < le; k++)
-                    if (!isws((unsigned char)text[k])) cur.push_back(text[k]);
             }
+            p = e + 1;
         }
-        p = e + 1;
-    }
-    if (named) recs.push_back({std::move(cur_name), std::move(cur)});
-    gxy_dseq_list *list = new_list(recs.size());
-    if (!list) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
-    for (const auto &rc_ : recs) {
-        const uint64_t n = rc_.seq.size();
-        gxy_dseq *d;
-        int rc = dseq_alloc(ctx, n, &d);
-        if (rc) {
-            gxy_dseq_list_free(list);
-            return rc;
+        if (named) recs.push_back({std::move(cur_name), std::move(cur)});
+        gxy_dseq_list *list = new_list(recs.size());
+        if (!list) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+        for (const auto &rc_ : recs) {
+            const uint64_t n = rc_.seq.size();
+            gxy_dseq *d;
+            int rc = dseq_alloc(ctx, n, &d);
+            if (rc) {
+                gxy_dseq_list_free(list);
+                return rc;
+            }
+            list->names[list->n] = dup((const uint8_t *)rc_.name.data(), rc_.name.size());
+            list->seqs[list->n] = d;
+            list->n++;
+            auto *stage = (uint8_t *)scratch(ctx, kSlotMisc0, n + 16);
+            if (!stage) {
+                gxy_dseq_list_free(list);
+                return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+            }
+            if (n) GXY_HIP(ctx, hipMemcpyAsync(stage, rc_.seq.data(), n, hipMemcpyHostToDevice, ctx->stream));
+            k_pack_ascii<<<grid_for(d->nblk, 256), 256, 0, ctx->stream>>>(stage, n, d->d, d->nblk);
+            GXY_LAUNCH_CHECK(ctx);
+            GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
-        list->names[list->n] = dup((const uint8_t *)rc_.name.data(), rc_.name.size());
-        list->seqs[list->n] = d;
-        list->n++;
-        auto *stage = (uint8_t *)scratch(ctx, kSlotMisc0, n + 16);
-        if (!stage) {
-            gxy_dseq_list_free(list);
-            return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-        }
-        if (n) GXY_HIP(ctx, hipMemcpyAsync(stage, rc_.seq.data(), n, hipMemcpyHostToDevice, ctx->stream));
-        k_pack_ascii<<<grid_for(d->nblk, 256), 256, 0, ctx->stream>>>(stage, n, d->d, d->nblk);
-        GXY_LAUNCH_CHECK(ctx);
-        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    *out = list;
-    return GXY_OK;
+        *out = list;
+        return GXY_OK;
+    });
 }
 
 }  // extern "C"

@@ -255,128 +255,138 @@ extern "C" {
 
 int gxy_postable_build_dev(gxy_ctx *ctx, const gxy_dseq *target, const gxy_seed_pattern *pattern,
                            uint64_t step, gxy_postable **out) {
-    if (!ctx) return GXY_EINVAL;
-    if (!target || !out) return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    return postable_build_impl(ctx, target, pattern, step, 1, out);
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!target || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        return postable_build_impl(ctx, target, pattern, step, 1, out);
+    });
 }
 
 int gxy_postable_build(gxy_ctx *ctx, const gxy_seq *target, const gxy_seed_pattern *pattern,
                        uint64_t step, gxy_postable **out) {
-    if (!ctx) return GXY_EINVAL;
-    if (!target || !out) return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    gxy_dseq *d;
-    int rc = dseq_upload_impl(ctx, target, &d);
-    if (rc) return rc;
-    rc = postable_build_impl(ctx, d, pattern, step, 1, out);
-    if (rc) {
-        gxy_dseq_free(d);
-        return rc;
-    }
-    (*out)->owned_target = d;
-    (*out)->host_codes = target->codes;
-    (*out)->host_len = target->len;
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!target || !out) return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        gxy_dseq *d;
+        int rc = dseq_upload_impl(ctx, target, &d);
+        if (rc) return rc;
+        rc = postable_build_impl(ctx, d, pattern, step, 1, out);
+        if (rc) {
+            gxy_dseq_free(d);
+            return rc;
+        }
+        (*out)->owned_target = d;
+        (*out)->host_codes = target->codes;
+        (*out)->host_len = target->len;
+        return GXY_OK;
+    });
 }
 
 int gxy_postable_prune(gxy_ctx *ctx, gxy_postable *tab, uint32_t limit) {
-    // PosTable::prune_hot_words (src/pos_table.rs:113-136)
-    if (!ctx) return GXY_EINVAL;
-    if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
-    const uint64_t nw = tab->num_words;
-    auto *keep = (uint32_t *)scratch(ctx, kSlotCounts, nw * sizeof(uint32_t));
-    uint32_t *ns = nullptr, *np = nullptr;
-    if (!keep) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-    ns = (uint32_t *)dev_alloc_from(tab->pool, (nw + 1) * sizeof(uint32_t));
-    if (!ns) return set_err(ctx, GXY_ENOMEM, "table allocation failed");
-    k_prune_counts<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, limit, keep);
-    GXY_LAUNCH_CHECK(ctx);
-    size_t tmp_bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep, ns, (int)nw, ctx->stream);
-    void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
-    if (!tmp) {
-        dev_free(tab->pool, ns);
-        return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-    }
-    GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, keep, ns, (int)nw, ctx->stream));
-    k_set_last<<<1, 1, 0, ctx->stream>>>(ns, keep, nw);
-    GXY_LAUNCH_CHECK(ctx);
-    uint32_t total = 0;
-    GXY_HIP(ctx, hipMemcpyAsync(&total, ns + nw, 4, hipMemcpyDeviceToHost, ctx->stream));
-    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    np = (uint32_t *)dev_alloc_from(tab->pool, std::max<uint32_t>(1, total) * sizeof(uint32_t));
-    if (!np) {
-        dev_free(tab->pool, ns);
-        return set_err(ctx, GXY_ENOMEM, "table allocation failed");
-    }
-    k_prune_copy<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, ns, keep,
-                                                             nw, np);
-    GXY_LAUNCH_CHECK(ctx);
-    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    dev_free(tab->pool, tab->d_starts);
-    dev_free(tab->pool, tab->d_positions);
-    tab->d_starts = ns;
-    tab->d_positions = np;
-    tab->npos = total;
-    k_nonempty<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_nonempty);
-    GXY_LAUNCH_CHECK(ctx);
-    k_cmap<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_cmap);
-    GXY_LAUNCH_CHECK(ctx);
-    k_spos<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, nw, tab->d_spos);
-    GXY_LAUNCH_CHECK(ctx);
-    if (tab->d_ncnt) {
-        k_ncnt<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_cmap, nw, tab->pattern.weight, tab->d_ncnt);
+    return guarded(ctx, [&]() -> int {
+        // PosTable::prune_hot_words (src/pos_table.rs:113-136)
+        if (!ctx) return GXY_EINVAL;
+        if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
+        const uint64_t nw = tab->num_words;
+        auto *keep = (uint32_t *)scratch(ctx, kSlotCounts, nw * sizeof(uint32_t));
+        uint32_t *ns = nullptr, *np = nullptr;
+        if (!keep) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        ns = (uint32_t *)dev_alloc_from(tab->pool, (nw + 1) * sizeof(uint32_t));
+        if (!ns) return set_err(ctx, GXY_ENOMEM, "table allocation failed");
+        k_prune_counts<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, limit, keep);
         GXY_LAUNCH_CHECK(ctx);
-    }
-    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return GXY_OK;
+        size_t tmp_bytes = 0;
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep, ns, (int)nw, ctx->stream);
+        void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
+        if (!tmp) {
+            dev_free(tab->pool, ns);
+            return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        }
+        GXY_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, keep, ns, (int)nw, ctx->stream));
+        k_set_last<<<1, 1, 0, ctx->stream>>>(ns, keep, nw);
+        GXY_LAUNCH_CHECK(ctx);
+        uint32_t total = 0;
+        GXY_HIP(ctx, hipMemcpyAsync(&total, ns + nw, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        np = (uint32_t *)dev_alloc_from(tab->pool, std::max<uint32_t>(1, total) * sizeof(uint32_t));
+        if (!np) {
+            dev_free(tab->pool, ns);
+            return set_err(ctx, GXY_ENOMEM, "table allocation failed");
+        }
+        k_prune_copy<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, ns, keep,
+                                                                 nw, np);
+        GXY_LAUNCH_CHECK(ctx);
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        dev_free(tab->pool, tab->d_starts);
+        dev_free(tab->pool, tab->d_positions);
+        tab->d_starts = ns;
+        tab->d_positions = np;
+        tab->npos = total;
+        k_nonempty<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_nonempty);
+        GXY_LAUNCH_CHECK(ctx);
+        k_cmap<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, nw, tab->d_cmap);
+        GXY_LAUNCH_CHECK(ctx);
+        k_spos<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_starts, tab->d_positions, nw, tab->d_spos);
+        GXY_LAUNCH_CHECK(ctx);
+        if (tab->d_ncnt) {
+            k_ncnt<<<grid_for(nw, 256), 256, 0, ctx->stream>>>(tab->d_cmap, nw, tab->pattern.weight, tab->d_ncnt);
+            GXY_LAUNCH_CHECK(ctx);
+        }
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return GXY_OK;
+    });
 }
 
 int gxy_postable_lookup(gxy_ctx *ctx, const gxy_postable *tab, uint64_t word, uint32_t *out, uint64_t cap,
                         uint64_t *n_out) {
-    // PosTable::lookup (src/pos_table.rs:102-108)
-    if (!ctx) return GXY_EINVAL;
-    if (!tab || !n_out || (cap && !out)) return set_err(ctx, GXY_EINVAL, "null argument");
-    if (word >= tab->num_words) return set_err(ctx, GXY_EINVAL, "word out of range");
-    uint32_t se[2];
-    const uint64_t idx = word_to_index(word, tab->pattern.weight);
-    GXY_HIP(ctx, hipMemcpyAsync(se, tab->d_starts + idx, 8, hipMemcpyDeviceToHost, ctx->stream));
-    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const uint64_t n = se[1] - se[0];
-    *n_out = n;
-    const uint64_t c = std::min(n, cap);
-    if (c) {
-        GXY_HIP(ctx, hipMemcpyAsync(out, tab->d_positions + se[0], c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    return guarded(ctx, [&]() -> int {
+        // PosTable::lookup (src/pos_table.rs:102-108)
+        if (!ctx) return GXY_EINVAL;
+        if (!tab || !n_out || (cap && !out)) return set_err(ctx, GXY_EINVAL, "null argument");
+        if (word >= tab->num_words) return set_err(ctx, GXY_EINVAL, "word out of range");
+        uint32_t se[2];
+        const uint64_t idx = word_to_index(word, tab->pattern.weight);
+        GXY_HIP(ctx, hipMemcpyAsync(se, tab->d_starts + idx, 8, hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    return GXY_OK;
+        const uint64_t n = se[1] - se[0];
+        *n_out = n;
+        const uint64_t c = std::min(n, cap);
+        if (c) {
+            GXY_HIP(ctx, hipMemcpyAsync(out, tab->d_positions + se[0], c * 4, hipMemcpyDeviceToHost, ctx->stream));
+            GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        return GXY_OK;
+    });
 }
 
 uint64_t gxy_postable_total_positions(const gxy_postable *tab) { return tab ? tab->npos : 0; }
 
 int gxy_postable_export(gxy_ctx *ctx, const gxy_postable *tab, uint32_t *starts, uint64_t starts_cap,
                         uint32_t *positions, uint64_t positions_cap) {
-    // The device buckets are in index order (gather_index); the reference
-    // layout orders them by word.  Rebuild it on the host.
-    if (!ctx) return GXY_EINVAL;
-    if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
-    const uint64_t nw = tab->num_words;
-    std::vector<uint32_t> ds(nw + 1), dp(std::max<uint64_t>(1, tab->npos));
-    GXY_HIP(ctx, hipMemcpyAsync(ds.data(), tab->d_starts, (nw + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (tab->npos)
-        GXY_HIP(ctx, hipMemcpyAsync(dp.data(), tab->d_positions, tab->npos * 4, hipMemcpyDeviceToHost, ctx->stream));
-    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    uint64_t acc = 0;
-    for (uint64_t w = 0; w < nw; w++) {
-        const uint64_t idx = word_to_index(w, tab->pattern.weight);
-        const uint32_t b = ds[idx], e = ds[idx + 1];
-        if (starts && w < starts_cap) starts[w] = (uint32_t)acc;
-        for (uint32_t k = b; k < e; k++, acc++)
-            if (positions && acc < positions_cap) positions[acc] = dp[k];
-    }
-    if (starts && nw < starts_cap) starts[nw] = (uint32_t)acc;
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        // The device buckets are in index order (gather_index); the reference
+        // layout orders them by word.  Rebuild it on the host.
+        if (!ctx) return GXY_EINVAL;
+        if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
+        const uint64_t nw = tab->num_words;
+        std::vector<uint32_t> ds(nw + 1), dp(std::max<uint64_t>(1, tab->npos));
+        GXY_HIP(ctx, hipMemcpyAsync(ds.data(), tab->d_starts, (nw + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+        if (tab->npos)
+            GXY_HIP(ctx, hipMemcpyAsync(dp.data(), tab->d_positions, tab->npos * 4, hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        uint64_t acc = 0;
+        for (uint64_t w = 0; w < nw; w++) {
+            const uint64_t idx = word_to_index(w, tab->pattern.weight);
+            const uint32_t b = ds[idx], e = ds[idx + 1];
+            if (starts && w < starts_cap) starts[w] = (uint32_t)acc;
+            for (uint32_t k = b; k < e; k++, acc++)
+                if (positions && acc < positions_cap) positions[acc] = dp[k];
+        }
+        if (starts && nw < starts_cap) starts[nw] = (uint32_t)acc;
+        return GXY_OK;
+    });
 }
 
 void gxy_postable_free(gxy_postable *tab) {

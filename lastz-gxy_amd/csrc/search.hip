@@ -1241,109 +1241,117 @@ extern "C" {
 
 int gxy_search_dev(gxy_ctx *ctx, const gxy_postable *table, const gxy_dseq *query, const gxy_scoring *scoring,
                    const gxy_search_params *params, gxy_hsp **out, uint64_t *n_out, gxy_search_stats *stats) {
-    if (!ctx) return GXY_EINVAL;
-    if (!out || !n_out) return set_err(ctx, GXY_EINVAL, "null output");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    std::vector<gxy_hsp> v;
-    int rc = search_dev_impl(ctx, table, query, scoring, params, 1, &v, stats);
-    if (rc) return rc;
-    *n_out = v.size();
-    *out = (gxy_hsp *)std::malloc(std::max<size_t>(1, v.size()) * sizeof(gxy_hsp));
-    if (!*out) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
-    if (!v.empty()) std::memcpy(*out, v.data(), v.size() * sizeof(gxy_hsp));
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!out || !n_out) return set_err(ctx, GXY_EINVAL, "null output");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        std::vector<gxy_hsp> v;
+        int rc = search_dev_impl(ctx, table, query, scoring, params, 1, &v, stats);
+        if (rc) return rc;
+        *n_out = v.size();
+        *out = (gxy_hsp *)std::malloc(std::max<size_t>(1, v.size()) * sizeof(gxy_hsp));
+        if (!*out) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
+        if (!v.empty()) std::memcpy(*out, v.data(), v.size() * sizeof(gxy_hsp));
+        return GXY_OK;
+    });
 }
 
 int gxy_search(gxy_ctx *ctx, const gxy_postable *table, const gxy_seq *target, const gxy_seq *query,
                const gxy_scoring *scoring, const gxy_search_params *params, gxy_hsp **out, uint64_t *n_out,
                gxy_search_stats *stats) {
-    if (!ctx) return GXY_EINVAL;
-    if (!table || !target || !query) return set_err(ctx, GXY_EINVAL, "null argument");
-    if (target->len != table->target->len)
-        return set_err(ctx, GXY_EINVAL, "target is not the sequence the table was built from");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    gxy_dseq *dq;
-    int rc = dseq_upload_impl(ctx, query, &dq);
-    if (rc) return rc;
-    rc = gxy_search_dev(ctx, table, dq, scoring, params, out, n_out, stats);
-    gxy_dseq_free(dq);
-    return rc;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!table || !target || !query) return set_err(ctx, GXY_EINVAL, "null argument");
+        if (target->len != table->target->len)
+            return set_err(ctx, GXY_EINVAL, "target is not the sequence the table was built from");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        gxy_dseq *dq;
+        int rc = dseq_upload_impl(ctx, query, &dq);
+        if (rc) return rc;
+        rc = gxy_search_dev(ctx, table, dq, scoring, params, out, n_out, stats);
+        gxy_dseq_free(dq);
+        return rc;
+    });
 }
 
 int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *query, const gxy_scoring *scoring,
                          const gxy_hsp_params *params, const uint32_t *t_pos, const uint32_t *q_pos, uint64_t n,
                          uint32_t seed_len, gxy_hsp *out, uint8_t *found) {
-    if (!ctx) return GXY_EINVAL;
-    if (!target || !query || !scoring || !params || (n && (!t_pos || !q_pos || !out || !found)))
-        return set_err(ctx, GXY_EINVAL, "null argument");
-    GXY_HIP(ctx, hipSetDevice(ctx->device));
-    if (n == 0) return GXY_OK;
-    gxy_dseq *dt = nullptr, *dq = nullptr;
-    int rc = dseq_upload_impl(ctx, target, &dt);
-    if (rc) return rc;
-    rc = dseq_upload_impl(ctx, query, &dq);
-    if (rc) {
-        gxy_dseq_free(dt);
-        return rc;
-    }
-    auto *dtp = (uint32_t *)scratch(ctx, kSlotMisc0, n * 4);
-    auto *dqp = (uint32_t *)scratch(ctx, kSlotMisc1, n * 4);
-    auto *dout = (gxy_hsp *)scratch(ctx, kSlotMisc2, n * sizeof(gxy_hsp));
-    auto *dfound = (uint8_t *)scratch(ctx, kSlotMisc3, n);
-    hipError_t e = hipSuccess;
-    if (!dtp || !dqp || !dout || !dfound) {
-        rc = set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-    } else {
-        e = hipMemcpyAsync(dtp, t_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(dqp, q_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) {
-            if (scoring_byte_ok(*scoring) && params->x_drop >= 0) {
-                auto *tb = (uint8_t *)scratch(ctx, kSlotTBytes, byte_buffer_size(dt->len));
-                auto *qb = (uint8_t *)scratch(ctx, kSlotQBytes, byte_buffer_size(dq->len));
-                if (!tb || !qb) {
-                    gxy_dseq_free(dt);
-                    gxy_dseq_free(dq);
-                    return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-                }
-                rc = seq_to_bytes(ctx, dt, true, tb);
-                if (!rc) rc = seq_to_bytes(ctx, dq, false, qb);
-                if (rc) {
-                    gxy_dseq_free(dt);
-                    gxy_dseq_free(dq);
-                    return rc;
-                }
-                XdropArgs xa{tb + kBytePad, qb + kBytePad, dt->len, dq->len, dt->d, make_byte_lut(*scoring),
-                             params->x_drop, params->hsp_threshold, seed_len, 0, 0};
-                PairSrc src{dtp, dqp, n};
-                PassOut none{nullptr, nullptr, nullptr};
-                auto *cs = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
-                k_xdrop<PairSrc, false><<<std::min<unsigned>(grid_for(n, 256), kXdropBlocks), 256, 0,
-                                          ctx->stream>>>(xa, src, none, dout, dfound, cs);
-            } else
-                k_extend_batch<ScorerI32><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
-                    dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i32(*scoring), params->x_drop,
-                    params->hsp_threshold, seed_len, dout, dfound);
-            e = hipGetLastError();
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!target || !query || !scoring || !params || (n && (!t_pos || !q_pos || !out || !found)))
+            return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        if (n == 0) return GXY_OK;
+        gxy_dseq *dt = nullptr, *dq = nullptr;
+        int rc = dseq_upload_impl(ctx, target, &dt);
+        if (rc) return rc;
+        rc = dseq_upload_impl(ctx, query, &dq);
+        if (rc) {
+            gxy_dseq_free(dt);
+            return rc;
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(found, dfound, n, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) rc = set_err(ctx, GXY_EDEVICE, "extend_hit_batch: %s", hipGetErrorString(e));
-    }
-    gxy_dseq_free(dt);
-    gxy_dseq_free(dq);
-    return rc;
+        auto *dtp = (uint32_t *)scratch(ctx, kSlotMisc0, n * 4);
+        auto *dqp = (uint32_t *)scratch(ctx, kSlotMisc1, n * 4);
+        auto *dout = (gxy_hsp *)scratch(ctx, kSlotMisc2, n * sizeof(gxy_hsp));
+        auto *dfound = (uint8_t *)scratch(ctx, kSlotMisc3, n);
+        hipError_t e = hipSuccess;
+        if (!dtp || !dqp || !dout || !dfound) {
+            rc = set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        } else {
+            e = hipMemcpyAsync(dtp, t_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(dqp, q_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) {
+                if (scoring_byte_ok(*scoring) && params->x_drop >= 0) {
+                    auto *tb = (uint8_t *)scratch(ctx, kSlotTBytes, byte_buffer_size(dt->len));
+                    auto *qb = (uint8_t *)scratch(ctx, kSlotQBytes, byte_buffer_size(dq->len));
+                    if (!tb || !qb) {
+                        gxy_dseq_free(dt);
+                        gxy_dseq_free(dq);
+                        return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+                    }
+                    rc = seq_to_bytes(ctx, dt, true, tb);
+                    if (!rc) rc = seq_to_bytes(ctx, dq, false, qb);
+                    if (rc) {
+                        gxy_dseq_free(dt);
+                        gxy_dseq_free(dq);
+                        return rc;
+                    }
+                    XdropArgs xa{tb + kBytePad, qb + kBytePad, dt->len, dq->len, dt->d, make_byte_lut(*scoring),
+                                 params->x_drop, params->hsp_threshold, seed_len, 0, 0};
+                    PairSrc src{dtp, dqp, n};
+                    PassOut none{nullptr, nullptr, nullptr};
+                    auto *cs = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
+                    k_xdrop<PairSrc, false><<<std::min<unsigned>(grid_for(n, 256), kXdropBlocks), 256, 0,
+                                              ctx->stream>>>(xa, src, none, dout, dfound, cs);
+                } else
+                    k_extend_batch<ScorerI32><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+                        dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i32(*scoring), params->x_drop,
+                        params->hsp_threshold, seed_len, dout, dfound);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(found, dfound, n, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = set_err(ctx, GXY_EDEVICE, "extend_hit_batch: %s", hipGetErrorString(e));
+        }
+        gxy_dseq_free(dt);
+        gxy_dseq_free(dq);
+        return rc;
+    });
 }
 
 int gxy_extend_hit(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *query, const gxy_scoring *scoring,
                    const gxy_hsp_params *params, uint32_t t_pos, uint32_t q_pos, uint32_t seed_len, gxy_hsp *out,
                    int *found) {
-    if (!ctx) return GXY_EINVAL;
-    if (!out || !found) return set_err(ctx, GXY_EINVAL, "null output");
-    uint8_t f = 0;
-    int rc = gxy_extend_hit_batch(ctx, target, query, scoring, params, &t_pos, &q_pos, 1, seed_len, out, &f);
-    *found = f;
-    return rc;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!out || !found) return set_err(ctx, GXY_EINVAL, "null output");
+        uint8_t f = 0;
+        int rc = gxy_extend_hit_batch(ctx, target, query, scoring, params, &t_pos, &q_pos, 1, seed_len, out, &f);
+        *found = f;
+        return rc;
+    });
 }
 
 }  // extern "C"

@@ -383,55 +383,81 @@ extern "C" {
 const char *gxy_version(void) { return "gxy-mi355x 0.1 (gfx950)"; }
 
 int gxy_ctx_create(int device, gxy_ctx **out) {
-    if (!out) return set_err(nullptr, GXY_EINVAL, "gxy_ctx_create: out is NULL");
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        (void)hipGetLastError();
-        return set_err(nullptr, GXY_EDEVICE, "no HIP device available");
-    }
-    if (device < 0 || device >= n) return set_err(nullptr, GXY_EINVAL, "device %d out of range", device);
-    if (hipSetDevice(device) != hipSuccess) return set_err(nullptr, GXY_EDEVICE, "hipSetDevice failed");
-    auto *c = new (std::nothrow) gxy_ctx();
-    if (!c) return set_err(nullptr, GXY_ENOMEM, "host allocation failed");
-    c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return set_err(nullptr, GXY_EDEVICE, "hipStreamCreate failed");
-    }
-    int pools = 0;
-    if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) != hipSuccess) {
-        (void)hipGetLastError();
-        pools = 0;
-    }
-    c->async_alloc = pools != 0;
-    *out = c;
-    return GXY_OK;
+    return guarded(nullptr, [&]() -> int {
+        if (!out) return set_err(nullptr, GXY_EINVAL, "gxy_ctx_create: out is NULL");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            (void)hipGetLastError();
+            return set_err(nullptr, GXY_EDEVICE, "no HIP device available");
+        }
+        if (device < 0 || device >= n) return set_err(nullptr, GXY_EINVAL, "device %d out of range", device);
+        if (hipSetDevice(device) != hipSuccess) return set_err(nullptr, GXY_EDEVICE, "hipSetDevice failed");
+        auto *c = new (std::nothrow) gxy_ctx();
+        if (!c) return set_err(nullptr, GXY_ENOMEM, "host allocation failed");
+        c->device = device;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return set_err(nullptr, GXY_EDEVICE, "hipStreamCreate failed");
+        }
+        int pools = 0;
+        if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) != hipSuccess) {
+            (void)hipGetLastError();
+            pools = 0;
+        }
+        c->async_alloc = pools != 0;
+        *out = c;
+        return GXY_OK;
+    });
 }
 
 int gxy_ctx_set_timing(gxy_ctx *ctx, int enabled) {
-    if (!ctx) return GXY_EINVAL;
-    ctx->timing = enabled != 0;
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        ctx->timing = enabled != 0;
+        return GXY_OK;
+    });
+}
+
+int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        ctx->dp_pool_rows = rows;
+        return GXY_OK;
+    });
+}
+
+int gxy_ctx_dp_pool_stats(gxy_ctx *ctx, uint64_t *passes, uint64_t *truncated, uint64_t *alone) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !passes || !truncated || !alone) return GXY_EINVAL;
+        *passes = ctx->dp_passes;
+        *truncated = ctx->dp_truncated;
+        *alone = ctx->dp_alone;
+        return GXY_OK;
+    });
 }
 
 int gxy_ctx_kernel_time(gxy_ctx *ctx, const char *name, double *ms, uint64_t *launches) {
-    if (!ctx || !name || !ms || !launches) return GXY_EINVAL;
-    timing_flush(ctx);
-    *ms = 0.0;
-    *launches = 0;
-    for (auto &t : ctx->totals)
-        if (t.first == name) {
-            *ms = t.second.first;
-            *launches = t.second.second;
-        }
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !name || !ms || !launches) return GXY_EINVAL;
+        timing_flush(ctx);
+        *ms = 0.0;
+        *launches = 0;
+        for (auto &t : ctx->totals)
+            if (t.first == name) {
+                *ms = t.second.first;
+                *launches = t.second.second;
+            }
+        return GXY_OK;
+    });
 }
 
 int gxy_ctx_reset_timing(gxy_ctx *ctx) {
-    if (!ctx) return GXY_EINVAL;
-    timing_flush(ctx);
-    ctx->totals.clear();
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        timing_flush(ctx);
+        ctx->totals.clear();
+        return GXY_OK;
+    });
 }
 
 void gxy_ctx_destroy(gxy_ctx *ctx) {
@@ -455,96 +481,108 @@ const char *gxy_last_error(const gxy_ctx *ctx) {
 }
 
 int gxy_ctx_synchronize(gxy_ctx *ctx) {
-    if (!ctx) return GXY_EINVAL;
-    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return GXY_OK;
+    });
 }
 
 int gxy_pack_ascii(const char *ascii, uint64_t len, uint8_t *codes, uint8_t *valid, uint8_t *mask) {
-    // PackedSeq::from_ascii / push_ascii (src/sequences.rs:164-206) with
-    // encode_base (src/dna.rs:12-20).
-    if ((len && !ascii) || !codes || !valid || !mask) return GXY_EINVAL;
-    std::memset(codes, 0, (len + 3) / 4);
-    std::memset(valid, 0, (len + 7) / 8);
-    std::memset(mask, 0, (len + 7) / 8);
-    for (uint64_t i = 0; i < len; i++) {
-        const unsigned char b = (unsigned char)ascii[i];
-        uint8_t c = 0;
-        bool v = true;
-        switch (b) {
-            case 'A': case 'a': c = 0; break;
-            case 'C': case 'c': c = 1; break;
-            case 'G': case 'g': c = 2; break;
-            case 'T': case 't': case 'U': case 'u': c = 3; break;
-            default: v = false; break;
+    return guarded(nullptr, [&]() -> int {
+        // PackedSeq::from_ascii / push_ascii (src/sequences.rs:164-206) with
+        // encode_base (src/dna.rs:12-20).
+        if ((len && !ascii) || !codes || !valid || !mask) return GXY_EINVAL;
+        std::memset(codes, 0, (len + 3) / 4);
+        std::memset(valid, 0, (len + 7) / 8);
+        std::memset(mask, 0, (len + 7) / 8);
+        for (uint64_t i = 0; i < len; i++) {
+            const unsigned char b = (unsigned char)ascii[i];
+            uint8_t c = 0;
+            bool v = true;
+            switch (b) {
+                case 'A': case 'a': c = 0; break;
+                case 'C': case 'c': c = 1; break;
+                case 'G': case 'g': c = 2; break;
+                case 'T': case 't': case 'U': case 'u': c = 3; break;
+                default: v = false; break;
+            }
+            if (!v) continue;
+            codes[i >> 2] |= (uint8_t)(c << (2 * (i & 3)));
+            valid[i >> 3] |= (uint8_t)(1u << (i & 7));
+            if (b >= 'a' && b <= 'z') mask[i >> 3] |= (uint8_t)(1u << (i & 7));
         }
-        if (!v) continue;
-        codes[i >> 2] |= (uint8_t)(c << (2 * (i & 3)));
-        valid[i >> 3] |= (uint8_t)(1u << (i & 7));
-        if (b >= 'a' && b <= 'z') mask[i >> 3] |= (uint8_t)(1u << (i & 7));
-    }
-    return GXY_OK;
+        return GXY_OK;
+    });
 }
 
 int gxy_dseq_upload(gxy_ctx *ctx, const gxy_seq *seq, gxy_dseq **out) {
-    if (!ctx) return GXY_EINVAL;
-    return dseq_upload_impl(ctx, seq, out);
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        return dseq_upload_impl(ctx, seq, out);
+    });
 }
 
 int gxy_dseq_reverse_complement(gxy_ctx *ctx, const gxy_dseq *seq, gxy_dseq **out) {
-    if (!ctx || !seq || !out) return ctx ? set_err(ctx, GXY_EINVAL, "null argument") : GXY_EINVAL;
-    gxy_dseq *r;
-    int rc = dseq_alloc(ctx, seq->len, &r);
-    if (rc) return rc;
-    const unsigned th = 256;
-    k_rc<<<(unsigned)((r->nblk + th - 1) / th), th, 0, ctx->stream>>>(seq->d, r->d, seq->len, r->nblk);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-        gxy_dseq_free(r);
-        return set_err(ctx, GXY_EDEVICE, "reverse complement failed: %s", hipGetErrorString(e));
-    }
-    *out = r;
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !seq || !out) return ctx ? set_err(ctx, GXY_EINVAL, "null argument") : GXY_EINVAL;
+        gxy_dseq *r;
+        int rc = dseq_alloc(ctx, seq->len, &r);
+        if (rc) return rc;
+        const unsigned th = 256;
+        k_rc<<<(unsigned)((r->nblk + th - 1) / th), th, 0, ctx->stream>>>(seq->d, r->d, seq->len, r->nblk);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            gxy_dseq_free(r);
+            return set_err(ctx, GXY_EDEVICE, "reverse complement failed: %s", hipGetErrorString(e));
+        }
+        *out = r;
+        return GXY_OK;
+    });
 }
 
 int gxy_dseq_slice(gxy_ctx *ctx, const gxy_dseq *seq, uint64_t start, uint64_t end, gxy_dseq **out) {
-    if (!ctx || !seq || !out) return ctx ? set_err(ctx, GXY_EINVAL, "null argument") : GXY_EINVAL;
-    if (start > end || end > seq->len)  // src/sequences.rs:278
-        return set_err(ctx, GXY_EINVAL, "slice [%llu, %llu) out of range (len %llu)",
-                       (unsigned long long)start, (unsigned long long)end,
-                       (unsigned long long)seq->len);
-    gxy_dseq *r;
-    int rc = dseq_alloc(ctx, end - start, &r);
-    if (rc) return rc;
-    const unsigned th = 256;
-    k_slice<<<(unsigned)((r->nblk + th - 1) / th), th, 0, ctx->stream>>>(seq->d, r->d, start, end - start,
-                                                                          r->nblk);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-        gxy_dseq_free(r);
-        return set_err(ctx, GXY_EDEVICE, "slice failed: %s", hipGetErrorString(e));
-    }
-    *out = r;
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !seq || !out) return ctx ? set_err(ctx, GXY_EINVAL, "null argument") : GXY_EINVAL;
+        if (start > end || end > seq->len)  // src/sequences.rs:278
+            return set_err(ctx, GXY_EINVAL, "slice [%llu, %llu) out of range (len %llu)",
+                           (unsigned long long)start, (unsigned long long)end,
+                           (unsigned long long)seq->len);
+        gxy_dseq *r;
+        int rc = dseq_alloc(ctx, end - start, &r);
+        if (rc) return rc;
+        const unsigned th = 256;
+        k_slice<<<(unsigned)((r->nblk + th - 1) / th), th, 0, ctx->stream>>>(seq->d, r->d, start, end - start,
+                                                                              r->nblk);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            gxy_dseq_free(r);
+            return set_err(ctx, GXY_EDEVICE, "slice failed: %s", hipGetErrorString(e));
+        }
+        *out = r;
+        return GXY_OK;
+    });
 }
 
 uint64_t gxy_dseq_len(const gxy_dseq *seq) { return seq ? seq->len : 0; }
 
 int gxy_dseq_download(gxy_ctx *ctx, const gxy_dseq *seq, uint8_t *codes, uint8_t *valid, uint8_t *mask) {
-    if (!ctx || !seq || !codes || !valid || !mask) return GXY_EINVAL;
-    std::vector<Blk> host(seq->nblk);
-    GXY_HIP(ctx, hipMemcpyAsync(host.data(), seq->d, seq->nblk * sizeof(Blk), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const uint64_t len = seq->len, ncode = (len + 3) / 4, nbit = (len + 7) / 8;
-    for (uint64_t x = 0; x < ncode; x++) codes[x] = (uint8_t)(host[x / 8].codes >> (8 * (x % 8)));
-    for (uint64_t x = 0; x < nbit; x++) {
-        valid[x] = (uint8_t)(host[x / 4].valid >> (8 * (x % 4)));
-        mask[x] = (uint8_t)(host[x / 4].mask >> (8 * (x % 4)));
-    }
-    return GXY_OK;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !seq || !codes || !valid || !mask) return GXY_EINVAL;
+        std::vector<Blk> host(seq->nblk);
+        GXY_HIP(ctx, hipMemcpyAsync(host.data(), seq->d, seq->nblk * sizeof(Blk), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const uint64_t len = seq->len, ncode = (len + 3) / 4, nbit = (len + 7) / 8;
+        for (uint64_t x = 0; x < ncode; x++) codes[x] = (uint8_t)(host[x / 8].codes >> (8 * (x % 8)));
+        for (uint64_t x = 0; x < nbit; x++) {
+            valid[x] = (uint8_t)(host[x / 4].valid >> (8 * (x % 4)));
+            mask[x] = (uint8_t)(host[x / 4].mask >> (8 * (x % 4)));
+        }
+        return GXY_OK;
+    });
 }
 
 void gxy_dseq_free(gxy_dseq *seq) {

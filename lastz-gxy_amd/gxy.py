@@ -32,6 +32,8 @@ _SIGS = {
     "gxy_ctx_synchronize": (C.c_int, [_vp]),
     "gxy_version": (C.c_char_p, []),
     "gxy_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
+    "gxy_ctx_set_dp_pool_rows": (C.c_int, [_vp, C.c_uint64]),
+    "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
     "gxy_ctx_reset_timing": (C.c_int, [_vp]),
     "gxy_pack_ascii": (C.c_int, [C.c_char_p, C.c_uint64, _vp, _vp, _vp]),
@@ -134,6 +136,16 @@ class Context:
 
     def set_timing(self, on: bool = True):
         self.check(lib().gxy_ctx_set_timing(self._h, 1 if on else 0), "gxy_ctx_set_timing")
+
+    def set_dp_pool_rows(self, rows: int):
+        """Cap the gapped extension's traceback pool at `rows` rows (0: default)."""
+        self.check(lib().gxy_ctx_set_dp_pool_rows(self._h, rows), "gxy_ctx_set_dp_pool_rows")
+
+    def dp_pool_stats(self):
+        """(passes, truncated sides, passes run with one side alone) since creation."""
+        v = [C.c_uint64() for _ in range(3)]
+        self.check(lib().gxy_ctx_dp_pool_stats(self._h, *[C.byref(x) for x in v]), "gxy_ctx_dp_pool_stats")
+        return tuple(x.value for x in v)
 
     def reset_timing(self):
         self.check(lib().gxy_ctx_reset_timing(self._h), "gxy_ctx_reset_timing")

@@ -168,3 +168,32 @@ def test_search_entropy_gate_on_low_complexity(ctx, thr):
     assert gst.hits == ost.hits
     assert gst.emitted == ost.emitted
     assert gh == oh
+
+
+def test_load_2bit_rejects_corrupt_counts(ctx):
+    """A record whose N-block or mask-block count exceeds the bytes left, or a
+    file cut short, is an error (src/bit2.rs:115-164 returns Bit2Error): the
+    counts are checked before anything is sized from them."""
+    import os
+    import struct
+    sys.path.insert(0, os.path.dirname(__file__))
+    from twobit import build_2bit
+    good = build_2bit([("a", b"ACGTNNNNACGTacgtacgtACGT"), ("b", b"GGCCAATT")])
+    assert len(gxy.load_2bit(good, ctx)) == 2
+    # record "a" starts at the offset in its index entry: dna_size, then n_count
+    off = struct.unpack("<I", good[16 + 2:16 + 6])[0]
+    for at, val in ((off + 4, 0xFFFFFFFF), (off + 4, 0x40000000)):
+        bad = bytearray(good)
+        bad[at:at + 4] = struct.pack("<I", val)
+        with pytest.raises(gxy.GxyError, match=r"\(-1\)"):
+            gxy.load_2bit(bytes(bad), ctx)
+    # mask-block count (after the one N block: start and size)
+    bad = bytearray(good)
+    mat = off + 8 + 8
+    assert struct.unpack("<I", good[mat:mat + 4])[0] == 1  # one lowercase run
+    bad[mat:mat + 4] = struct.pack("<I", 0xFFFFFFF0)
+    with pytest.raises(gxy.GxyError, match=r"\(-1\)"):
+        gxy.load_2bit(bytes(bad), ctx)
+    for cut in (3, 17, off + 6, len(good) - 1):
+        with pytest.raises(gxy.GxyError):
+            gxy.load_2bit(good[:cut], ctx)

@@ -79,3 +79,22 @@ def test_writers_match_oracle_on_oracle_records():
         assert len(recs.tuples()) > 0, (gapped, chain)
         assert gxy.format_maf(targets, queries, recs) == O.format_maf(targets, queries, recs)
         assert gxy.format_paf(targets, queries, recs) == O.format_paf(targets, queries, recs)
+
+
+def test_writers_reject_records_whose_script_does_not_fit():
+    """A record whose edit script consumes more (or fewer) bases than its spans,
+    or names an unknown op, is refused with GXY_EINVAL instead of reading past
+    the sequences (EditScript::target_span / query_span, src/edit_script.rs:104-121)."""
+    import pytest
+    from outcheck import ManualRecords, padded
+    targets = [("t", padded(b"ACGTACGT", 20))]
+    queries = [("q", padded(b"ACGTACGT", 20))]
+    ok = ManualRecords([(0, 0, 0, 0, 0, 8, 8, 10, [(0, 8)])])
+    assert gxy.format_maf(targets, queries, ok).count(b"a score=10") == 1
+    cases = [([(0, 9)], 8, 8), ([(0, 4), (1, 30)], 4, 4), ([(0, 7)], 8, 8), ([(3, 8)], 8, 8),
+             ([(0, 4), (2, 4)], 4, 8)]
+    for script, tsp, qsp in cases:
+        bad = ManualRecords([(0, 0, 0, 0, 0, tsp, qsp, 10, script)])
+        for fmt in (gxy.format_maf, gxy.format_paf):
+            with pytest.raises(gxy.GxyError):
+                fmt(targets, queries, bad)
