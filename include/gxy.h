@@ -255,6 +255,21 @@ int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *que
                          const gxy_scoring *scoring, const gxy_hsp_params *params,
                          const uint32_t *t_pos, const uint32_t *q_pos, uint64_t n,
                          uint32_t seed_len, gxy_hsp *out, uint8_t *found);
+/* gxy_extend_hit_batch without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as gxy_extend_hit_batch; every device
+ * buffer it needs (the staged sequences, their block and byte layouts, the
+ * hit positions and results) lives in a 64 MiB array that the library holds
+ * on each device it runs on until the process ends.  Inputs that need more
+ * than that (about 2 bytes per target and query base plus 25 bytes per hit)
+ * fail with GXY_ERANGE.  Calls on one device share the array: the library
+ * serialises them (a per-device lock held for the whole call, which returns
+ * only when the device is done with the array). */
+int gxy_extend_hit_batch_noalloc(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *query,
+                                 const gxy_scoring *scoring, const gxy_hsp_params *params,
+                                 const uint32_t *t_pos, const uint32_t *q_pos, uint64_t n,
+                                 uint32_t seed_len, gxy_hsp *out, uint8_t *found);
 
 /* ---- anchor ------------------------------------------------------------- */
 /* choose_anchor(Hsp, &PackedSeq, &PackedSeq, &ScoringMatrix, window) -> Anchor

@@ -21,6 +21,23 @@ int postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pattern 
 // caller (gxy_dseq_free); from the block cache while ctx->pool_on.
 int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out);
 int seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf);
+// The same from raw blocks of a sequence of `len` bases.
+int blocks_to_bytes(gxy_ctx *ctx, const Blk *b, uint64_t len, bool as_target, uint8_t *buf);
+// Blocks (nblocks(len) of them, the last one zero) from the reference's
+// PackedSeq byte arrays already in device memory (src/sequences.rs:128-141).
+int blocks_from_arrays(gxy_ctx *ctx, const uint8_t *codes, const uint8_t *valid, const uint8_t *mask,
+                       uint64_t len, Blk *out);
+
+// extend_hit (src/hsp.rs:66-211) over n hits whose positions, results and
+// buffers are all in device memory the caller provides: t/q blocks of the
+// pair, byte buffers of byte_buffer_size(len) for each (filled here; used
+// when extend_uses_bytes), an 8-byte column counter.  Allocates nothing;
+// launches on ctx->stream without waiting.
+bool extend_uses_bytes(const gxy_scoring &sc, const gxy_hsp_params &hp);
+int extend_hits_dev(gxy_ctx *ctx, const Blk *t, uint64_t tlen, const Blk *q, uint64_t qlen, uint8_t *tbytes,
+                    uint8_t *qbytes, const uint32_t *dtp, const uint32_t *dqp, uint64_t n, const gxy_scoring &sc,
+                    const gxy_hsp_params &hp, uint32_t seed_len, gxy_hsp *dout, uint8_t *dfound,
+                    unsigned long long *colsum);
 
 // Seed search on device sequences.  `qbytes` may hold the query's byte copy
 // (seq_to_bytes buffer + kBytePad) when the caller searches the same query

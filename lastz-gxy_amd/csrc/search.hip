@@ -1237,6 +1237,39 @@ uint64_t dedup_hsps(std::vector<gxy_hsp> *v) {
 
 }  // namespace gxy
 
+namespace gxy {
+
+bool extend_uses_bytes(const gxy_scoring &sc, const gxy_hsp_params &hp) {
+    return scoring_byte_ok(sc) && hp.x_drop >= 0;
+}
+
+int extend_hits_dev(gxy_ctx *ctx, const Blk *t, uint64_t tlen, const Blk *q, uint64_t qlen, uint8_t *tbytes,
+                    uint8_t *qbytes, const uint32_t *dtp, const uint32_t *dqp, uint64_t n, const gxy_scoring &sc,
+                    const gxy_hsp_params &hp, uint32_t seed_len, gxy_hsp *dout, uint8_t *dfound,
+                    unsigned long long *colsum) {
+    if (n == 0) return GXY_OK;
+    if (extend_uses_bytes(sc, hp)) {
+        // persistent lanes over byte buffers (the search's x-drop kernel)
+        int rc = blocks_to_bytes(ctx, t, tlen, true, tbytes);
+        if (!rc) rc = blocks_to_bytes(ctx, q, qlen, false, qbytes);
+        if (rc) return rc;
+        XdropArgs xa{tbytes + kBytePad, qbytes + kBytePad, tlen, qlen, t, make_byte_lut(sc), hp.x_drop,
+                     hp.hsp_threshold, seed_len, 0, 0};
+        PairSrc src{dtp, dqp, n};
+        PassOut none{nullptr, nullptr, nullptr};
+        k_xdrop<PairSrc, false><<<std::min<unsigned>(grid_for(n, 256), kXdropBlocks), 256, 0, ctx->stream>>>(
+            xa, src, none, dout, dfound, colsum);
+    } else {
+        k_extend_batch<ScorerI32><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+            dtp, dqp, n, t, tlen, q, qlen, make_scorer_i32(sc), hp.x_drop, hp.hsp_threshold, seed_len, dout,
+            dfound);
+    }
+    GXY_LAUNCH_CHECK(ctx);
+    return GXY_OK;
+}
+
+}  // namespace gxy
+
 extern "C" {
 
 int gxy_search_dev(gxy_ctx *ctx, const gxy_postable *table, const gxy_dseq *query, const gxy_scoring *scoring,
@@ -1291,49 +1324,31 @@ int gxy_extend_hit_batch(gxy_ctx *ctx, const gxy_seq *target, const gxy_seq *que
             gxy_dseq_free(dt);
             return rc;
         }
+        const bool bytes = extend_uses_bytes(*scoring, *params);
         auto *dtp = (uint32_t *)scratch(ctx, kSlotMisc0, n * 4);
         auto *dqp = (uint32_t *)scratch(ctx, kSlotMisc1, n * 4);
         auto *dout = (gxy_hsp *)scratch(ctx, kSlotMisc2, n * sizeof(gxy_hsp));
         auto *dfound = (uint8_t *)scratch(ctx, kSlotMisc3, n);
-        hipError_t e = hipSuccess;
-        if (!dtp || !dqp || !dout || !dfound) {
+        auto *cs = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
+        auto *tb = bytes ? (uint8_t *)scratch(ctx, kSlotTBytes, byte_buffer_size(dt->len)) : nullptr;
+        auto *qb = bytes ? (uint8_t *)scratch(ctx, kSlotQBytes, byte_buffer_size(dq->len)) : nullptr;
+        if (!dtp || !dqp || !dout || !dfound || !cs || (bytes && (!tb || !qb))) {
             rc = set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
         } else {
-            e = hipMemcpyAsync(dtp, t_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
+            hipError_t e = hipMemcpyAsync(dtp, t_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(dqp, q_pos, n * 4, hipMemcpyHostToDevice, ctx->stream);
-            if (e == hipSuccess) {
-                if (scoring_byte_ok(*scoring) && params->x_drop >= 0) {
-                    auto *tb = (uint8_t *)scratch(ctx, kSlotTBytes, byte_buffer_size(dt->len));
-                    auto *qb = (uint8_t *)scratch(ctx, kSlotQBytes, byte_buffer_size(dq->len));
-                    if (!tb || !qb) {
-                        gxy_dseq_free(dt);
-                        gxy_dseq_free(dq);
-                        return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
-                    }
-                    rc = seq_to_bytes(ctx, dt, true, tb);
-                    if (!rc) rc = seq_to_bytes(ctx, dq, false, qb);
-                    if (rc) {
-                        gxy_dseq_free(dt);
-                        gxy_dseq_free(dq);
-                        return rc;
-                    }
-                    XdropArgs xa{tb + kBytePad, qb + kBytePad, dt->len, dq->len, dt->d, make_byte_lut(*scoring),
-                                 params->x_drop, params->hsp_threshold, seed_len, 0, 0};
-                    PairSrc src{dtp, dqp, n};
-                    PassOut none{nullptr, nullptr, nullptr};
-                    auto *cs = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
-                    k_xdrop<PairSrc, false><<<std::min<unsigned>(grid_for(n, 256), kXdropBlocks), 256, 0,
-                                              ctx->stream>>>(xa, src, none, dout, dfound, cs);
-                } else
-                    k_extend_batch<ScorerI32><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
-                        dtp, dqp, n, dt->d, dt->len, dq->d, dq->len, make_scorer_i32(*scoring), params->x_drop,
-                        params->hsp_threshold, seed_len, dout, dfound);
-                e = hipGetLastError();
+            if (e != hipSuccess) {
+                rc = set_err(ctx, GXY_EDEVICE, "extend_hit_batch: %s", hipGetErrorString(e));
+            } else {
+                rc = extend_hits_dev(ctx, dt->d, dt->len, dq->d, dq->len, tb, qb, dtp, dqp, n, *scoring, *params,
+                                     seed_len, dout, dfound, cs);
             }
-            if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(found, dfound, n, hipMemcpyDeviceToHost, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) rc = set_err(ctx, GXY_EDEVICE, "extend_hit_batch: %s", hipGetErrorString(e));
+            if (!rc) {
+                e = hipMemcpyAsync(out, dout, n * sizeof(gxy_hsp), hipMemcpyDeviceToHost, ctx->stream);
+                if (e == hipSuccess) e = hipMemcpyAsync(found, dfound, n, hipMemcpyDeviceToHost, ctx->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+                if (e != hipSuccess) rc = set_err(ctx, GXY_EDEVICE, "extend_hit_batch: %s", hipGetErrorString(e));
+            }
         }
         gxy_dseq_free(dt);
         gxy_dseq_free(dq);

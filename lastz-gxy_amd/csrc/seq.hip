@@ -309,6 +309,35 @@ __global__ void k_slice(const Blk *__restrict__ src, Blk *__restrict__ dst, uint
     dst[b] = {w.codes, w.valid, w.mask};
 }
 
+// Blocks from the reference's byte arrays already in device memory (the
+// device form of interleave above): one thread per block.
+__global__ void k_from_arrays(const uint8_t *__restrict__ codes, const uint8_t *__restrict__ valid,
+                              const uint8_t *__restrict__ mask, uint64_t len, uint64_t nblk, Blk *__restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblk) return;
+    const uint64_t ncode = (len + 3) / 4, nbit = (len + 7) / 8;
+    uint64_t c = 0;
+    uint32_t v = 0, m = 0;
+    for (uint64_t x = 8 * k; x < ncode && x < 8 * k + 8; x++) c |= (uint64_t)codes[x] << (8 * (x - 8 * k));
+    for (uint64_t x = 4 * k; x < nbit && x < 4 * k + 4; x++) {
+        v |= (uint32_t)valid[x] << (8 * (x - 4 * k));
+        m |= (uint32_t)mask[x] << (8 * (x - 4 * k));
+    }
+    const uint64_t first = 32 * k;
+    if (first >= len) {
+        c = 0;
+        v = m = 0;
+    } else if (len - first < 32) {
+        const uint32_t keep = (uint32_t)(len - first);
+        v &= (1u << keep) - 1u;
+        m &= (1u << keep) - 1u;
+        c &= (1ull << (2 * keep)) - 1ull;
+    }
+    m &= v;
+    c &= spread2(v);
+    out[k] = {c, v, m};
+}
+
 // Byte-per-base buffer (x-drop layout, target or query codes) including both
 // pads: one thread per output dword.
 __global__ void k_to_bytes(const Blk *__restrict__ src, uint64_t len, uint64_t ndw, int as_target,
@@ -351,10 +380,22 @@ int dseq_alloc(gxy_ctx *ctx, uint64_t len, gxy_dseq **out) {
 
 }  // namespace gxy
 
-int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf) {
-    const uint64_t ndw = (byte_buffer_size(s->len) + 3) / 4;
-    k_to_bytes<<<(unsigned)((ndw + 255) / 256), 256, 0, ctx->stream>>>(s->d, s->len, ndw, as_target ? 1 : 0,
+int gxy::blocks_to_bytes(gxy_ctx *ctx, const Blk *b, uint64_t len, bool as_target, uint8_t *buf) {
+    const uint64_t ndw = (byte_buffer_size(len) + 3) / 4;
+    k_to_bytes<<<(unsigned)((ndw + 255) / 256), 256, 0, ctx->stream>>>(b, len, ndw, as_target ? 1 : 0,
                                                                        (uint32_t *)buf);
+    GXY_LAUNCH_CHECK(ctx);
+    return GXY_OK;
+}
+
+int gxy::seq_to_bytes(gxy_ctx *ctx, const gxy_dseq *s, bool as_target, uint8_t *buf) {
+    return blocks_to_bytes(ctx, s->d, s->len, as_target, buf);
+}
+
+int gxy::blocks_from_arrays(gxy_ctx *ctx, const uint8_t *codes, const uint8_t *valid, const uint8_t *mask,
+                            uint64_t len, Blk *out) {
+    const uint64_t nblk = nblocks(len);
+    k_from_arrays<<<(unsigned)((nblk + 255) / 256), 256, 0, ctx->stream>>>(codes, valid, mask, len, nblk, out);
     GXY_LAUNCH_CHECK(ctx);
     return GXY_OK;
 }

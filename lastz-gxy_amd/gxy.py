@@ -58,6 +58,8 @@ _SIGS = {
                                  C.c_uint32, C.c_uint32, C.c_uint32, _P(T.Hsp), _P(C.c_int)]),
     "gxy_extend_hit_batch": (C.c_int, [_vp, _P(T.Seq), _P(T.Seq), _P(T.Scoring), _P(T.HspParams),
                                        _vp, _vp, C.c_uint64, C.c_uint32, _vp, _vp]),
+    "gxy_extend_hit_batch_noalloc": (C.c_int, [_vp, _P(T.Seq), _P(T.Seq), _P(T.Scoring), _P(T.HspParams),
+                                       _vp, _vp, C.c_uint64, C.c_uint32, _vp, _vp]),
     "gxy_choose_anchor": (C.c_int, [_vp, _P(T.Hsp), _P(T.Seq), _P(T.Seq), _P(T.Scoring),
                                     C.c_uint32, _P(T.Anchor)]),
     "gxy_choose_anchor_batch": (C.c_int, [_vp, _vp, C.c_uint64, _P(T.Seq), _P(T.Seq),
@@ -336,6 +338,22 @@ def extend_hit_batch(target, query, matrix, params, t_pos, q_pos, seed_len, ctx=
                                          C.byref(matrix), C.byref(params), t_pos.ctypes.data,
                                          q_pos.ctypes.data, n, seed_len, C.addressof(out),
                                          found.ctypes.data), "gxy_extend_hit_batch")
+    return [out[i].astuple() if found[i] else None for i in range(n)]
+
+
+def extend_hit_batch_noalloc(target, query, matrix, params, t_pos, q_pos, seed_len, ctx=None):
+    """gxy_extend_hit_batch_noalloc: extend_hit_batch with every device buffer
+    in the library's fixed per-device array (no device allocation)."""
+    ctx = ctx or default_context()
+    t_pos = np.ascontiguousarray(t_pos, np.uint32)
+    q_pos = np.ascontiguousarray(q_pos, np.uint32)
+    n = len(t_pos)
+    out = (T.Hsp * max(1, n))()
+    found = np.zeros(max(1, n), np.uint8)
+    ctx.check(lib().gxy_extend_hit_batch_noalloc(ctx._h, C.byref(target.view()), C.byref(query.view()),
+                                                 C.byref(matrix), C.byref(params), t_pos.ctypes.data,
+                                                 q_pos.ctypes.data, n, seed_len, C.addressof(out),
+                                                 found.ctypes.data), "gxy_extend_hit_batch_noalloc")
     return [out[i].astuple() if found[i] else None for i in range(n)]
 
 
