@@ -94,7 +94,7 @@ def measured_copy_gbs(nbytes=1 << 30, reps=10):
     return round(2.0 * nbytes * reps / (ms.value / 1e3) / 1e9, 1)
 
 
-def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3):
+def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0):
     """Batched gapped DP over n_anchors synthetic anchored pairs.  Returns a
     dict (GCUPS, kernel times, VALU roofline, oracle baseline on a stratified
     sample whose alignments are compared with the GPU's)."""
@@ -108,6 +108,7 @@ def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3):
     code, qcode, toff, qoff, tpos, qpos = synth.dp_anchor_arrays(n_anchors, seed, packed=False)
     log(f"dp: generated {n_anchors} pairs ({len(code)} + {len(qcode)} bp) in {time.time() - t0:.1f}s")
     ctx = gxy.default_context()
+    ctx.set_dp_kernel(kernel)
     t0 = time.time()
     dcat_t = gxy.DeviceSeq.upload(synth._packed(code), ctx)
     dcat_q = gxy.DeviceSeq.upload(synth._packed(qcode), ctx)
@@ -493,6 +494,9 @@ def main():
     ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.8])
     ap.add_argument("--dp-anchors", type=int, default=50_000)
     ap.add_argument("--dp-cpu-anchors", type=int, default=400)
+    ap.add_argument("--dp-kernel", type=int, default=0,
+                    help="DP forward kernel (0 library's choice, 1 four wavefronts per side, 2/3/4 one "
+                         "wavefront per side with 1/2/4 columns per lane per slot)")
     ap.add_argument("--no-dp", action="store_true", help="omit the batched_dp section of the chrom line")
     ap.add_argument("--target-bp", type=int, default=48_000_000)
     ap.add_argument("--cpu-sample-bp", type=int, default=16_000_000)
@@ -517,7 +521,8 @@ def main():
         return
 
     if args.workload == "dp":
-        d = dp_leg(args.dp_anchors, args.steps, args.warmup, 0 if args.no_cpu else args.dp_cpu_anchors)
+        d = dp_leg(args.dp_anchors, args.steps, args.warmup, 0 if args.no_cpu else args.dp_cpu_anchors,
+                   kernel=args.dp_kernel)
         out = {"metric": METRIC, "value": d["dp_gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": d["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",

@@ -183,6 +183,20 @@ int gxy_ctx_reset_timing(gxy_ctx *ctx);
  * side that needs more rows than the whole pool fails with GXY_ENOMEM.  Results
  * do not depend on the cap.  rows = 0 restores the default. */
 int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows);
+/* DP forward kernel of the gapped extension (no counterpart in the reference;
+ * for measurement and tests).  Every kernel gives identical results; they
+ * differ in how a side's row is spread over the GPU: GXY_DP_BLOCK4 runs each
+ * side on four wavefronts with the row in LDS, GXY_DP_WAVE_K* on one
+ * wavefront with the row in registers, 1, 2 or 4 columns per lane per slot.
+ * GXY_DP_AUTO (the default) lets the library choose. */
+#define GXY_DP_AUTO 0
+#define GXY_DP_BLOCK4 1
+#define GXY_DP_WAVE_K1 2
+#define GXY_DP_WAVE_K2 3
+#define GXY_DP_WAVE_K4 4
+#define GXY_DP_WAVE_K2_W4 5 /* K2 / K4 compiled for four wavefronts per SIMD */
+#define GXY_DP_WAVE_K4_W4 6
+int gxy_ctx_set_dp_kernel(gxy_ctx *ctx, int kernel);
 /* Gapped-extension passes on the context since it was created: passes run,
  * sides stopped by a full pool (swept again later) and passes that ran one
  * side alone. */

@@ -374,6 +374,7 @@ struct gxy_ctx {
     gxy_ctx *helper = nullptr;
     uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(96 GiB, device / 3))
     uint64_t dp_pool_rows = 0; // traceback pool rows set by the caller (0: no row cap)
+    int dp_kernel = 0;         // DP forward kernel (GXY_DP_*), 0: chosen by the library
     // gapped passes since creation: passes, sides stopped by a full pool, passes run with one side alone
     uint64_t dp_passes = 0, dp_truncated = 0, dp_alone = 0;
     std::multimap<size_t, void *> pool_free;

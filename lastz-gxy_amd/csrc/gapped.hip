@@ -36,13 +36,17 @@ constexpr int kMaxW = 1280;                 // widest row window
 constexpr int kDpThreads = 256;             // lanes per side-problem
 constexpr int kDpWaves = kDpThreads / 64;
 constexpr int kCMax = kMaxW / kDpThreads;   // cells per lane
-// A traceback row packs the 4-bit from-states of its cells in window order:
-// cell idx = j - jl at bits [4 idx, 4 idx + 4) (lane t's five cells are bits
-// [20 t, 20 t + 20)), 256 lanes x 20 bits = 640 bytes.
-constexpr int kRowWords = kDpThreads * kCMax * 4 / 32;
-constexpr int kRowBytes = kRowWords * 4;
-static_assert(kRowWords % kDpWaves == 0 && kRowWords / kDpWaves * 32 == 64 * kCMax * 4,
+// Traceback rows hold 4-bit from-states, one per cell, in one of two layouts
+// (DpRes::fmt): k_dp_forward packs them in window order, cell idx = j - jl at
+// bits [4 idx, 4 idx + 4) (lane t's five cells are bits [20 t, 20 t + 20)),
+// 256 lanes x 20 bits = 640 bytes; k_dp_fwd1 stores them lane-major, 96 bits
+// per lane (see there), 768 bytes.  Rows sit kRowBytes apart in the pool.
+constexpr int kRowWordsW4 = kDpThreads * kCMax * 4 / 32;
+static_assert(kRowWordsW4 % kDpWaves == 0 && kRowWordsW4 / kDpWaves * 32 == 64 * kCMax * 4,
               "a wavefront's lanes fill whole words of the packed row");
+constexpr int kRowWords = 64 * 3;           // lane-major layout: three words per lane
+constexpr int kRowBytes = kRowWords * 4;
+static_assert(kRowWordsW4 <= kRowWords, "both layouts fit one row");
 constexpr int kRecs = kMaxW + 2;            // previous-row records in LDS: [0] NEG, [1, 1+W) the row, NEG after
 constexpr int kQCache = 4096;               // query columns cached in LDS, one byte each
 constexpr int kQMargin = 512;               // cached columns kept left of the window on a refill
@@ -68,7 +72,7 @@ struct DpRes {
     uint32_t rows;       // rows swept
     uint32_t truncated;  // 1 when the traceback pool ran out before the sweep ended
     uint32_t last_chunk; // pool chunk holding the last swept row
-    uint32_t pad;
+    uint32_t fmt;        // traceback row layout: 0 window order, 1 + log2(K) lane-major (k_dp_fwd1<K>)
     uint64_t cells;
 };
 
@@ -93,6 +97,29 @@ __device__ __forceinline__ int32_t wave_scan_max(int32_t v) {
     v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x118, 0xF, 0xF, false));  // row_shr:8
     v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
     v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return v;
+}
+
+// The same in place: each step is one DPP max whose missing source lanes
+// keep their own value (v_max_i32_dpp with destination = both sources), with
+// the two wait states a DPP read of a just-written VGPR needs.  Every lane
+// must be active.
+__device__ __forceinline__ int32_t wave_scan_max_self(int32_t v) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_max_i32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1\n"
+        "v_max_i32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1\n"
+        "v_max_i32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1\n"
+        "v_max_i32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1\n"
+        "v_max_i32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n"
+        "s_nop 1\n"
+        "v_max_i32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n"
+        "s_nop 1\n"
+        : "+v"(v));
     return v;
 }
 
@@ -482,9 +509,9 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
             const uint32_t v2 = (uint32_t)__shfl((int)word, (int)min(a + 2u, 63u));
             uint32_t pw = (v0 >> o) | (v1 << (20u - o));
             if (o > 8u) pw |= v2 << (40u - o);
-            if (lane < kRowWords / kDpWaves)
+            if (lane < kRowWordsW4 / kDpWaves)
                 *(uint32_t *)(pool.tb + chunk * kChunkBytes + ir * kRowBytes +
-                              ((uint32_t)wv * (kRowWords / kDpWaves) + (uint32_t)lane) * 4) = pw;
+                              ((uint32_t)wv * (kRowWordsW4 / kDpWaves) + (uint32_t)lane) * 4) = pw;
         }
         if (t == 0) pool.win[(uint64_t)chunk * kChunkRows + ir] = make_uint2(jl, jh);
         rows_done = i + 1;
@@ -515,6 +542,434 @@ __global__ __launch_bounds__(kDpThreads) __attribute__((amdgpu_waves_per_eu(5)))
         out.last_chunk = chunk;
         out.cells = cells;
         res[p] = out;
+    }
+}
+
+// ---- one wavefront per side, rows in registers --------------------------------
+//
+// k_dp_fwd1<S, K>: the same row recurrence as k_dp_forward, for one side per
+// wavefront, with the row held in registers instead of LDS.  Columns map to
+// lanes by absolute position, so a cell of the previous row sits in the same
+// lane and register as the cell of this row below it: column j belongs to
+// slot k = j / SW - B (SW = 64 K columns per slot, B = jl / SW the row's base
+// block), lane (j % SW) / K and cell j % K of that lane.  NS slots cover any
+// window of at most kMaxW columns wherever it starts.  Per lane and cell the
+// registers carry, from the previous row:
+//   dd = max(M, X, Y)                  (what M of the column to the right adds to)
+//   xn = max(M - (O+E), X - E)         (X of the same column in this row)
+//   sb = arg of dd (2 bits) | from-bit of xn << 2
+// with cells outside the previous row's window or dead at its end holding the
+// values the reference reads there (NEG for M, X and Y).  The window moves
+// right as the alignment proceeds: when jl crosses a slot boundary the slots
+// shift down by one (rarely up, when the band grows left), once per SW rows.
+//
+// Y is the max-plus prefix scan of k_dp_forward with offsets relative to the
+// row's base column B * SW: each cell contributes c = M - (O+E) + (j + 1 -
+// B SW) E to every cell to its right, Z[j] = max(zinit, c over [jl, j)), Y[j]
+// = Z[j] - (j - B SW) E; inside a lane over its K cells, across lanes with a
+// DPP scan, across slots through a carry.  The Y from-bit of cell j is
+// Z[j] != c(j - 1) (ties go to M), and the window's first cell takes the
+// reference's from-state of a cell whose left neighbour lies outside.
+//
+// Traceback rows are stored lane-major: lane l's 96 bits at bit 96 l, cell
+// (k, p) of the lane at bits [4 (k K + p), + 4).
+template <int K>
+struct Fwd1Shape {
+    static constexpr int SW = 64 * K;                     // columns per slot
+    static constexpr int NS = (kMaxW + SW - 1) / SW + 1;  // slots: any window of <= kMaxW columns
+    static_assert(NS * K * 4 <= 96, "a lane's traceback cells fit 96 bits");
+};
+
+// Query bytes 8 * (code | 4 * invalid) (as in the LDS cache of k_dp_forward)
+// of query indices xs .. xs + 31 into out[0..8) (byte t of the 32 at byte t),
+// zero for indices outside [0, m).
+__device__ __forceinline__ void fwd1_qchunk(const DpProb &pr, int64_t xs, uint32_t *out) {
+    uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (xs + 32 > 0 && xs < (int64_t)pr.m) {
+        const int64_t lx = xs < 0 ? 0 : xs;  // first index read
+        const uint32_t sh = (uint32_t)(lx - xs);
+        const Win w = dir_win(pr.Q, pr.qdir > 0 ? pr.q0 + (uint64_t)lx : pr.q0 - (uint64_t)lx, pr.qdir);
+#pragma unroll
+        for (int t = 0; t < 32; t++) {
+            const int64_t x = xs + t;
+            if ((uint32_t)t >= sh && x < (int64_t)pr.m) {
+                const uint32_t u = (uint32_t)t - sh;
+                const uint32_t code = (uint32_t)(w.codes >> (2 * u)) & 3u;
+                const uint32_t inv = ((w.valid >> u) & 1u) ^ 1u;
+                v[t >> 2] |= (8u * (code | (inv << 2))) << (8 * (t & 3));
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) out[t] = v[t];
+}
+
+template <class S, int K, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dp_fwd1(
+    const DpProb *__restrict__ probs, const uint32_t *__restrict__ list, uint32_t nprob, S sc, int32_t oe, int32_t e,
+    int32_t ydrop, DpPool pool, DpRes *__restrict__ res) {
+    constexpr int SW = Fwd1Shape<K>::SW, NS = Fwd1Shape<K>::NS;
+    constexpr uint32_t kFmt = K == 1 ? 1u : (K == 2 ? 2u : 3u);
+    const uint32_t side = blockIdx.x;
+    if (side >= nprob) return;
+    const int lane = threadIdx.x;
+    const DpProb pr = probs[list[side]];
+    const uint32_t n = pr.n, m = pr.m;
+    DpRes out = {0, 0, 0, kStM, 0, 0, kNoChunk, kFmt, 0};
+    if (n == 0 || m == 0) {  // src/gapped_extend.rs:188-195
+        if (lane == 0) res[side] = out;
+        return;
+    }
+    // band radius / width (src/gapped_extend.rs:214-233)
+    uint32_t br;
+    if (e > 0) {
+        const int32_t r = sat_sub(ydrop, oe) / e;
+        br = (uint32_t)(max(r, 0) + 2);
+    } else {
+        br = m + 1;
+    }
+    const uint32_t bw = (uint32_t)min((uint64_t)4 * br + 4, (uint64_t)m + 1);  // host guarantees bw <= kMaxW
+    // what a cell outside the previous row's window (or dead) yields
+    const int32_t negx = max(kNegInf - oe, kNegInf - e);
+    const uint32_t xb_neg = (kNegInf - oe >= kNegInf - e) ? 0u : 1u;
+    uint32_t sb_negw = 0;  // arg M, X from-bit of NEG, in every cell field
+#pragma unroll
+    for (int p = 0; p < K; p++) sb_negw |= (xb_neg << 2) << (4 * p);
+    const int32_t lead_y = negx;  // Y of a cell whose left neighbour is outside
+    const uint32_t ybit_lead = xb_neg;
+
+    // query bytes of query indices [qc0, qc0 + kQC1): column j reads index j - 1
+    constexpr int kQC1 = 4096;
+    __shared__ uint32_t qcache[kQC1 / 4];
+    int64_t qc0 = INT64_MIN / 2;  // nothing cached yet
+
+    // Row 0 from the same code: every column scores `ambig` in row 0 (no
+    // target base), so D = NEG - ambig gives M = NEG exactly, X starts at NEG
+    // and row 0 has no M or X from-states (src/gapped_extend.rs:245).
+    int32_t dd[NS][K], xn[NS][K];
+    uint32_t sb[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+#pragma unroll
+        for (int p = 0; p < K; p++) {
+            dd[k][p] = kNegInf - sc.ambig;
+            xn[k][p] = kNegInf;
+        }
+        sb[k] = 0;
+    }
+    auto reset_slot = [&](int k) {
+#pragma unroll
+        for (int p = 0; p < K; p++) {
+            dd[k][p] = kNegInf;
+            xn[k][p] = negx;
+        }
+        sb[k] = sb_negw;
+    };
+    uint32_t B = 0;  // base block: slot k holds columns [(B + k) SW, (B + k + 1) SW)
+
+    int32_t best_score = 0;
+    uint32_t bi = 0, bj = 0, bst = kStM;
+    uint32_t lo = 0, hi = 0;
+    Win tw = {0, 0, 0};
+    uint64_t cells = 0;
+    uint32_t rows_done = 0, chunk = kNoChunk, truncated = 0;
+    for (uint32_t i = 0; i <= n; i++) {
+        // a fresh traceback chunk every kChunkRows rows
+        if (i % kChunkRows == 0) {
+            uint32_t c = 0;
+            if (lane == 0) {
+                c = atomicAdd(pool.next, 1u);
+                if (c < pool.nchunks) pool.prev[c] = chunk;
+            }
+            c = __builtin_amdgcn_readfirstlane(c);
+            if (c >= pool.nchunks) {  // pool exhausted: the side is swept again later
+                truncated = 1;
+                break;
+            }
+            chunk = c;
+        }
+        const uint32_t jl = lo >= br ? lo - br : 0;
+        const uint32_t jh = (uint32_t)min(min((uint64_t)hi + br, (uint64_t)m), (uint64_t)jl + bw - 1);
+        cells += (jh - jl + 1) - ((i == 0 && jl == 0) ? 1 : 0);
+
+        // move the slots with the window; the column left of the new base
+        // keeps its previous-row value (it may lie inside the previous window)
+        const uint32_t Bn = jl / SW;
+        int32_t carry_dd = kNegInf;  // D and sb word of the column left of slot 0 (lane 63 of a slot)
+        uint32_t carry_sb = sb_negw;
+        if (Bn > B) {
+            const uint32_t d = Bn - B;
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                if ((uint32_t)k + 1 == d) {
+                    carry_dd = __builtin_amdgcn_readlane(dd[k][K - 1], 63);
+                    carry_sb = __builtin_amdgcn_readlane(sb[k], 63);
+                }
+            }
+            const uint32_t steps = min(d, (uint32_t)NS);
+            for (uint32_t x = 0; x < steps; x++) {
+#pragma unroll
+                for (int k = 0; k + 1 < NS; k++) {
+#pragma unroll
+                    for (int p = 0; p < K; p++) {
+                        dd[k][p] = dd[k + 1][p];
+                        xn[k][p] = xn[k + 1][p];
+                    }
+                    sb[k] = sb[k + 1];
+                }
+                reset_slot(NS - 1);
+            }
+            B = Bn;
+        } else if (Bn < B) {
+            const uint32_t steps = min(B - Bn, (uint32_t)NS);
+            for (uint32_t x = 0; x < steps; x++) {
+#pragma unroll
+                for (int k = NS - 1; k > 0; k--) {
+#pragma unroll
+                    for (int p = 0; p < K; p++) {
+                        dd[k][p] = dd[k - 1][p];
+                        xn[k][p] = xn[k - 1][p];
+                    }
+                    sb[k] = sb[k - 1];
+                }
+                reset_slot(0);
+            }
+            B = Bn;
+        }
+        const uint32_t kmax = jh / SW - B;  // last slot the window reaches (< NS)
+        {
+            // the active slots read query indices [B SW - 1, (B + kmax + 1) SW - 1);
+            // a refill starts the cache at B SW - 1, so a slot's K bytes stay aligned
+            const int64_t xlo = (int64_t)B * SW - 1, xhi = (int64_t)(B + kmax + 1) * SW - 1;
+            if (xlo < qc0 || xhi > qc0 + kQC1) {
+                qc0 = xlo;
+#pragma unroll
+                for (int h = 0; h < kQC1 / 64 / 32; h++) {
+                    const int c = lane * (kQC1 / 64 / 32) + h;  // 32-index chunk of this lane
+                    uint32_t w[8];
+                    fwd1_qchunk(pr, qc0 + 32 * c, w);
+                    *(uint4 *)&qcache[8 * c] = make_uint4(w[0], w[1], w[2], w[3]);
+                    *(uint4 *)&qcache[8 * c + 4] = make_uint4(w[4], w[5], w[6], w[7]);
+                }
+                __syncthreads();
+            }
+        }
+
+        // target base of row i (T[i-1]); uniform
+        uint32_t tc = 0, tv = 0;
+        if (i >= 1) {
+            const uint32_t r = (i - 1) & 31u;
+            if (r == 0) tw = dir_win(pr.T, pr.tdir > 0 ? pr.t0 + (i - 1) : pr.t0 - (i - 1), pr.tdir);
+            tc = (uint32_t)(tw.codes >> (2 * r)) & 3u;
+            tv = (tw.valid >> r) & 1u;
+        }
+        const auto rs = row_scores(sc, tc, tv);
+        // Per-row copies of the gap costs and the lane id: the column offsets
+        // below are computed per row instead of being hoisted out of the row
+        // loop into dozens of live registers.
+        int32_t er = e, oer = oe;
+        int32_t lr = (int32_t)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));  // lane id
+        asm volatile("" : "+s"(er), "+s"(oer), "+v"(lr));
+        const int32_t lc = K * lr;  // the lane's first column within a slot
+        const int32_t lane_e = lc * er;
+
+        const int32_t base_col = (int32_t)(B * SW);
+        // Z at the window's first cell: for jl >= 1 both of Y[jl]'s candidates
+        // lie outside the window; Y[0] is not computed
+        int32_t zc = (jl >= 1 ? lead_y : kNegInf) + ((int32_t)jl - base_col) * er;
+        // lane 0 of these holds the previous-row D / sb word, and the Y-scan
+        // term, of the column left of the current slot (lane 63 of the slot
+        // before, moved round with a DPP rotate)
+        int32_t ldd = carry_dd, lsb = (int32_t)carry_sb, lcv = INT32_MIN;
+        int32_t rmax = INT32_MIN;
+        uint32_t nw0 = 0, nw1 = 0, nw2 = 0;  // the lane's traceback cells of this row
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            if ((uint32_t)k <= kmax) {
+                const int32_t scol = base_col + k * SW;  // first column of the slot
+                const int32_t jl_rel = (int32_t)jl - scol, jh_rel = (int32_t)jh - scol;
+                // previous-row D and sb of column j - 1 for cell 0: lane - 1, or the slot before
+                const int32_t dl = __builtin_amdgcn_update_dpp(ldd, dd[k][K - 1], 0x138, 0xF, 0xF, false);
+                const uint32_t al = (uint32_t)__builtin_amdgcn_update_dpp(lsb, (int)sb[k], 0x138, 0xF, 0xF, false);
+                ldd = __builtin_amdgcn_update_dpp(0, dd[k][K - 1], 0x13C, 0xF, 0xF, false);  // wave_ror:1
+                lsb = __builtin_amdgcn_update_dpp(0, (int)sb[k], 0x13C, 0xF, 0xF, false);
+                // this lane's K query bytes: index j - 1 - qc0 = (B + k) SW + lc - 1 - qc0, a multiple of K
+                const uint32_t qoff = (uint32_t)((int64_t)scol - 1 - qc0) + (uint32_t)lc;
+                uint32_t qk;
+                if (K == 1)
+                    qk = ((const uint8_t *)qcache)[qoff];
+                else if (K == 2)
+                    qk = ((const uint16_t *)qcache)[qoff / 2];
+                else
+                    qk = qcache[qoff / 4];
+                int32_t Mv[K], cv[K];
+#pragma unroll
+                for (int p = 0; p < K; p++) {
+                    const int32_t dprev = p == 0 ? dl : dd[k][p - 1];
+                    Mv[p] = dprev + rs((qk >> (8 * p)) & 0xFFu);  // src/gapped_extend.rs:339-363
+                    cv[p] = Mv[p] + lane_e + ((k * SW + p + 1) * er - oer);
+                }
+                uint32_t mfix = 0;  // column 0 (no M, no Y): lane 0 cell 0 of the slot at column 0
+                if (k == 0 && scol == 0) {
+                    if (lr == 0) {
+                        Mv[0] = i == 0 ? 0 : kNegInf;
+                        cv[0] = Mv[0] + ((1 * er) - oer);
+                    }
+                    mfix = lr == 0 ? 1u : 0u;
+                }
+                if (k == 0) {  // columns left of the window add nothing to the Y scan
+#pragma unroll
+                    for (int p = 0; p < K; p++)
+                        if (lc + p < jl_rel) cv[p] = INT32_MIN;
+                }
+                // Y: lane-local exclusive prefix, DPP scan across lanes, carry across slots
+                int32_t E[K];
+                E[0] = INT32_MIN;
+#pragma unroll
+                for (int p = 1; p < K; p++) E[p] = max(E[p - 1], cv[p - 1]);
+                const int32_t incl = wave_scan_max_self(max(E[K - 1], cv[K - 1]));
+                const int32_t excl = max(__builtin_amdgcn_update_dpp(zc, incl, 0x138, 0xF, 0xF, false), zc);
+                const int32_t cl0 = __builtin_amdgcn_update_dpp(lcv, cv[K - 1], 0x138, 0xF, 0xF, false);
+                lcv = __builtin_amdgcn_update_dpp(0, cv[K - 1], 0x13C, 0xF, 0xF, false);
+                zc = max(zc, __builtin_amdgcn_readlane(incl, 63));
+                const bool full = jl_rel <= 0 && jh_rel >= SW - 1;
+                uint32_t sbn = 0;
+#pragma unroll
+                for (int p = 0; p < K; p++) {
+                    const int32_t z = max(excl, E[p]);
+                    int32_t yv = z - lane_e - (k * SW + p) * er;
+                    uint32_t yb = z != (p == 0 ? cl0 : cv[p - 1]) ? 8u : 0u;
+                    // from-states of the previous row: M from column j - 1, X from column j
+                    const uint32_t sprev = p == 0 ? (al >> (4 * (K - 1))) : (sb[k] >> (4 * (p - 1)));
+                    uint32_t nib = __builtin_amdgcn_ubfe(sprev, 0, 2) | (sb[k] >> (4 * p) & 4u);
+                    if (k == 0) {
+                        if (lc + p == jl_rel) yb = ybit_lead << 3;
+                        if (p == 0 && mfix) {
+                            yv = kNegInf;
+                            yb = 0;
+                            nib &= 4u;
+                        }
+                    }
+                    nib |= yb;
+                    const int cidx = k * K + p;
+                    if (cidx < 8)
+                        nw0 |= nib << (4 * cidx);
+                    else if (cidx < 16)
+                        nw1 |= nib << (4 * (cidx - 8));
+                    else
+                        nw2 |= nib << (4 * (cidx - 16));
+                    const int32_t xv = xn[k][p];
+                    const int32_t cb = max(max(Mv[p], xv), yv);
+                    const uint32_t st = Mv[p] != cb ? (xv != cb ? kStY : kStX) : kStM;
+                    const int32_t tm = Mv[p] - oer, tx = xv - er;
+                    xn[k][p] = max(tm, tx);
+                    dd[k][p] = cb;
+                    sbn |= (st | (tm >= tx ? 0u : 4u)) << (4 * p);
+                    if (full)
+                        rmax = max(rmax, cb);
+                    else
+                        rmax = max(rmax, (uint32_t)(lc + p - jl_rel) <= (uint32_t)(jh_rel - jl_rel) ? cb : INT32_MIN);
+                }
+                sb[k] = sbn;
+            } else {
+                reset_slot(k);  // outside the window: NEG for the next row
+            }
+            // keep the scheduler from interleaving slots: it would hold every
+            // slot's temporaries at once
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+        // ---- row best (first strict maximum in j order, src/gapped_extend.rs:365-380)
+        const int32_t wmax = __builtin_amdgcn_readlane(wave_scan_max_self(rmax), 63);
+        if (wmax > best_score) {
+            bool found = false;
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                if ((uint32_t)k <= kmax && !found) {
+                    const int32_t scol = base_col + k * SW;
+                    const int32_t jl_rel = (int32_t)jl - scol, jh_rel = (int32_t)jh - scol;
+                    uint64_t hit[K], any = 0;
+#pragma unroll
+                    for (int p = 0; p < K; p++) {
+                        const bool in = (uint32_t)(lc + p - jl_rel) <= (uint32_t)(jh_rel - jl_rel);
+                        hit[p] = __ballot(in && dd[k][p] == wmax);
+                        any |= hit[p];
+                    }
+                    if (any) {  // the first lane holding the maximum, then its first cell
+                        const int l = __ffsll((long long)any) - 1;
+                        int pf = K - 1;
+#pragma unroll
+                        for (int p = K - 1; p >= 0; p--)
+                            if ((hit[p] >> l) & 1ull) pf = p;
+                        best_score = wmax;
+                        bi = i;
+                        bj = (uint32_t)(scol + K * l + pf);
+                        bst = ((uint32_t)__builtin_amdgcn_readlane((int)sb[k], l) >> (4 * pf)) & 3u;
+                        found = true;
+                    }
+                }
+            }
+        }
+        const int32_t thr = sat_sub(best_score, ydrop);
+
+        // ---- alive band; dead cells become NEG for the next row (src/gapped_extend.rs:388-412)
+        uint32_t alo = 0xFFFFFFFFu, ahi = 0;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            if ((uint32_t)k <= kmax) {
+                const int32_t scol = base_col + k * SW;
+                const int32_t jl_rel = (int32_t)jl - scol, jh_rel = (int32_t)jh - scol;
+                uint64_t al[K], any = 0;
+                uint32_t sbk = sb[k];
+#pragma unroll
+                for (int p = 0; p < K; p++) {
+                    const bool in = (uint32_t)(lc + p - jl_rel) <= (uint32_t)(jh_rel - jl_rel);
+                    const bool alive = in && dd[k][p] >= thr;
+                    al[p] = __ballot(alive);
+                    any |= al[p];
+                    dd[k][p] = alive ? dd[k][p] : kNegInf;
+                    xn[k][p] = alive ? xn[k][p] : negx;
+                    const uint32_t f = 0xFu << (4 * p);
+                    sbk = alive ? sbk : ((sbk & ~f) | (sb_negw & f));
+                }
+                sb[k] = sbk;
+                if (any) {
+                    const int lf = __ffsll((long long)any) - 1, ll = 63 - __clzll((long long)any);
+                    int pf = K - 1, pl = 0;
+#pragma unroll
+                    for (int p = K - 1; p >= 0; p--)
+                        if ((al[p] >> lf) & 1ull) pf = p;
+#pragma unroll
+                    for (int p = 0; p < K; p++)
+                        if ((al[p] >> ll) & 1ull) pl = p;
+                    alo = min(alo, (uint32_t)(scol + K * lf + pf));
+                    ahi = (uint32_t)(scol + K * ll + pl);
+                }
+            }
+        }
+        const uint32_t ir = i % kChunkRows;
+        {
+            uint32_t *row = (uint32_t *)(pool.tb + (uint64_t)chunk * kChunkBytes + ir * kRowBytes) + 3 * lr;
+            row[0] = nw0;
+            if (NS * K > 8) row[1] = nw1;
+            if (NS * K > 16) row[2] = nw2;
+        }
+        if (lane == 0) pool.win[(uint64_t)chunk * kChunkRows + ir] = make_uint2(jl, jh);
+        rows_done = i + 1;
+        if (alo == 0xFFFFFFFFu) break;  // whole row dead (src/gapped_extend.rs:407-409)
+        lo = alo;
+        hi = ahi;
+    }
+    if (lane == 0) {
+        out.score = best_score;
+        out.bi = bi;
+        out.bj = bj;
+        out.bstate = bst;
+        out.rows = rows_done;
+        out.truncated = truncated;
+        out.last_chunk = chunk;
+        out.cells = cells;
+        res[side] = out;
     }
 }
 
@@ -644,8 +1099,18 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
             const uint2 win = twin[ci - (uint32_t)tl];
             if (edge_ok && cj >= win.x && cj <= win.y) {
                 valid = true;
-                const uint32_t idx = cj - win.x;  // packed: cell idx at bits [4 idx, 4 idx + 4)
-                const uint32_t nb = (tile[ci - (uint32_t)tl][idx >> 3] >> (4 * (idx & 7u))) & 0xFu;
+                uint32_t word, sh;
+                if (r.fmt == 0) {  // window order: cell idx = j - jl at bits [4 idx, 4 idx + 4)
+                    const uint32_t idx = cj - win.x;
+                    word = idx >> 3;
+                    sh = 4 * (idx & 7u);
+                } else {  // lane-major (k_dp_fwd1<K>, K = 2^(fmt-1)): slot, lane, cell of column cj
+                    const uint32_t ks = r.fmt - 1u, sws = 6u + ks;
+                    const uint32_t c = (((cj >> sws) - (win.x >> sws)) << ks) + (cj & ((1u << ks) - 1u));
+                    word = 3u * ((cj >> ks) & 63u) + (c >> 3);
+                    sh = 4 * (c & 7u);
+                }
+                const uint32_t nb = (tile[ci - (uint32_t)tl][word] >> sh) & 0xFu;
                 next = st == kStM ? (nb & 3u)
                                   : st == kStX ? (((nb >> 2) & 1u) ? kStX : kStM) : (((nb >> 3) & 1u) ? kStY : kStM);
             }
@@ -749,6 +1214,35 @@ __global__ void k_anchor_cols(const Blk *const *__restrict__ Ts, const Blk *cons
 }
 
 inline unsigned grid_for(uint64_t n, unsigned th) { return (unsigned)((n + th - 1) / th); }
+
+// DP forward kernels (GXY_DP_* in gxy.h): 1 = four wavefronts per side with
+// the row in LDS, 2/3/4 = one wavefront per side with the row in registers
+// and 1/2/4 columns per lane per slot.  Every variant gives the same results.
+constexpr int kDpDefault = GXY_DP_WAVE_K2_W4;
+template <class S>
+void launch_dp_forward(int variant, unsigned npp, hipStream_t st, const DpProb *probs, const uint32_t *list, S sc,
+                       int32_t oe, int32_t e, int32_t ydrop, DpPool pool, DpRes *res) {
+    switch (variant) {
+    case GXY_DP_BLOCK4:
+        k_dp_forward<S><<<npp, kDpThreads, 0, st>>>(probs, list, npp, sc, oe, e, ydrop, pool, res);
+        break;
+    case GXY_DP_WAVE_K1:
+        k_dp_fwd1<S, 1, 3><<<npp, 64, 0, st>>>(probs, list, npp, sc, oe, e, ydrop, pool, res);
+        break;
+    case GXY_DP_WAVE_K4:
+        k_dp_fwd1<S, 4, 3><<<npp, 64, 0, st>>>(probs, list, npp, sc, oe, e, ydrop, pool, res);
+        break;
+    case GXY_DP_WAVE_K2_W4:
+        k_dp_fwd1<S, 2, 4><<<npp, 64, 0, st>>>(probs, list, npp, sc, oe, e, ydrop, pool, res);
+        break;
+    case GXY_DP_WAVE_K4_W4:
+        k_dp_fwd1<S, 4, 4><<<npp, 64, 0, st>>>(probs, list, npp, sc, oe, e, ydrop, pool, res);
+        break;
+    default:
+        k_dp_fwd1<S, 2, 3><<<npp, 64, 0, st>>>(probs, list, npp, sc, oe, e, ydrop, pool, res);
+        break;
+    }
+}
 
 // The DP's per-row score table packs the target row and the ambiguity score
 // as signed bytes; other matrices use the int32 row table.
@@ -928,13 +1422,14 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             GXY_HIP(ctx, hipMemcpyAsync(dlist, pending.data() + head, npp * 4, hipMemcpyHostToDevice, ctx->stream));
             GXY_HIP(ctx, hipMemsetAsync(pool.next, 0, 4, ctx->stream));
             {
+                const int variant = ctx->dp_kernel ? ctx->dp_kernel : kDpDefault;
                 KernelTimer kt_(ctx, "dp_forward");
                 if (dp_scores_i8(sc))
-                    k_dp_forward<ScorerI8><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(
-                        dprob, dlist, (uint32_t)npp, make_scorer_i8(sc), oe, e, gp.y_drop, pool, dres);
+                    launch_dp_forward(variant, (unsigned)npp, ctx->stream, dprob, dlist, make_scorer_i8(sc), oe, e,
+                                      gp.y_drop, pool, dres);
                 else
-                    k_dp_forward<ScorerI32><<<(unsigned)npp, kDpThreads, 0, ctx->stream>>>(
-                        dprob, dlist, (uint32_t)npp, make_scorer_i32(sc), oe, e, gp.y_drop, pool, dres);
+                    launch_dp_forward(variant, (unsigned)npp, ctx->stream, dprob, dlist, make_scorer_i32(sc), oe, e,
+                                      gp.y_drop, pool, dres);
             }
             GXY_LAUNCH_CHECK(ctx);
             std::vector<DpRes> pr(npp);

@@ -467,6 +467,13 @@ int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows) {
     });
 }
 
+int gxy_ctx_set_dp_kernel(gxy_ctx *ctx, int kernel) {
+    if (!ctx) return GXY_EINVAL;
+    if (kernel < 0 || kernel > GXY_DP_WAVE_K4_W4) return set_err(ctx, GXY_EINVAL, "unknown DP kernel %d", kernel);
+    ctx->dp_kernel = kernel;
+    return GXY_OK;
+}
+
 int gxy_ctx_dp_pool_stats(gxy_ctx *ctx, uint64_t *passes, uint64_t *truncated, uint64_t *alone) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !passes || !truncated || !alone) return GXY_EINVAL;

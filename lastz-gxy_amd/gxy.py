@@ -33,6 +33,7 @@ _SIGS = {
     "gxy_version": (C.c_char_p, []),
     "gxy_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_set_dp_pool_rows": (C.c_int, [_vp, C.c_uint64]),
+    "gxy_ctx_set_dp_kernel": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
     "gxy_ctx_reset_timing": (C.c_int, [_vp]),
@@ -142,6 +143,11 @@ class Context:
     def set_dp_pool_rows(self, rows: int):
         """Cap the gapped extension's traceback pool at `rows` rows (0: default)."""
         self.check(lib().gxy_ctx_set_dp_pool_rows(self._h, rows), "gxy_ctx_set_dp_pool_rows")
+
+    def set_dp_kernel(self, kernel: int):
+        """Choose the DP forward kernel (GXY_DP_*: 0 auto, 1 four wavefronts per side,
+        2/3/4 one wavefront per side with 1/2/4 columns per lane per slot)."""
+        self.check(lib().gxy_ctx_set_dp_kernel(self._h, kernel), "gxy_ctx_set_dp_kernel")
 
     def dp_pool_stats(self):
         """(passes, truncated sides, passes run with one side alone) since creation."""

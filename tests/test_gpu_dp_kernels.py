@@ -1,0 +1,125 @@
+"""Every DP forward kernel (gxy_ctx_set_dp_kernel) against the oracle: the
+four-wavefront kernel with the row in LDS and the one-wavefront kernels with
+the row in registers (1, 2 and 4 columns per lane per slot).  Alignments
+(coordinates, score, edit script) and DP cell counts must be identical.  The
+cases cover short and long sides, windows that move right across many slots
+and jump left (long insertions and deletions), N bases, three y-drops, a
+substitution matrix outside int8 (the int32 score path) and an asymmetric one."""
+import sys
+
+import numpy as np
+import pytest
+
+import gxy
+import gxy_types as T
+import oracle_py as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = {"block4": 1, "wave_k1": 2, "wave_k2": 3, "wave_k4": 4, "wave_k2_w4": 5, "wave_k4_w4": 6}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pairs(rng, lens, indel_blocks=False, pn=0.01):
+    out = []
+    for k, L in enumerate(lens):
+        code = rng.integers(0, 4, 2 * L + 1)
+        valid = rng.random(2 * L + 1) >= pn
+        t = T.PackedSeq.from_arrays(code, valid, np.zeros(2 * L + 1, bool))
+        c, v, m = t.unpack()
+        qc, qv, qm = synth.mutate(c, v, m, rng, float(rng.uniform(0.02, 0.3)), float(rng.uniform(0.0, 0.03)))
+        if indel_blocks and len(qc) > 600:
+            p = int(rng.integers(len(qc) // 2 + 1, len(qc) - 250))
+            if k % 2 == 0:  # a long insertion in the query: the window jumps right
+                ins = rng.integers(0, 4, int(rng.integers(60, 250)))
+                qc = np.concatenate([qc[:p], ins, qc[p:]])
+                qv = np.concatenate([qv[:p], np.ones(len(ins), bool), qv[p:]])
+                qm = np.concatenate([qm[:p], np.zeros(len(ins), bool), qm[p:]])
+            else:  # a long deletion: the band grows left
+                d = int(rng.integers(60, 240))
+                qc, qv, qm = (np.concatenate([a[:p], a[p + d:]]) for a in (qc, qv, qm))
+        q = T.PackedSeq.from_arrays(qc, qv, qm)
+        out.append((t, q, (L, min(L, len(q) - 1))))
+    return out
+
+
+def run_all(ctx, cases, sc, gp):
+    ts = [gxy.DeviceSeq.upload(t, ctx) for t, _, _ in cases]
+    qs = [gxy.DeviceSeq.upload(q, ctx) for _, q, _ in cases]
+    anc = [a for _, _, a in cases]
+    return gxy.gapped_extend_pairs_dev(ts, qs, anc, sc, gp, ctx)
+
+
+def oracle_all(cases, sc, gp):
+    res, cells = [], 0
+    for t, q, a in cases:
+        r, c = O.gapped_extend(a, t, q, sc, gp)
+        res.append(r)
+        cells += c
+    return res, cells
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return gxy.Context(0)
+
+
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
+def test_dp_kernel_random_pairs(ctx, kernel):
+    ctx.set_dp_kernel(KERNELS[kernel])
+    rng = np.random.default_rng(101)
+    hox = T.scoring_hoxd70()
+    cases = pairs(rng, [int(x) for x in rng.integers(10, 1500, 60)])
+    for ydrop in (9400, 3000, 800):
+        gp = T.GappedParams(ydrop, 1000)
+        say(kernel, "random pairs ydrop", ydrop)
+        got, cells = run_all(ctx, cases, hox, gp)
+        exp, ecells = oracle_all(cases, hox, gp)
+        assert cells == ecells
+        for k in range(len(cases)):
+            assert got[k] == exp[k], (kernel, ydrop, k)
+    ctx.set_dp_kernel(0)
+
+
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
+def test_dp_kernel_long_sides_and_indel_blocks(ctx, kernel):
+    ctx.set_dp_kernel(KERNELS[kernel])
+    rng = np.random.default_rng(102)
+    hox = T.scoring_hoxd70()
+    cases = pairs(rng, [3000, 2500, 4000, 1800, 3500, 600, 5000, 2200], indel_blocks=True)
+    gp = T.GappedParams(9400, 1000)
+    say(kernel, "long sides with indel blocks")
+    got, cells = run_all(ctx, cases, hox, gp)
+    exp, ecells = oracle_all(cases, hox, gp)
+    assert cells == ecells
+    for k in range(len(cases)):
+        assert got[k] == exp[k], (kernel, k)
+    ctx.set_dp_kernel(0)
+
+
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
+def test_dp_kernel_int32_and_asymmetric_matrices(ctx, kernel):
+    ctx.set_dp_kernel(KERNELS[kernel])
+    rng = np.random.default_rng(103)
+    wide = T.scoring_hoxd70()  # scores outside int8: the int32 row table
+    asym = T.scoring_hoxd70()
+    for a in range(4):
+        for b in range(4):
+            wide.sub[a][b] = 3 * wide.sub[a][b]
+            if a != b:
+                asym.sub[a][b] += 7 if a < b else -11
+    wide.gap_open, wide.gap_extend = 1200, 90
+    cases = pairs(rng, [int(x) for x in rng.integers(20, 900, 30)], pn=0.03)
+    for sc, gp in ((wide, T.GappedParams(9400 * 3, 3000)), (asym, T.GappedParams(9400, 1000))):
+        say(kernel, "matrix", sc.sub[0][0], sc.sub[0][1], sc.sub[1][0])
+        got, cells = run_all(ctx, cases, sc, gp)
+        exp, ecells = oracle_all(cases, sc, gp)
+        assert cells == ecells
+        for k in range(len(cases)):
+            assert got[k] == exp[k], (kernel, k)
+    ctx.set_dp_kernel(0)
