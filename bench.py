@@ -1,44 +1,35 @@
-"""Benchmark: end-to-end query Mbp/s of the seed -> HSP hot path on one MI355X.
+"""Benchmark: the seed -> HSP -> gapped hot path on one MI355X, all five workloads.
 
-Workload (default, `--workload chrom`): the chromosome-scale HSP-only pair —
-48 Mbp uniform target with 50 % soft-masked blocks and 2 % N-runs, query =
-target plus 5 % segmental duplications at 15 % substitutions and 2 % indel
-events (~50.4 Mbp); seed 12of19, transitions 1, x-drop 910, hspthresh 3000,
---nogapped, --chunk-size 10000000 --halo 50000.  Both sequences are resident
-in HBM before the timed region; one step is a full driver::run over them
-(5 chunk tables, 10 chunk x strand searches, records merged on the host).
+The default run (`--workload all`) prints one JSON line whose headline
+(`value`, `roofline`, `cpu_baseline`) is the chromosome-scale HSP-only pair,
+with one sub-object per other workload, each carrying its own `roofline` (for
+its dominant kernel, measured with HIP events over its timed region) and
+`cpu_baseline` (the oracle, the C++ restatement of the reference's path, on a
+bounded sample, whose output is also compared with the GPU's on that sample):
 
-`--workload pair30k`: the 30 kbp pair, gapped, MAF (latency-bound).
+* headline (`--workload chrom`): 48 Mbp uniform target with 50 % soft-masked
+  blocks and 2 % N-runs, query = target plus 5 % segmental duplications at
+  15 % substitutions and 2 % indel events (~50.4 Mbp); 12of19, transitions 1,
+  x-drop 910, hspthresh 3000, --nogapped, --chunk-size 10000000 --halo 50000.
+  One step is a full driver::run over the resident pair.
+* `pair30k` (`--workload pair30k`): the 30 kbp pair, gapped, MAF.
+* `batched_dp` (`--workload dp`): anchored pairs with extension lengths
+  LogNormal(median 2 kbp, sigma 1) clipped to 100 bp-50 kbp at 70-95 %
+  identity, HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000, all pairs in one
+  gxy_gapped_extend_slices_dev call (1,000,000 pairs in the default run).
+* `serve` (`--workload serve`): the serving loop (gxy_server) on a resident
+  100 Mbp target with 40 % soft-masked blocks; queries of log-uniform length,
+  80 % mutated excerpts (identity 75-95 %, either strand) and 20 % decoys,
+  full seed -> HSP -> gapped per query, Poisson arrivals; p50 / p99 latency
+  (arrival -> records) at fractions of the served and of the plain capacity.
+* `contigs` (`--workload contigs`): a 200 Mbp unmasked target against contigs
+  of log-uniform length 1 kbp-1 Mbp at 70-90 % identity with ~3 % of bases in
+  indels; 12of19, --chain, --inner (match12 tweener), gapped, PAF; batched (one
+  gxy_run_dev) and served.
 
-`--workload serve`: the serving loop (gxy_server) on a resident uniform target
-with 40 % soft-masked blocks; queries of log-uniform length, 80 % mutated
-excerpts (identity 75-95 %, either strand) and 20 % random decoys, full seed ->
-HSP -> gapped per query, arriving as a Poisson stream.  The plain per-query
-driver's capacity is measured first (all queries submitted at once, one query
-at a time); arrivals then run at fractions of it (--serve-loads) through the
-co-batched server and through the plain one.  `value` is the co-batched
-server's throughput (query Mbp/s) at the highest load; each run reports p50 /
-p99 latency (arrival -> records).
-
-`--workload contigs`: many-contig full sensitivity — a uniform unmasked target
-against contigs of log-uniform length, excerpts at 70-90 % identity with ~3 %
-of bases in indels, 12of19 with --chain and --inner (match12 tweener), gapped,
-PAF out; the batched driver (one gxy_run_dev over all contigs) and the server
-(all contigs submitted at once), each query-Mbp/s; oracle baseline and PAF
-parity on a sample of contigs.
-
-`--workload dp`: batched gapped DP alone — anchored pairs with extension
-lengths LogNormal(median 2 kbp, sigma 1) clipped to 100 bp-50 kbp at 70-95 %
-identity, HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000, every pair in one
-gxy_gapped_extend_pairs_dev call; `value` is DP GCUPS (cells / s).  The default
-chromosome line carries the same measurement for --dp-anchors pairs in its
-`batched_dp` section.
-
-Prints one JSON line (rank 0, N = 1).  `roofline` is for the kernel with the
-largest device time, measured with HIP events on the library's stream over the
-timed steps; `cpu_baseline` is the oracle (the C++ restatement, AVX2 x-drop,
-all host threads) timed on a bounded sample of the same workload, whose output
-is also checked against the GPU's on that sample.
+Size cuts that keep the default run inside the harness's time limit are named
+in each sub-object's `config` (`full_size` gives the configuration's own size).
+A leg that would not finish inside `--budget-s` is skipped and says so.
 """
 import argparse
 import json
@@ -94,47 +85,59 @@ def measured_copy_gbs(nbytes=1 << 30, reps=10):
     return round(2.0 * nbytes * reps / (ms.value / 1e3) / 1e9, 1)
 
 
-def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0):
-    """Batched gapped DP over n_anchors synthetic anchored pairs.  Returns a
-    dict (GCUPS, kernel times, VALU roofline, oracle baseline on a stratified
-    sample whose alignments are compared with the GPU's)."""
-    import ctypes as C
+def valu_roofline(cells, kernel_ms, launches, kernel="k_dp_forward"):
+    """VALU roofline of the DP forward kernel: 11 int32 ops per cell against
+    the int32 VALU peak."""
+    in_kernel = cells / (kernel_ms / 1e3) / 1e9 if kernel_ms else None
+    peak_gcups = VALU_I32_PEAK_TOPS * 1e3 / DP_OPS_PER_CELL
+    return {"bound": "valu", "kernel": kernel,
+            "achieved": round(in_kernel * DP_OPS_PER_CELL / 1e3, 3) if in_kernel else None,
+            "peak": round(VALU_I32_PEAK_TOPS, 2), "unit": "T int32 ops/s",
+            "frac": round(in_kernel / peak_gcups, 5) if in_kernel else None,
+            "gcups_in_kernel": round(in_kernel, 2) if in_kernel else None,
+            "peak_gcups": round(peak_gcups, 1), "ops_per_cell": DP_OPS_PER_CELL,
+            "launches": launches, "avg_launch_ms": round(kernel_ms / max(1, launches), 4) if kernel_ms else None,
+            "traffic": None}
+
+
+def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0, workers=16):
+    """Batched gapped DP over n_anchors synthetic anchored pairs held as slices
+    of two resident device sequences.  Returns a dict (GCUPS, kernel times,
+    VALU roofline, oracle baseline on a stratified sample whose alignments are
+    compared with those the timed call returned)."""
     import numpy as np
     import gxy
     import gxy_types as T
     import synth
 
     t0 = time.time()
-    code, qcode, toff, qoff, tpos, qpos = synth.dp_anchor_arrays(n_anchors, seed, packed=False)
-    log(f"dp: generated {n_anchors} pairs ({len(code)} + {len(qcode)} bp) in {time.time() - t0:.1f}s")
-    ctx = gxy.default_context()
+    Tq, Qq, toff, tlen, qoff, qlen, tpos, qpos = synth.dp_anchor_slices(n_anchors, seed,
+                                                                        workers=min(workers, os.cpu_count() or 1))
+    log(f"dp: generated {n_anchors} pairs ({int(tlen.sum())} + {int(qlen.sum())} bp) in {time.time() - t0:.1f}s")
+    ctx = gxy.Context(0)  # its own context: its traceback pool goes with it
     ctx.set_dp_kernel(kernel)
     t0 = time.time()
-    dcat_t = gxy.DeviceSeq.upload(synth._packed(code), ctx)
-    dcat_q = gxy.DeviceSeq.upload(synth._packed(qcode), ctx)
-    dts = [dcat_t.slice(int(toff[k]), int(toff[k + 1])) for k in range(n_anchors)]
-    dqs = [dcat_q.slice(int(qoff[k]), int(qoff[k + 1])) for k in range(n_anchors)]
-    del dcat_t, dcat_q
+    dT = gxy.DeviceSeq.upload(Tq, ctx)
+    dQ = gxy.DeviceSeq.upload(Qq, ctx)
     ctx.synchronize()
-    log(f"dp: resident as {2 * n_anchors} device sequences in {time.time() - t0:.1f}s")
+    log(f"dp: resident in {time.time() - t0:.1f}s")
     hox = T.scoring_hoxd70()
     gp = T.GappedParams(9400, 3000)
-    L = gxy.lib()
-    tv = (C.c_void_p * n_anchors)(*[t._h for t in dts])
-    qv = (C.c_void_p * n_anchors)(*[q._h for q in dqs])
-    arr = (T.Anchor * n_anchors)()
-    for k in range(n_anchors):
-        arr[k].t_pos, arr[k].q_pos = int(tpos[k]), int(qpos[k])
+    anchors = np.stack([tpos, qpos], axis=1)
+    # stratified sample (equal counts from each decile of extension length)
+    # for the CPU baseline, checked against the timed call's own results
+    order = np.argsort(tpos, kind="stable")
+    per = max(1, cpu_anchors // 10)
+    pick = []
+    if cpu_anchors:
+        for b in range(10):
+            seg = order[b * n_anchors // 10:(b + 1) * n_anchors // 10]
+            if len(seg):
+                pick.extend(seg[np.linspace(0, len(seg) - 1, min(per, len(seg))).astype(int)].tolist())
 
-    def once():
-        sp = C.POINTER(T.AlignmentSet)()
-        ctx.check(L.gxy_gapped_extend_pairs_dev(ctx._h, C.addressof(tv), C.addressof(qv), C.addressof(arr),
-                                                n_anchors, C.byref(hox), C.byref(gp), C.byref(sp)),
-                  "gxy_gapped_extend_pairs_dev")
-        cells = sp.contents.cells
-        found = int(np.ctypeslib.as_array(sp.contents.found, (n_anchors,)).astype(bool).sum())
-        L.gxy_alignment_set_free(sp)
-        return cells, found
+    def once(pk=None):
+        return gxy.gapped_extend_slices_dev(dT, dQ, toff, tlen, qoff, qlen, anchors, hox, gp, ctx, raw=True,
+                                            pick=pk)
 
     for _ in range(warmup):
         once()
@@ -142,29 +145,23 @@ def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0):
     ctx.set_timing(True)
     ctx.synchronize()
     t1 = time.perf_counter()
-    for _ in range(steps):
-        cells, found = once()
+    for st in range(steps):
+        found, cells, picked = once(pick if st == steps - 1 else None)
     ctx.synchronize()
     el = time.perf_counter() - t1
     ctx.set_timing(False)
     kt = {nm: ctx.kernel_time(nm) for nm in ("dp_forward", "dp_traceback")}
     gcups = cells * steps / el / 1e9
     fwd_ms, fwd_n = kt["dp_forward"]
-    in_kernel = cells * steps / (fwd_ms / 1e3) / 1e9 if fwd_ms else None
-    peak_gcups = VALU_I32_PEAK_TOPS * 1e3 / DP_OPS_PER_CELL
     out = {
         "workload": f"{n_anchors} anchored pairs, extension LogNormal(median 2000, sigma 1) clipped 100-50000 bp, "
-                    f"identity U[0.70, 0.95], HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000",
-        "anchors": n_anchors, "cells_per_step": cells, "found": found, "steps": steps,
+                    f"identity U[0.70, 0.95], HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000, one call",
+        "anchors": n_anchors, "cells_per_step": cells, "found": found, "steps": steps, "warmup": warmup,
         "ms_per_step": round(1000 * el / steps, 2), "dp_gcups": round(gcups, 3),
         "kernel_ms_per_step": {k: round(v[0] / steps, 3) for k, v in kt.items() if v[1]},
-        "roofline": {"bound": "valu", "kernel": "k_dp_forward",
-                     "achieved": round(in_kernel * DP_OPS_PER_CELL / 1e3, 3) if in_kernel else None,
-                     "peak": round(VALU_I32_PEAK_TOPS, 2), "unit": "T int32 ops/s",
-                     "frac": round(in_kernel / peak_gcups, 5) if in_kernel else None,
-                     "gcups_in_kernel": round(in_kernel, 2) if in_kernel else None,
-                     "peak_gcups": round(peak_gcups, 1), "ops_per_cell": DP_OPS_PER_CELL,
-                     "launches": fwd_n, "traffic": None},
+        "roofline": valu_roofline(cells * steps, fwd_ms, fwd_n),
+        "dp_autotune": ctx.dp_autotune_state(),
+        "dp_pool_passes": ctx.dp_pool_stats()[0],
     }
     log(f"dp: {cells} cells/step, {found}/{n_anchors} found, {gcups:.2f} GCUPS end to end, kernels {kt}")
 
@@ -172,16 +169,8 @@ def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
         from concurrent.futures import ThreadPoolExecutor
-        # stratified: equal counts from each decile of extension length
-        order = np.argsort(tpos, kind="stable")
-        per = max(1, cpu_anchors // 10)
-        pick = []
-        for b in range(10):
-            seg = order[b * n_anchors // 10:(b + 1) * n_anchors // 10]
-            if len(seg):
-                pick.extend(seg[np.linspace(0, len(seg) - 1, min(per, len(seg))).astype(int)].tolist())
-        hp = [(synth._packed(code[toff[k]:toff[k + 1]]), synth._packed(qcode[qoff[k]:qoff[k + 1]]),
-               int(tpos[k]), int(qpos[k])) for k in pick]
+        hp = [(synth.slice_of(Tq, toff[k], tlen[k]), synth.slice_of(Qq, qoff[k], qlen[k]), int(tpos[k]), int(qpos[k]))
+              for k in pick]
         nthreads = os.cpu_count() or 1
 
         def one(p):
@@ -192,20 +181,43 @@ def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0):
             ores = list(ex.map(one, hp))
         cpu_s = time.perf_counter() - t1
         ocells = sum(c for _, c in ores)
-        gres, gcells = gxy.gapped_extend_pairs_dev([dts[k] for k in pick], [dqs[k] for k in pick],
-                                                   [(p[2], p[3]) for p in hp], hox, gp, ctx)
-        same = gres == [r for r, _ in ores] and gcells == ocells
+        same = all(picked[k] == r for k, (r, _) in zip(pick, ores))
         out["cpu_baseline"] = {
             "value": round(ocells / cpu_s / 1e9, 4), "unit": "GCUPS", "cores": nthreads, "kind": "port",
             "sample": f"oracle (lazy-row DP) on {len(pick)} of the pairs, {per} from each decile of "
                       f"extension length: {ocells} cells in {cpu_s:.2f} s",
-            "gpu_output_identical_on_sample": bool(same)}
+            "gpu_output_identical_on_sample": bool(same),
+            "gpu_output_checked": "the sampled pairs' alignments as returned by the last timed call"}
         log(f"dp: cpu baseline {ocells / cpu_s / 1e9:.3f} GCUPS on {len(pick)} pairs, identical: {same}")
+    del dT, dQ
+    ctx.close()
     return out
 
 
-def serve_leg(args):
-    """The serving workload; returns the bench line."""
+def hbm_xdrop_roofline(hits, columns, passing, kernel_ms, launches):
+    """HBM roofline of k_xdrop: 8 B per hit record + 0.75 B per x-drop column
+    (2 code bits + 1 valid bit per side) + 24 B per passing extension."""
+    alg = 8 * hits + 0.75 * columns + 24 * passing
+    achieved = alg / (kernel_ms / 1e3) / 1e9 if kernel_ms else None
+    return {"bound": "hbm", "kernel": "k_xdrop", "achieved": round(achieved, 2) if achieved else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
+            "launches": launches, "avg_launch_ms": round(kernel_ms / max(1, launches), 4) if kernel_ms else None,
+            "algorithmic_bytes_per_launch": round(alg / max(1, launches)), "traffic": None}
+
+
+def dominant_roofline(ktimes, st):
+    """Roofline of whichever of the DP forward and x-drop kernels took longer."""
+    dp_ms, dp_n = ktimes.get("dp_forward", (0.0, 0))
+    xd_ms, xd_n = ktimes.get("extend", (0.0, 0))
+    if dp_ms >= xd_ms and dp_ms > 0:
+        return valu_roofline(st.dp_cells, dp_ms, dp_n)
+    if xd_ms > 0:
+        return hbm_xdrop_roofline(st.hits, st.columns, st.passing, xd_ms, xd_n)
+    return None
+
+
+def serve_leg(args, budget_left=None):
+    """The serving workload; returns its measurements."""
     import threading
     import numpy as np
     import gxy
@@ -223,75 +235,124 @@ def serve_leg(args):
     dt = gxy.DeviceSeq.upload(target, ctx)
     ctx.synchronize()
     cfg = T.config_default()
+    rng = np.random.default_rng(12)
+    gaps = rng.exponential(1.0, len(queries))
+    keep = {}  # records of sampled queries from the first served run: {index: tuples}
 
-    def run(co_batch, arrivals):
+    def run(co_batch, idx, rate, timing=False, keep_idx=()):
+        """Serve queries idx (arriving as a Poisson stream of `rate` queries/s,
+        or all at once when rate is None); latency = arrival -> records."""
         srv = gxy.Server([dt], cfg, co_batch=co_batch)
-        tickets = [None] * len(queries)
+        if timing:
+            srv.set_timing(True)
+        arrivals = None if rate is None else np.cumsum(gaps[:len(idx)]) / rate
+        tickets = [None] * len(idx)
         start = time.perf_counter()
 
         def submitter():
-            for k, q in enumerate(queries):
+            for x, k in enumerate(idx):
                 if arrivals is not None:
-                    dt_s = arrivals[k] - (time.perf_counter() - start)
+                    dt_s = arrivals[x] - (time.perf_counter() - start)
                     if dt_s > 0:
                         time.sleep(dt_s)
-                tickets[k] = srv.submit(q)
+                tickets[x] = srv.submit(queries[k])
 
         th = threading.Thread(target=submitter)
         th.start()
         th.join()
         lat, sub, done, nrec = [], [], [], 0
-        for k in range(len(queries)):
-            recs, st = srv.wait(tickets[k])
+        for x, k in enumerate(idx):
+            recs, rst = srv.wait(tickets[x])
             nrec += len(recs)
-            lat.append(st.done_ms - st.submit_ms)
-            sub.append(st.submit_ms)
-            done.append(st.done_ms)
+            lat.append(rst.done_ms - rst.submit_ms)
+            sub.append(rst.submit_ms)
+            done.append(rst.done_ms)
+            if k in keep_idx:
+                keep[k] = recs.tuples()
+        extra = {}
+        if timing:
+            st = srv.stats()
+            kt = {nm: srv.kernel_time(nm) for nm in ("extend", "qexpand", "qcount", "dp_forward", "dp_traceback")}
+            extra = {"kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]},
+                     "dp_cells": st.dp_cells, "seed_hits": st.hits, "roofline": dominant_roofline(kt, st)}
         srv.close()
         span_s = (max(done) - min(sub)) / 1e3
         lat = np.array(lat)
-        return {"throughput_mbps": round(lens.sum() / 1e6 / span_s, 4), "span_s": round(span_s, 2),
-                "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
-                "mean_ms": round(float(lat.mean()), 1), "records": nrec}
+        qbp = float(lens[list(idx)].sum())
+        r = {"queries": len(idx), "query_bp": int(qbp), "throughput_mbps": round(qbp / 1e6 / span_s, 4),
+             "span_s": round(span_s, 2), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+             "p99_ms": round(float(np.percentile(lat, 99)), 1), "max_ms": round(float(lat.max()), 1),
+             "mean_ms": round(float(lat.mean()), 1), "records": nrec}
+        r.update(extra)
+        return r
 
-    # capacity of the plain per-query driver: everything queued at once
-    cap = run(False, None)
-    log(f"serve: plain capacity {cap}")
-    rate_cap = cap["throughput_mbps"] * 1e6 / lens.mean()  # queries per second
-    runs = {"plain_closed_loop": cap}
-    scap = run(True, None)
+    allq = list(range(len(queries)))
+    sub_n = min(args.serve_plain_requests, len(queries))
+    subq = allq[:sub_n]
+    # oracle sample: queries from the 20th length percentile up, ~args.serve_cpu_bp bases
+    order = np.argsort(lens)
+    pick, tot = [], 0
+    for k in order[len(order) // 5:]:
+        if tot + lens[k] > args.serve_cpu_bp:
+            break
+        pick.append(int(k))
+        tot += lens[k]
+    runs = {}
+    scap = run(True, allq, None, timing=True, keep_idx=set(pick))
     log(f"serve: served capacity {scap}")
     runs["served_closed_loop"] = scap
-    rng = np.random.default_rng(12)
-    gaps = rng.exponential(1.0, len(queries))
-    best = None
+    rate_s = scap["throughput_mbps"] * 1e6 / lens.mean()  # queries per second
     for f in args.serve_loads:
-        lam = f * rate_cap
-        arrivals = np.cumsum(gaps) / lam
-        for co in (True, False):
-            r = run(co, arrivals)
-            r["load"] = f
-            r["qps"] = round(lam, 4)
-            key = f"{'served' if co else 'plain'}_load{f:g}"
-            runs[key] = r
-            log(f"serve: {key} {r}")
-            if co:
-                best = r
+        r = run(True, allq, f * rate_s)
+        r["load_of_served_capacity"] = f
+        r["qps"] = round(f * rate_s, 4)
+        runs[f"served_load{f:g}_of_served"] = r
+        log(f"serve: served at {f} of its capacity {r}")
+    pcap = run(False, subq, None)
+    log(f"serve: plain capacity (first {sub_n} queries) {pcap}")
+    runs["plain_closed_loop"] = pcap
+    rate_p = pcap["throughput_mbps"] * 1e6 / lens[subq].mean()
+    for co in (False, True):
+        r = run(co, subq, args.serve_plain_load * rate_p)
+        r["load_of_plain_capacity"] = args.serve_plain_load
+        r["qps"] = round(args.serve_plain_load * rate_p, 4)
+        runs[f"{'served' if co else 'plain'}_load{args.serve_plain_load:g}_of_plain"] = r
+        log(f"serve: {'served' if co else 'plain'} at {args.serve_plain_load} of plain capacity {r}")
+    cpu = None
+    if not args.no_cpu and pick:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        nthreads = os.cpu_count() or 1
+        names = [(f"q{k}", queries[k]) for k in pick]
+        t1 = time.perf_counter()
+        orec, _ = O.run([("t", target)], names, cfg, threads=nthreads)
+        cpu_s = time.perf_counter() - t1
+        per = {k: [] for k in pick}
+        for x in orec.tuples():
+            per[pick[x[1]]].append((x[0], 0) + tuple(x[2:]))
+        same = all(keep.get(k) == per[k] for k in pick)
+        cpu = {"value": round(tot / 1e6 / cpu_s, 4), "unit": "query-Mbp/s", "cores": nthreads, "kind": "port",
+               "sample": f"oracle (lazy-row DP) on {len(pick)} queries ({int(tot)} bp, the 20th length percentile "
+                         f"up) against the whole target, one run: {cpu_s:.2f} s (throughput only, no latency claim)",
+               "gpu_output_identical_on_sample": bool(same),
+               "gpu_output_checked": "each sampled query's records from the served capacity run"}
+        log(f"serve: cpu {cpu}")
+    best = runs.get(f"served_load{max(args.serve_loads):g}_of_served", scap)
     wl = {"workload": f"serving: {len(target)} bp uniform target, 40% soft-masked; {len(queries)} queries, "
-                      f"length log-uniform 1 kbp-{args.serve_max_len} bp, 80% excerpts (identity 75-95%, "
-                      f"both strands) / 20% decoys, full seed->HSP->gapped, Poisson arrivals",
-          "target_bp": len(target), "queries": len(queries), "query_bp": int(lens.sum())}
-    return {"metric": METRIC, "value": best["throughput_mbps"] if best else None, "unit": "query-Mbp/s",
-            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(best["span_s"] * 1e3, 1) if best else None,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
-            "data": "synthetic", "config": wl, "p50_ms": best["p50_ms"] if best else None,
-            "p99_ms": best["p99_ms"] if best else None,
-            "capacity_mbps": {"served": scap["throughput_mbps"], "plain": cap["throughput_mbps"]},
-            "runs": runs, "roofline": None, "cpu_baseline": None}
+                      f"length log-uniform 1 kbp-{args.serve_max_len} bp, 80% excerpts (identity 75-95%, both "
+                      f"strands) / 20% decoys, full seed->HSP->gapped, Poisson arrivals; loads as fractions of the "
+                      f"served capacity (all queries) and of the plain per-query driver's capacity (first "
+                      f"{sub_n} queries)",
+          "target_bp": len(target), "queries": len(queries), "query_bp": int(lens.sum()),
+          "full_size": "the configuration's >= 1000 requests of log-uniform 1 kbp-5 Mbp (--serve-max-len 5000000)"}
+    return {"config": wl, "value": best["throughput_mbps"], "unit": "query-Mbp/s",
+            "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
+            "capacity_mbps": {"served": scap["throughput_mbps"], "plain": pcap["throughput_mbps"]},
+            "runs": runs, "roofline": scap.get("roofline"), "cpu_baseline": cpu}
 
 
 def contigs_leg(args):
-    """The many-contig workload; returns the bench line."""
+    """The many-contig workload; returns its measurements."""
     import numpy as np
     import gxy
     import gxy_types as T
@@ -312,9 +373,6 @@ def contigs_leg(args):
     names_t = [("target", target)]
     names_q = [(f"contig{k}", q) for k, q in enumerate(contigs)]
 
-    # The batched run gets a context of its own, closed before the server starts:
-    # a two-strand run leaves a helper context (its own stream) on its context,
-    # and a server started beside it answers this workload ~20 % slower.
     bctx = gxy.Context(0)
     bctx.reset_timing()
     bctx.set_timing(True)
@@ -328,9 +386,15 @@ def contigs_leg(args):
     paf_s = time.perf_counter() - t1
     log(f"contigs: batched {batched_s:.2f}s, {len(recs)} records, PAF {len(paf)} bytes in {paf_s:.2f}s")
     kt = {nm: bctx.kernel_time(nm) for nm in ("extend", "qexpand", "qcount", "dp_forward", "dp_traceback")}
+    roof = dominant_roofline(kt, st)
+    tune = bctx.dp_autotune_state()
     bctx.close()
+    rec_by_q = {}
+    for x in recs.tuples():
+        rec_by_q.setdefault(x[1], []).append(x)
 
     srv = gxy.Server([dt], cfg)
+    srv.set_timing(True)
     t1 = time.perf_counter()
     tickets = [srv.submit(q) for q in contigs]
     lat, nrec = [], 0
@@ -342,6 +406,7 @@ def contigs_leg(args):
         if (k + 1) % step == 0:
             log(f"contigs: served {k + 1}/{len(tickets)} answered, {time.perf_counter() - t1:.1f}s")
     served_s = time.perf_counter() - t1
+    skt = {nm: srv.kernel_time(nm) for nm in ("extend", "dp_forward")}
     srv.close()
     lat = np.array(lat)
     log(f"contigs: served {served_s:.2f}s, {nrec} records")
@@ -350,7 +415,7 @@ def contigs_leg(args):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
-        # sample: contigs at the 20th-80th length percentiles, up to ~2 Mbp in all
+        # sample: contigs from the 20th length percentile up, up to ~contig_cpu_bp bases in all
         order = np.argsort([len(q) for q in contigs])
         pick, tot = [], 0
         for k in order[len(order) // 5:]:
@@ -368,8 +433,10 @@ def contigs_leg(args):
         t1 = time.perf_counter()
         orec, _ = O.run(names_t, sample, cfg, threads=nthreads)
         cpu_s = time.perf_counter() - t1
-        grec, _ = gxy.run_dev([dt], [dqs[k] for k in pick], cfg, ctx)
-        same = gxy.format_paf(names_t, sample, grec) == O.format_paf(names_t, sample, orec)
+        oper = {}
+        for x in orec.tuples():
+            oper.setdefault(pick[x[1]], []).append((x[0], pick[x[1]]) + tuple(x[2:]))
+        same = all(rec_by_q.get(k, []) == oper.get(k, []) for k in pick)
         per_bp = max(cpu_s - build_s, 1e-9) / tot
         full_s = build_s + per_bp * qbp
         cpu = {"value": round(qbp / 1e6 / full_s, 4), "unit": "query-Mbp/s", "cores": nthreads, "kind": "port",
@@ -377,22 +444,25 @@ def contigs_leg(args):
                          f"percentile of length upward) against the same target: {cpu_s:.2f} s, of which "
                          f"{build_s:.2f} s the table; value = all contigs extrapolated ({full_s:.1f} s)",
                "sample_query_mbps": round(tot / 1e6 / cpu_s, 4),
-               "gpu_output_identical_on_sample": bool(same)}
+               "gpu_output_identical_on_sample": bool(same),
+               "gpu_output_checked": "the sampled contigs' records from the batched run (coordinates, scores, "
+                                     "edit scripts; the PAF writer is pinned separately)"}
         log(f"contigs: cpu {cpu}")
     wl = {"workload": f"many contigs: {len(target)} bp uniform target vs {len(contigs)} contigs, length log-uniform "
                       f"1 kbp-{args.contig_max_len} bp, identity 70-90%, ~3% of bases in indels; 12of19, --chain, "
                       f"--inner (match12), gapped, PAF",
-          "target_bp": len(target), "contigs": len(contigs), "query_bp": qbp}
-    return {"metric": METRIC, "value": round(qbp / 1e6 / batched_s, 4), "unit": "query-Mbp/s", "n_gpus": 1,
-            "steps": 1, "warmup": 0, "ms_per_step": round(batched_s * 1e3, 1), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic", "config": wl,
+          "target_bp": len(target), "contigs": len(contigs), "query_bp": qbp,
+          "full_size": "the configuration's 10,000 contigs (--contigs 10000)"}
+    return {"config": wl, "value": round(qbp / 1e6 / batched_s, 4), "unit": "query-Mbp/s",
+            "ms_per_step": round(batched_s * 1e3, 1),
             "records": len(recs), "paf_bytes": len(paf), "paf_format_s": round(paf_s, 3),
             "dp_cells": st.dp_cells, "dp_gcups": round(st.dp_cells / 1e9 / batched_s, 3),
-            "kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]},
+            "kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]}, "dp_autotune": tune,
             "served": {"query_mbps": round(qbp / 1e6 / served_s, 4), "wall_s": round(served_s, 2),
                        "records": nrec, "p50_ms": round(float(np.percentile(lat, 50)), 1),
-                       "p99_ms": round(float(np.percentile(lat, 99)), 1)},
-            "roofline": None, "cpu_baseline": cpu}
+                       "p99_ms": round(float(np.percentile(lat, 99)), 1),
+                       "kernel_ms": {k: round(v[0], 1) for k, v in skt.items() if v[1]}},
+            "roofline": roof, "cpu_baseline": cpu}
 
 
 def plain_dropin_leg(target, query, cfg, steps, recs):
@@ -478,63 +548,16 @@ def plain_dropin_leg(target, query, cfg, steps, recs):
                     "coordinate shift and sort on the host"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="chrom", choices=["chrom", "pair30k", "dp", "serve", "contigs"])
-    ap.add_argument("--contig-target-bp", type=int, default=200_000_000)
-    ap.add_argument("--contigs", type=int, default=200)
-    ap.add_argument("--contig-max-len", type=int, default=1_000_000)
-    ap.add_argument("--contig-cpu-bp", type=int, default=1_500_000)
-    ap.add_argument("--serve-target-bp", type=int, default=100_000_000)
-    ap.add_argument("--serve-requests", type=int, default=100)
-    ap.add_argument("--serve-max-len", type=int, default=1_000_000)
-    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.8])
-    ap.add_argument("--dp-anchors", type=int, default=50_000)
-    ap.add_argument("--dp-cpu-anchors", type=int, default=400)
-    ap.add_argument("--dp-kernel", type=int, default=0,
-                    help="DP forward kernel (0 library's choice, 1 four wavefronts per side, 2/3/4 one "
-                         "wavefront per side with 1/2/4 columns per lane per slot)")
-    ap.add_argument("--no-dp", action="store_true", help="omit the batched_dp section of the chrom line")
-    ap.add_argument("--target-bp", type=int, default=48_000_000)
-    ap.add_argument("--cpu-sample-bp", type=int, default=16_000_000)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="omit the plain drop-in leg of the chrom line")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(this library drives one device per process)"}))
-        sys.exit(2)
-
+def pair_leg(args, workload, steps, warmup):
+    """The chromosome-scale HSP-only pair (workload "chrom") or the 30 kbp pair
+    ("pair30k"): `steps` driver runs over the resident pair."""
     import numpy as np
     import gxy
     import gxy_types as T
     import synth
 
-    if args.workload == "serve":
-        print(json.dumps(serve_leg(args)), flush=True)
-        return
-    if args.workload == "contigs":
-        print(json.dumps(contigs_leg(args)), flush=True)
-        return
-
-    if args.workload == "dp":
-        d = dp_leg(args.dp_anchors, args.steps, args.warmup, 0 if args.no_cpu else args.dp_cpu_anchors,
-                   kernel=args.dp_kernel)
-        out = {"metric": METRIC, "value": d["dp_gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-               "warmup": args.warmup, "ms_per_step": d["ms_per_step"], "higher_is_better": True,
-               "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-               "config": {"workload": d["workload"], "anchors": d["anchors"]},
-               "roofline": d["roofline"], "cpu_baseline": d.get("cpu_baseline"),
-               "cells_per_step": d["cells_per_step"], "found": d["found"],
-               "kernel_ms_per_step": d["kernel_ms_per_step"]}
-        print(json.dumps(out), flush=True)
-        return
-
     t0 = time.time()
-    if args.workload == "chrom":
+    if workload == "chrom":
         target, query = synth.chromosome_pair(n=args.target_bp, seed=7)
         cfg = T.config_default()
         cfg.gapped_enabled = 0
@@ -549,14 +572,14 @@ def main():
         cfg = T.config_default()
         wl = {"workload": "30 kbp pair, 20% substitutions, 1% indel events, gapped, MAF",
               "target_bp": len(target), "query_bp": len(query)}
-    log(f"generated {len(target)} + {len(query)} bp in {time.time() - t0:.1f}s")
+    log(f"{workload}: generated {len(target)} + {len(query)} bp in {time.time() - t0:.1f}s")
 
     ctx = gxy.default_context()
     dt = gxy.DeviceSeq.upload(target, ctx)
     dq = gxy.DeviceSeq.upload(query, ctx)
     ctx.synchronize()
 
-    for w in range(args.warmup):
+    for w in range(warmup):
         t1 = time.time()
         recs, st = gxy.run_dev([dt], [dq], cfg, ctx)
         log(f"warmup {w}: {time.time() - t1:.3f}s, {len(recs)} records")
@@ -567,29 +590,29 @@ def main():
     ctx.set_timing(True)
     ctx.synchronize()
     tstart = time.perf_counter()
-    for s in range(args.steps):
+    for s in range(steps):
         recs, st = gxy.run_dev([dt], [dq], cfg, ctx)
         log(f"step {s}: {len(recs)} records")
     ctx.synchronize()
     elapsed = time.perf_counter() - tstart
     ctx.set_timing(False)
     ktime = {n: ctx.kernel_time(n) for n in names}
-    ms_per_step = 1000.0 * elapsed / args.steps
+    ms_per_step = 1000.0 * elapsed / steps
     qbp = len(query)
-    value = qbp / 1e6 / (elapsed / args.steps)
-    hsp_gcups = st.columns / 1e9 / (elapsed / args.steps)
-    dp_gcups = st.dp_cells / 1e9 / (elapsed / args.steps)
+    value = qbp / 1e6 / (elapsed / steps)
+    hsp_gcups = st.columns / 1e9 / (elapsed / steps)
+    dp_gcups = st.dp_cells / 1e9 / (elapsed / steps)
 
     dom_name, (dom_ms, dom_launches) = max(ktime.items(), key=lambda kv: kv[1][0])
     roof = None
     if dom_name == "extend":
-        alg = (8 * st.hits + 0.75 * st.columns + 24 * st.passing) * args.steps
+        alg = (8 * st.hits + 0.75 * st.columns + 24 * st.passing) * steps
         achieved = alg / (dom_ms / 1e3) / 1e9
         # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
         # runs of this workload (scaled by a 1 GiB copy), when present
         traffic = None
         prof = os.path.join(ROOT, "profiles", "k_xdrop_chrom48M.json")
-        if args.workload == "chrom" and os.path.exists(prof):
+        if workload == "chrom" and os.path.exists(prof):
             traffic = json.load(open(prof))["traffic_bytes_per_launch"]
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
@@ -597,24 +620,18 @@ def main():
                 "launches": dom_launches,
                 "algorithmic_bytes_per_launch": round(alg / max(1, dom_launches))}
     elif dom_name == "dp_forward":
-        # 0.5 B/cell of traceback written is the HBM floor; DP is VALU-bound
-        alg = 0.5 * st.dp_cells * args.steps
-        achieved = alg / (dom_ms / 1e3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_dp_forward", "avg_launch_ms": round(dom_ms / max(1, dom_launches), 4),
-                "launches": dom_launches,
-                "dp_gcups_in_kernel": round(st.dp_cells * args.steps / (dom_ms / 1e3) / 1e9, 2)}
+        roof = valu_roofline(st.dp_cells * steps, dom_ms, dom_launches)
     else:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None, "kernel": dom_name,
                 "avg_launch_ms": round(dom_ms / max(1, dom_launches), 4), "launches": dom_launches}
 
-    # `peak` stays the published 8 TB/s; a measured streaming-copy rate rides beside it
-    copy_gbs = measured_copy_gbs()
-    roof["copy_peak_measured_gbs"] = copy_gbs
-    if copy_gbs and roof.get("achieved"):
-        roof["frac_of_measured_copy"] = round(roof["achieved"] / copy_gbs, 5)
+    if roof["bound"] == "hbm":
+        # `peak` stays the published 8 TB/s; a measured streaming-copy rate rides beside it
+        copy_gbs = measured_copy_gbs()
+        roof["copy_peak_measured_gbs"] = copy_gbs
+        if copy_gbs and roof.get("achieved"):
+            roof["frac_of_measured_copy"] = round(roof["achieved"] / copy_gbs, 5)
 
     # PCIe-inclusive rate (host buffers handed over): reported, never `value`
     t1 = time.perf_counter()
@@ -622,7 +639,7 @@ def main():
     pcie_s = time.perf_counter() - t1
 
     plain = None
-    if args.workload == "chrom" and not args.no_plain:
+    if workload == "chrom" and not args.no_plain:
         plain = plain_dropin_leg(target, query, cfg, 2, recs)
         plain["batched_over_plain"] = round(value / plain["query_mbps"], 2)
         plain["pcie_inclusive_over_plain"] = round(qbp / 1e6 / pcie_s / plain["query_mbps"], 2)
@@ -634,25 +651,30 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
         nthreads = os.cpu_count() or 1
-        if args.workload == "chrom":
+        if workload == "chrom":
             sample = query.slice(0, min(args.cpu_sample_bp, len(query)))
         else:
             sample = query
-        # fixed cost: the five chunk tables (a 1 kbp query barely adds search work)
+        # fixed cost: the chunk tables (a 1 kbp query barely adds search work);
+        # the 30 kbp pair runs the reference's as-written DP storage
+        lazy = workload != "pair30k"
         t1 = time.perf_counter()
-        O.run([("t", target)], [("q", query.slice(0, 1000))], cfg, threads=nthreads)
+        O.run([("t", target)], [("q", query.slice(0, 1000))], cfg, threads=nthreads, lazy=lazy)
         build_s = time.perf_counter() - t1
         t1 = time.perf_counter()
-        orec, ost = O.run([("t", target)], [("q", sample)], cfg, threads=nthreads)
+        orec, ost = O.run([("t", target)], [("q", sample)], cfg, threads=nthreads, lazy=lazy)
         cpu_s = time.perf_counter() - t1
-        grec, _ = gxy.run_dev([dt], [gxy.DeviceSeq.upload(sample, ctx)], cfg, ctx)
+        # the whole query: the last timed run's records; a prefix: a run of its own
+        grec = recs if len(sample) == len(query) else gxy.run_dev([dt], [gxy.DeviceSeq.upload(sample, ctx)],
+                                                                   cfg, ctx)[0]
         same = (gxy.format_maf([("t", target)], [("q", sample)], grec)
                 == O.format_maf([("t", target)], [("q", sample)], orec))
         per_bp = max(cpu_s - build_s, 1e-9) / len(sample)
         full_s = build_s + per_bp * len(query)
         cpu = {"value": round(len(query) / 1e6 / full_s, 4), "unit": "query-Mbp/s",
                "cores": nthreads, "kind": "port",
-               "sample": f"oracle on the same target ({len(target)} bp, all chunks, both strands) "
+               "sample": f"oracle ({'lazy-row' if lazy else 'as-written'} DP storage) on the same target "
+                         f"({len(target)} bp, all chunks, both strands) "
                          f"with the first {len(sample)} bp of the query: {cpu_s:.2f} s wall, of which "
                          f"{build_s:.2f} s table builds; value = whole query extrapolated "
                          f"({full_s:.1f} s)",
@@ -660,23 +682,117 @@ def main():
                "gpu_output_identical_on_sample": bool(same)}
         log(f"cpu baseline {cpu_s:.2f}s on {len(sample)} bp, GPU output identical: {same}")
 
-    batched_dp = None
-    if args.workload == "chrom" and not args.no_dp:
-        del dt, dq
-        batched_dp = dp_leg(args.dp_anchors, 2, 1, 0 if args.no_cpu else args.dp_cpu_anchors)
-
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "query-Mbp/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic", "config": wl,
+        "value": round(value, 3), "unit": "query-Mbp/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(ms_per_step, 2), "config": wl,
         "hsp_gcups": round(hsp_gcups, 3), "dp_gcups": round(dp_gcups, 3),
         "records": len(recs), "seed_hits_per_step": st.hits, "xdrop_columns_per_step": st.columns,
         "pcie_inclusive_query_mbps": round(qbp / 1e6 / pcie_s, 3),
-        "kernel_ms_per_step": {n: round(v[0] / args.steps, 3) for n, v in ktime.items() if v[1]},
-        "timed_kernels_ms_per_step": round(sum(v[0] for v in ktime.values()) / args.steps, 3),
-        "roofline": roof, "cpu_baseline": cpu, "plain_dropin": plain, "batched_dp": batched_dp,
+        "kernel_ms_per_step": {n: round(v[0] / steps, 3) for n, v in ktime.items() if v[1]},
+        "timed_kernels_ms_per_step": round(sum(v[0] for v in ktime.values()) / steps, 3),
+        "roofline": roof, "cpu_baseline": cpu,
     }
+    if plain is not None:
+        out["plain_dropin"] = plain
+    if workload == "pair30k":
+        out["dp_autotune"] = ctx.dp_autotune_state()
+        out["dp_cells_per_step"] = st.dp_cells
+        out["dp_pool_passes_truncated_alone"] = ctx.dp_pool_stats()
+    return out
+
+
+LEG_ESTIMATE_S = {"pair30k": 20, "dp": 150, "serve": 240, "contigs": 120}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="all", choices=["all", "chrom", "pair30k", "dp", "serve", "contigs"])
+    ap.add_argument("--budget-s", type=float, default=540.0,
+                    help="wall-clock budget of an `all` run: a leg whose estimate would overrun it is skipped")
+    ap.add_argument("--contig-target-bp", type=int, default=200_000_000)
+    ap.add_argument("--contigs", type=int, default=2000)
+    ap.add_argument("--contig-max-len", type=int, default=1_000_000)
+    ap.add_argument("--contig-cpu-bp", type=int, default=1_500_000)
+    ap.add_argument("--serve-target-bp", type=int, default=100_000_000)
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-max-len", type=int, default=1_000_000)
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.5, 0.8])
+    ap.add_argument("--serve-plain-requests", type=int, default=100)
+    ap.add_argument("--serve-plain-load", type=float, default=0.8)
+    ap.add_argument("--serve-cpu-bp", type=int, default=1_000_000)
+    ap.add_argument("--dp-anchors", type=int, default=1_000_000)
+    ap.add_argument("--dp-cpu-anchors", type=int, default=400)
+    ap.add_argument("--dp-kernel", type=int, default=0,
+                    help="DP forward kernel (gxy_ctx_set_dp_kernel: 0 autotuned, 1 four wavefronts per side, "
+                         "2/3/4 one wavefront per side with 1/2/4 columns per lane per slot, 5/6 = 3/4 at four "
+                         "wavefronts per SIMD)")
+    ap.add_argument("--pair-steps", type=int, default=5)
+    ap.add_argument("--target-bp", type=int, default=48_000_000)
+    ap.add_argument("--cpu-sample-bp", type=int, default=16_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="omit the plain drop-in leg of the chrom line")
+    ap.add_argument("--legs", nargs="+", default=["pair30k", "dp", "serve", "contigs"],
+                    help="sub-workloads of an `all` run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(this library drives one device per process)"}))
+        sys.exit(2)
+    t_start = time.time()
+
+    import gxy
+    gxy.default_context().set_dp_kernel(args.dp_kernel)
+    head = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32", "data": "synthetic"}
+
+    def run_leg(name, steps, warmup):
+        if name == "pair30k":
+            return pair_leg(args, "pair30k", steps, warmup)
+        if name == "dp":
+            return dp_leg(args.dp_anchors, steps, warmup, 0 if args.no_cpu else args.dp_cpu_anchors,
+                          kernel=args.dp_kernel)
+        if name == "serve":
+            return serve_leg(args)
+        return contigs_leg(args)
+
+    if args.workload != "all":
+        if args.workload == "chrom":
+            d = pair_leg(args, "chrom", args.steps, args.warmup)
+        else:
+            d = run_leg(args.workload, args.steps, args.warmup)
+        out = dict(head)
+        out.update({"steps": args.steps, "warmup": args.warmup})
+        out.update(d)
+        if args.workload == "dp":
+            out["value"], out["unit"] = d["dp_gcups"], "GCUPS"
+            out["config"] = {"workload": d["workload"], "anchors": d["anchors"]}
+        print(json.dumps(out), flush=True)
+        return
+
+    d = pair_leg(args, "chrom", args.steps, args.warmup)
+    out = dict(head)
+    out.update(d)
+    for name in args.legs:
+        left = args.budget_s - (time.time() - t_start)
+        if left < LEG_ESTIMATE_S.get(name, 60):
+            out[name] = {"skipped": f"time budget: {left:.0f} s left of {args.budget_s:.0f} s, the leg needs "
+                                    f"about {LEG_ESTIMATE_S.get(name, 60)} s"}
+            log(f"{name}: skipped ({left:.0f} s left)")
+            continue
+        t1 = time.time()
+        key = {"dp": "batched_dp"}.get(name, name)
+        if name == "pair30k":
+            out[key] = run_leg(name, args.pair_steps, 2)
+        elif name == "dp":
+            out[key] = run_leg(name, 1, 1)
+        else:
+            out[key] = run_leg(name, 1, 0)
+        out[key]["wall_s"] = round(time.time() - t1, 1)
+        log(f"{name}: done in {time.time() - t1:.1f}s")
+    out["bench_wall_s"] = round(time.time() - t_start, 1)
     print(json.dumps(out), flush=True)
 
 

@@ -197,6 +197,16 @@ int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows);
 #define GXY_DP_WAVE_K2_W4 5 /* K2 / K4 compiled for four wavefronts per SIMD */
 #define GXY_DP_WAVE_K4_W4 6
 int gxy_ctx_set_dp_kernel(gxy_ctx *ctx, int kernel);
+/* The autotuner behind GXY_DP_AUTO (no counterpart in the reference).  Passes
+ * of the gapped extension fall in four classes by the number of sides they
+ * sweep together (<= 8, <= 512, <= 8192, more); for each class the context
+ * measures the candidate kernels (GXY_DP_WAVE_K2_W4, GXY_DP_WAVE_K4_W4,
+ * GXY_DP_WAVE_K2, GXY_DP_WAVE_K4, GXY_DP_BLOCK4) on its first passes of that
+ * class (big passes lend a slice of their sides to each probe) and keeps the
+ * fastest.  chosen[4] receives each class's kernel (0 while probing) and
+ * gcups[4 * 5] the forward kernel's measured rate of each candidate, in the
+ * order above (0 where not yet measured).  Results never depend on it. */
+int gxy_ctx_dp_autotune_state(gxy_ctx *ctx, int *chosen, double *gcups);
 /* Gapped-extension passes on the context since it was created: passes run,
  * sides stopped by a full pool (swept again later) and passes that ran one
  * side alone. */
@@ -317,6 +327,16 @@ int gxy_gapped_extend_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *targets,
                                 const gxy_dseq *const *queries, const gxy_anchor *anchors,
                                 uint64_t n, const gxy_scoring *scoring,
                                 const gxy_gapped_params *params, gxy_alignment_set **out);
+/* The same for n pairs that are slices of two device sequences: pair k is
+ * target[t_off[k], t_off[k] + t_len[k]) against query[q_off[k], q_off[k] +
+ * q_len[k]), anchors[k] is relative to those slices, and so are the results.
+ * (Many short pairs need no device sequence each.)  Each slice is shorter
+ * than 2^32 bases; the sequences may be longer. */
+int gxy_gapped_extend_slices_dev(gxy_ctx *ctx, const gxy_dseq *target, const gxy_dseq *query,
+                                 const uint64_t *t_off, const uint64_t *t_len, const uint64_t *q_off,
+                                 const uint64_t *q_len, const gxy_anchor *anchors, uint64_t n,
+                                 const gxy_scoring *scoring, const gxy_gapped_params *params,
+                                 gxy_alignment_set **out);
 void gxy_alignment_set_free(gxy_alignment_set *set);
 
 /* ---- driver (src/driver.rs:50-111, 199-392) ----------------------------- */
@@ -499,6 +519,14 @@ int gxy_server_wait(gxy_server *server, uint64_t ticket, gxy_record_set **out,
 /* Message of the last failure inside the server (or of the failed request).
  * The text is a copy owned by the calling thread, valid until its next call. */
 const char *gxy_server_last_error(gxy_server *server);
+/* Kernel timing over the server's contexts (no counterpart in the reference;
+ * for measurement only), as gxy_ctx_set_timing / gxy_ctx_kernel_time.  Both
+ * return GXY_EINVAL while requests are in flight (submitted, not yet waited). */
+int gxy_server_set_timing(gxy_server *server, int enabled);
+/* Work done so far over all requests (seed hits, x-drop columns, passing
+ * extensions, HSPs, gapped extensions and DP cells; for measurement). */
+int gxy_server_stats(gxy_server *server, gxy_run_stats *out);
+int gxy_server_kernel_time(gxy_server *server, const char *name, double *ms, uint64_t *launches);
 /* Stops the worker after the queued queries are served; frees everything. */
 int gxy_server_destroy(gxy_server *server);
 

@@ -375,6 +375,19 @@ struct gxy_ctx {
     uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(96 GiB, device / 3))
     uint64_t dp_pool_rows = 0; // traceback pool rows set by the caller (0: no row cap)
     int dp_kernel = 0;         // DP forward kernel (GXY_DP_*), 0: chosen by the library
+    // DP autotuner (gapped.hip): per class of pass size, the forward kernel
+    // chosen, or the candidates' measured rates while it is still probing.
+    static constexpr int kDpClasses = 4, kDpCands = 5;
+    // Passes of at most 8 sides are not probed: a lone side is bound by its
+    // row latency, which the four-wavefront kernel halves (the one-wavefront
+    // kernels measured 0.24-0.27 against 0.49 GCUPS on a 30 kbp pair, and a
+    // probe pass there costs a whole sweep).
+    struct DpTune {
+        int chosen = 0;                 // GXY_DP_* once decided, else 0
+        int tried = 0;                  // candidates measured so far
+        double gcups[kDpCands] = {};    // forward-kernel cells / ns of each candidate's probe
+    } dp_tune[kDpClasses] = {{GXY_DP_BLOCK4, 0, {}}, {}, {}, {}};
+    hipEvent_t dp_ev[2] = {nullptr, nullptr};
     // gapped passes since creation: passes, sides stopped by a full pool, passes run with one side alone
     uint64_t dp_passes = 0, dp_truncated = 0, dp_alone = 0;
     std::multimap<size_t, void *> pool_free;

@@ -816,6 +816,12 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         if (strands.size() == 2) {
             if (!ctx->helper && gxy_ctx_create(ctx->device, &ctx->helper) == GXY_OK)
                 ctx->helper->pool_on = true;  // its own block cache: nothing it frees waits on other streams
+            if (ctx->helper && !ctx->helper->stream &&
+                hipStreamCreateWithFlags(&ctx->helper->stream, hipStreamNonBlocking) != hipSuccess) {
+                (void)hipGetLastError();
+                gxy_ctx_destroy(ctx->helper);
+                ctx->helper = nullptr;
+            }
             helper = ctx->helper;
             if (helper) helper->timing = ctx->timing;
         }
@@ -947,6 +953,16 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             }
             helper->totals.clear();
         }
+        // The helper's stream ends with the run (its scratch and block cache
+        // stay for the next run): a stream kept would take a share of the
+        // device's four hardware queues from streams that other contexts create
+        // later (a server started after a two-strand run answered the
+        // many-contig workload ~25 % slower).
+        if (ctx->helper && ctx->helper->stream) {
+            (void)hipStreamSynchronize(ctx->helper->stream);
+            (void)hipStreamDestroy(ctx->helper->stream);
+            ctx->helper->stream = nullptr;
+        }
         // A traceback pool grown past 48 GiB by this run goes back to the device
         // (the next large run allocates it again), so it does not hold memory that
         // other contexts on the device need.
@@ -1061,6 +1077,7 @@ struct gxy_server {
     uint64_t next_id = 1, iteration = 0;
     bool stop = false, search_done = false;
     std::string err;
+    gxy_run_stats totals{};  // work of every iteration so far (under mu)
     std::thread worker;
     std::vector<std::thread> dp_workers;
     std::chrono::steady_clock::time_point t0;
@@ -1195,6 +1212,13 @@ int serve_search(gxy_server *S, std::vector<ServeReq *> &batch) {
                 int rc = search_dev_impl(ctx, S->tables[c].get(), units[u], &cfg->scoring, &sp, respect, &hsps, &st,
                                          byte_ok ? ub[u].p + kBytePad : nullptr, cfg->gapped_enabled != 0);
                 if (rc) return rc;
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    S->totals.hits += st.hits;
+                    S->totals.columns += st.columns;
+                    S->totals.passing += st.passing;
+                    S->totals.hsps += hsps.size();
+                }
                 if (!concat) {
                     per[u] = std::move(hsps);
                     continue;
@@ -1385,6 +1409,11 @@ void dp_loop(gxy_server *S, gxy_ctx *ctx) {
                     gxy_run_stats trs;
                     std::memset(&trs, 0, sizeof trs);
                     rc = tweener_groups_dev(ctx, tw, &S->cfg, &run_store, &trs);
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    S->totals.dp_problems += trs.dp_problems;
+                    S->totals.dp_cells += trs.dp_cells;
+                    S->totals.hits += trs.hits;
+                    S->totals.columns += trs.columns;
                 }
                 for (size_t x : now) mine[x]->tweens.clear();
             }
@@ -1393,6 +1422,11 @@ void dp_loop(gxy_server *S, gxy_ctx *ctx) {
         gxy_run_stats rs;
         std::memset(&rs, 0, sizeof rs);
         const int rc = gapped_groups_dev(ctx, all, &S->cfg, &run_store, &rs, [&] { settle(0); });
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->totals.dp_problems += rs.dp_problems;
+            S->totals.dp_cells += rs.dp_cells;
+        }
         settle(rc);
     }
 }
@@ -1533,6 +1567,47 @@ const char *gxy_server_last_error(gxy_server *S) {
     std::lock_guard<std::mutex> lk(S->mu);
     copy = S->err;
     return copy.c_str();
+}
+
+int gxy_server_stats(gxy_server *S, gxy_run_stats *out) {
+    return guarded(nullptr, [&]() -> int {
+        if (!S || !out) return GXY_EINVAL;
+        std::lock_guard<std::mutex> lk(S->mu);
+        *out = S->totals;
+        return GXY_OK;
+    });
+}
+
+int gxy_server_set_timing(gxy_server *S, int enabled) {
+    return guarded(nullptr, [&]() -> int {
+        if (!S) return GXY_EINVAL;
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!S->queue.empty() || !S->dpq.empty() || !S->reqs.empty()) return GXY_EINVAL;  // requests in flight
+        S->ctx->timing = enabled != 0;
+        for (gxy_ctx *c : S->dp_ctx) c->timing = enabled != 0;
+        return GXY_OK;
+    });
+}
+
+int gxy_server_kernel_time(gxy_server *S, const char *name, double *ms, uint64_t *launches) {
+    return guarded(nullptr, [&]() -> int {
+        if (!S || !name || !ms || !launches) return GXY_EINVAL;
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!S->queue.empty() || !S->dpq.empty() || !S->reqs.empty()) return GXY_EINVAL;  // requests in flight
+        *ms = 0;
+        *launches = 0;
+        std::vector<gxy_ctx *> all(1, S->ctx);
+        all.insert(all.end(), S->dp_ctx.begin(), S->dp_ctx.end());
+        for (gxy_ctx *c : all) {
+            double m = 0;
+            uint64_t l = 0;
+            int rc = gxy_ctx_kernel_time(c, name, &m, &l);
+            if (rc) return rc;
+            *ms += m;
+            *launches += l;
+        }
+        return GXY_OK;
+    });
 }
 
 int gxy_server_destroy(gxy_server *S) {

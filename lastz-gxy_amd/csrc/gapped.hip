@@ -743,7 +743,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             const int64_t xlo = (int64_t)B * SW - 1, xhi = (int64_t)(B + kmax + 1) * SW - 1;
             if (xlo < qc0 || xhi > qc0 + kQC1) {
                 qc0 = xlo;
-#pragma unroll
+#pragma unroll 1
                 for (int h = 0; h < kQC1 / 64 / 32; h++) {
                     const int c = lane * (kQC1 / 64 / 32) + h;  // 32-index chunk of this lane
                     uint32_t w[8];
@@ -1215,10 +1215,16 @@ __global__ void k_anchor_cols(const Blk *const *__restrict__ Ts, const Blk *cons
 
 inline unsigned grid_for(uint64_t n, unsigned th) { return (unsigned)((n + th - 1) / th); }
 
-// DP forward kernels (GXY_DP_* in gxy.h): 1 = four wavefronts per side with
-// the row in LDS, 2/3/4 = one wavefront per side with the row in registers
-// and 1/2/4 columns per lane per slot.  Every variant gives the same results.
-constexpr int kDpDefault = GXY_DP_WAVE_K2_W4;
+// DP forward kernels (GXY_DP_* in gxy.h): four wavefronts per side with the
+// row in LDS, or one wavefront per side with the row in registers, 1/2/4
+// columns per lane per slot, compiled for 3 or 4 wavefronts per SIMD.  Every
+// variant gives the same results.
+// Autotuner candidates, and the pass-size classes they are tuned for.
+constexpr int kDpCand[gxy_ctx::kDpCands] = {GXY_DP_WAVE_K2_W4, GXY_DP_WAVE_K4_W4, GXY_DP_WAVE_K2, GXY_DP_WAVE_K4,
+                                           GXY_DP_BLOCK4};
+inline int dp_class(uint64_t sides) { return sides <= 8 ? 0 : sides <= 512 ? 1 : sides <= 8192 ? 2 : 3; }
+// sides a probe pass of a big pass takes: enough to fill the device
+constexpr uint64_t kDpProbeSides = 8192;
 template <class S>
 void launch_dp_forward(int variant, unsigned npp, hipStream_t st, const DpProb *probs, const uint32_t *list, S sc,
                        int32_t oe, int32_t e, int32_t ydrop, DpPool pool, DpRes *res) {
@@ -1274,8 +1280,16 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     results->assign(n, GappedResult{false, {}});
     if (n == 0) return GXY_OK;
     const int32_t oe = sc.gap_open + sc.gap_extend, e = sc.gap_extend;
+    // each item's target and query slice
+    auto tlen_of = [](const GappedItem &it) { return it.t_len == kWholeSeq ? it.t->len - it.t_off : it.t_len; };
+    auto qlen_of = [](const GappedItem &it) { return it.q_len == kWholeSeq ? it.q->len - it.q_off : it.q_len; };
     for (const auto &it : items) {
-        if (it.a.t_pos >= it.t->len || it.a.q_pos >= it.q->len)
+        if (it.t_off > it.t->len || it.q_off > it.q->len || tlen_of(it) > it.t->len - it.t_off ||
+            qlen_of(it) > it.q->len - it.q_off)
+            return set_err(ctx, GXY_EINVAL, "pair slice outside its sequences");
+        if ((tlen_of(it) >> 32) || (qlen_of(it) >> 32))
+            return set_err(ctx, GXY_ERANGE, "a gapped pair of 2^32 bases or more");
+        if (it.a.t_pos >= tlen_of(it) || it.a.q_pos >= qlen_of(it))
             return set_err(ctx, GXY_EINVAL, "anchor outside its sequences");
     }
     // Two side-problems per item: 2k = right, 2k+1 = left.
@@ -1288,18 +1302,18 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         r = DpProb{};
         r.T = it.t->d;
         r.Q = it.q->d;
-        r.t0 = at + 1;
-        r.q0 = aq + 1;
+        r.t0 = it.t_off + at + 1;
+        r.q0 = it.q_off + aq + 1;
         r.tdir = 1;
         r.qdir = 1;
-        r.n = (uint32_t)(it.t->len - at - 1);
-        r.m = (uint32_t)(it.q->len - aq - 1);
+        r.n = (uint32_t)(tlen_of(it) - at - 1);
+        r.m = (uint32_t)(qlen_of(it) - aq - 1);
         DpProb &l = probs[2 * k + 1];
         l = DpProb{};
         l.T = it.t->d;
         l.Q = it.q->d;
-        l.t0 = at >= 1 ? at - 1 : 0;
-        l.q0 = aq >= 1 ? aq - 1 : 0;
+        l.t0 = it.t_off + (at >= 1 ? at - 1 : 0);
+        l.q0 = it.q_off + (aq >= 1 ? aq - 1 : 0);
         l.tdir = -1;
         l.qdir = -1;
         l.n = (uint32_t)at;
@@ -1313,9 +1327,12 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     std::vector<const Blk *> tp(2 * n);
     std::vector<gxy_anchor> anc(n);
     for (uint64_t k = 0; k < n; k++) {
-        tp[k] = items[k].t->d;
-        tp[n + k] = items[k].q->d;
+        // the slice's first block, and the anchor relative to it
+        tp[k] = items[k].t->d + (items[k].t_off >> 5);
+        tp[n + k] = items[k].q->d + (items[k].q_off >> 5);
         anc[k] = items[k].a;
+        anc[k].t_pos += (uint32_t)(items[k].t_off & 31);
+        anc[k].q_pos += (uint32_t)(items[k].q_off & 31);
     }
     auto *dptrs = (const Blk **)scratch(ctx, kSlotMisc0, 2 * n * sizeof(void *));
     auto *danc = (gxy_anchor *)scratch(ctx, kSlotMisc1, n * sizeof(gxy_anchor));
@@ -1408,6 +1425,20 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                 npp++;
                 if (alone) break;
             }
+            // the forward kernel: the caller's, the class's tuned one, or the
+            // next candidate to measure (a big pass lends a probe-sized slice)
+            int variant = ctx->dp_kernel;
+            gxy_ctx::DpTune *tune = nullptr;
+            if (!variant) {
+                tune = &ctx->dp_tune[dp_class(npp)];
+                if (tune->chosen) {
+                    variant = tune->chosen;
+                    tune = nullptr;
+                } else {
+                    variant = kDpCand[tune->tried];
+                    if (npp > 2 * kDpProbeSides) npp = kDpProbeSides;
+                }
+            }
             // per-pass arrays: results | run offsets | packed offsets | run counts | side list | counter
             const size_t o_res = 0, o_roff = o_res + npp * sizeof(DpRes), o_poff = o_roff + npp * 8,
                          o_nr = o_poff + npp * 8, o_list = o_nr + npp * 4, o_ctr = (o_list + npp * 4 + 255) & ~255ull;
@@ -1421,8 +1452,12 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             pool.next = (uint32_t *)(work + o_ctr);
             GXY_HIP(ctx, hipMemcpyAsync(dlist, pending.data() + head, npp * 4, hipMemcpyHostToDevice, ctx->stream));
             GXY_HIP(ctx, hipMemsetAsync(pool.next, 0, 4, ctx->stream));
+            if (tune && !ctx->dp_ev[0]) {
+                GXY_HIP(ctx, hipEventCreate(&ctx->dp_ev[0]));
+                GXY_HIP(ctx, hipEventCreate(&ctx->dp_ev[1]));
+            }
+            if (tune) GXY_HIP(ctx, hipEventRecord(ctx->dp_ev[0], ctx->stream));
             {
-                const int variant = ctx->dp_kernel ? ctx->dp_kernel : kDpDefault;
                 KernelTimer kt_(ctx, "dp_forward");
                 if (dp_scores_i8(sc))
                     launch_dp_forward(variant, (unsigned)npp, ctx->stream, dprob, dlist, make_scorer_i8(sc), oe, e,
@@ -1432,9 +1467,23 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                                       gp.y_drop, pool, dres);
             }
             GXY_LAUNCH_CHECK(ctx);
+            if (tune) GXY_HIP(ctx, hipEventRecord(ctx->dp_ev[1], ctx->stream));
             std::vector<DpRes> pr(npp);
             GXY_HIP(ctx, hipMemcpyAsync(pr.data(), dres, npp * sizeof(DpRes), hipMemcpyDeviceToHost, ctx->stream));
             GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (tune) {  // the candidate's rate on this pass; all measured: keep the fastest
+                float ms = 0;
+                GXY_HIP(ctx, hipEventElapsedTime(&ms, ctx->dp_ev[0], ctx->dp_ev[1]));
+                uint64_t pcells = 0;
+                for (uint64_t y = 0; y < npp; y++) pcells += pr[y].cells;
+                tune->gcups[tune->tried] = ms > 0 ? (double)pcells / (ms * 1e6) : 0.0;
+                if (++tune->tried == gxy_ctx::kDpCands) {
+                    int b = 0;
+                    for (int c = 1; c < gxy_ctx::kDpCands; c++)
+                        if (tune->gcups[c] > tune->gcups[b]) b = c;
+                    tune->chosen = kDpCand[b];
+                }
+            }
             // reserve bi + bj runs per completed side and walk them back
             std::vector<uint64_t> roff(npp, 0);
             uint64_t total_runs = 0, done = 0;
@@ -1664,6 +1713,30 @@ int gxy_gapped_extend_batch(gxy_ctx *ctx, const gxy_anchor *anchors, uint64_t n,
     });
 }
 
+int gxy_gapped_extend_slices_dev(gxy_ctx *ctx, const gxy_dseq *target, const gxy_dseq *query, const uint64_t *t_off,
+                                 const uint64_t *t_len, const uint64_t *q_off, const uint64_t *q_len,
+                                 const gxy_anchor *anchors, uint64_t n, const gxy_scoring *scoring,
+                                 const gxy_gapped_params *params, gxy_alignment_set **out) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        if (!target || !query || !scoring || !params || !out ||
+            (n && (!t_off || !t_len || !q_off || !q_len || !anchors)))
+            return set_err(ctx, GXY_EINVAL, "null argument");
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        std::vector<GappedItem> items(n);
+        for (uint64_t k = 0; k < n; k++)
+            items[k] = GappedItem{target, query, anchors[k], t_off[k], t_len[k], q_off[k], q_len[k]};
+        std::vector<GappedResult> res;
+        std::vector<gxy_edit_run> runs;
+        uint64_t cells = 0;
+        int rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
+        // a traceback pool grown past 48 GiB goes back to the device, as after gxy_run_dev
+        scratch_release(ctx, kSlotDpTb, 48ull << 30);
+        if (rc) return rc;
+        return make_set(ctx, res, runs, cells, out);
+    });
+}
+
 int gxy_gapped_extend_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, const gxy_dseq *const *queries,
                                 const gxy_anchor *anchors, uint64_t n, const gxy_scoring *scoring,
                                 const gxy_gapped_params *params, gxy_alignment_set **out) {
@@ -1682,6 +1755,8 @@ int gxy_gapped_extend_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, co
         std::vector<gxy_edit_run> runs;
         uint64_t cells = 0;
         int rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
+        // a traceback pool grown past 48 GiB goes back to the device, as after gxy_run_dev
+        scratch_release(ctx, kSlotDpTb, 48ull << 30);
         if (rc) return rc;
         return make_set(ctx, res, runs, cells, out);
     });

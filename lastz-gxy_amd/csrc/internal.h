@@ -51,10 +51,14 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
 // Vec::dedup of a search's emitted HSPs (in place); returns the tie count.
 uint64_t dedup_hsps(std::vector<gxy_hsp> *v);
 
+constexpr uint64_t kWholeSeq = ~0ull;
 struct GappedItem {
     const gxy_dseq *t;
     const gxy_dseq *q;
-    gxy_anchor a;
+    gxy_anchor a;  // relative to the slices below
+    // the pair is t[t_off, t_off + t_len) against q[q_off, q_off + q_len)
+    // (kWholeSeq: to the end of the sequence)
+    uint64_t t_off = 0, t_len = kWholeSeq, q_off = 0, q_len = kWholeSeq;
 };
 
 struct GappedResult {

@@ -474,6 +474,17 @@ int gxy_ctx_set_dp_kernel(gxy_ctx *ctx, int kernel) {
     return GXY_OK;
 }
 
+int gxy_ctx_dp_autotune_state(gxy_ctx *ctx, int *chosen, double *gcups) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !chosen || !gcups) return GXY_EINVAL;
+        for (int c = 0; c < gxy_ctx::kDpClasses; c++) {
+            chosen[c] = ctx->dp_tune[c].chosen;
+            for (int k = 0; k < gxy_ctx::kDpCands; k++) gcups[c * gxy_ctx::kDpCands + k] = ctx->dp_tune[c].gcups[k];
+        }
+        return GXY_OK;
+    });
+}
+
 int gxy_ctx_dp_pool_stats(gxy_ctx *ctx, uint64_t *passes, uint64_t *truncated, uint64_t *alone) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !passes || !truncated || !alone) return GXY_EINVAL;
@@ -515,12 +526,14 @@ void gxy_ctx_destroy(gxy_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     timing_flush(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->dp_ev)
+        if (e) (void)hipEventDestroy(e);
     for (int k = 0; k < gxy_ctx::kSlots; k++)
         if (ctx->slot[k]) (void)(ctx->async_alloc ? hipFreeAsync(ctx->slot[k], ctx->stream) : hipFree(ctx->slot[k]));
     (void)hipStreamSynchronize(ctx->stream);
     pool_trim(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);  // a run's helper may hold none between runs
     delete ctx;
 }
 

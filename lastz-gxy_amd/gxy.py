@@ -34,6 +34,7 @@ _SIGS = {
     "gxy_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_set_dp_pool_rows": (C.c_int, [_vp, C.c_uint64]),
     "gxy_ctx_set_dp_kernel": (C.c_int, [_vp, C.c_int]),
+    "gxy_ctx_dp_autotune_state": (C.c_int, [_vp, _P(C.c_int), _P(C.c_double)]),
     "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
     "gxy_ctx_reset_timing": (C.c_int, [_vp]),
@@ -73,6 +74,8 @@ _SIGS = {
                                           _P(_P(T.AlignmentSet))]),
     "gxy_gapped_extend_pairs_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint64, _P(T.Scoring),
                                               _P(T.GappedParams), _P(_P(T.AlignmentSet))]),
+    "gxy_gapped_extend_slices_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint64,
+                                               _P(T.Scoring), _P(T.GappedParams), _P(_P(T.AlignmentSet))]),
     "gxy_alignment_set_free": (None, [_P(T.AlignmentSet)]),
     "gxy_config_default": (None, [_P(T.Config)]),
     "gxy_pattern_parse": (C.c_int, [C.c_char_p, _P(T.SeedPatternC)]),
@@ -92,6 +95,9 @@ _SIGS = {
     "gxy_best_chain": (C.c_int, [_vp, C.c_uint64, _vp, _P(C.c_uint64)]),
     "gxy_interpolate": (C.c_int, [_vp, _P(T.RecordSet), _vp, _vp, _P(T.Config), _P(_P(T.RecordSet))]),
     "gxy_server_create": (C.c_int, [C.c_int, _vp, C.c_uint64, _P(T.Config), _P(T.ServerParams), _P(_vp)]),
+    "gxy_server_set_timing": (C.c_int, [_vp, C.c_int]),
+    "gxy_server_stats": (C.c_int, [_vp, _P(T.RunStats)]),
+    "gxy_server_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
     "gxy_server_submit": (C.c_int, [_vp, _P(T.Seq), _P(C.c_uint64)]),
     "gxy_server_wait": (C.c_int, [_vp, C.c_uint64, _P(_P(T.RecordSet)), _P(T.RequestStats)]),
     "gxy_server_last_error": (C.c_char_p, [_vp]),
@@ -148,6 +154,19 @@ class Context:
         """Choose the DP forward kernel (GXY_DP_*: 0 auto, 1 four wavefronts per side,
         2/3/4 one wavefront per side with 1/2/4 columns per lane per slot)."""
         self.check(lib().gxy_ctx_set_dp_kernel(self._h, kernel), "gxy_ctx_set_dp_kernel")
+
+    DP_CANDIDATES = ("wave_k2_w4", "wave_k4_w4", "wave_k2", "wave_k4", "block4")
+    DP_KERNEL_NAMES = {1: "block4", 2: "wave_k1", 3: "wave_k2", 4: "wave_k4", 5: "wave_k2_w4", 6: "wave_k4_w4"}
+
+    def dp_autotune_state(self):
+        """The DP autotuner per pass-size class (<= 8, <= 512, <= 8192, more sides):
+        [(chosen kernel name or None, {candidate: measured GCUPS})]."""
+        chosen = (C.c_int * 4)()
+        g = (C.c_double * 20)()
+        self.check(lib().gxy_ctx_dp_autotune_state(self._h, chosen, g), "gxy_ctx_dp_autotune_state")
+        return [(self.DP_KERNEL_NAMES.get(chosen[c]),
+                 {n: round(g[5 * c + k], 2) for k, n in enumerate(self.DP_CANDIDATES) if g[5 * c + k] > 0})
+                for c in range(4)]
 
     def dp_pool_stats(self):
         """(passes, truncated sides, passes run with one side alone) since creation."""
@@ -443,6 +462,44 @@ def gapped_extend_pairs_dev(targets, queries, anchors, matrix, params, ctx=None)
     return res, cells
 
 
+def gapped_extend_slices_dev(target, query, t_off, t_len, q_off, q_len, anchors, matrix, params, ctx=None,
+                             raw=False, pick=None):
+    """Batched extend over pairs that are slices of two device sequences
+    (gxy_gapped_extend_slices_dev); anchors relative to the slices.  Returns
+    (results, cells), or with raw=True (found count, cells, {k: result for k
+    in pick}) decoding only the alignments listed in `pick`."""
+    ctx = ctx or default_context()
+    n = len(anchors)
+    arrs = [np.ascontiguousarray(a, np.uint64) for a in (t_off, t_len, q_off, q_len)]
+    an = np.asarray(anchors, np.uint32).reshape(n, 2) if n else np.zeros((0, 2), np.uint32)
+    arr = (T.Anchor * max(1, n))()
+    view = np.ctypeslib.as_array(C.cast(arr, _P(C.c_uint32)), (max(1, n) * C.sizeof(T.Anchor) // 4,))
+    view = view.reshape(max(1, n), C.sizeof(T.Anchor) // 4)
+    view[:n, 0] = an[:, 0]
+    view[:n, 1] = an[:, 1]
+    sp = _P(T.AlignmentSet)()
+    ctx.check(lib().gxy_gapped_extend_slices_dev(ctx._h, target._h, query._h, *[a.ctypes.data for a in arrs],
+                                                 C.addressof(arr), n, C.byref(matrix), C.byref(params),
+                                                 C.byref(sp)), "gxy_gapped_extend_slices_dev")
+    cells = sp.contents.cells
+    if raw:
+        st = sp.contents
+        found = int(np.ctypeslib.as_array(st.found, (n,)).astype(bool).sum()) if n else 0
+        picked = {}
+        for k in (pick or ()):
+            if not st.found[k]:
+                picked[k] = None
+                continue
+            a = st.alns[k]
+            runs = [(st.runs[a.runs_offset + x].op, st.runs[a.runs_offset + x].n) for x in range(a.n_runs)]
+            picked[k] = (a.t_start, a.q_start, a.t_len, a.q_len, a.score, runs)
+        lib().gxy_alignment_set_free(sp)
+        return found, cells, picked
+    res = _alignment_list(sp.contents)
+    lib().gxy_alignment_set_free(sp)
+    return res, cells
+
+
 def gapped_extend(anchor, target, query, matrix, params, ctx=None):
     """gapped_extend::extend (src/gapped_extend.rs:78-84)."""
     res, cells = gapped_extend_batch([anchor], target, query, matrix, params, ctx)
@@ -588,6 +645,29 @@ class Server:
             msg = lib().gxy_server_last_error(self._h)
             raise GxyError(f"gxy_server_wait failed ({rc}): {msg.decode() if msg else ''}")
         return Records(out), st
+
+    def set_timing(self, on: bool = True):
+        """Kernel timing on the server's contexts (only with no request in flight)."""
+        rc = lib().gxy_server_set_timing(self._h, 1 if on else 0)
+        if rc != 0:
+            raise GxyError(f"gxy_server_set_timing failed ({rc})")
+
+    def stats(self) -> T.RunStats:
+        """Work done so far over all requests."""
+        st = T.RunStats()
+        rc = lib().gxy_server_stats(self._h, C.byref(st))
+        if rc != 0:
+            raise GxyError(f"gxy_server_stats failed ({rc})")
+        return st
+
+    def kernel_time(self, name: str):
+        """(milliseconds, launches) of kernel `name` over the server's contexts."""
+        ms = C.c_double()
+        n = C.c_uint64()
+        rc = lib().gxy_server_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n))
+        if rc != 0:
+            raise GxyError(f"gxy_server_kernel_time failed ({rc})")
+        return ms.value, n.value
 
     def close(self):
         if getattr(self, "_h", None):

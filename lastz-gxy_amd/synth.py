@@ -346,3 +346,95 @@ def contig_workload(n_target: int = 200_000_000, n_contigs: int = 10_000, seed: 
         out.append(_packed(q))
     return target, out
 
+
+
+def _pack2(code: np.ndarray) -> np.ndarray:
+    """2-bit codes (length a multiple of 4) into PackedSeq code bytes."""
+    c4 = code.reshape(-1, 4)
+    return c4[:, 0] | (c4[:, 1] << 2) | (c4[:, 2] << 4) | (c4[:, 3] << 6)
+
+
+def _dp_slice_worker(args):
+    """Pairs [a, b) of dp_anchor_slices: generate them (dp_anchor_arrays'
+    model, its own seed) and write their packed codes at the worker's offsets
+    of the shared target and query code buffers.  Returns the query lengths
+    and anchor query positions."""
+    from multiprocessing import shared_memory
+    a, b, seed, tname, qname, tbase, qbase, qcap = args
+    t, q, toff, qoff, tpos, qpos = dp_anchor_arrays(b - a, seed=(seed, a), packed=False)
+    tsh = shared_memory.SharedMemory(name=tname)
+    qsh = shared_memory.SharedMemory(name=qname)
+    try:
+        tb = np.ndarray((tsh.size,), np.uint8, tsh.buf)
+        qb = np.ndarray((qsh.size,), np.uint8, qsh.buf)
+        if len(q) > qcap:
+            raise RuntimeError("query segment bound exceeded")
+        pad = lambda x: np.concatenate([x, np.zeros((-len(x)) % 4, np.uint8)])
+        pt, pq = _pack2(pad(t)), _pack2(pad(q))
+        tb[tbase // 4: tbase // 4 + len(pt)] = pt
+        qb[qbase // 4: qbase // 4 + len(pq)] = pq
+        del tb, qb
+    finally:
+        tsh.close()
+        qsh.close()
+    return np.diff(toff), np.diff(qoff), tpos, qpos
+
+
+def dp_anchor_slices(n: int, seed: int = 3, workers: int = 16):
+    """dp_anchor_arrays' workload for many pairs, generated by `workers`
+    processes: pairs are split in contiguous ranges, range r drawn with seed
+    (seed, first pair of r).  Returns (target, query, t_off, t_len, q_off,
+    q_len, t_pos, q_pos): two PackedSeqs holding the pairs (each range's
+    pairs end to end, ranges 32-base aligned, queries in segments sized for
+    their range with room to spare) and each pair's slices and anchor."""
+    from concurrent.futures import ProcessPoolExecutor
+    from multiprocessing import get_context, shared_memory
+    workers = max(1, min(workers, n))
+    bounds = [n * w // workers for w in range(workers + 1)]
+    # each range's target length is fixed by its draw of extension lengths:
+    # draw them first (dp_anchor_arrays draws them first from its seed)
+    tlens = []
+    for w in range(workers):
+        a, b = bounds[w], bounds[w + 1]
+        rng = np.random.default_rng((seed, a))
+        Ls = np.clip(np.round(2000.0 * np.exp(rng.standard_normal(b - a))), 100, 50_000).astype(np.int64)
+        tlens.append(int((2 * Ls + 1).sum()))
+    al = lambda x: (x + 31) // 32 * 32
+    tbase = np.concatenate([[0], np.cumsum([al(x) for x in tlens])])
+    qcap = [al(int(x * 1.1) + 4096) for x in tlens]
+    qbase = np.concatenate([[0], np.cumsum(qcap)])
+    tsh = shared_memory.SharedMemory(create=True, size=int(tbase[-1]) // 4)
+    qsh = shared_memory.SharedMemory(create=True, size=int(qbase[-1]) // 4)
+    try:
+        jobs = [(bounds[w], bounds[w + 1], seed, tsh.name, qsh.name, int(tbase[w]), int(qbase[w]), qcap[w])
+                for w in range(workers)]
+        with ProcessPoolExecutor(workers, mp_context=get_context("fork")) as ex:
+            parts = list(ex.map(_dp_slice_worker, jobs))
+        tcodes = np.ndarray((tsh.size,), np.uint8, tsh.buf).copy()
+        qcodes = np.ndarray((qsh.size,), np.uint8, qsh.buf).copy()
+    finally:
+        tsh.close()
+        tsh.unlink()
+        qsh.close()
+        qsh.unlink()
+    t_len = np.concatenate([p[0] for p in parts])
+    q_len = np.concatenate([p[1] for p in parts])
+    t_off = np.concatenate([tbase[w] + np.concatenate([[0], np.cumsum(parts[w][0])[:-1]])
+                            for w in range(workers)])
+    q_off = np.concatenate([qbase[w] + np.concatenate([[0], np.cumsum(parts[w][1])[:-1]])
+                            for w in range(workers)])
+    t_pos = np.concatenate([p[2] for p in parts])
+    q_pos = np.concatenate([p[3] for p in parts])
+    ones = lambda m: np.full((m + 7) // 8, 0xFF, np.uint8)
+    T = PackedSeq(tcodes, ones(int(tbase[-1])), np.zeros((int(tbase[-1]) + 7) // 8, np.uint8), int(tbase[-1]))
+    Q = PackedSeq(qcodes, ones(int(qbase[-1])), np.zeros((int(qbase[-1]) + 7) // 8, np.uint8), int(qbase[-1]))
+    return T, Q, t_off, t_len, q_off, q_len, t_pos, q_pos
+
+
+def slice_of(seq: PackedSeq, off: int, length: int) -> PackedSeq:
+    """seq[off, off + length) as a PackedSeq, unpacking only that range."""
+    off, length = int(off), int(length)
+    a, b = off // 32 * 32, min(seq.len, (off + length + 31) // 32 * 32)
+    part = PackedSeq(seq.codes[a // 4:(b + 3) // 4], seq.valid[a // 8:(b + 7) // 8], seq.mask[a // 8:(b + 7) // 8],
+                     b - a)
+    return part.slice(off - a, off - a + length)

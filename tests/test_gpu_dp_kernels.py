@@ -123,3 +123,53 @@ def test_dp_kernel_int32_and_asymmetric_matrices(ctx, kernel):
         for k in range(len(cases)):
             assert got[k] == exp[k], (kernel, k)
     ctx.set_dp_kernel(0)
+
+
+def test_slices_equal_pairs_and_oracle(ctx):
+    """gxy_gapped_extend_slices_dev over pairs laid end to end in two device
+    sequences equals the oracle on each pair (and the per-pair entry point)."""
+    ctx.set_dp_kernel(0)
+    rng = np.random.default_rng(104)
+    hox = T.scoring_hoxd70()
+    cases = pairs(rng, [int(x) for x in rng.integers(1, 1200, 80)])
+    tcat = np.concatenate([t.unpack()[0] for t, _, _ in cases])
+    qcat = np.concatenate([q.unpack()[0] for _, q, _ in cases])
+    tval = np.concatenate([t.unpack()[1] for t, _, _ in cases])
+    qval = np.concatenate([q.unpack()[1] for _, q, _ in cases])
+    tl = [len(t) for t, _, _ in cases]
+    ql = [len(q) for _, q, _ in cases]
+    to = np.concatenate([[0], np.cumsum(tl)[:-1]])
+    qo = np.concatenate([[0], np.cumsum(ql)[:-1]])
+    z = np.zeros
+    dt = gxy.DeviceSeq.upload(T.PackedSeq.from_arrays(tcat, tval, z(len(tcat), bool)), ctx)
+    dq = gxy.DeviceSeq.upload(T.PackedSeq.from_arrays(qcat, qval, z(len(qcat), bool)), ctx)
+    gp = T.GappedParams(9400, 500)
+    got, cells = gxy.gapped_extend_slices_dev(dt, dq, to, tl, qo, ql, [a for _, _, a in cases], hox, gp, ctx)
+    exp, ecells = oracle_all(cases, hox, gp)
+    assert cells == ecells
+    assert got == exp
+    per, pcells = run_all(ctx, cases, hox, gp)
+    assert per == got and pcells == cells
+
+
+def test_autotuner_probes_then_keeps_the_fastest():
+    """With no kernel forced, a context measures every candidate on its first
+    passes of a size class (one pass each), keeps the fastest, and every pass —
+    probe or not — gives the oracle's alignments.  Passes of at most 8 sides
+    use the four-wavefront kernel without probing."""
+    ctx = gxy.Context(0)
+    rng = np.random.default_rng(105)
+    hox = T.scoring_hoxd70()
+    gp = T.GappedParams(9400, 1000)
+    cases = pairs(rng, [int(x) for x in rng.integers(50, 800, 40)])  # 80 sides: class 1
+    exp, ecells = oracle_all(cases, hox, gp)
+    n_cand = len(gxy.Context.DP_CANDIDATES)
+    for k in range(n_cand + 1):
+        got, cells = run_all(ctx, cases, hox, gp)
+        assert cells == ecells and got == exp, k
+    state = ctx.dp_autotune_state()
+    assert state[0][0] == "block4" and state[0][1] == {}
+    chosen, rates = state[1]
+    assert set(rates) == set(gxy.Context.DP_CANDIDATES) and all(v > 0 for v in rates.values())
+    assert chosen == max(rates, key=rates.get)
+    ctx.close()
