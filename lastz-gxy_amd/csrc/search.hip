@@ -769,10 +769,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                 }
                 kstop = 32 - nst;
             } else {  // saturating form of the same walk (src/hsp.rs:186)
+                // its own copy of the score words: byte extracts shared with
+                // the fast form would be hoisted above the branch as 32
+                // separate instructions instead of folding into the adds
+                uint32_t sv[8];
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    sv[w] = sw[w];
+                    asm volatile("" : "+v"(sv[w]));
+                }
                 uint32_t stopbits = 0;
 #pragma unroll
                 for (int k = 0; k < 32; k++) {
-                    const int32_t sc = __builtin_amdgcn_sbfe(sw[k >> 2], 8 * (k & 3), 8);
+                    const int32_t sc = __builtin_amdgcn_sbfe(sv[k >> 2], 8 * (k & 3), 8);
                     const int32_t r = sat_add(run, sc);
                     kbest = r > best ? (uint32_t)(k + 1) : kbest;
                     best = max(best, r);
