@@ -42,9 +42,11 @@ sys.path.insert(0, os.path.join(ROOT, "lastz-gxy_amd"))
 
 METRIC = "end-to-end query Mbp/s + HSP/DP GCUPS; p50/p99 serve latency vs plain GPU"
 HBM_PEAK_GBS = 8000.0
-# int32 VALU: 256 CUs x 64 lanes x 2.4 GHz (the 157.3 TFLOP/s fp32 figure
-# counts packed pairs and FMA as 2 each)
-VALU_I32_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# int32 VALU: 256 CUs x 128 lane-ops per clock x 2.4 GHz = 78.6 T ops/s, the
+# lane-op rate behind the published 157.3 TFLOP/s vector fp32 figure (an FMA
+# counts 2).  A 16-lane-per-SIMD model (39.3 T) is too low: an int32 add/xor
+# loop over 8192 wavefronts ran 51.4 T lane-ops/s on the MI355X.
+VALU_I32_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 DP_OPS_PER_CELL = 11  # X: 2 sub + max; Y: 2 sub + max; M: add + 2 max; cell best: 2 max
 
 

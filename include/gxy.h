@@ -183,6 +183,12 @@ int gxy_ctx_reset_timing(gxy_ctx *ctx);
  * side that needs more rows than the whole pool fails with GXY_ENOMEM.  Results
  * do not depend on the cap.  rows = 0 restores the default. */
 int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows);
+/* Returns the context's cached device memory to the device (no counterpart in
+ * the reference): the scratch buffers that calls grow and keep for the next
+ * call — the traceback pool among them, up to a third of the device after a
+ * large gapped batch — and freed sequence blocks kept for reuse.  Sequences,
+ * tables and results stay valid; the next call allocates what it needs again. */
+int gxy_ctx_release_cache(gxy_ctx *ctx);
 /* DP forward kernel of the gapped extension (no counterpart in the reference;
  * for measurement and tests).  Every kernel gives identical results; they
  * differ in how a side's row is spread over the GPU: GXY_DP_BLOCK4 runs each

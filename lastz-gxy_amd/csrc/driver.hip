@@ -963,10 +963,6 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             (void)hipStreamDestroy(ctx->helper->stream);
             ctx->helper->stream = nullptr;
         }
-        // A traceback pool grown past 48 GiB by this run goes back to the device
-        // (the next large run allocates it again), so it does not hold memory that
-        // other contexts on the device need.
-        scratch_release(ctx, kSlotDpTb, 48ull << 30);
         rs.tie_pairs += ties.load();
         for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();
 

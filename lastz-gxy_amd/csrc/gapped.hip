@@ -1730,8 +1730,6 @@ int gxy_gapped_extend_slices_dev(gxy_ctx *ctx, const gxy_dseq *target, const gxy
         std::vector<gxy_edit_run> runs;
         uint64_t cells = 0;
         int rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
-        // a traceback pool grown past 48 GiB goes back to the device, as after gxy_run_dev
-        scratch_release(ctx, kSlotDpTb, 48ull << 30);
         if (rc) return rc;
         return make_set(ctx, res, runs, cells, out);
     });
@@ -1755,8 +1753,6 @@ int gxy_gapped_extend_pairs_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, co
         std::vector<gxy_edit_run> runs;
         uint64_t cells = 0;
         int rc = gapped_batch_dev(ctx, items, *scoring, *params, &res, &runs, &cells);
-        // a traceback pool grown past 48 GiB goes back to the device, as after gxy_run_dev
-        scratch_release(ctx, kSlotDpTb, 48ull << 30);
         if (rc) return rc;
         return make_set(ctx, res, runs, cells, out);
     });

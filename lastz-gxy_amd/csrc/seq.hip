@@ -459,6 +459,22 @@ int gxy_ctx_set_timing(gxy_ctx *ctx, int enabled) {
     });
 }
 
+int gxy_ctx_release_cache(gxy_ctx *ctx) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return GXY_EINVAL;
+        GXY_HIP(ctx, hipSetDevice(ctx->device));
+        for (gxy_ctx *c : {ctx, ctx->helper}) {
+            if (!c) continue;
+            // a helper holds no stream between runs: its work has completed
+            if (c->stream) GXY_HIP(ctx, hipStreamSynchronize(c->stream));
+            for (int k = 0; k < gxy_ctx::kSlots; k++) scratch_release(c, k, 0);
+            pool_trim(c);
+            if (c->stream) GXY_HIP(ctx, hipStreamSynchronize(c->stream));
+        }
+        return GXY_OK;
+    });
+}
+
 int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows) {
     return guarded(ctx, [&]() -> int {
         if (!ctx) return GXY_EINVAL;

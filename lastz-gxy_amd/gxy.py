@@ -33,6 +33,7 @@ _SIGS = {
     "gxy_version": (C.c_char_p, []),
     "gxy_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_set_dp_pool_rows": (C.c_int, [_vp, C.c_uint64]),
+    "gxy_ctx_release_cache": (C.c_int, [_vp]),
     "gxy_ctx_set_dp_kernel": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_dp_autotune_state": (C.c_int, [_vp, _P(C.c_int), _P(C.c_double)]),
     "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
@@ -167,6 +168,11 @@ class Context:
         return [(self.DP_KERNEL_NAMES.get(chosen[c]),
                  {n: round(g[5 * c + k], 2) for k, n in enumerate(self.DP_CANDIDATES) if g[5 * c + k] > 0})
                 for c in range(4)]
+
+    def release_cache(self):
+        """Return cached device memory (scratch buffers, traceback pool, freed
+        sequence blocks) to the device."""
+        self.check(lib().gxy_ctx_release_cache(self._h), "gxy_ctx_release_cache")
 
     def dp_pool_stats(self):
         """(passes, truncated sides, passes run with one side alone) since creation."""

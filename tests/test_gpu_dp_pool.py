@@ -84,3 +84,30 @@ def test_side_longer_than_the_pool_is_an_error():
     ctx.set_dp_pool_rows(0)
     g, _ = gxy.gapped_extend_pairs_dev(ts, qs, anc, HOX, gp, ctx)
     assert g[0] == O.gapped_extend(anc[0], host[0][0], host[0][1], HOX, gp)[0]
+
+
+def test_release_cache_keeps_sequences_and_results():
+    """gxy_ctx_release_cache returns the scratch buffers and the traceback pool
+    between calls; resident sequences stay valid and the next call, which
+    allocates again, gives the same alignments (including a two-strand driver
+    run, whose helper context is released too)."""
+    ctx = gxy.Context(0)
+    rng = np.random.default_rng(43)
+    ts, qs, anc, host = pairs(rng, [int(x) for x in rng.integers(200, 1500, 12)], ctx)
+    gp = T.GappedParams(9400, 3000)
+    g1, c1 = gxy.gapped_extend_pairs_dev(ts, qs, anc, HOX, gp, ctx)
+    say("released after a gapped batch")
+    ctx.release_cache()
+    g2, c2 = gxy.gapped_extend_pairs_dev(ts, qs, anc, HOX, gp, ctx)
+    assert g2 == g1 and c2 == c1
+    target, query = synth.pair_30k(seed=5)
+    dt, dq = gxy.DeviceSeq.upload(target, ctx), gxy.DeviceSeq.upload(query, ctx)
+    cfg = T.config_default()
+    r1 = gxy.run_dev([dt], [dq], cfg, ctx)[0].tuples()
+    say("released after a two-strand driver run")
+    ctx.release_cache()
+    r2 = gxy.run_dev([dt], [dq], cfg, ctx)[0].tuples()
+    assert r1 == r2 and len(r1) > 0
+    ctx.release_cache()
+    g3, c3 = gxy.gapped_extend_pairs_dev(ts, qs, anc, HOX, gp, ctx)
+    assert g3 == g1 and c3 == c1
