@@ -146,24 +146,36 @@ def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0, workers=16):
     ctx.reset_timing()
     ctx.set_timing(True)
     ctx.synchronize()
+    p0 = ctx.dp_persist_stats()
     t1 = time.perf_counter()
     for st in range(steps):
         found, cells, picked = once(pick if st == steps - 1 else None)
     ctx.synchronize()
     el = time.perf_counter() - t1
     ctx.set_timing(False)
-    kt = {nm: ctx.kernel_time(nm) for nm in ("dp_forward", "dp_traceback")}
+    kt = {nm: ctx.kernel_time(nm) for nm in ("dp_forward", "dp_traceback", "dp_persist")}
     gcups = cells * steps / el / 1e9
     fwd_ms, fwd_n = kt["dp_forward"]
+    p_launch, p_sides, p_deferred, p_cells = (b - a for a, b in zip(p0, ctx.dp_persist_stats()))
+    if kt["dp_persist"][1]:
+        # the persistent launch (sweep and walk of most sides) dominates; its
+        # cells are those of the sides it finished in the timed steps
+        p_ms, p_n = kt["dp_persist"]
+        roof = valu_roofline(p_cells, p_ms, p_n, kernel="k_dp_persist")
+        roof["note"] = "k_dp_persist's time includes each side's traceback walk"
+    else:
+        roof = valu_roofline(cells * steps, fwd_ms, fwd_n)
     out = {
         "workload": f"{n_anchors} anchored pairs, extension LogNormal(median 2000, sigma 1) clipped 100-50000 bp, "
                     f"identity U[0.70, 0.95], HOXD70, gap 400/30, ydrop 9400, gappedthresh 3000, one call",
         "anchors": n_anchors, "cells_per_step": cells, "found": found, "steps": steps, "warmup": warmup,
         "ms_per_step": round(1000 * el / steps, 2), "dp_gcups": round(gcups, 3),
         "kernel_ms_per_step": {k: round(v[0] / steps, 3) for k, v in kt.items() if v[1]},
-        "roofline": valu_roofline(cells * steps, fwd_ms, fwd_n),
+        "roofline": roof,
         "dp_autotune": ctx.dp_autotune_state(),
         "dp_pool_passes": ctx.dp_pool_stats()[0],
+        "dp_persistent_timed_steps": {"launches": p_launch, "sides": p_sides, "sides_left_to_passes": p_deferred,
+                                      "cells": p_cells},
     }
     log(f"dp: {cells} cells/step, {found}/{n_anchors} found, {gcups:.2f} GCUPS end to end, kernels {kt}")
 

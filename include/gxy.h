@@ -217,6 +217,21 @@ int gxy_ctx_dp_autotune_state(gxy_ctx *ctx, int *chosen, double *gcups);
  * sides stopped by a full pool (swept again later) and passes that ran one
  * side alone. */
 int gxy_ctx_dp_pool_stats(gxy_ctx *ctx, uint64_t *passes, uint64_t *truncated, uint64_t *alone);
+/* Persistent form of the gapped extension (no counterpart in the reference,
+ * whose driver extends anchors one at a time: src/driver.rs:305-331).  Once
+ * the forward kernel of a batch is a one-wavefront variant (GXY_DP_WAVE_K2,
+ * _K4, _K2_W4, _K4_W4), its sides run as one launch of as many wavefronts as
+ * the GPU holds: each takes sides from a device queue (longest first), sweeps
+ * a side into its own share of the traceback pool, walks it back at once and
+ * appends its edit runs, with no host work between sides.  Sides that
+ * outgrow a share go on through passes.  mode 0 (the default) does this for
+ * batches of 16384 sides or more, 1 for every batch, -1 never.  Results do
+ * not depend on it. */
+int gxy_ctx_set_dp_persist(gxy_ctx *ctx, int mode);
+/* Persistent launches on the context since creation, sides given to them,
+ * sides they left to passes and the DP cells of the sides they finished. */
+int gxy_ctx_dp_persist_stats(gxy_ctx *ctx, uint64_t *launches, uint64_t *sides, uint64_t *deferred,
+                             uint64_t *cells);
 
 /* ---- sequences ---------------------------------------------------------- */
 /* PackedSeq::from_ascii (src/sequences.rs:164-206) into caller buffers sized

@@ -390,6 +390,13 @@ struct gxy_ctx {
     hipEvent_t dp_ev[2] = {nullptr, nullptr};
     // gapped passes since creation: passes, sides stopped by a full pool, passes run with one side alone
     uint64_t dp_passes = 0, dp_truncated = 0, dp_alone = 0;
+    // Persistent DP launches (gapped.hip, k_dp_persist): 0 for batches of at
+    // least kPersistMinSides sides once the kernel is a one-wavefront variant,
+    // 1 for every batch, -1 never.  Counters: launches, sides given to them,
+    // sides they left to passes, DP cells of the sides they finished.
+    int dp_persist = 0;
+    uint64_t dp_persist_launches = 0, dp_persist_sides = 0, dp_persist_deferred = 0, dp_persist_cells = 0;
+    int cus = 0;  // compute units of the device (ctx_cus)
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_used;
     // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.
@@ -423,6 +430,18 @@ enum TracePhase : int {
 namespace gxy {
 
 int set_err(gxy_ctx *ctx, int code, const char *fmt, ...);
+// Compute units of the context's device (queried once; 256 if the query fails).
+inline int ctx_cus(gxy_ctx *ctx) {
+    if (!ctx->cus) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || v <= 0)
+            v = 256;
+        ctx->cus = v;
+    }
+    return ctx->cus;
+}
+// Sides a batch must hold for the library to sweep it with one persistent launch.
+constexpr uint64_t kPersistMinSides = 16384;
 // Runs the body of a C entry point: a C++ exception (host allocation failure
 // or a library invariant) becomes an error code and message instead of
 // crossing the ABI.
@@ -504,6 +523,7 @@ enum Slot {
     kSlotHits2,
     kSlotBins,
     kSlotBinOff,
+    kSlotDpPersist,
 };
 
 }  // namespace gxy

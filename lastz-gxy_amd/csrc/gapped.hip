@@ -26,6 +26,8 @@
 // wavefront per problem, and emits run-length edit runs tip-to-root.
 #include <algorithm>
 
+#include <thread>
+
 #include "internal.h"
 
 using namespace gxy;
@@ -604,22 +606,49 @@ __device__ __forceinline__ void fwd1_qchunk(const DpProb &pr, int64_t xs, uint32
     for (int t = 0; t < 8; t++) out[t] = v[t];
 }
 
-template <class S, int K, int WPE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dp_fwd1(
-    const DpProb *__restrict__ probs, const uint32_t *__restrict__ list, uint32_t nprob, S sc, int32_t oe, int32_t e,
-    int32_t ydrop, DpPool pool, DpRes *__restrict__ res) {
+// Where a one-wavefront sweep takes the traceback chunk of its rows
+// 64 x .. 64 x + 63 (take(x, prev): the chunk, linked to the side's previous
+// chunk `prev`, or kNoChunk when none is left).
+// The pass kernels share the pool: chunks in arrival order from one counter.
+struct PoolChunks {
+    DpPool pool;
+    __device__ __forceinline__ uint32_t take(uint32_t, uint32_t prev) const {
+        uint32_t c = 0;
+        if (threadIdx.x == 0) {
+            c = atomicAdd(pool.next, 1u);
+            if (c < pool.nchunks) pool.prev[c] = prev;
+        }
+        c = __builtin_amdgcn_readfirstlane(c);
+        return c < pool.nchunks ? c : kNoChunk;
+    }
+};
+// The persistent kernel: each block owns `count` consecutive chunks from
+// `base` and reuses them for every side it sweeps.
+struct RegionChunks {
+    DpPool pool;
+    uint32_t base, count;
+    __device__ __forceinline__ uint32_t take(uint32_t x, uint32_t prev) const {
+        if (x >= count) return kNoChunk;
+        const uint32_t c = base + x;
+        if (threadIdx.x == 0) pool.prev[c] = prev;
+        return c;
+    }
+};
+
+// query bytes a one-wavefront sweep caches in LDS
+constexpr int kQC1 = 4096;
+
+// One side swept by one wavefront (the body of k_dp_fwd1 and k_dp_persist);
+// every lane returns the side's result.  qcache: kQC1 bytes of LDS.
+template <class S, int K, class Chunks>
+__device__ __forceinline__ DpRes fwd1_sweep(const DpProb &pr, S sc, int32_t oe, int32_t e, int32_t ydrop,
+                                            DpPool pool, const Chunks &chunks, uint32_t *__restrict__ qcache) {
     constexpr int SW = Fwd1Shape<K>::SW, NS = Fwd1Shape<K>::NS;
     constexpr uint32_t kFmt = K == 1 ? 1u : (K == 2 ? 2u : 3u);
-    const uint32_t side = blockIdx.x;
-    if (side >= nprob) return;
     const int lane = threadIdx.x;
-    const DpProb pr = probs[list[side]];
     const uint32_t n = pr.n, m = pr.m;
     DpRes out = {0, 0, 0, kStM, 0, 0, kNoChunk, kFmt, 0};
-    if (n == 0 || m == 0) {  // src/gapped_extend.rs:188-195
-        if (lane == 0) res[side] = out;
-        return;
-    }
+    if (n == 0 || m == 0) return out;  // src/gapped_extend.rs:188-195
     // band radius / width (src/gapped_extend.rs:214-233)
     uint32_t br;
     if (e > 0) {
@@ -639,8 +668,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const uint32_t ybit_lead = xb_neg;
 
     // query bytes of query indices [qc0, qc0 + kQC1): column j reads index j - 1
-    constexpr int kQC1 = 4096;
-    __shared__ uint32_t qcache[kQC1 / 4];
     int64_t qc0 = INT64_MIN / 2;  // nothing cached yet
 
     // Row 0 from the same code: every column scores `ambig` in row 0 (no
@@ -676,13 +703,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     for (uint32_t i = 0; i <= n; i++) {
         // a fresh traceback chunk every kChunkRows rows
         if (i % kChunkRows == 0) {
-            uint32_t c = 0;
-            if (lane == 0) {
-                c = atomicAdd(pool.next, 1u);
-                if (c < pool.nchunks) pool.prev[c] = chunk;
-            }
-            c = __builtin_amdgcn_readfirstlane(c);
-            if (c >= pool.nchunks) {  // pool exhausted: the side is swept again later
+            const uint32_t c = chunks.take(i / kChunkRows, chunk);
+            if (c == kNoChunk) {  // no chunk left: the side is swept again later
                 truncated = 1;
                 break;
             }
@@ -960,17 +982,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         lo = alo;
         hi = ahi;
     }
-    if (lane == 0) {
-        out.score = best_score;
-        out.bi = bi;
-        out.bj = bj;
-        out.bstate = bst;
-        out.rows = rows_done;
-        out.truncated = truncated;
-        out.last_chunk = chunk;
-        out.cells = cells;
-        res[side] = out;
-    }
+    out.score = best_score;
+    out.bi = bi;
+    out.bj = bj;
+    out.bstate = bst;
+    out.rows = rows_done;
+    out.truncated = truncated;
+    out.last_chunk = chunk;
+    out.cells = cells;
+    return out;
+}
+
+template <class S, int K, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dp_fwd1(
+    const DpProb *__restrict__ probs, const uint32_t *__restrict__ list, uint32_t nprob, S sc, int32_t oe, int32_t e,
+    int32_t ydrop, DpPool pool, DpRes *__restrict__ res) {
+    __shared__ uint32_t qcache[kQC1 / 4];
+    const uint32_t side = blockIdx.x;
+    if (side >= nprob) return;
+    const DpProb pr = probs[list[side]];
+    const DpRes out = fwd1_sweep<S, K>(pr, sc, oe, e, ydrop, pool, PoolChunks{pool}, qcache);
+    if (threadIdx.x == 0) res[side] = out;
 }
 
 // One lane's words of kTileRows consecutive traceback rows.  Named members
@@ -1011,26 +1043,31 @@ __device__ __forceinline__ void tb_store(uint32_t (*tile)[kRowWords], int lane, 
 // run of one op at once (its lanes read the run's next 64 steps in parallel
 // and a ballot finds where it ends), so a run costs one pair of LDS reads
 // instead of one dependent read per step.  Runs are emitted tip-to-root,
-// merged while the op repeats, into run_off[p] (room for bi + bj runs).
-__global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ res, uint32_t nprob, DpPool pool,
-                                                     const uint64_t *__restrict__ run_off,
-                                                     gxy_edit_run *__restrict__ runs, uint32_t *__restrict__ nruns) {
-    __shared__ __align__(16) uint32_t tile[kTileRows][kRowWords];
-    __shared__ uint2 twin[kTileRows];
-    __shared__ uint32_t tlink;  // chunk below the staged tile's chunk
-    const uint32_t p = blockIdx.x;
-    if (p >= nprob) return;
-    const int lane = threadIdx.x;
-    const DpRes r = res[p];
-    if (r.truncated || (r.bi == 0 && r.bj == 0) || r.rows == 0) {
-        if (lane == 0) nruns[p] = 0;
-        return;
+// merged while the op repeats, through out.put(k, run) for k = 0, 1, ... (at
+// most bi + bj of them); the walk returns their count.
+struct TbLds {
+    uint32_t tile[kTileRows][kRowWords];
+    uint2 twin[kTileRows];
+    uint32_t tlink;  // chunk below the staged tile's chunk
+};
+// Run sinks of the walk: lane 0 writes run k.
+struct RunsOut {  // a side's reserved range in device memory
+    gxy_edit_run *p;
+    __device__ __forceinline__ void put(uint32_t k, gxy_edit_run r) const {
+        if (threadIdx.x == 0) p[k] = r;
     }
+};
+template <class Out>
+__device__ __forceinline__ uint32_t tb_walk(const DpRes &r, DpPool pool, TbLds &L, const Out &out) {
+    uint32_t(*tile)[kRowWords] = L.tile;
+    uint2 *twin = L.twin;
+    uint32_t &tlink = L.tlink;
+    const int lane = threadIdx.x;
+    if (r.truncated || (r.bi == 0 && r.bj == 0) || r.rows == 0) return 0;
     uint32_t i = r.bi, j = r.bj, st = r.bstate;
     // chunk holding row bi
     uint32_t c = r.last_chunk;
     for (uint32_t d = (r.rows - 1) / kChunkRows; d > r.bi / kChunkRows; d--) c = pool.prev[c];
-    gxy_edit_run *out = runs + run_off[p];
     uint32_t nr = 0;
     uint32_t cur_op = 0xFFFFFFFFu, cur_n = 0;
     const uint32_t cap = r.bi + r.bj;  // runs reserved for this side
@@ -1139,7 +1176,7 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
             } else {
                 if (cur_n) {
                     if (nr >= cap) break;
-                    if (lane == 0) out[nr] = {cur_op, cur_n};
+                    out.put(nr, gxy_edit_run{cur_op, cur_n});
                     nr++;
                 }
                 cur_op = op;
@@ -1152,10 +1189,100 @@ __global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ r
         st = nst;
     }
     if (cur_n && nr < cap) {
-        if (lane == 0) out[nr] = {cur_op, cur_n};
+        out.put(nr, gxy_edit_run{cur_op, cur_n});
         nr++;
     }
-    if (lane == 0) nruns[p] = nr;
+    return nr;
+}
+
+// The walk of side p into runs[run_off[p] ..].
+__global__ __launch_bounds__(64) void k_dp_traceback(const DpRes *__restrict__ res, uint32_t nprob, DpPool pool,
+                                                     const uint64_t *__restrict__ run_off,
+                                                     gxy_edit_run *__restrict__ runs, uint32_t *__restrict__ nruns) {
+    __shared__ __align__(16) TbLds L;
+    const uint32_t p = blockIdx.x;
+    if (p >= nprob) return;
+    const DpRes r = res[p];
+    const uint32_t nr = tb_walk(r, pool, L, RunsOut{runs + run_off[p]});
+    if (threadIdx.x == 0) nruns[p] = nr;
+}
+
+// ---- persistent form of a batch -------------------------------------------
+//
+// One launch of as many one-wavefront blocks as the GPU holds at once.  Each
+// block takes sides from a device queue in list order (the host sorts the
+// list longest first, so the long sweeps start early and the short ones fill
+// the tail), sweeps a side forward into its own run of pool chunks, walks it
+// back at once and appends the side's runs to one array, then takes the next
+// side: no pass boundaries and no host work between sides.  A side whose
+// sweep outgrows the block's chunks is left truncated, like a pass kernel's
+// side that finds the pool empty, and the host sweeps it again in a pass.
+constexpr uint32_t kRunStage = 256;  // runs a block stages in LDS per side
+struct PersistArgs {
+    uint32_t *head;                   // next list index to take
+    uint32_t region;                  // pool chunks per block
+    unsigned long long *run_ctr;      // runs appended so far
+    gxy_edit_run *runs;
+    uint64_t run_cap;
+    uint64_t *run_off;                // per list index: first run
+    uint32_t *nruns;                  //                 run count
+};
+struct StageOut {  // the first kRunStage runs into LDS; the walk counts the rest
+    gxy_edit_run *stage;
+    __device__ __forceinline__ void put(uint32_t k, gxy_edit_run r) const {
+        if (threadIdx.x == 0 && k < kRunStage) stage[k] = r;
+    }
+};
+
+template <class S, int K, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dp_persist(
+    const DpProb *__restrict__ probs, const uint32_t *__restrict__ list, uint32_t nlist, S sc, int32_t oe, int32_t e,
+    int32_t ydrop, DpPool pool, DpRes *__restrict__ res, PersistArgs pa) {
+    // the forward sweep's query cache and the walk's tiles share the LDS
+    union Lds {
+        uint32_t qcache[kQC1 / 4];
+        struct {
+            TbLds tb;
+            gxy_edit_run stage[kRunStage];
+        } w;
+    };
+    __shared__ __align__(16) Lds L;
+    const RegionChunks chunks{pool, blockIdx.x * pa.region, pa.region};
+    for (;;) {
+        uint32_t x = 0;
+        if (threadIdx.x == 0) x = atomicAdd(pa.head, 1u);
+        x = __builtin_amdgcn_readfirstlane(x);
+        if (x >= nlist) break;
+        const DpProb pr = probs[list[x]];
+        DpRes r = fwd1_sweep<S, K>(pr, sc, oe, e, ydrop, pool, chunks, L.qcache);
+        // the rows, windows and links written by the sweep, for the walk's loads
+        __threadfence();
+        __syncthreads();  // the query cache's LDS becomes the walk's
+        uint32_t nr = tb_walk(r, pool, L.w.tb, StageOut{L.w.stage});
+        uint64_t off = 0;
+        if (nr) {
+            unsigned long long o = 0;
+            if (threadIdx.x == 0) o = atomicAdd(pa.run_ctr, (unsigned long long)nr);
+            off = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(o >> 32)) << 32) |
+                  (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)o);
+            if (off + nr > pa.run_cap) {  // no room for the runs: swept again in a pass
+                r.truncated = 1;
+                nr = 0;
+                off = 0;
+            } else if (nr <= kRunStage) {
+                __syncthreads();  // lane 0's staged runs
+                for (uint32_t k = threadIdx.x; k < nr; k += 64) pa.runs[off + k] = L.w.stage[k];
+            } else {  // more runs than the stage holds: walk again into the reserved range
+                tb_walk(r, pool, L.w.tb, RunsOut{pa.runs + off});
+            }
+        }
+        if (threadIdx.x == 0) {
+            res[x] = r;
+            pa.run_off[x] = off;
+            pa.nruns[x] = nr;
+        }
+        __syncthreads();  // the walk's LDS becomes the next sweep's query cache
+    }
 }
 
 // Copy each side's runs from its reserved range to a packed range.
@@ -1250,6 +1377,40 @@ void launch_dp_forward(int variant, unsigned npp, hipStream_t st, const DpProb *
     }
 }
 
+// One-wavefront variants the persistent kernel runs, and their waves per SIMD
+// (0: not run persistently).
+inline int persist_wpe(int variant) {
+    switch (variant) {
+    case GXY_DP_WAVE_K2:
+    case GXY_DP_WAVE_K4:
+        return 3;
+    case GXY_DP_WAVE_K2_W4:
+    case GXY_DP_WAVE_K4_W4:
+        return 4;
+    default:
+        return 0;
+    }
+}
+template <class S>
+void launch_dp_persist(int variant, unsigned grid, hipStream_t st, const DpProb *probs, const uint32_t *list,
+                       uint32_t nlist, S sc, int32_t oe, int32_t e, int32_t ydrop, DpPool pool, DpRes *res,
+                       PersistArgs pa) {
+    switch (variant) {
+    case GXY_DP_WAVE_K2:
+        k_dp_persist<S, 2, 3><<<grid, 64, 0, st>>>(probs, list, nlist, sc, oe, e, ydrop, pool, res, pa);
+        break;
+    case GXY_DP_WAVE_K4:
+        k_dp_persist<S, 4, 3><<<grid, 64, 0, st>>>(probs, list, nlist, sc, oe, e, ydrop, pool, res, pa);
+        break;
+    case GXY_DP_WAVE_K2_W4:
+        k_dp_persist<S, 2, 4><<<grid, 64, 0, st>>>(probs, list, nlist, sc, oe, e, ydrop, pool, res, pa);
+        break;
+    default:
+        k_dp_persist<S, 4, 4><<<grid, 64, 0, st>>>(probs, list, nlist, sc, oe, e, ydrop, pool, res, pa);
+        break;
+    }
+}
+
 // The DP's per-row score table packs the target row and the ambiguity score
 // as signed bytes; other matrices use the int32 row table.
 bool dp_scores_i8(const gxy_scoring &m) { return scoring_is_i8(m) && m.ambig >= -128 && m.ambig <= 127; }
@@ -1279,6 +1440,17 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     const uint64_t n = items.size();
     results->assign(n, GappedResult{false, {}});
     if (n == 0) return GXY_OK;
+    // host phase times, printed when GXY_TRACE is set: setup, persistent
+    // launch (kernel), its results on the host, passes, script assembly,
+    // anchor columns (part of setup)
+    double tr_ms[6] = {};
+    auto tr_t = std::chrono::steady_clock::now();
+    auto lap = [&](int k) {
+        if (!ctx->trace) return;
+        const auto now = std::chrono::steady_clock::now();
+        tr_ms[k] += std::chrono::duration<double, std::milli>(now - tr_t).count();
+        tr_t = now;
+    };
     const int32_t oe = sc.gap_open + sc.gap_extend, e = sc.gap_extend;
     // each item's target and query slice
     auto tlen_of = [](const GappedItem &it) { return it.t_len == kWholeSeq ? it.t->len - it.t_off : it.t_len; };
@@ -1351,10 +1523,12 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     GXY_HIP(ctx, hipMemcpyAsync(col.data(), dcol, n * 4, hipMemcpyDeviceToHost, ctx->stream));
     GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
 
+    lap(5);
     std::vector<DpRes> res(np);
-    // each side's runs: side_runs[side_off[x] .. side_off[x] + res[x]'s count)
-    std::vector<gxy_edit_run> side_runs;
-    std::vector<uint64_t> side_off(np, 0);
+    // each side's runs: side_ptr[x][0 .. side_n[x]), inside one of the run
+    // buffers downloaded by the passes (kept until the scripts are assembled)
+    std::vector<std::vector<gxy_edit_run>> run_store;
+    std::vector<const gxy_edit_run *> side_ptr(np, nullptr);
     std::vector<uint32_t> side_n(np, 0);
     std::vector<uint32_t> pending;
     uint64_t need_rows = 0;
@@ -1372,15 +1546,29 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     // the last good cell, so a contig against a long target does not count
     // the whole rest of the target.  Only pass packing uses it: a side that
     // outruns it is truncated when the pool runs out and swept again.
-    auto est_rows = [&](uint32_t x) {
+    std::vector<uint32_t> est_v(np);  // < 2^32: n < 2^32
+    for (uint64_t x = 0; x < np; x++) {
         const uint64_t m = probs[x].m;
-        return std::min<uint64_t>((uint64_t)probs[x].n + 1, m + m / 8 + 512);
+        est_v[x] = (uint32_t)std::min<uint64_t>((uint64_t)probs[x].n + 1, m + m / 8 + 512);
+    }
+    auto est_rows = [&](uint32_t x) -> uint64_t { return est_v[x]; };
+    // longest first, ties in their order in v: a stable LSD radix sort on the
+    // complemented estimate, two 16-bit digits
+    auto longest_first = [&](std::vector<uint32_t> &v) {
+        std::vector<uint32_t> tmp(v.size()), cnt(65537);
+        for (int pass = 0; pass < 2; pass++) {
+            auto digit = [&](uint32_t x) { return ((~est_v[x]) >> (16 * pass)) & 0xFFFFu; };
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (uint32_t x : v) cnt[digit(x) + 1]++;
+            for (int k = 0; k < 65536; k++) cnt[k + 1] += cnt[k];
+            for (uint32_t x : v) tmp[cnt[digit(x)]++] = x;
+            v.swap(tmp);
+        }
     };
     if (!pending.empty()) {
         // longest sides first, so the long sweeps start early instead of
         // forming the tail of a pass
-        std::stable_sort(pending.begin(), pending.end(),
-                         [&](uint32_t a, uint32_t b) { return est_rows(a) > est_rows(b); });
+        longest_first(pending);
         // The traceback pool: up to a third of the device (at most 96 GiB),
         // no more than every side could use.  Its size bounds how many long
         // sweeps run at once (rows in flight x kRowBytes).
@@ -1412,7 +1600,9 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         auto est = [&](uint32_t x) {
             return std::min<uint64_t>((est_rows(x) + kChunkRows - 1) / kChunkRows + 1, est_cap);
         };
+        lap(0);
         bool alone = false;  // run only the first pending side (progress guarantee)
+        bool persist_done = false;  // at most one persistent launch per call
         // pending[head ..] waits; a pass takes its sides from the head, and the
         // sides it truncates go back just before the new head (first in line)
         size_t head = 0;
@@ -1430,6 +1620,15 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             int variant = ctx->dp_kernel;
             gxy_ctx::DpTune *tune = nullptr;
             if (!variant) {
+                // A pass whose longest side holds more rows than an even share of
+                // the pass's rows over the one-wavefront slots the GPU runs at
+                // once is bound by that side's row latency, which the
+                // four-wavefront kernel halves: no probe, no one-wavefront kernel.
+                uint64_t rows_sum = 0;
+                for (uint64_t y = 0; y < npp; y++) rows_sum += est_rows(pending[head + y]);
+                if (est_rows(pending[head]) * (uint64_t)(ctx_cus(ctx) * 16) >= rows_sum) variant = GXY_DP_BLOCK4;
+            }
+            if (!variant) {
                 tune = &ctx->dp_tune[dp_class(npp)];
                 if (tune->chosen) {
                     variant = tune->chosen;
@@ -1437,6 +1636,111 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                 } else {
                     variant = kDpCand[tune->tried];
                     if (npp > 2 * kDpProbeSides) npp = kDpProbeSides;
+                }
+            }
+            // Once the kernel is settled on a one-wavefront variant, the rest of a
+            // large batch (or any batch, when the caller asks) runs as one
+            // persistent launch; what it leaves goes on through passes.
+            if (!tune && !persist_done && persist_wpe(variant) && ctx->dp_persist >= 0 &&
+                (ctx->dp_persist > 0 || pending.size() - head >= kPersistMinSides)) {
+                persist_done = true;
+                const uint64_t slots = (uint64_t)ctx_cus(ctx) * 4 * persist_wpe(variant);
+                uint64_t grid = std::min<uint64_t>(slots, pending.size() - head);
+                uint64_t region = nchunks / grid;
+                if (region < 2) {  // a small pool: fewer blocks, two chunks each
+                    region = 2;
+                    grid = nchunks / 2;
+                }
+                // sides expected to fit a block's chunks; the others stay for passes
+                std::vector<uint32_t> plist, rest;
+                uint64_t prow = 0;
+                for (size_t y = head; y < pending.size(); y++) {
+                    const uint32_t x = pending[y];
+                    if (est_rows(x) + kChunkRows <= region * kChunkRows) {
+                        plist.push_back(x);
+                        prow += est_rows(x);
+                    } else {
+                        rest.push_back(x);
+                    }
+                }
+                if (!plist.empty()) {
+                    const uint64_t pn = plist.size();
+                    grid = std::min<uint64_t>(grid, pn);
+                    // results | run offsets | run counts | side list | counters
+                    const size_t q_res = 0, q_roff = q_res + pn * sizeof(DpRes), q_nr = q_roff + pn * 8,
+                                 q_list = q_nr + pn * 4, q_ctr = (q_list + pn * 4 + 255) & ~255ull;
+                    auto *pw = (uint8_t *)scratch(ctx, kSlotDpPersist, q_ctr + 256);
+                    // room for the runs: a side's runs are far fewer than its rows
+                    uint64_t run_cap = std::max<uint64_t>(prow / 16, 1u << 16);
+                    gxy_edit_run *pruns = nullptr;
+                    while (!(pruns = (gxy_edit_run *)scratch(ctx, kSlotDpRuns, run_cap * sizeof(gxy_edit_run))) &&
+                           run_cap > (1u << 16))
+                        run_cap /= 2;
+                    if (!pw || !pruns) return set_err(ctx, GXY_ENOMEM, "DP scratch allocation failed");
+                    auto *dres = (DpRes *)(pw + q_res);
+                    PersistArgs pa;
+                    pa.head = (uint32_t *)(pw + q_ctr);
+                    pa.region = (uint32_t)region;
+                    pa.run_ctr = (unsigned long long *)(pw + q_ctr + 8);
+                    pa.runs = pruns;
+                    pa.run_cap = run_cap;
+                    pa.run_off = (uint64_t *)(pw + q_roff);
+                    pa.nruns = (uint32_t *)(pw + q_nr);
+                    auto *dlist = (uint32_t *)(pw + q_list);
+                    GXY_HIP(ctx, hipMemcpyAsync(dlist, plist.data(), pn * 4, hipMemcpyHostToDevice, ctx->stream));
+                    GXY_HIP(ctx, hipMemsetAsync(pw + q_ctr, 0, 16, ctx->stream));
+                    {
+                        KernelTimer kt_(ctx, "dp_persist");
+                        if (dp_scores_i8(sc))
+                            launch_dp_persist(variant, (unsigned)grid, ctx->stream, dprob, dlist, (uint32_t)pn,
+                                              make_scorer_i8(sc), oe, e, gp.y_drop, pool, dres, pa);
+                        else
+                            launch_dp_persist(variant, (unsigned)grid, ctx->stream, dprob, dlist, (uint32_t)pn,
+                                              make_scorer_i32(sc), oe, e, gp.y_drop, pool, dres, pa);
+                    }
+                    GXY_LAUNCH_CHECK(ctx);
+                    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                    lap(1);
+                    std::vector<DpRes> pr(pn);
+                    std::vector<uint64_t> proff(pn);
+                    std::vector<uint32_t> pnr(pn);
+                    unsigned long long nruns_all = 0;
+                    GXY_HIP(ctx, hipMemcpyAsync(pr.data(), dres, pn * sizeof(DpRes), hipMemcpyDeviceToHost, ctx->stream));
+                    GXY_HIP(ctx, hipMemcpyAsync(proff.data(), pa.run_off, pn * 8, hipMemcpyDeviceToHost, ctx->stream));
+                    GXY_HIP(ctx, hipMemcpyAsync(pnr.data(), pa.nruns, pn * 4, hipMemcpyDeviceToHost, ctx->stream));
+                    GXY_HIP(ctx, hipMemcpyAsync(&nruns_all, pa.run_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
+                    GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                    const uint64_t got = std::min<uint64_t>(nruns_all, run_cap);
+                    std::vector<gxy_edit_run> prun(got);
+                    if (got) {
+                        GXY_HIP(ctx, hipMemcpyAsync(prun.data(), pruns, got * sizeof(gxy_edit_run),
+                                                    hipMemcpyDeviceToHost, ctx->stream));
+                        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                    }
+                    ctx->dp_persist_launches++;
+                    ctx->dp_persist_sides += pn;
+                    for (uint64_t y = 0; y < pn; y++) {
+                        const uint32_t x = plist[y];
+                        if (pr[y].truncated) {  // outgrew its block's chunks, or no room for its runs
+                            rest.push_back(x);
+                            ctx->dp_persist_deferred++;
+                            continue;
+                        }
+                        if (pnr[y] && proff[y] + pnr[y] > got)
+                            return set_err(ctx, GXY_EDEVICE, "persistent DP: runs outside the run array");
+                        cells += pr[y].cells;
+                        ctx->dp_persist_cells += pr[y].cells;
+                        res[x] = pr[y];
+                        side_ptr[x] = prun.data() + proff[y];
+                        side_n[x] = pnr[y];
+                    }
+                    run_store.push_back(std::move(prun));
+                    // the rest (longest first) goes through passes
+                    longest_first(rest);
+                    pending.swap(rest);
+                    head = 0;
+                    lap(2);
+                    continue;
                 }
             }
             // per-pass arrays: results | run offsets | packed offsets | run counts | side list | counter
@@ -1538,10 +1842,10 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
                 }
                 cells += pr[y].cells;
                 res[x] = pr[y];
-                side_off[x] = side_runs.size();
+                side_ptr[x] = sr.data() + poff[y];
                 side_n[x] = nr[y];
-                side_runs.insert(side_runs.end(), sr.begin() + poff[y], sr.begin() + poff[y] + nr[y]);
             }
+            run_store.push_back(std::move(sr));
             if (done == 0) {
                 // nothing finished: the pool is shared by too many long sweeps, or
                 // one sweep alone needs more than the pool
@@ -1557,41 +1861,89 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             std::copy(again.begin(), again.end(), pending.begin() + head);
         }
     }
+    lap(3);
     if (cells_out) *cells_out += cells;
 
     // Assemble (src/gapped_extend.rs:107-133): score = col + L + R (saturating);
     // script = rev(L.script) + M1 + R.script.  The walk emits tip-to-root runs,
-    // so the left side contributes its raw order and the right side reversed.
-    for (uint64_t k = 0; k < n; k++) {
-        const DpRes &R = res[2 * k], &L = res[2 * k + 1];
-        int64_t t64 = (int64_t)col[k] + L.score;
-        t64 = std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, t64));
-        t64 = std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, t64 + R.score));
-        const int32_t tot = (int32_t)t64;
-        if (tot < gp.gapped_threshold) continue;
-        GappedResult &g = (*results)[k];
-        g.found = true;
-        g.aln.score = tot;
-        g.aln.t_start = items[k].a.t_pos - L.bi;
-        g.aln.q_start = items[k].a.q_pos - L.bj;
-        g.aln.t_len = L.bi + 1 + R.bi;
-        g.aln.q_len = L.bj + 1 + R.bj;
-        g.aln.source = items[k].a;
-        g.aln.runs_offset = runs->size();
-        auto push = [&](uint32_t op, uint32_t cnt) {
-            if (cnt == 0) return;
-            if (runs->size() > g.aln.runs_offset && runs->back().op == op)
-                runs->back().n += cnt;
-            else
-                runs->push_back({op, cnt});
-        };
-        const gxy_edit_run *lr = side_runs.data() + side_off[2 * k + 1];
-        for (uint32_t x = 0; x < side_n[2 * k + 1]; x++) push(lr[x].op, lr[x].n);
-        push(GXY_OP_MATCH, 1);
-        const gxy_edit_run *rr = side_runs.data() + side_off[2 * k];
-        for (uint32_t x = side_n[2 * k]; x-- > 0;) push(rr[x].op, rr[x].n);
-        g.aln.n_runs = (uint32_t)(runs->size() - g.aln.runs_offset);
-    }
+    // so the left side contributes its raw order and the right side reversed;
+    // equal neighbouring ops merge (EditScript::push, src/edit_script.rs:65-74).
+    // Items are independent: one pass counts each item's merged runs, a
+    // prefix sum places them, a second pass writes them, both over threads.
+    std::vector<uint64_t> item_off(n + 1, 0);
+    auto for_items = [&](auto &&fn) {
+        const uint64_t nt = std::min<uint64_t>(std::max(1u, std::thread::hardware_concurrency()),
+                                               std::max<uint64_t>(1, n / 4096));
+        if (nt <= 1) {
+            fn(0, n);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (uint64_t w = 0; w < nt; w++) th.emplace_back(fn, n * w / nt, n * (w + 1) / nt);
+        for (auto &t : th) t.join();
+    };
+    std::vector<int32_t> total(n);
+    // runs of item k in order, handed to `sink(op, n)` before merging
+    auto item_runs = [&](uint64_t k, auto &&sink) {
+        const gxy_edit_run *lr = side_ptr[2 * k + 1];
+        for (uint32_t x = 0; x < side_n[2 * k + 1]; x++) sink(lr[x].op, lr[x].n);
+        sink(GXY_OP_MATCH, 1);
+        const gxy_edit_run *rr = side_ptr[2 * k];
+        for (uint32_t x = side_n[2 * k]; x-- > 0;) sink(rr[x].op, rr[x].n);
+    };
+    for_items([&](uint64_t a, uint64_t b) {
+        for (uint64_t k = a; k < b; k++) {
+            const DpRes &R = res[2 * k], &L = res[2 * k + 1];
+            int64_t t64 = (int64_t)col[k] + L.score;
+            t64 = std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, t64));
+            t64 = std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, t64 + R.score));
+            total[k] = (int32_t)t64;
+            uint64_t cnt = 0;
+            if (total[k] >= gp.gapped_threshold) {
+                uint32_t last = 0xFFFFFFFFu;
+                item_runs(k, [&](uint32_t op, uint32_t c) {
+                    if (c == 0) return;
+                    if (op != last) cnt++;
+                    last = op;
+                });
+            }
+            item_off[k + 1] = cnt;
+        }
+    });
+    for (uint64_t k = 0; k < n; k++) item_off[k + 1] += item_off[k];
+    const uint64_t base = runs->size();
+    runs->resize(base + item_off[n]);
+    gxy_edit_run *dst = runs->data() + base;
+    for_items([&](uint64_t a, uint64_t b) {
+        for (uint64_t k = a; k < b; k++) {
+            if (total[k] < gp.gapped_threshold) continue;
+            const DpRes &R = res[2 * k], &L = res[2 * k + 1];
+            GappedResult &g = (*results)[k];
+            g.found = true;
+            g.aln.score = total[k];
+            g.aln.t_start = items[k].a.t_pos - L.bi;
+            g.aln.q_start = items[k].a.q_pos - L.bj;
+            g.aln.t_len = L.bi + 1 + R.bi;
+            g.aln.q_len = L.bj + 1 + R.bj;
+            g.aln.source = items[k].a;
+            g.aln.runs_offset = base + item_off[k];
+            gxy_edit_run *o = dst + item_off[k];
+            uint64_t w = 0;
+            item_runs(k, [&](uint32_t op, uint32_t c) {
+                if (c == 0) return;
+                if (w > 0 && o[w - 1].op == op)
+                    o[w - 1].n += c;
+                else
+                    o[w++] = {op, c};
+            });
+            g.aln.n_runs = (uint32_t)w;
+        }
+    });
+    lap(4);
+    if (ctx->trace)
+        std::fprintf(stderr, "[gxy dp] %llu items: setup %.1f (anchor columns %.1f) persistent %.1f "
+                             "persistent-host %.1f passes %.1f assemble %.1f ms\n",
+                     (unsigned long long)n, tr_ms[0] + tr_ms[5], tr_ms[5], tr_ms[1], tr_ms[2], tr_ms[3], tr_ms[4]);
     return GXY_OK;
 }
 

@@ -511,6 +511,25 @@ int gxy_ctx_dp_pool_stats(gxy_ctx *ctx, uint64_t *passes, uint64_t *truncated, u
     });
 }
 
+int gxy_ctx_set_dp_persist(gxy_ctx *ctx, int mode) {
+    if (!ctx) return GXY_EINVAL;
+    if (mode < -1 || mode > 1) return set_err(ctx, GXY_EINVAL, "unknown persistent-DP mode %d", mode);
+    ctx->dp_persist = mode;
+    return GXY_OK;
+}
+
+int gxy_ctx_dp_persist_stats(gxy_ctx *ctx, uint64_t *launches, uint64_t *sides, uint64_t *deferred,
+                             uint64_t *cells) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !launches || !sides || !deferred || !cells) return GXY_EINVAL;
+        *launches = ctx->dp_persist_launches;
+        *sides = ctx->dp_persist_sides;
+        *deferred = ctx->dp_persist_deferred;
+        *cells = ctx->dp_persist_cells;
+        return GXY_OK;
+    });
+}
+
 int gxy_ctx_kernel_time(gxy_ctx *ctx, const char *name, double *ms, uint64_t *launches) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !name || !ms || !launches) return GXY_EINVAL;

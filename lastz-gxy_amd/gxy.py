@@ -37,6 +37,8 @@ _SIGS = {
     "gxy_ctx_set_dp_kernel": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_dp_autotune_state": (C.c_int, [_vp, _P(C.c_int), _P(C.c_double)]),
     "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
+    "gxy_ctx_set_dp_persist": (C.c_int, [_vp, C.c_int]),
+    "gxy_ctx_dp_persist_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
     "gxy_ctx_reset_timing": (C.c_int, [_vp]),
     "gxy_pack_ascii": (C.c_int, [C.c_char_p, C.c_uint64, _vp, _vp, _vp]),
@@ -178,6 +180,17 @@ class Context:
         """(passes, truncated sides, passes run with one side alone) since creation."""
         v = [C.c_uint64() for _ in range(3)]
         self.check(lib().gxy_ctx_dp_pool_stats(self._h, *[C.byref(x) for x in v]), "gxy_ctx_dp_pool_stats")
+        return tuple(x.value for x in v)
+
+    def set_dp_persist(self, mode: int):
+        """Persistent DP launches: 0 for large batches (default), 1 always, -1 never."""
+        self.check(lib().gxy_ctx_set_dp_persist(self._h, mode), "gxy_ctx_set_dp_persist")
+
+    def dp_persist_stats(self):
+        """(persistent launches, sides given to them, sides left to passes, DP
+        cells of the sides they finished) since creation."""
+        v = [C.c_uint64() for _ in range(4)]
+        self.check(lib().gxy_ctx_dp_persist_stats(self._h, *[C.byref(x) for x in v]), "gxy_ctx_dp_persist_stats")
         return tuple(x.value for x in v)
 
     def reset_timing(self):

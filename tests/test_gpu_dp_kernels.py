@@ -155,21 +155,45 @@ def test_slices_equal_pairs_and_oracle(ctx):
 def test_autotuner_probes_then_keeps_the_fastest():
     """With no kernel forced, a context measures every candidate on its first
     passes of a size class (one pass each), keeps the fastest, and every pass —
-    probe or not — gives the oracle's alignments.  Passes of at most 8 sides
-    use the four-wavefront kernel without probing."""
+    probe or not — gives the oracle's alignments.  A pass bound by its longest
+    side's row latency (fewer rows in all than that side's rows times the
+    one-wavefront slots of the GPU) uses the four-wavefront kernel without
+    probing: 80 sides of up to 800 rows here."""
     ctx = gxy.Context(0)
     rng = np.random.default_rng(105)
     hox = T.scoring_hoxd70()
-    gp = T.GappedParams(9400, 1000)
-    cases = pairs(rng, [int(x) for x in rng.integers(50, 800, 40)])  # 80 sides: class 1
+    gp = T.GappedParams(9400, 300)
+    few = pairs(rng, [int(x) for x in rng.integers(50, 800, 40)])  # 80 long-ish sides
+    got, cells = run_all(ctx, few, hox, gp)
+    exp, ecells = oracle_all(few, hox, gp)
+    assert cells == ecells and got == exp
+    assert all(c == (None, {}) for c in ctx.dp_autotune_state()[1:])
+    # 8400 sides of 40-60 rows: rows enough to keep every slot busy (class 3)
+    cases = pairs(rng, [int(x) for x in rng.integers(40, 60, 4200)])
+    t, q, to, tl, qo, ql, anc = slices(cases)
+    dt, dq = gxy.DeviceSeq.upload(t, ctx), gxy.DeviceSeq.upload(q, ctx)
     exp, ecells = oracle_all(cases, hox, gp)
     n_cand = len(gxy.Context.DP_CANDIDATES)
     for k in range(n_cand + 1):
-        got, cells = run_all(ctx, cases, hox, gp)
+        got, cells = gxy.gapped_extend_slices_dev(dt, dq, to, tl, qo, ql, anc, hox, gp, ctx)
         assert cells == ecells and got == exp, k
     state = ctx.dp_autotune_state()
     assert state[0][0] == "block4" and state[0][1] == {}
-    chosen, rates = state[1]
+    chosen, rates = state[3]
     assert set(rates) == set(gxy.Context.DP_CANDIDATES) and all(v > 0 for v in rates.values())
     assert chosen == max(rates, key=rates.get)
     ctx.close()
+
+
+def slices(cases):
+    """Pairs laid end to end in two host sequences: (t, q, t_off, t_len, q_off, q_len, anchors)."""
+    tu = [t.unpack() for t, _, _ in cases]
+    qu = [q.unpack() for _, q, _ in cases]
+    tl = [len(u[0]) for u in tu]
+    ql = [len(u[0]) for u in qu]
+    cat = lambda us, k: np.concatenate([u[k] for u in us])
+    t = T.PackedSeq.from_arrays(cat(tu, 0), cat(tu, 1), np.zeros(sum(tl), bool))
+    q = T.PackedSeq.from_arrays(cat(qu, 0), cat(qu, 1), np.zeros(sum(ql), bool))
+    to = np.concatenate([[0], np.cumsum(tl)[:-1]])
+    qo = np.concatenate([[0], np.cumsum(ql)[:-1]])
+    return t, q, to, tl, qo, ql, [a for _, _, a in cases]
