@@ -599,7 +599,7 @@ def pair_leg(args, workload, steps, warmup):
         log(f"warmup {w}: {time.time() - t1:.3f}s, {len(recs)} records")
 
     names = ["table_items", "table_sort", "table_starts", "table_bytes", "qcount", "qscan", "qexpand", "bin",
-             "extend", "pass_sort", "admit", "emit", "dp_forward", "dp_traceback"]
+             "extend", "extend_cont", "pass_sort", "admit", "emit", "dp_forward", "dp_traceback"]
     ctx.reset_timing()
     ctx.set_timing(True)
     ctx.synchronize()
@@ -743,6 +743,9 @@ def main():
                     help="DP forward kernel (gxy_ctx_set_dp_kernel: 0 autotuned, 1 four wavefronts per side, "
                          "2/3/4 one wavefront per side with 1/2/4 columns per lane per slot, 5/6 = 3/4 at four "
                          "wavefronts per SIMD)")
+    ap.add_argument("--xdrop-form", type=int, default=0,
+                    help="x-drop form of the search (gxy_ctx_set_xdrop_form: 0 first step + resumed walks, "
+                         "1 one persistent kernel)")
     ap.add_argument("--pair-steps", type=int, default=5)
     ap.add_argument("--target-bp", type=int, default=48_000_000)
     ap.add_argument("--cpu-sample-bp", type=int, default=16_000_000)
@@ -759,6 +762,7 @@ def main():
 
     import gxy
     gxy.default_context().set_dp_kernel(args.dp_kernel)
+    gxy.default_context().set_xdrop_form(args.xdrop_form)
     head = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "int32", "data": "synthetic"}
 

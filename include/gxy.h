@@ -232,6 +232,16 @@ int gxy_ctx_set_dp_persist(gxy_ctx *ctx, int mode);
  * sides they left to passes and the DP cells of the sides they finished. */
 int gxy_ctx_dp_persist_stats(gxy_ctx *ctx, uint64_t *launches, uint64_t *sides, uint64_t *deferred,
                              uint64_t *cells);
+/* Form of the search's x-drop extension (no counterpart in the reference,
+ * which extends one hit at a time: src/seed_search.rs:98-112).  0 (the
+ * default): a straight-line first step takes the seed and the first 32
+ * columns of both walks of every hit, and a persistent kernel resumes the
+ * hits whose walks go on; 1: the persistent kernel alone, one 32-column chunk
+ * of one walk per lane per step.  Results do not depend on it. */
+int gxy_ctx_set_xdrop_form(gxy_ctx *ctx, int form);
+/* x-drop columns evaluated by first steps (form 0) since the context was
+ * created, the minus-strand helper's included. */
+int gxy_ctx_xdrop_stats(gxy_ctx *ctx, uint64_t *first_step_columns);
 
 /* ---- sequences ---------------------------------------------------------- */
 /* PackedSeq::from_ascii (src/sequences.rs:164-206) into caller buffers sized

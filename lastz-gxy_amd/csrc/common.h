@@ -397,6 +397,10 @@ struct gxy_ctx {
     int dp_persist = 0;
     uint64_t dp_persist_launches = 0, dp_persist_sides = 0, dp_persist_deferred = 0, dp_persist_cells = 0;
     int cus = 0;  // compute units of the device (ctx_cus)
+    // x-drop in one persistent kernel per hit batch instead of a first step
+    // for every hit and a kernel resuming the longer walks (for measurement)
+    bool xdrop_single = false;
+    uint64_t xd_first_cols = 0;  // x-drop columns evaluated by first steps since creation
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_used;
     // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.
@@ -524,6 +528,8 @@ enum Slot {
     kSlotBins,
     kSlotBinOff,
     kSlotDpPersist,
+    kSlotXCont,
+    kSlotXContCnt,
 };
 
 }  // namespace gxy

@@ -823,7 +823,10 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 ctx->helper = nullptr;
             }
             helper = ctx->helper;
-            if (helper) helper->timing = ctx->timing;
+            if (helper) {
+                helper->timing = ctx->timing;
+                helper->xdrop_single = ctx->xdrop_single;
+            }
         }
         HostJobs helper_job(helper ? 1 : 0);
 
@@ -937,6 +940,10 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             int rc = tweener_groups_dev(ctx, tween_groups, cfg, &run_store, &rs);
             if (rc) return rc;
             for (auto &tg : tween_groups) finalize_group(*tg.g);
+        }
+        if (helper) {  // the helper's x-drop counts as this context's
+            ctx->xd_first_cols += helper->xd_first_cols;
+            helper->xd_first_cols = 0;
         }
         if (helper && helper->timing) {  // the helper's kernel times count as this context's
             timing_flush(helper);

@@ -578,6 +578,7 @@ struct XdropArgs {
 // Hit sources: next(t, q) yields the lane's next seed hit, `cur` its index
 // where results are written per hit.
 struct HitArraySrc {  // (q << 32 | t) records
+    static constexpr bool kResume = false;
     const uint64_t *hits;
     uint64_t n, h, stride, cur;
     __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim, uint64_t *) {
@@ -596,6 +597,7 @@ struct HitArraySrc {  // (q << 32 | t) records
 };
 
 struct PairSrc {  // separate t and q position arrays
+    static constexpr bool kResume = false;
     const uint32_t *tpos, *qpos;
     uint64_t n, h, stride, cur;
     __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim, uint64_t *) {
@@ -620,6 +622,7 @@ struct PairSrc {  // separate t and q position arrays
 // wavefront holds at any time are neighbours in query order and its query
 // reads share cache lines.
 struct BinnedHitSrc {
+    static constexpr bool kResume = false;
     const uint64_t *hits;
     const uint32_t *binoff;  // bin k starts at binoff[k * nblk_exp]
     uint32_t nblk_exp;
@@ -650,6 +653,86 @@ struct BinnedHitSrc {
     }
 };
 
+// A hit whose walks are not both over after their first 32 columns
+// (k_xdrop_first): its seed score and each side's state after that chunk,
+// resumed by k_xdrop through ContSrc.
+struct XdropCont {
+    uint32_t t, q;
+    int32_t seed;
+    int32_t rrun, rbest, lrun, lbest;
+    uint32_t st;  // right extent (bits 0-7), left extent (8-15), kContREnded, kContLEnded
+};
+constexpr uint32_t kContREnded = 1u << 16, kContLEnded = 1u << 17;
+
+// Continuation records, laid out by the first step's wavefronts: wavefront
+// w of the first step writes its records from cont_base(w) on and its count
+// to counts[w]; wavefront w of the resuming launch (same grid, so the same
+// XCD and the same target bin) takes exactly those, in order, handing them
+// to its lanes through an LDS head as BinnedHitSrc does.
+struct ContLayout {
+    const uint32_t *binoff;  // bin k's hits start at binoff[k * nblk_exp]; nullptr: not binned
+    uint32_t nblk_exp;
+    uint64_t nh;
+    // first record of wavefront `wv` of block `blk` in a grid of `nblk`
+    // blocks of 256: the wavefront's hit groups of 64 (one per grid-stride
+    // step over its bin) bound its records; bin k's region is padded by one
+    // grid-stride of hits, so regions never meet
+    __device__ __forceinline__ uint64_t base(uint32_t blk, uint32_t nblk, uint32_t wv, uint64_t *lo_out,
+                                             uint64_t *hi_out) const {
+        uint64_t lo = 0, hi = nh, pad = 0, j, J;
+        if (binoff) {
+            const uint32_t bin = blk % kBins;
+            lo = binoff[bin * nblk_exp];
+            hi = bin + 1 < kBins ? binoff[(bin + 1) * nblk_exp] : nh;
+            J = (uint64_t)(nblk / kBins) * 4;
+            j = (uint64_t)(blk / kBins) * 4 + wv;
+            pad = (uint64_t)bin * J * 64;
+        } else {
+            J = (uint64_t)nblk * 4;
+            j = (uint64_t)blk * 4 + wv;
+        }
+        const uint64_t K = (hi - lo + J * 64 - 1) / (J * 64);  // grid-stride steps over the bin
+        *lo_out = lo;
+        *hi_out = hi;
+        return lo + pad + j * K * 64;
+    }
+};
+// records the layout needs for nh hits in a grid of nblk blocks
+inline uint64_t cont_capacity(uint64_t nh, uint32_t nblk) { return nh + (uint64_t)nblk * 256; }
+
+struct ContSrc {
+    static constexpr bool kResume = true;
+    const XdropCont *recs;
+    ContLayout lay;
+    const uint32_t *counts;  // per wavefront of the grid
+    uint64_t end, cur;
+    volatile uint64_t *head;  // this wavefront's next record (LDS)
+    XdropCont rec;
+    __device__ void init(uint32_t blk, uint32_t nblk, uint32_t tid, uint32_t bdim, uint64_t *lds) {
+        uint64_t lo, hi;
+        const uint32_t wv = tid / 64;
+        const uint64_t b = lay.base(blk, nblk, wv, &lo, &hi);
+        end = b + counts[(uint64_t)blk * (bdim / 64) + wv];
+        head = lds;
+        if ((tid & 63) == 0) *head = b;
+    }
+    // Every lane that needs a record must make this call together.
+    __device__ __forceinline__ bool next(uint32_t &t, uint32_t &q) {
+        const uint64_t asking = __ballot(1);
+        const int lane = threadIdx.x & 63;
+        const uint32_t rank = (uint32_t)__popcll(asking & ((1ull << lane) - 1ull));
+        const uint64_t h0 = *head;
+        if (rank == 0) *head = h0 + (uint64_t)__popcll(asking);
+        const uint64_t h = h0 + rank;
+        if (h >= end) return false;
+        rec = recs[h];
+        t = rec.t;
+        q = rec.q;
+        cur = h;
+        return true;
+    }
+};
+
 template <class Src, bool kAppend>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop(
     XdropArgs a, Src src, PassOut po, gxy_hsp *__restrict__ out, uint8_t *__restrict__ found,
@@ -668,6 +751,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     // seed span in score words; bytes of the last word inside the span
     const uint32_t nsw = (a.span + 3) >> 2;
     const uint32_t lastmask = (a.span & 3) ? ((1u << (8 * (a.span & 3))) - 1u) : 0xFFFFFFFFu;
+    // a resumed hit's left side after its first chunk (k_xdrop_first)
+    int32_t lrun = 0, lbest = 0;
+    uint32_t lext = 0;
+    bool lres = false, lend = false;
 
     auto begin_left = [&]() {  // left walk from (t-1, q-1), src/hsp.rs:129
         phase = kPhLeft;
@@ -676,10 +763,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         rem = min((uint64_t)tp, (uint64_t)qp);
         run = best = 0;
         bext = base = 0;
+        if (lres) {  // go on from the state after the first 32 columns
+            lres = false;
+            best = lbest;
+            bext = lext;
+            if (lend) {
+                rem = 0;
+            } else {
+                pt -= 32;
+                pq -= 32;
+                rem -= 32;
+                run = lrun;
+                base = 32;
+            }
+        }
     };
     auto fetch = [&]() {
         phase = kPhDone;
         while (src.next(tp, qp)) {
+            if constexpr (Src::kResume) {
+                const XdropCont &c = src.rec;
+                seed = c.seed;
+                lres = true;
+                lend = (c.st & kContLEnded) != 0;
+                lrun = c.lrun;
+                lbest = c.lbest;
+                lext = (c.st >> 8) & 0xFFu;
+                if (c.st & kContREnded) {  // only the left walk goes on
+                    rbest = c.rbest;
+                    rext = c.st & 0xFFu;
+                    begin_left();
+                } else {  // the right walk goes on from column 32 past the seed
+                    phase = kPhRight;
+                    pt = (uint64_t)tp + a.span + 32;
+                    pq = (uint64_t)qp + a.span + 32;
+                    rem = min(a.tlen - pt, a.qlen - pq);
+                    run = c.rrun;
+                    best = c.rbest;
+                    bext = c.st & 0xFFu;
+                    base = 32;
+                }
+                return;
+            }
             if ((uint64_t)tp + a.span <= a.tlen && (uint64_t)qp + a.span <= a.qlen) {  // src/hsp.rs:113-115
                 // seed score (src/hsp.rs:118-124); |sum| <= 32*128, no saturation
                 uint32_t tw[8], qw[8];
@@ -809,11 +934,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                     rbest = best;
                     rext = bext;
                     begin_left();
-                    if (rem == 0) {  // no left columns at all
-                        best = 0;
-                        bext = 0;
-                        finished = true;
-                    }
+                    // no left columns (left): best and bext are its result
+                    if (rem == 0) finished = true;
                 } else {
                     finished = true;
                 }
@@ -854,6 +976,153 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     unsigned long long c = cols;
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
     if (lane == 0 && c) atomicAdd(colsum, c);
+}
+
+// ---- first step of every hit, straight-line --------------------------------
+//
+// Most seed hits are chance matches whose walks both stop within ~21 columns
+// (HOXD70 averages -43.4 per random column against an x-drop of 910): in
+// 82 % of them both walks end inside their first 32 columns.  This kernel
+// gives every lane one hit at a time and does, with no per-lane state
+// machine: the seed score, the first 32 columns right of the seed and the 32
+// left of it (the same byte loads and score words as k_xdrop), both scans
+// side by side.  A hit whose walks are both over is finished here (passing
+// ones appended as in k_xdrop); the others are written as XdropCont records
+// for k_xdrop to resume.  Needs x_drop < 2^29 (plain adds are exact then).
+// The 32-column scan of k_xdrop's fast form from (run, best): kbest is the
+// 1-based column of the last improvement (0: none), kstop the first stopping
+// column (32: none); run is pinned after a stop.
+__device__ __forceinline__ void scan32(const uint32_t sw[8], int32_t xdrop, int32_t &run, int32_t &best,
+                                       uint32_t &kbest, uint32_t &kstop) {
+    constexpr int32_t kPinned = -(1 << 30);
+    uint32_t nst = 0;
+    kbest = 0;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+        const int32_t sc = (int32_t)(int8_t)(sw[k >> 2] >> (8 * (k & 3)));
+        const int32_t r = run + sc;
+        kbest = r > best ? (uint32_t)(k + 1) : kbest;
+        best = max(best, r);
+        const bool stop = best - r > xdrop;
+        nst += stop ? 1u : 0u;
+        run = stop ? kPinned : r;
+    }
+    kstop = 32 - nst;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop_first(
+    XdropArgs a, const uint64_t *__restrict__ hits, ContLayout lay, PassOut po, XdropCont *__restrict__ cont,
+    uint32_t *__restrict__ cont_counts, unsigned long long *__restrict__ colsum) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wv = threadIdx.x >> 6;
+    // this block's hits: its bin's (block b walks bin b % 8 with the other
+    // blocks of its XCD, 256 consecutive hits per block and grid-stride step)
+    // or all of them; the wavefront's continuation records from cbase on
+    uint64_t lo, hi;
+    const uint64_t cbase = lay.base(blockIdx.x, gridDim.x, wv, &lo, &hi);
+    const uint64_t nbin = lay.binoff ? gridDim.x / kBins : gridDim.x;
+    uint64_t h = lo + (uint64_t)(lay.binoff ? blockIdx.x / kBins : blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t stride = nbin * 256;
+    uint32_t ncont = 0;  // this wavefront's records so far (uniform)
+    const uint32_t nsw = (a.span + 3) >> 2;
+    const uint32_t lastmask = (a.span & 3) ? ((1u << (8 * (a.span & 3))) - 1u) : 0xFFFFFFFFu;
+    uint64_t cols = 0;
+    for (; h < hi; h += stride) {
+        const uint64_t x = hits[h];
+        const uint32_t q = (uint32_t)(x >> 32), t = (uint32_t)x;
+        bool pass = false, more = false;
+        gxy_hsp res = {0, 0, 0, 0};
+        XdropCont c;
+        if ((uint64_t)t + a.span <= a.tlen && (uint64_t)q + a.span <= a.qlen) {  // src/hsp.rs:113-115
+            // bytes [t - 32, t + 32): the left chunk, then the seed
+            uint32_t tw[16], qw[16], tr[8], qr[8];
+            load_bytes32(a.tf, (int64_t)t - 32, tw);
+            load_bytes32(a.tf, (int64_t)t, tw + 8);
+            load_bytes32(a.qf, (int64_t)q - 32, qw);
+            load_bytes32(a.qf, (int64_t)q, qw + 8);
+            // the right chunk: 32 bytes from the seed end
+            load_bytes32(a.tf, (int64_t)t + a.span, tr);
+            load_bytes32(a.qf, (int64_t)q + a.span, qr);
+            int32_t seed = 0;  // src/hsp.rs:118-124; |sum| <= 32 * 128
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                if ((uint32_t)w < nsw) {
+                    uint32_t sw = score4(a.lut, tw[8 + w], qw[8 + w]);
+                    if ((uint32_t)w == nsw - 1) sw &= lastmask;
+                    seed = __builtin_amdgcn_sdot4((int)sw, 0x01010101, seed, false);
+                }
+            }
+            uint32_t rs[8], ls[8];
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                rs[w] = score4(a.lut, tr[w], qr[w]);
+                // column k of the left walk is byte 31 - k of [t - 32, t)
+                ls[w] = __builtin_bswap32(score4(a.lut, tw[7 - w], qw[7 - w]));
+            }
+            // columns left on each side (src/hsp.rs:126-129)
+            const uint64_t pt = (uint64_t)t + a.span, pq = (uint64_t)q + a.span;
+            const uint64_t remR = (pt < a.tlen && pq < a.qlen) ? min(a.tlen - pt, a.qlen - pq) : 0;
+            const uint64_t remL = min(t, q);
+            const uint32_t nR = (uint32_t)min<uint64_t>(32, remR), nL = (uint32_t)min<uint64_t>(32, remL);
+            if (nR < 32 || nL < 32) {  // columns past a sequence end must not move the best
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    const uint32_t liveR = nR > 4u * w ? min(4u, nR - 4u * w) : 0u;
+                    const uint32_t keepR = liveR >= 4 ? 0xFFFFFFFFu : ((1u << (8 * liveR)) - 1u);
+                    rs[w] = (rs[w] & keepR) | (0x80808080u & ~keepR);
+                    const uint32_t liveL = nL > 4u * w ? min(4u, nL - 4u * w) : 0u;
+                    const uint32_t keepL = liveL >= 4 ? 0xFFFFFFFFu : ((1u << (8 * liveL)) - 1u);
+                    ls[w] = (ls[w] & keepL) | (0x80808080u & ~keepL);
+                }
+            }
+            int32_t rrun = 0, rbest = 0, lrun = 0, lbest = 0;
+            uint32_t rk, rstop, lk, lstop;
+            scan32(rs, a.xdrop, rrun, rbest, rk, rstop);
+            scan32(ls, a.xdrop, lrun, lbest, lk, lstop);
+            const bool rlive = rstop < nR, llive = lstop < nL;
+            const bool rend = rlive || remR <= 32, lend = llive || remL <= 32;
+            cols += a.span + (rlive ? rstop + 1 : nR) + (llive ? lstop + 1 : nL);
+            if (rend && lend) {  // src/hsp.rs:131-147
+                const int32_t score = sat_add(sat_add(seed, lbest), rbest);
+                res.t_start = t - lk;
+                res.q_start = q - lk;
+                res.length = a.span + lk + rk;
+                res.score = score;
+                pass = score >= a.thresh;
+                if (pass && a.entropy_on)
+                    pass = entropy_pass(a.tblk, a.tlen, res.t_start, res.t_start + res.length, res.score,
+                                        a.entropy_thresh);
+            } else {
+                more = true;
+                c = XdropCont{t, q, seed, rrun, rbest, lrun, lbest,
+                              rk | (lk << 8) | (rend ? kContREnded : 0u) | (lend ? kContLEnded : 0u)};
+            }
+        }
+        // appends, one atomic per wavefront each
+        const unsigned long long bp = __ballot(pass);
+        if (bp) {
+            unsigned long long basei = 0;
+            const int first = __ffsll((long long)bp) - 1;
+            if (lane == first) basei = atomicAdd(po.count, (unsigned long long)__popcll(bp));
+            basei = __shfl(basei, first);
+            if (pass) {
+                const uint64_t slot = basei + __popcll(bp & ((1ull << lane) - 1ull));
+                const uint64_t d = (uint64_t)((int64_t)t - (int64_t)q + (int64_t)a.qlen);
+                po.keys[slot] = (d << 32) | q;
+                po.hsp[slot] = res;
+            }
+        }
+        const unsigned long long bm = __ballot(more);
+        if (more) cont[cbase + ncont + __popcll(bm & ((1ull << lane) - 1ull))] = c;
+        ncont += (uint32_t)__popcll(bm);
+    }
+    // lanes past the end of the bin left the loop early; every lane holds the
+    // same count of the steps it took part in, lane 0 the whole wavefront's
+    ncont = (uint32_t)__builtin_amdgcn_readfirstlane(ncont);
+    if (lane == 0) cont_counts[(uint64_t)blockIdx.x * 4 + wv] = ncont;
+    unsigned long long cs = cols;
+    for (int o = 32; o > 0; o >>= 1) cs += __shfl_xor(cs, o);
+    if (lane == 0 && cs) atomicAdd(colsum, cs);
 }
 
 __global__ void k_iota(uint32_t *__restrict__ v, uint64_t n) {
@@ -964,6 +1233,8 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     auto *words = (uint32_t *)scratch(ctx, kSlotWords, nwin * 4);
     auto *cnt = (uint32_t *)scratch(ctx, kSlotCounts, nwin * 4);
     auto *off = (uint64_t *)scratch(ctx, kSlotOffsets, (nwin + 1) * 8);
+    // counters: [1] passing extensions, [2] x-drop columns past the first
+    // step, [3] first-step columns
     auto *ctr = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
     const bool fast_var = var.transitions <= 1 && nvar <= (uint32_t)kMaxV1;
     auto *vcnt = (uint32_t *)scratch(ctx, kSlotVMask, nwin * 4);  // per-variant bucket sizes, 2 bits each
@@ -1079,6 +1350,14 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
         auto *hits2 = binned ? (uint64_t *)scratch(ctx, kSlotHits2, maxb * 8) : nullptr;
         if (!bincnt || !binoff || (binned && !hits2))
             return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)", (unsigned long long)H);
+        // the straight-line first step (k_xdrop_first) needs plain adds to be exact
+        const bool first_step = byte_ok && sp->hsp.x_drop < (1 << 29) && !ctx->xdrop_single;
+        auto *xcont = first_step
+                          ? (XdropCont *)scratch(ctx, kSlotXCont, cont_capacity(maxb, kXdropBlocks) * sizeof(XdropCont))
+                          : nullptr;
+        auto *xccount = first_step ? (uint32_t *)scratch(ctx, kSlotXContCnt, kXdropBlocks * 4 * 4) : nullptr;
+        if (first_step && (!xcont || !xccount))
+            return set_err(ctx, GXY_ENOMEM, "scratch allocation failed (%llu hits)", (unsigned long long)H);
         uint64_t written = 0;  // passing extensions written before this batch
         for (uint32_t k = 0; k < nb; k++) {
             const uint64_t w0 = bounds[k] * unit_win, w1 = std::min(bounds[k + 1] * unit_win, nwin);
@@ -1126,6 +1405,30 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
                 k_bin_hits<<<nblk, 64 * kExpWaves, 0, ctx->stream>>>(bounds[k], off, boff[k], hits, binoff, mbin, hits2);
                 GXY_LAUNCH_CHECK(ctx);
             }
+            if (byte_ok && first_step) {
+                // the first 32 columns of both walks of every hit, then the hits
+                // whose walks go on, resumed by the persistent kernel (the
+                // continuation counts stay on the device)
+                XdropArgs xa{tab->d_tbytes + kBytePad, qb, tlen, qlen, target->d,
+                             make_byte_lut(*scm), sp->hsp.x_drop, sp->hsp.hsp_threshold, pat.span,
+                             sp->entropy_enabled, sp->entropy_threshold};
+                const unsigned g = std::min<unsigned>(
+                    kXdropBlocks, std::max<unsigned>(kBins, (grid_for(nh, 256) + kBins - 1) / kBins * kBins));
+                const ContLayout lay{binned ? binoff : nullptr, (uint32_t)nblk, nh};
+                {
+                    KernelTimer kt_(ctx, "extend");
+                    k_xdrop_first<<<g, 256, 0, ctx->stream>>>(xa, binned ? hits2 : hits, lay, po, xcont, xccount,
+                                                              ctr + 3);
+                }
+                GXY_LAUNCH_CHECK(ctx);
+                {
+                    KernelTimer kt_(ctx, "extend_cont");
+                    const ContSrc src{xcont, lay, xccount};
+                    k_xdrop<ContSrc, true><<<g, 256, 0, ctx->stream>>>(xa, src, po, nullptr, nullptr, ctr + 2);
+                }
+                GXY_LAUNCH_CHECK(ctx);
+                continue;
+            }
             KernelTimer kt_ext(ctx, "extend");
             if (byte_ok) {
                 XdropArgs xa{tab->d_tbytes + kBytePad, qb, tlen, qlen, target->d,
@@ -1149,11 +1452,12 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             }
             GXY_LAUNCH_CHECK(ctx);
         }
-        unsigned long long c2[2];
-        GXY_HIP(ctx, hipMemcpyAsync(c2, ctr + 1, 16, hipMemcpyDeviceToHost, ctx->stream));
+        unsigned long long c2[3];
+        GXY_HIP(ctx, hipMemcpyAsync(c2, ctr + 1, 24, hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         npass = c2[0];
-        st.columns = c2[1];
+        st.columns = c2[1] + c2[2];
+        ctx->xd_first_cols += c2[2];
         clk.lap(kTrSearchExtend);
     }
     st.passing = npass;

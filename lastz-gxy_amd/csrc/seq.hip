@@ -530,6 +530,21 @@ int gxy_ctx_dp_persist_stats(gxy_ctx *ctx, uint64_t *launches, uint64_t *sides, 
     });
 }
 
+int gxy_ctx_set_xdrop_form(gxy_ctx *ctx, int form) {
+    if (!ctx) return GXY_EINVAL;
+    if (form < 0 || form > 1) return set_err(ctx, GXY_EINVAL, "unknown x-drop form %d", form);
+    ctx->xdrop_single = form == 1;
+    return GXY_OK;
+}
+
+int gxy_ctx_xdrop_stats(gxy_ctx *ctx, uint64_t *first_step_columns) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !first_step_columns) return GXY_EINVAL;
+        *first_step_columns = ctx->xd_first_cols;
+        return GXY_OK;
+    });
+}
+
 int gxy_ctx_kernel_time(gxy_ctx *ctx, const char *name, double *ms, uint64_t *launches) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !name || !ms || !launches) return GXY_EINVAL;

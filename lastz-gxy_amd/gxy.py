@@ -38,6 +38,8 @@ _SIGS = {
     "gxy_ctx_dp_autotune_state": (C.c_int, [_vp, _P(C.c_int), _P(C.c_double)]),
     "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_set_dp_persist": (C.c_int, [_vp, C.c_int]),
+    "gxy_ctx_set_xdrop_form": (C.c_int, [_vp, C.c_int]),
+    "gxy_ctx_xdrop_stats": (C.c_int, [_vp, _P(C.c_uint64)]),
     "gxy_ctx_dp_persist_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
     "gxy_ctx_reset_timing": (C.c_int, [_vp]),
@@ -181,6 +183,16 @@ class Context:
         v = [C.c_uint64() for _ in range(3)]
         self.check(lib().gxy_ctx_dp_pool_stats(self._h, *[C.byref(x) for x in v]), "gxy_ctx_dp_pool_stats")
         return tuple(x.value for x in v)
+
+    def set_xdrop_form(self, form: int):
+        """x-drop form of the search: 0 first step + resumed walks (default), 1 one persistent kernel."""
+        self.check(lib().gxy_ctx_set_xdrop_form(self._h, form), "gxy_ctx_set_xdrop_form")
+
+    def xdrop_first_step_columns(self) -> int:
+        """x-drop columns evaluated by first steps since creation."""
+        v = C.c_uint64()
+        self.check(lib().gxy_ctx_xdrop_stats(self._h, C.byref(v)), "gxy_ctx_xdrop_stats")
+        return v.value
 
     def set_dp_persist(self, mode: int):
         """Persistent DP launches: 0 for large batches (default), 1 always, -1 never."""

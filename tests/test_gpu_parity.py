@@ -190,6 +190,40 @@ def test_search_matches_oracle(ctx, spec, step, trans, xdrop, thr, entropy):
     assert gh == oh
 
 
+@pytest.mark.parametrize("trans,tlen,qlen", [(1, 150000, 0), (2, 150000, 0), (1, 700, 300), (0, 2000, 90)])
+def test_search_xdrop_forms_agree(trans, tlen, qlen):
+    """The two x-drop forms (a straight-line first step of every hit plus the
+    resumed longer walks, and the single persistent kernel) give the same HSPs
+    and the same statistics — columns included, since both extend every hit —
+    and the oracle's HSPs: binned (T=1) and unbinned (T=0, 2) hit lists, and
+    short sequences where most walks meet a sequence end."""
+    say("x-drop forms", trans, tlen, qlen)
+    rng = np.random.default_rng(4242 + trans + tlen)
+    if qlen == 0:
+        t, q = homologous_pair(rng, tlen, 0.12, 0.01, 0.2)
+    else:
+        t = rand_seq(rng, tlen, 0.01, 0.0)
+        c, v, m = t.unpack()
+        qc, qv, qm = synth.mutate(c[50:50 + qlen], v[50:50 + qlen], m[50:50 + qlen], rng, 0.05, 0.0)
+        q = T.PackedSeq.from_arrays(qc, qv, qm)
+    pat = T.pattern_from_spec("12of19")
+    sp = T.SearchParams(1, T.HspParams(910, 1500), trans, 0, 1500)
+    res = {}
+    for form in (0, 1):
+        c = gxy.Context(0)
+        c.set_xdrop_form(form)
+        tab = gxy.PosTable.build(t, pat, 1, c)
+        res[form] = gxy.search(tab, t, q, HOX, sp)
+        first = c.xdrop_first_step_columns()
+        assert (first > 0) == (form == 0 and res[form][1].hits > 0)
+        c.close()
+    (h0, s0), (h1, s1) = res[0], res[1]
+    assert h0 == h1
+    assert (s0.hits, s0.columns, s0.passing, s0.emitted) == (s1.hits, s1.columns, s1.passing, s1.emitted)
+    oh, ost = O.search(O.PosTable(t, pat, 1), t, q, HOX, sp)
+    assert h0 == oh and s0.emitted == ost.emitted
+
+
 def test_search_dense_table(ctx):
     """A dense table (match8 on 300 kbp unmasked: ~4.6 positions per word, ~9
     non-empty variant buckets per window) overflows the expansion kernel's
