@@ -208,25 +208,30 @@ def dp_leg(n_anchors, steps, warmup, cpu_anchors, seed=3, kernel=0, workers=16):
     return out
 
 
-def hbm_xdrop_roofline(hits, columns, passing, kernel_ms, launches):
-    """HBM roofline of k_xdrop: 8 B per hit record + 0.75 B per x-drop column
-    (2 code bits + 1 valid bit per side) + 24 B per passing extension."""
-    alg = 8 * hits + 0.75 * columns + 24 * passing
+def hbm_xdrop_roofline(hits, xds, kernel_ms, launches, traffic=None):
+    """HBM roofline of k_xdrop_first, the x-drop's first step (the kernel
+    timed as "extend"): per hit 8 B of record, per column it evaluates 0.75 B
+    (2 code bits + 1 valid bit on each side), 36 B per continuation record it
+    writes and 24 B per passing extension it appends.  xds: the context's
+    (first-step columns, resumed hits, first-step passing) over the same calls."""
+    cols, resumed, fpass = xds
+    alg = 8 * hits + 0.75 * cols + 36 * resumed + 24 * fpass
     achieved = alg / (kernel_ms / 1e3) / 1e9 if kernel_ms else None
-    return {"bound": "hbm", "kernel": "k_xdrop", "achieved": round(achieved, 2) if achieved else None,
+    return {"bound": "hbm", "kernel": "k_xdrop_first", "achieved": round(achieved, 2) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
             "launches": launches, "avg_launch_ms": round(kernel_ms / max(1, launches), 4) if kernel_ms else None,
-            "algorithmic_bytes_per_launch": round(alg / max(1, launches)), "traffic": None}
+            "algorithmic_bytes_per_launch": round(alg / max(1, launches)), "traffic": traffic,
+            "bytes_model": "8 B/hit + 0.75 B/first-step column + 36 B/resumed hit + 24 B/first-step passing HSP"}
 
 
-def dominant_roofline(ktimes, st):
-    """Roofline of whichever of the DP forward and x-drop kernels took longer."""
+def dominant_roofline(ktimes, st, xds):
+    """Roofline of whichever of the DP forward and x-drop first-step kernels took longer."""
     dp_ms, dp_n = ktimes.get("dp_forward", (0.0, 0))
     xd_ms, xd_n = ktimes.get("extend", (0.0, 0))
     if dp_ms >= xd_ms and dp_ms > 0:
         return valu_roofline(st.dp_cells, dp_ms, dp_n)
     if xd_ms > 0:
-        return hbm_xdrop_roofline(st.hits, st.columns, st.passing, xd_ms, xd_n)
+        return hbm_xdrop_roofline(st.hits, xds, xd_ms, xd_n)
     return None
 
 
@@ -286,9 +291,11 @@ def serve_leg(args, budget_left=None):
         extra = {}
         if timing:
             st = srv.stats()
-            kt = {nm: srv.kernel_time(nm) for nm in ("extend", "qexpand", "qcount", "dp_forward", "dp_traceback")}
+            kt = {nm: srv.kernel_time(nm) for nm in ("extend", "extend_cont", "qexpand", "qcount", "dp_forward",
+                                                      "dp_traceback")}
             extra = {"kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]},
-                     "dp_cells": st.dp_cells, "seed_hits": st.hits, "roofline": dominant_roofline(kt, st)}
+                     "dp_cells": st.dp_cells, "seed_hits": st.hits,
+                     "roofline": dominant_roofline(kt, st, srv.xdrop_stats())}
         srv.close()
         span_s = (max(done) - min(sub)) / 1e3
         lat = np.array(lat)
@@ -399,8 +406,9 @@ def contigs_leg(args):
     paf = gxy.format_paf(names_t, names_q, recs)
     paf_s = time.perf_counter() - t1
     log(f"contigs: batched {batched_s:.2f}s, {len(recs)} records, PAF {len(paf)} bytes in {paf_s:.2f}s")
-    kt = {nm: bctx.kernel_time(nm) for nm in ("extend", "qexpand", "qcount", "dp_forward", "dp_traceback")}
-    roof = dominant_roofline(kt, st)
+    kt = {nm: bctx.kernel_time(nm) for nm in ("extend", "extend_cont", "qexpand", "qcount", "dp_forward",
+                                               "dp_traceback")}
+    roof = dominant_roofline(kt, st, bctx.xdrop_stats())
     tune = bctx.dp_autotune_state()
     bctx.close()
     rec_by_q = {}
@@ -603,6 +611,7 @@ def pair_leg(args, workload, steps, warmup):
     ctx.reset_timing()
     ctx.set_timing(True)
     ctx.synchronize()
+    xd0 = ctx.xdrop_stats()
     tstart = time.perf_counter()
     for s in range(steps):
         recs, st = gxy.run_dev([dt], [dq], cfg, ctx)
@@ -617,22 +626,21 @@ def pair_leg(args, workload, steps, warmup):
     hsp_gcups = st.columns / 1e9 / (elapsed / steps)
     dp_gcups = st.dp_cells / 1e9 / (elapsed / steps)
 
+    xds = tuple(b - a for a, b in zip(xd0, ctx.xdrop_stats()))
     dom_name, (dom_ms, dom_launches) = max(ktime.items(), key=lambda kv: kv[1][0])
     roof = None
     if dom_name == "extend":
-        alg = (8 * st.hits + 0.75 * st.columns + 24 * st.passing) * steps
-        achieved = alg / (dom_ms / 1e3) / 1e9
         # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
         # runs of this workload (scaled by a 1 GiB copy), when present
         traffic = None
-        prof = os.path.join(ROOT, "profiles", "k_xdrop_chrom48M.json")
+        prof = os.path.join(ROOT, "profiles", "k_xdrop_first_chrom48M.json")
         if workload == "chrom" and os.path.exists(prof):
             traffic = json.load(open(prof))["traffic_bytes_per_launch"]
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_xdrop", "avg_launch_ms": round(dom_ms / max(1, dom_launches), 4),
-                "launches": dom_launches,
-                "algorithmic_bytes_per_launch": round(alg / max(1, dom_launches))}
+        roof = hbm_xdrop_roofline(st.hits * steps, xds, dom_ms, dom_launches, traffic)
+        ems, en = ktime.get("extend_cont", (0.0, 0))
+        if en:
+            roof["resume_kernel"] = {"kernel": "k_xdrop (resumed walks)", "launches": en,
+                                     "avg_launch_ms": round(ems / en, 4)}
     elif dom_name == "dp_forward":
         roof = valu_roofline(st.dp_cells * steps, dom_ms, dom_launches)
     else:

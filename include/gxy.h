@@ -239,9 +239,10 @@ int gxy_ctx_dp_persist_stats(gxy_ctx *ctx, uint64_t *launches, uint64_t *sides, 
  * hits whose walks go on; 1: the persistent kernel alone, one 32-column chunk
  * of one walk per lane per step.  Results do not depend on it. */
 int gxy_ctx_set_xdrop_form(gxy_ctx *ctx, int form);
-/* x-drop columns evaluated by first steps (form 0) since the context was
- * created, the minus-strand helper's included. */
-int gxy_ctx_xdrop_stats(gxy_ctx *ctx, uint64_t *first_step_columns);
+/* First steps of the x-drop (form 0) since the context was created, the
+ * minus-strand helper's included: columns they evaluated, hits they handed on
+ * to be resumed and passing extensions they finished. */
+int gxy_ctx_xdrop_stats(gxy_ctx *ctx, uint64_t *first_step_columns, uint64_t *resumed, uint64_t *first_step_passing);
 
 /* ---- sequences ---------------------------------------------------------- */
 /* PackedSeq::from_ascii (src/sequences.rs:164-206) into caller buffers sized
@@ -559,6 +560,9 @@ int gxy_server_set_timing(gxy_server *server, int enabled);
 int gxy_server_stats(gxy_server *server, gxy_run_stats *out);
 int gxy_server_kernel_time(gxy_server *server, const char *name, double *ms, uint64_t *launches);
 /* Stops the worker after the queued queries are served; frees everything. */
+/* gxy_ctx_xdrop_stats summed over the server's contexts. */
+int gxy_server_xdrop_stats(gxy_server *server, uint64_t *first_step_columns, uint64_t *resumed,
+                           uint64_t *first_step_passing);
 int gxy_server_destroy(gxy_server *server);
 
 #ifdef __cplusplus

@@ -400,7 +400,9 @@ struct gxy_ctx {
     // x-drop in one persistent kernel per hit batch instead of a first step
     // for every hit and a kernel resuming the longer walks (for measurement)
     bool xdrop_single = false;
-    uint64_t xd_first_cols = 0;  // x-drop columns evaluated by first steps since creation
+    // first steps since creation: x-drop columns evaluated, hits handed on to
+    // the resuming kernel, passing extensions finished
+    uint64_t xd_first_cols = 0, xd_conts = 0, xd_first_pass = 0;
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_used;
     // Host phase clock (GXY_TRACE set in the environment): milliseconds per phase.

@@ -943,7 +943,9 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
         }
         if (helper) {  // the helper's x-drop counts as this context's
             ctx->xd_first_cols += helper->xd_first_cols;
-            helper->xd_first_cols = 0;
+            ctx->xd_conts += helper->xd_conts;
+            ctx->xd_first_pass += helper->xd_first_pass;
+            helper->xd_first_cols = helper->xd_conts = helper->xd_first_pass = 0;
         }
         if (helper && helper->timing) {  // the helper's kernel times count as this context's
             timing_flush(helper);
@@ -1608,6 +1610,23 @@ int gxy_server_kernel_time(gxy_server *S, const char *name, double *ms, uint64_t
             if (rc) return rc;
             *ms += m;
             *launches += l;
+        }
+        return GXY_OK;
+    });
+}
+
+int gxy_server_xdrop_stats(gxy_server *S, uint64_t *first_step_columns, uint64_t *resumed,
+                           uint64_t *first_step_passing) {
+    return guarded(nullptr, [&]() -> int {
+        if (!S || !first_step_columns || !resumed || !first_step_passing) return GXY_EINVAL;
+        std::lock_guard<std::mutex> lk(S->mu);
+        *first_step_columns = *resumed = *first_step_passing = 0;
+        std::vector<gxy_ctx *> all(1, S->ctx);
+        all.insert(all.end(), S->dp_ctx.begin(), S->dp_ctx.end());
+        for (gxy_ctx *c : all) {
+            *first_step_columns += c->xd_first_cols;
+            *resumed += c->xd_conts;
+            *first_step_passing += c->xd_first_pass;
         }
         return GXY_OK;
     });

@@ -1012,7 +1012,7 @@ __device__ __forceinline__ void scan32(const uint32_t sw[8], int32_t xdrop, int3
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop_first(
     XdropArgs a, const uint64_t *__restrict__ hits, ContLayout lay, PassOut po, XdropCont *__restrict__ cont,
-    uint32_t *__restrict__ cont_counts, unsigned long long *__restrict__ colsum) {
+    uint32_t *__restrict__ cont_counts, unsigned long long *__restrict__ stats) {
     const int lane = threadIdx.x & 63;
     const uint32_t wv = threadIdx.x >> 6;
     // this block's hits: its bin's (block b walks bin b % 8 with the other
@@ -1024,6 +1024,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     uint64_t h = lo + (uint64_t)(lay.binoff ? blockIdx.x / kBins : blockIdx.x) * 256 + threadIdx.x;
     const uint64_t stride = nbin * 256;
     uint32_t ncont = 0;  // this wavefront's records so far (uniform)
+    uint32_t npass = 0;  // this wavefront's passing extensions (uniform)
     const uint32_t nsw = (a.span + 3) >> 2;
     const uint32_t lastmask = (a.span & 3) ? ((1u << (8 * (a.span & 3))) - 1u) : 0xFFFFFFFFu;
     uint64_t cols = 0;
@@ -1111,6 +1112,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                 po.keys[slot] = (d << 32) | q;
                 po.hsp[slot] = res;
             }
+            npass += (uint32_t)__popcll(bp);
         }
         const unsigned long long bm = __ballot(more);
         if (more) cont[cbase + ncont + __popcll(bm & ((1ull << lane) - 1ull))] = c;
@@ -1119,10 +1121,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     // lanes past the end of the bin left the loop early; every lane holds the
     // same count of the steps it took part in, lane 0 the whole wavefront's
     ncont = (uint32_t)__builtin_amdgcn_readfirstlane(ncont);
+    npass = (uint32_t)__builtin_amdgcn_readfirstlane(npass);
     if (lane == 0) cont_counts[(uint64_t)blockIdx.x * 4 + wv] = ncont;
+    // stats: [0] columns, [1] continuation records, [2] passing extensions
     unsigned long long cs = cols;
     for (int o = 32; o > 0; o >>= 1) cs += __shfl_xor(cs, o);
-    if (lane == 0 && cs) atomicAdd(colsum, cs);
+    if (lane == 0) {
+        if (cs) atomicAdd(stats, cs);
+        if (ncont) atomicAdd(stats + 1, (unsigned long long)ncont);
+        if (npass) atomicAdd(stats + 2, (unsigned long long)npass);
+    }
 }
 
 __global__ void k_iota(uint32_t *__restrict__ v, uint64_t n) {
@@ -1234,7 +1242,8 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     auto *cnt = (uint32_t *)scratch(ctx, kSlotCounts, nwin * 4);
     auto *off = (uint64_t *)scratch(ctx, kSlotOffsets, (nwin + 1) * 8);
     // counters: [1] passing extensions, [2] x-drop columns past the first
-    // step, [3] first-step columns
+    // step, [3] first-step columns, [4] continuation records, [5] first-step
+    // passing extensions
     auto *ctr = (unsigned long long *)scratch(ctx, kSlotCounters, 64);
     const bool fast_var = var.transitions <= 1 && nvar <= (uint32_t)kMaxV1;
     auto *vcnt = (uint32_t *)scratch(ctx, kSlotVMask, nwin * 4);  // per-variant bucket sizes, 2 bits each
@@ -1452,12 +1461,14 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
             }
             GXY_LAUNCH_CHECK(ctx);
         }
-        unsigned long long c2[3];
-        GXY_HIP(ctx, hipMemcpyAsync(c2, ctr + 1, 24, hipMemcpyDeviceToHost, ctx->stream));
+        unsigned long long c2[5];
+        GXY_HIP(ctx, hipMemcpyAsync(c2, ctr + 1, 40, hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         npass = c2[0];
         st.columns = c2[1] + c2[2];
         ctx->xd_first_cols += c2[2];
+        ctx->xd_conts += c2[3];
+        ctx->xd_first_pass += c2[4];
         clk.lap(kTrSearchExtend);
     }
     st.passing = npass;

@@ -537,10 +537,12 @@ int gxy_ctx_set_xdrop_form(gxy_ctx *ctx, int form) {
     return GXY_OK;
 }
 
-int gxy_ctx_xdrop_stats(gxy_ctx *ctx, uint64_t *first_step_columns) {
+int gxy_ctx_xdrop_stats(gxy_ctx *ctx, uint64_t *first_step_columns, uint64_t *resumed, uint64_t *first_step_passing) {
     return guarded(ctx, [&]() -> int {
-        if (!ctx || !first_step_columns) return GXY_EINVAL;
+        if (!ctx || !first_step_columns || !resumed || !first_step_passing) return GXY_EINVAL;
         *first_step_columns = ctx->xd_first_cols;
+        *resumed = ctx->xd_conts;
+        *first_step_passing = ctx->xd_first_pass;
         return GXY_OK;
     });
 }

@@ -39,7 +39,7 @@ _SIGS = {
     "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_set_dp_persist": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_set_xdrop_form": (C.c_int, [_vp, C.c_int]),
-    "gxy_ctx_xdrop_stats": (C.c_int, [_vp, _P(C.c_uint64)]),
+    "gxy_ctx_xdrop_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_dp_persist_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
     "gxy_ctx_reset_timing": (C.c_int, [_vp]),
@@ -103,6 +103,7 @@ _SIGS = {
     "gxy_server_set_timing": (C.c_int, [_vp, C.c_int]),
     "gxy_server_stats": (C.c_int, [_vp, _P(T.RunStats)]),
     "gxy_server_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
+    "gxy_server_xdrop_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_server_submit": (C.c_int, [_vp, _P(T.Seq), _P(C.c_uint64)]),
     "gxy_server_wait": (C.c_int, [_vp, C.c_uint64, _P(_P(T.RecordSet)), _P(T.RequestStats)]),
     "gxy_server_last_error": (C.c_char_p, [_vp]),
@@ -188,11 +189,12 @@ class Context:
         """x-drop form of the search: 0 first step + resumed walks (default), 1 one persistent kernel."""
         self.check(lib().gxy_ctx_set_xdrop_form(self._h, form), "gxy_ctx_set_xdrop_form")
 
-    def xdrop_first_step_columns(self) -> int:
-        """x-drop columns evaluated by first steps since creation."""
-        v = C.c_uint64()
-        self.check(lib().gxy_ctx_xdrop_stats(self._h, C.byref(v)), "gxy_ctx_xdrop_stats")
-        return v.value
+    def xdrop_stats(self):
+        """First steps of the x-drop since creation: (columns evaluated, hits
+        handed on to be resumed, passing extensions finished)."""
+        v = [C.c_uint64() for _ in range(3)]
+        self.check(lib().gxy_ctx_xdrop_stats(self._h, *[C.byref(x) for x in v]), "gxy_ctx_xdrop_stats")
+        return tuple(x.value for x in v)
 
     def set_dp_persist(self, mode: int):
         """Persistent DP launches: 0 for large batches (default), 1 always, -1 never."""
@@ -699,6 +701,14 @@ class Server:
         if rc != 0:
             raise GxyError(f"gxy_server_kernel_time failed ({rc})")
         return ms.value, n.value
+
+    def xdrop_stats(self):
+        """Context.xdrop_stats summed over the server's contexts."""
+        v = [C.c_uint64() for _ in range(3)]
+        rc = lib().gxy_server_xdrop_stats(self._h, *[C.byref(x) for x in v])
+        if rc != 0:
+            raise GxyError(f"gxy_server_xdrop_stats failed ({rc})")
+        return tuple(x.value for x in v)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -214,8 +214,9 @@ def test_search_xdrop_forms_agree(trans, tlen, qlen):
         c.set_xdrop_form(form)
         tab = gxy.PosTable.build(t, pat, 1, c)
         res[form] = gxy.search(tab, t, q, HOX, sp)
-        first = c.xdrop_first_step_columns()
+        first, resumed, fpass = c.xdrop_stats()
         assert (first > 0) == (form == 0 and res[form][1].hits > 0)
+        assert resumed <= res[form][1].hits and fpass <= res[form][1].passing
         c.close()
     (h0, s0), (h1, s1) = res[0], res[1]
     assert h0 == h1
