@@ -989,6 +989,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
 // side by side.  A hit whose walks are both over is finished here (passing
 // ones appended as in k_xdrop); the others are written as XdropCont records
 // for k_xdrop to resume.  Needs x_drop < 2^29 (plain adds are exact then).
+// query bytes a wavefront of k_xdrop_first keeps in LDS
+constexpr int kQLds = 1024;
+// 32 consecutive bytes at byte offset `off` of an LDS word array.
+__device__ __forceinline__ void lds_bytes32(const uint32_t *w, uint32_t off, uint32_t out[8]) {
+    const uint32_t *p = w + (off >> 2);
+    const uint32_t sh = off & 3u;
+    uint32_t d[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) d[i] = p[i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) out[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+}
+
 // The 32-column scan of k_xdrop's fast form from (run, best): kbest is the
 // 1-based column of the last improvement (0: none), kstop the first stopping
 // column (32: none); run is pinned after a stop.
@@ -1010,6 +1023,55 @@ __device__ __forceinline__ void scan32(const uint32_t sw[8], int32_t xdrop, int3
     kstop = 32 - nst;
 }
 
+// Both first chunks of a hit at once, in packed 16-bit lanes: the right walk
+// in the low halves, the left walk in the high halves.  From a fresh start
+// every value fits 16 bits (|run| <= 32 * 128; best - run <= 8192), so the
+// walk's arithmetic is exact there; x_drop is clamped to 16383, above any
+// drop a chunk can reach.  Per column and both sides: the two score bytes
+// placed as the high bytes of the halves and shifted down (sign-extended),
+// the run, best - run before the column (< 0: the column improves the best,
+// > x_drop: it stops the walk), the new best, the last improving column
+// (bit-select on the sign of best - run), a sticky stopped mask and a count
+// of stopped columns.  The walk is not frozen at its first stop, so the
+// result is the reference's (src/hsp.rs:179-193) unless the best improves
+// after that stop — then kbest > kstop and the caller redoes the side with
+// scan32.  Per side: run, best, kbest, kstop as in scan32.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void scan32x2(const uint32_t rs[8], const uint32_t ls[8], int32_t xdrop, int32_t &rrun,
+                                         int32_t &rbest, uint32_t &rk, uint32_t &rstop, int32_t &lrun, int32_t &lbest,
+                                         uint32_t &lk, uint32_t &lstop) {
+    const short X = (short)min(xdrop, 16383);
+    const s16x2 xp = {X, X};
+    s16x2 r = {0, 0}, b = {0, 0}, c = {0, 0};
+    uint32_t kb = 0, stopped = 0;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+        const uint32_t by = (uint32_t)(k & 3);
+        // bytes: 0, right byte `by`, 0, left byte `by` (v_perm: rs is bytes 0-3, ls 4-7, 12 gives 0)
+        const uint32_t sel = 0x0Cu | (by << 8) | (0x0Cu << 16) | ((4u + by) << 24);
+        const s16x2 s = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(ls[k >> 2], rs[k >> 2], sel)) >> (short)8;
+        r = r + s;
+        const s16x2 d = b - r;
+        b = __builtin_elementwise_max(b, r);
+        const uint32_t imp = __builtin_bit_cast(uint32_t, d >> (short)15);
+        const uint32_t kk = (uint32_t)(k + 1) * 0x00010001u;
+        kb = (imp & kk) | (~imp & kb);
+        stopped |= __builtin_bit_cast(uint32_t, (xp - d) >> (short)15);
+        c = c - __builtin_bit_cast(s16x2, stopped);
+    }
+    rrun = r.x;
+    rbest = b.x;
+    rk = kb & 0xFFFFu;
+    rstop = 32u - (uint32_t)(uint16_t)c.x;
+    lrun = r.y;
+    lbest = b.y;
+    lk = kb >> 16;
+    lstop = 32u - (uint32_t)(uint16_t)c.y;
+}
+
+// kSpan: the seed span when it is known at compile time (the target bytes of
+// a hit are then one dword-aligned window), 0 for any span.
+template <int kSpan>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_xdrop_first(
     XdropArgs a, const uint64_t *__restrict__ hits, ContLayout lay, PassOut po, XdropCont *__restrict__ cont,
     uint32_t *__restrict__ cont_counts, unsigned long long *__restrict__ stats) {
@@ -1021,29 +1083,84 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     uint64_t lo, hi;
     const uint64_t cbase = lay.base(blockIdx.x, gridDim.x, wv, &lo, &hi);
     const uint64_t nbin = lay.binoff ? gridDim.x / kBins : gridDim.x;
-    uint64_t h = lo + (uint64_t)(lay.binoff ? blockIdx.x / kBins : blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t h = lo + (uint64_t)(lay.binoff ? blockIdx.x / kBins : blockIdx.x) * 256 + threadIdx.x;
     const uint64_t stride = nbin * 256;
     uint32_t ncont = 0;  // this wavefront's records so far (uniform)
     uint32_t npass = 0;  // this wavefront's passing extensions (uniform)
     const uint32_t nsw = (a.span + 3) >> 2;
     const uint32_t lastmask = (a.span & 3) ? ((1u << (8 * (a.span & 3))) - 1u) : 0xFFFFFFFFu;
     uint64_t cols = 0;
-    for (; h < hi; h += stride) {
-        const uint64_t x = hits[h];
+    // The wavefront's 64 hits come from neighbouring query windows, so their
+    // query bytes overlap: the wavefront loads the range they span once,
+    // coalesced, into LDS and each lane reads its windows there; only the
+    // target bytes (random per hit) go through the vector memory pipeline,
+    // which bounds this kernel.  Ranges wider than kQLds bytes read the query
+    // from global memory as well.
+    __shared__ __align__(16) uint32_t qlds[4][kQLds / 4];
+    uint32_t *ql = qlds[wv];
+    const uint64_t hw0 = h - (uint64_t)lane;  // the wavefront's first hit (uniform loop below)
+    for (uint64_t hw = hw0; hw < hi; hw += stride) {
+        const uint64_t hh = hw + (uint64_t)lane;
+        const bool act = hh < hi;
+        const uint64_t x = act ? hits[hh] : 0;
         const uint32_t q = (uint32_t)(x >> 32), t = (uint32_t)x;
+        // src/hsp.rs:113-115: a seed out of bounds has no HSP
+        const bool ok = act && (uint64_t)t + a.span <= a.tlen && (uint64_t)q + a.span <= a.qlen;
         bool pass = false, more = false;
         gxy_hsp res = {0, 0, 0, 0};
         XdropCont c;
-        if ((uint64_t)t + a.span <= a.tlen && (uint64_t)q + a.span <= a.qlen) {  // src/hsp.rs:113-115
-            // bytes [t - 32, t + 32): the left chunk, then the seed
+        // a hit reads query bytes [q - 32, q + span + 36)
+        uint32_t qmin = ok ? q : 0xFFFFFFFFu, qmax = ok ? q : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            qmin = min(qmin, (uint32_t)__shfl_xor((int)qmin, o));
+            qmax = max(qmax, (uint32_t)__shfl_xor((int)qmax, o));
+        }
+        const int64_t qb = ((int64_t)qmin - 32) & ~(int64_t)15;  // >= -47: inside the padding
+        const int64_t qend = (int64_t)qmax + a.span + 36;
+        const bool in_lds = qmin != 0xFFFFFFFFu && qend <= qb + kQLds;  // uniform
+        if (in_lds) {
+            const int64_t p = qb + 16 * lane;
+            if (p < qend) *(uint4 *)&ql[4 * lane] = *(const uint4 *)(a.qf + p);
+            // this wavefront's stores before its own loads below
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        if (ok) {
+            // bytes [t - 32, t + 32): the left chunk, then the seed; tr: the
+            // right chunk, 32 bytes from the seed end
             uint32_t tw[16], qw[16], tr[8], qr[8];
-            load_bytes32(a.tf, (int64_t)t - 32, tw);
-            load_bytes32(a.tf, (int64_t)t, tw + 8);
-            load_bytes32(a.qf, (int64_t)q - 32, qw);
-            load_bytes32(a.qf, (int64_t)q, qw + 8);
-            // the right chunk: 32 bytes from the seed end
-            load_bytes32(a.tf, (int64_t)t + a.span, tr);
-            load_bytes32(a.qf, (int64_t)q + a.span, qr);
+            if constexpr (kSpan > 0) {
+                // one window [t - 32, t + kSpan + 32) of dword-aligned loads;
+                // the right chunk starts at dword r0 or r0 + 1 of it
+                constexpr int ND = ((66 + kSpan) >> 2) + 2, r0 = (32 + kSpan) >> 2;
+                const int64_t p0 = (int64_t)t - 32;
+                const uint32_t *tp = (const uint32_t *)(a.tf + (p0 & ~(int64_t)3));
+                const uint32_t sh = (uint32_t)(p0 & 3);
+                uint32_t d[ND];
+#pragma unroll
+                for (int i = 0; i < ND; i++) d[i] = tp[i];
+#pragma unroll
+                for (int i = 0; i < 16; i++) tw[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+                const bool up = ((sh + 32 + kSpan) >> 2) != (uint32_t)r0;
+                const uint32_t rsh = (sh + 32 + kSpan) & 3u;
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    tr[i] = __builtin_amdgcn_alignbyte(up ? d[r0 + i + 2] : d[r0 + i + 1], up ? d[r0 + i + 1] : d[r0 + i],
+                                                       rsh);
+            } else {
+                load_bytes32(a.tf, (int64_t)t - 32, tw);
+                load_bytes32(a.tf, (int64_t)t, tw + 8);
+                load_bytes32(a.tf, (int64_t)t + a.span, tr);
+            }
+            if (in_lds) {
+                lds_bytes32(ql, (uint32_t)((int64_t)q - 32 - qb), qw);
+                lds_bytes32(ql, (uint32_t)((int64_t)q - qb), qw + 8);
+                lds_bytes32(ql, (uint32_t)((int64_t)q + a.span - qb), qr);
+            } else {
+                load_bytes32(a.qf, (int64_t)q - 32, qw);
+                load_bytes32(a.qf, (int64_t)q, qw + 8);
+                load_bytes32(a.qf, (int64_t)q + a.span, qr);
+            }
             int32_t seed = 0;  // src/hsp.rs:118-124; |sum| <= 32 * 128
 #pragma unroll
             for (int w = 0; w < 8; w++) {
@@ -1076,10 +1193,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                     ls[w] = (ls[w] & keepL) | (0x80808080u & ~keepL);
                 }
             }
-            int32_t rrun = 0, rbest = 0, lrun = 0, lbest = 0;
+            int32_t rrun, rbest, lrun, lbest;
             uint32_t rk, rstop, lk, lstop;
-            scan32(rs, a.xdrop, rrun, rbest, rk, rstop);
-            scan32(ls, a.xdrop, lrun, lbest, lk, lstop);
+            scan32x2(rs, ls, a.xdrop, rrun, rbest, rk, rstop, lrun, lbest, lk, lstop);
+            // the best improved after a walk's first stop (a climb of more than
+            // x_drop inside the chunk): that side again, frozen at the stop
+            if (rk > rstop) {
+                rrun = rbest = 0;
+                scan32(rs, a.xdrop, rrun, rbest, rk, rstop);
+            }
+            if (lk > lstop) {
+                lrun = lbest = 0;
+                scan32(ls, a.xdrop, lrun, lbest, lk, lstop);
+            }
             const bool rlive = rstop < nR, llive = lstop < nL;
             const bool rend = rlive || remR <= 32, lend = llive || remL <= 32;
             cols += a.span + (rlive ? rstop + 1 : nR) + (llive ? lstop + 1 : nL);
@@ -1118,8 +1244,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         if (more) cont[cbase + ncont + __popcll(bm & ((1ull << lane) - 1ull))] = c;
         ncont += (uint32_t)__popcll(bm);
     }
-    // lanes past the end of the bin left the loop early; every lane holds the
-    // same count of the steps it took part in, lane 0 the whole wavefront's
+    // every lane took part in every step: the counts are the wavefront's
     ncont = (uint32_t)__builtin_amdgcn_readfirstlane(ncont);
     npass = (uint32_t)__builtin_amdgcn_readfirstlane(npass);
     if (lane == 0) cont_counts[(uint64_t)blockIdx.x * 4 + wv] = ncont;
@@ -1426,8 +1551,13 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
                 const ContLayout lay{binned ? binoff : nullptr, (uint32_t)nblk, nh};
                 {
                     KernelTimer kt_(ctx, "extend");
-                    k_xdrop_first<<<g, 256, 0, ctx->stream>>>(xa, binned ? hits2 : hits, lay, po, xcont, xccount,
-                                                              ctr + 3);
+                    const uint64_t *hsrc = binned ? hits2 : hits;
+                    if (pat.span == 19)  // 12of19
+                        k_xdrop_first<19><<<g, 256, 0, ctx->stream>>>(xa, hsrc, lay, po, xcont, xccount, ctr + 3);
+                    else if (pat.span == 12)  // match12 (the tweener's inner seed)
+                        k_xdrop_first<12><<<g, 256, 0, ctx->stream>>>(xa, hsrc, lay, po, xcont, xccount, ctr + 3);
+                    else
+                        k_xdrop_first<0><<<g, 256, 0, ctx->stream>>>(xa, hsrc, lay, po, xcont, xccount, ctr + 3);
                 }
                 GXY_LAUNCH_CHECK(ctx);
                 {
