@@ -37,7 +37,7 @@ extern "C" {
 #define GXY_EINVAL (-1)   /* invalid argument (bad size, null pointer, ...)  */
 #define GXY_ENOMEM (-2)   /* host or device allocation failed                */
 #define GXY_EDEVICE (-3)  /* HIP runtime / kernel launch error               */
-#define GXY_EUNSUP (-4)   /* parameters outside what a kernel supports       */
+#define GXY_EUNSUP (-4)   /* parameters outside what a kernel supports (kept for the ABI; no entry point returns it now) */
 #define GXY_ERANGE (-5)   /* input exceeds a documented limit                */
 
 /* ---- POD views ---------------------------------------------------------- */
@@ -214,8 +214,8 @@ int gxy_ctx_set_dp_kernel(gxy_ctx *ctx, int kernel);
  * order above (0 where not yet measured).  Results never depend on it. */
 int gxy_ctx_dp_autotune_state(gxy_ctx *ctx, int *chosen, double *gcups);
 /* Gapped-extension passes on the context since it was created: passes run,
- * sides stopped by a full pool (swept again later) and passes that ran one
- * side alone. */
+ * sides stopped by a full pool or by their reserved traceback rows (swept
+ * again later) and passes that ran one side alone. */
 int gxy_ctx_dp_pool_stats(gxy_ctx *ctx, uint64_t *passes, uint64_t *truncated, uint64_t *alone);
 /* Persistent form of the gapped extension (no counterpart in the reference,
  * whose driver extends anchors one at a time: src/driver.rs:305-331).  Once

@@ -1299,6 +1299,409 @@ __global__ __launch_bounds__(64) void k_pack_runs(const uint64_t *__restrict__ r
 }
 
 // choose_anchor (src/anchor.rs:25-69), one lane per HSP.
+// ---- sides wider than kMaxW ---------------------------------------------------
+//
+// k_dp_band<S>: the same row recurrence for sides whose band is wider than the
+// kernels above hold (y_drop / gap_extend large, or gap_extend = 0, where the
+// band is the whole query; src/gapped_extend.rs:214-233 sets no limit).  One
+// block of four wavefronts per side; the row lives in global memory (two row
+// buffers of the side's band width, int2 {D = max(M, X, Y), X-term =
+// max(M - (O+E), X - E)} plus a byte of from-states per cell, as the
+// register kernels keep them), lane t owning C (even) consecutive cells of
+// the window.  Dead cells are not rewritten: the next row reads a cell as NEG
+// when it lies outside the previous window or below the previous row's
+// threshold.  Per row three passes over a lane's cells: (1) M and X from the
+// previous row, the Y-scan terms; block scan of the lane totals; (2) Y,
+// from-states, the new row and the traceback nibbles; block reduction of the
+// row best; (3) the alive band under the new threshold.
+// Traceback rows go to the side's own range (row i at i * rb bytes, nibble
+// of column j at index j - jl, low nibble first) with the row windows beside
+// them; a side that outruns its reserved rows stops (truncated) and is swept
+// again with room for every row.
+struct BandSide {
+    uint64_t tb;        // byte offset of the side's traceback rows
+    uint64_t win;       // index of its first row window
+    uint64_t rowbuf;    // index of its first row-buffer cell (two rows of bw cells)
+    uint32_t cap_rows;  // traceback rows reserved
+    uint32_t rb;        // traceback bytes per row
+};
+constexpr uint32_t kFmtBand = 16;  // DpRes::fmt of k_dp_band's sides
+
+// query byte 8 * (code | 4 * invalid) of base u of a window
+__device__ __forceinline__ uint32_t win_qbyte(const Win &w, uint32_t u) {
+    const uint32_t code = (uint32_t)(w.codes >> (2 * u)) & 3u;
+    const uint32_t inv = ((w.valid >> u) & 1u) ^ 1u;
+    return 8u * (code | (inv << 2));
+}
+
+template <class S>
+__global__ __launch_bounds__(kDpThreads) void k_dp_band(const DpProb *__restrict__ probs,
+                                                        const uint32_t *__restrict__ list,
+                                                        const BandSide *__restrict__ bsides, uint32_t nside, S sc,
+                                                        int32_t oe, int32_t e, int32_t ydrop, uint8_t *__restrict__ tb,
+                                                        uint2 *__restrict__ wins, int2 *__restrict__ rows,
+                                                        uint8_t *__restrict__ rows_sb, DpRes *__restrict__ res) {
+    __shared__ int32_t w_tot[kDpWaves], w_alast[kDpWaves], w_max[kDpWaves];
+    __shared__ uint32_t w_j[kDpWaves], w_st[kDpWaves], w_lo[kDpWaves], w_hi[kDpWaves];
+    const uint32_t sidx = blockIdx.x;
+    if (sidx >= nside) return;  // uniform per block
+    const int t = threadIdx.x, lane = t & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const DpProb pr = probs[list[sidx]];
+    const BandSide bs = bsides[sidx];
+    const uint32_t n = pr.n, m = pr.m;
+    DpRes out = {0, 0, 0, kStM, 0, 0, kNoChunk, kFmtBand, 0};
+    if (n == 0 || m == 0) {  // src/gapped_extend.rs:188-195
+        if (t == 0) res[sidx] = out;
+        return;
+    }
+    // band radius / width (src/gapped_extend.rs:214-233)
+    uint32_t br;
+    if (e > 0) {
+        const int32_t r = sat_sub(ydrop, oe) / e;
+        br = (uint32_t)(max(r, 0) + 2);
+    } else {
+        br = m + 1;
+    }
+    const uint32_t bw = (uint32_t)min((uint64_t)4 * br + 4, (uint64_t)m + 1);
+    const int32_t negx = max(kNegInf - oe, kNegInf - e);
+    const uint32_t xb_neg = (kNegInf - oe >= kNegInf - e) ? 0u : 1u;
+    const int32_t lead_y = negx;  // Y of the window's first cell (src/gapped_extend.rs:318-334)
+    const uint32_t ybit_lead = xb_neg;
+    int2 *buf[2] = {rows + bs.rowbuf, rows + bs.rowbuf + bw};
+    uint8_t *sbuf[2] = {rows_sb + bs.rowbuf, rows_sb + bs.rowbuf + bw};
+    uint8_t *const tb0 = tb + bs.tb;
+    uint2 *const win0 = wins + bs.win;
+
+    int32_t best_score = 0;
+    uint32_t bi = 0, bj = 0, bst = kStM;
+    uint32_t lo = 0, hi = 0, pjl = 0, pjh = 0;
+    int32_t thr_prev = 0;
+    Win tw = {0, 0, 0};
+    uint64_t cells = 0;
+    uint32_t rows_done = 0, truncated = 0, cur = 0;
+    for (uint32_t i = 0; i <= n; i++) {
+        if (i >= bs.cap_rows) {  // out of reserved rows: swept again with room for all
+            truncated = 1;
+            break;
+        }
+        const uint32_t jl = lo >= br ? lo - br : 0;
+        const uint32_t jh = (uint32_t)min(min((uint64_t)hi + br, (uint64_t)m), (uint64_t)jl + bw - 1);
+        const uint32_t W = jh - jl + 1;
+        cells += W - ((i == 0 && jl == 0) ? 1 : 0);
+        // lane t owns cells [c0, c0 + cnt) of the window, C even
+        const uint32_t C = 2u * ((W + 2u * kDpThreads - 1) / (2u * kDpThreads));
+        const uint32_t c0 = (uint32_t)t * C;
+        const uint32_t cnt = c0 < W ? min(C, W - c0) : 0;
+        // target base of row i (T[i-1]); uniform
+        uint32_t tc = 0, tv = 0;
+        if (i >= 1) {
+            const uint32_t r = (i - 1) & 31u;
+            if (r == 0) tw = dir_win(pr.T, pr.tdir > 0 ? pr.t0 + (i - 1) : pr.t0 - (i - 1), pr.tdir);
+            tc = (uint32_t)(tw.codes >> (2 * r)) & 3u;
+            tv = (tw.valid >> r) & 1u;
+        }
+        const auto rs = row_scores(sc, tc, tv);
+        int2 *const pb = buf[cur ^ 1u];
+        const uint8_t *const psb = sbuf[cur ^ 1u];
+        int2 *const cb_ = buf[cur];
+        uint8_t *const csb = sbuf[cur];
+        // the previous row's cell at column c as the reference reads it:
+        // NEG outside its window or below its threshold
+        auto prev = [&](uint32_t c, int32_t &d, int32_t &xv, uint32_t &s) {
+            d = kNegInf;
+            xv = negx;
+            s = xb_neg << 2;
+            if (i >= 1 && c >= pjl && c <= pjh) {
+                const int2 v = pb[c - pjl];
+                if (v.x >= thr_prev) {
+                    d = v.x;
+                    xv = v.y;
+                    s = psb[c - pjl];
+                }
+            }
+        };
+
+        // ---- pass 1: M, X, their from-states; the Y-scan terms ----------------
+        // Z[j] = Y[j] + (j - jl) E = max(Z[j-1], M[j-1] - (O+E) + (j - jl) E)
+        int32_t tot = INT32_MIN, alast = INT32_MIN;
+        {
+            Win qw = {0, 0, 0};
+            int64_t qbase = -64;  // query index of qw's base 0
+            int32_t dl = 0, xl = 0;
+            uint32_t sl = 0;
+            if (cnt) {
+                const uint32_t j0 = jl + c0;
+                if (j0 >= 1) prev(j0 - 1, dl, xl, sl);
+            }
+            for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t j = jl + c0 + k;
+                int32_t dj, xj;
+                uint32_t sj;
+                prev(j, dj, xj, sj);
+                int32_t mv;
+                uint32_t nib;
+                if (i == 0) {  // row 0: M only at the anchor cell, no X (src/gapped_extend.rs:245)
+                    mv = j == 0 ? 0 : kNegInf;
+                    xj = kNegInf;
+                    nib = 0;
+                } else {
+                    nib = sj & 4u;  // X from M or X (src/gapped_extend.rs:293-312)
+                    if (j == 0) {   // column 0 has no M
+                        mv = kNegInf;
+                    } else {        // src/gapped_extend.rs:339-363
+                        const int64_t qi = (int64_t)j - 1;
+                        if (qi - qbase >= 32 || qi < qbase) {
+                            qbase = qi;
+                            qw = dir_win(pr.Q, pr.qdir > 0 ? pr.q0 + (uint64_t)qi : pr.q0 - (uint64_t)qi, pr.qdir);
+                        }
+                        mv = dl + rs(win_qbyte(qw, (uint32_t)(qi - qbase)));
+                        nib |= sl & 3u;
+                    }
+                }
+                cb_[c0 + k] = make_int2(mv, xj);
+                csb[c0 + k] = (uint8_t)nib;
+                const int32_t a = mv - oe + (int32_t)(j - jl + 1) * e;
+                tot = max(tot, a);
+                alast = a;
+                dl = dj;
+                sl = sj;
+            }
+        }
+        const int32_t incl = wave_scan_max(tot);
+        int32_t excl = wave_shr1(incl, INT32_MIN);
+        int32_t aprev = wave_shr1(alast, INT32_MIN);
+        if (lane == 63) {
+            w_tot[wv] = incl;
+            w_alast[wv] = alast;
+        }
+        __syncthreads();  // (1) wavefront totals of the Y scan
+        for (int v = 0; v < wv; v++) excl = max(excl, w_tot[v]);
+        if (lane == 0 && wv > 0) aprev = w_alast[wv - 1];
+
+        // ---- pass 2: Y, the new row, traceback nibbles, the lane's row best ----
+        int32_t rmax = INT32_MIN;
+        uint32_t rj = 0, rst = kStY;
+        {
+            int32_t z = max(excl, jl >= 1 ? lead_y : kNegInf);
+            int32_t an = aprev;  // Y-scan term of the cell left of the current one
+            uint32_t byte = 0;
+            uint8_t *const trow = tb0 + (uint64_t)i * bs.rb;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t j = jl + c0 + k;
+                const int2 mx = cb_[c0 + k];
+                uint32_t nib = csb[c0 + k];
+                uint32_t yb;
+                if (k == 0) {
+                    yb = t == 0 ? ybit_lead : (z != an ? 1u : 0u);
+                } else {
+                    yb = an < z ? 1u : 0u;
+                    z = max(z, an);
+                }
+                int32_t y = z - (int32_t)(j - jl) * e;
+                if (j == 0) {  // column 0 has no Y
+                    y = kNegInf;
+                    yb = 0;
+                    nib &= 4u;
+                }
+                nib |= yb << 3;
+                const int32_t mv = mx.x, xv = mx.y;
+                const int32_t cb = max(max(mv, xv), y);
+                const uint32_t st = mv == cb ? kStM : (xv == cb ? kStX : kStY);
+                const int32_t tm = mv - oe, tx = xv - e;
+                cb_[c0 + k] = make_int2(cb, max(tm, tx));
+                csb[c0 + k] = (uint8_t)(st | (tm >= tx ? 0u : 4u));
+                if (cb > rmax) {  // first strict maximum
+                    rmax = cb;
+                    rj = j;
+                    rst = st;
+                }
+                an = mv - oe + (int32_t)(j - jl + 1) * e;
+                // nibbles in window order, low nibble first; C is even, so a
+                // lane's bytes are its own
+                byte |= nib << (4 * (k & 1u));
+                if ((k & 1u) || k + 1 == cnt) {
+                    trow[(c0 + k) >> 1] = (uint8_t)byte;
+                    byte = 0;
+                }
+            }
+        }
+        {
+            const int32_t wmax = __builtin_amdgcn_readlane(wave_scan_max(rmax), 63);
+            const unsigned long long hasmax = __ballot(cnt > 0 && rmax == wmax);
+            if (hasmax) {
+                const int src = __ffsll((long long)hasmax) - 1;
+                if (lane == src) {
+                    w_max[wv] = wmax;
+                    w_j[wv] = rj;
+                    w_st[wv] = rst;
+                }
+            } else if (lane == 0) {
+                w_max[wv] = INT32_MIN;
+            }
+        }
+        __syncthreads();  // (2) row best per wavefront
+        {
+            int32_t gmax = INT32_MIN;
+            uint32_t gw = 0;
+#pragma unroll
+            for (int v = 0; v < kDpWaves; v++) {
+                const int32_t x = w_max[v];
+                if (x > gmax) {  // strict: the earliest wavefront holding the maximum
+                    gmax = x;
+                    gw = v;
+                }
+            }
+            if (gmax > best_score) {  // src/gapped_extend.rs:365-380
+                best_score = gmax;
+                bi = i;
+                bj = w_j[gw];
+                bst = w_st[gw];
+            }
+        }
+        const int32_t thr = sat_sub(best_score, ydrop);
+
+        // ---- pass 3: the alive band (src/gapped_extend.rs:388-412) ------------
+        uint32_t alo = 0xFFFFFFFFu, ahi = 0;
+        for (uint32_t k = 0; k < cnt; k++) {
+            if (cb_[c0 + k].x >= thr) {
+                const uint32_t j = jl + c0 + k;
+                alo = min(alo, j);
+                ahi = j;
+            }
+        }
+        {
+            const unsigned long long anyalive = __ballot(alo != 0xFFFFFFFFu);
+            const int fl = anyalive ? __ffsll((long long)anyalive) - 1 : 0;
+            const int ll = anyalive ? 63 - __clzll((long long)anyalive) : 0;
+            const uint32_t wl = __builtin_amdgcn_readlane(alo, fl);
+            const uint32_t wh = __builtin_amdgcn_readlane(ahi, ll);
+            if (lane == 0) {
+                w_lo[wv] = anyalive ? wl : 0xFFFFFFFFu;
+                w_hi[wv] = wh;
+            }
+        }
+        if (t == 0) win0[i] = make_uint2(jl, jh);
+        rows_done = i + 1;
+        __syncthreads();  // (3) alive band; the new row is written
+        uint32_t nlo = 0xFFFFFFFFu, nhi = 0;
+        bool alive = false;
+#pragma unroll
+        for (int v = 0; v < kDpWaves; v++) {
+            const uint32_t l = w_lo[v];
+            if (l != 0xFFFFFFFFu) {
+                if (!alive) nlo = l;
+                nhi = w_hi[v];
+                alive = true;
+            }
+        }
+        if (!alive) break;  // whole row dead (src/gapped_extend.rs:407-409)
+        lo = nlo;
+        hi = nhi;
+        pjl = jl;
+        pjh = jh;
+        thr_prev = thr;
+        cur ^= 1u;  // every read of the row now overwritten was before (1)
+    }
+    if (t == 0) {
+        out.score = best_score;
+        out.bi = bi;
+        out.bj = bj;
+        out.bstate = bst;
+        out.rows = rows_done;
+        out.truncated = truncated;
+        out.cells = cells;
+        res[sidx] = out;
+    }
+}
+
+// The walk of a k_dp_band side (tb_walk's runs, one wavefront, reading the
+// side's traceback rows and windows straight from device memory).
+__global__ __launch_bounds__(64) void k_dp_traceback_band(const DpRes *__restrict__ res,
+                                                          const BandSide *__restrict__ bsides, uint32_t nside,
+                                                          const uint8_t *__restrict__ tb, const uint2 *__restrict__ wins,
+                                                          const uint64_t *__restrict__ run_off,
+                                                          gxy_edit_run *__restrict__ runs, uint32_t *__restrict__ nruns) {
+    const uint32_t p = blockIdx.x;
+    if (p >= nside) return;
+    const DpRes r = res[p];
+    const BandSide bs = bsides[p];
+    const int lane = threadIdx.x;
+    gxy_edit_run *const out = runs + run_off[p];
+    uint32_t nr = 0;
+    if (!r.truncated && !(r.bi == 0 && r.bj == 0) && r.rows != 0) {
+        const uint8_t *const tb0 = tb + bs.tb;
+        const uint2 *const win0 = wins + bs.win;
+        uint32_t i = r.bi, j = r.bj, st = r.bstate;
+        uint32_t cur_op = 0xFFFFFFFFu, cur_n = 0;
+        const uint32_t cap = r.bi + r.bj;
+        while (i > 0 || j > 0) {
+            // a consistent traceback never leaves the swept cells
+            if (i >= r.rows || (st == kStM && (i == 0 || j == 0)) || (st == kStX && i == 0) ||
+                (st == kStY && j == 0) || st > kStY || nr >= cap)
+                break;
+            // lane k takes step k of a run of the current op (as tb_walk)
+            const uint32_t dk = (uint32_t)lane;
+            const uint32_t di = st == kStY ? 0u : dk, dj = st == kStX ? 0u : dk;
+            bool valid = false;
+            uint32_t next = kStY + 1u;
+            if (di <= i && dj <= j) {
+                const uint32_t ci = i - di, cj = j - dj;
+                const bool edge_ok = st == kStM ? (ci > 0 && cj > 0) : st == kStX ? ci > 0 : cj > 0;
+                const uint2 w = win0[ci];
+                if (edge_ok && cj >= w.x && cj <= w.y) {
+                    valid = true;
+                    const uint32_t idx = cj - w.x;
+                    const uint32_t nb = (tb0[(uint64_t)ci * bs.rb + (idx >> 1)] >> (4 * (idx & 1u))) & 0xFu;
+                    next = st == kStM ? (nb & 3u)
+                                      : st == kStX ? (((nb >> 2) & 1u) ? kStX : kStM)
+                                                   : (((nb >> 3) & 1u) ? kStY : kStM);
+                }
+            }
+            const uint64_t ends = __ballot(!valid || next != st);
+            uint32_t nstep;
+            bool stop = false;
+            uint32_t nst = st;
+            if (ends == 0) {
+                nstep = 64;
+            } else {
+                const uint32_t kf = (uint32_t)__ffsll((long long)ends) - 1u;
+                const bool vf = (__ballot(valid) >> kf) & 1u;
+                if (vf) {
+                    nstep = kf + 1u;
+                    nst = (uint32_t)__shfl((int)next, (int)kf);
+                } else {
+                    nstep = kf;
+                    stop = true;
+                }
+            }
+            const uint32_t op = st == kStM ? GXY_OP_MATCH : st == kStX ? GXY_OP_DELETE : GXY_OP_INSERT;
+            if (nstep) {
+                if (op == cur_op) {
+                    cur_n += nstep;
+                } else {
+                    if (cur_n) {
+                        if (nr >= cap) break;
+                        if (lane == 0) out[nr] = gxy_edit_run{cur_op, cur_n};
+                        nr++;
+                    }
+                    cur_op = op;
+                    cur_n = nstep;
+                }
+                if (st != kStY) i -= nstep;
+                if (st != kStX) j -= nstep;
+            }
+            if (stop) break;
+            st = nst;
+        }
+        if (cur_n && nr < cap) {
+            if (lane == 0) out[nr] = gxy_edit_run{cur_op, cur_n};
+            nr++;
+        }
+    }
+    if (lane == 0) nruns[p] = nr;
+}
+
 template <class S>
 __global__ void k_choose_anchor(const gxy_hsp *__restrict__ hsps, uint64_t n, const Blk *const *__restrict__ Ts,
                                 const Blk *const *__restrict__ Qs, S sc, uint32_t window,
@@ -1428,6 +1831,136 @@ uint32_t band_width_host(int32_t oe, int32_t e, int32_t ydrop, uint64_t m) {
     return (uint32_t)std::min<uint64_t>(4 * br + 4, m + 1);
 }
 
+
+// The sides whose band is wider than kMaxW, through k_dp_band in groups that
+// fit a memory budget: each side reserves traceback rows for its expected
+// sweep (est_v) and, if it outruns them, is swept again with room for all of
+// its rows.  Results, runs and cells as the passes of gapped_batch_dev give
+// them.
+int band_sides_dev(gxy_ctx *ctx, const std::vector<DpProb> &probs, const std::vector<uint32_t> &wide,
+                   const std::vector<uint32_t> &est_v, const gxy_scoring &sc, int32_t oe, int32_t e, int32_t ydrop,
+                   std::vector<DpRes> *res, std::vector<std::vector<gxy_edit_run>> *run_store,
+                   std::vector<const gxy_edit_run *> *side_ptr, std::vector<uint32_t> *side_n, uint64_t *cells) {
+    const uint64_t np = probs.size();
+    auto *dprob = (DpProb *)scratch(ctx, kSlotDpProb, np * sizeof(DpProb));
+    if (!dprob) return set_err(ctx, GXY_ENOMEM, "DP scratch allocation failed");
+    GXY_HIP(ctx, hipMemcpyAsync(dprob, probs.data(), np * sizeof(DpProb), hipMemcpyHostToDevice, ctx->stream));
+    size_t free_b = 0, total_b = 0;
+    GXY_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+    const uint64_t budget = std::max<uint64_t>(256ull << 20, std::min<uint64_t>(16ull << 30, free_b / 4));
+    std::vector<uint32_t> cap(np, 0);
+    for (uint32_t x : wide) cap[x] = (uint32_t)std::min<uint64_t>((uint64_t)probs[x].n + 1, (uint64_t)est_v[x] + 64);
+    auto bw_of = [&](uint32_t x) { return band_width_host(oe, e, ydrop, probs[x].m); };
+    auto rb_of = [&](uint32_t x) { return (uint32_t)(((uint64_t)bw_of(x) + 1) / 2 + 15) & ~15u; };
+    auto bytes_of = [&](uint32_t x) {
+        return (uint64_t)cap[x] * (rb_of(x) + sizeof(uint2)) + 2ull * bw_of(x) * (sizeof(int2) + 1);
+    };
+    std::vector<uint32_t> todo = wide;
+    while (!todo.empty()) {
+        // a group within the budget (at least one side)
+        size_t ng = 0;
+        uint64_t used = 0;
+        while (ng < todo.size() && (ng == 0 || used + bytes_of(todo[ng]) <= budget)) used += bytes_of(todo[ng++]);
+        std::vector<BandSide> bs(ng);
+        uint64_t tb_b = 0, nwin = 0, ncell = 0;
+        for (size_t y = 0; y < ng; y++) {
+            const uint32_t x = todo[y];
+            bs[y].tb = tb_b;
+            bs[y].win = nwin;
+            bs[y].rowbuf = ncell;
+            bs[y].cap_rows = cap[x];
+            bs[y].rb = rb_of(x);
+            tb_b += (uint64_t)cap[x] * bs[y].rb;
+            nwin += cap[x];
+            ncell += 2ull * bw_of(x);
+        }
+        // side descriptors | list | results | run offsets | run counts
+        const size_t o_bs = 0, o_list = o_bs + ng * sizeof(BandSide), o_res = (o_list + ng * 4 + 15) & ~15ull,
+                     o_roff = o_res + ng * sizeof(DpRes), o_nr = o_roff + ng * 8, o_end = o_nr + ng * 4;
+        auto *work = (uint8_t *)scratch(ctx, kSlotDpOut, o_end);
+        auto *dtb = (uint8_t *)scratch(ctx, kSlotDpTb, std::max<uint64_t>(tb_b, 16));
+        auto *dwin = (uint2 *)scratch(ctx, kSlotDpRows, nwin * sizeof(uint2));
+        auto *drows = (uint8_t *)scratch(ctx, kSlotDpPersist, ncell * (sizeof(int2) + 1));
+        if (!work || !dtb || !dwin || !drows)
+            return set_err(ctx, GXY_ENOMEM, "gapped extension of a %u-cell band: device allocation failed",
+                           bw_of(todo[0]));
+        auto *dbs = (BandSide *)(work + o_bs);
+        auto *dlist = (uint32_t *)(work + o_list);
+        auto *dres = (DpRes *)(work + o_res);
+        auto *droff = (uint64_t *)(work + o_roff);
+        auto *dnr = (uint32_t *)(work + o_nr);
+        GXY_HIP(ctx, hipMemcpyAsync(dbs, bs.data(), ng * sizeof(BandSide), hipMemcpyHostToDevice, ctx->stream));
+        GXY_HIP(ctx, hipMemcpyAsync(dlist, todo.data(), ng * 4, hipMemcpyHostToDevice, ctx->stream));
+        {
+            KernelTimer kt_(ctx, "dp_forward");
+            if (dp_scores_i8(sc))
+                k_dp_band<ScorerI8><<<(unsigned)ng, kDpThreads, 0, ctx->stream>>>(
+                    dprob, dlist, dbs, (uint32_t)ng, make_scorer_i8(sc), oe, e, ydrop, dtb, dwin, (int2 *)drows,
+                    drows + ncell * sizeof(int2), dres);
+            else
+                k_dp_band<ScorerI32><<<(unsigned)ng, kDpThreads, 0, ctx->stream>>>(
+                    dprob, dlist, dbs, (uint32_t)ng, make_scorer_i32(sc), oe, e, ydrop, dtb, dwin, (int2 *)drows,
+                    drows + ncell * sizeof(int2), dres);
+        }
+        GXY_LAUNCH_CHECK(ctx);
+        std::vector<DpRes> pr(ng);
+        GXY_HIP(ctx, hipMemcpyAsync(pr.data(), dres, ng * sizeof(DpRes), hipMemcpyDeviceToHost, ctx->stream));
+        GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // walk the finished sides back into bi + bj reserved runs each
+        std::vector<uint64_t> roff(ng, 0);
+        uint64_t total_runs = 0;
+        for (size_t y = 0; y < ng; y++) {
+            roff[y] = total_runs;
+            if (!pr[y].truncated) total_runs += (uint64_t)pr[y].bi + pr[y].bj;
+        }
+        std::vector<uint32_t> nr(ng, 0);
+        std::vector<gxy_edit_run> sr;
+        std::vector<uint64_t> poff(ng, 0);
+        if (total_runs > 0) {
+            auto *druns = (gxy_edit_run *)scratch(ctx, kSlotDpRuns, total_runs * sizeof(gxy_edit_run));
+            if (!druns) return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+            GXY_HIP(ctx, hipMemcpyAsync(droff, roff.data(), ng * 8, hipMemcpyHostToDevice, ctx->stream));
+            {
+                KernelTimer kt_(ctx, "dp_traceback");
+                k_dp_traceback_band<<<(unsigned)ng, 64, 0, ctx->stream>>>(dres, dbs, (uint32_t)ng, dtb, dwin, droff,
+                                                                          druns, dnr);
+            }
+            GXY_LAUNCH_CHECK(ctx);
+            GXY_HIP(ctx, hipMemcpyAsync(nr.data(), dnr, ng * 4, hipMemcpyDeviceToHost, ctx->stream));
+            GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            std::vector<gxy_edit_run> all(total_runs);
+            GXY_HIP(ctx, hipMemcpyAsync(all.data(), druns, total_runs * sizeof(gxy_edit_run), hipMemcpyDeviceToHost,
+                                        ctx->stream));
+            GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            uint64_t packed = 0;
+            for (size_t y = 0; y < ng; y++) {
+                poff[y] = packed;
+                packed += nr[y];
+            }
+            sr.resize(packed);
+            for (size_t y = 0; y < ng; y++)
+                std::copy(all.begin() + roff[y], all.begin() + roff[y] + nr[y], sr.begin() + poff[y]);
+        }
+        std::vector<uint32_t> again;
+        for (size_t y = 0; y < ng; y++) {
+            const uint32_t x = todo[y];
+            if (pr[y].truncated) {  // more rows than reserved: again, with room for every row
+                cap[x] = probs[x].n + 1;
+                again.push_back(x);
+                ctx->dp_truncated++;
+                continue;
+            }
+            *cells += pr[y].cells;
+            (*res)[x] = pr[y];
+            (*side_ptr)[x] = sr.data() + poff[y];
+            (*side_n)[x] = nr[y];
+        }
+        run_store->push_back(std::move(sr));
+        todo.erase(todo.begin(), todo.begin() + ng);
+        todo.insert(todo.begin(), again.begin(), again.end());
+    }
+    return GXY_OK;
+}
 }  // namespace
 
 namespace gxy {
@@ -1490,10 +2023,6 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         l.qdir = -1;
         l.n = (uint32_t)at;
         l.m = (uint32_t)aq;
-        for (int s2 = 0; s2 < 2; s2++)
-            if (band_width_host(oe, e, gp.y_drop, probs[2 * k + s2].m) > (uint32_t)kMaxW)
-                return set_err(ctx, GXY_EUNSUP, "band width exceeds %d cells (y_drop/gap_extend too large)",
-                               kMaxW);
     }
     // anchor column scores
     std::vector<const Blk *> tp(2 * n);
@@ -1530,11 +2059,13 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
     std::vector<std::vector<gxy_edit_run>> run_store;
     std::vector<const gxy_edit_run *> side_ptr(np, nullptr);
     std::vector<uint32_t> side_n(np, 0);
-    std::vector<uint32_t> pending;
+    std::vector<uint32_t> pending, wide;  // wide: bands over kMaxW cells, swept by k_dp_band
     uint64_t need_rows = 0;
     for (uint64_t x = 0; x < np; x++) {
         if (probs[x].n == 0 || probs[x].m == 0) {
             res[x] = DpRes{0, 0, 0, kStM, 0, 0, kNoChunk, 0, 0};
+        } else if (band_width_host(oe, e, gp.y_drop, probs[x].m) > (uint32_t)kMaxW) {
+            wide.push_back((uint32_t)x);
         } else {
             pending.push_back((uint32_t)x);
             need_rows += std::min<uint64_t>((uint64_t)probs[x].n + 1, 1u << 20);
@@ -1860,6 +2391,11 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
             head += npp - again.size();
             std::copy(again.begin(), again.end(), pending.begin() + head);
         }
+    }
+    if (!wide.empty()) {
+        const int rc = band_sides_dev(ctx, probs, wide, est_v, sc, oe, e, gp.y_drop, &res, &run_store, &side_ptr,
+                                      &side_n, &cells);
+        if (rc != GXY_OK) return rc;
     }
     lap(3);
     if (cells_out) *cells_out += cells;
