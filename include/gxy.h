@@ -526,7 +526,7 @@ typedef struct gxy_server_params {
     uint64_t hbm_budget_bytes; /* device bytes the queries of one iteration may hold (0: 16 GiB) */
     uint64_t max_batch_bp;     /* query bases admitted per iteration (0: 256 Mbp)                */
     int32_t co_batch;          /* 0: one query per iteration (the plain per-query driver)          */
-    int32_t dp_workers;        /* gapped-stage threads, one device stream each (0: 4)              */
+    int32_t dp_workers;        /* gapped-stage threads, one device stream each (0: 3)              */
 } gxy_server_params;
 
 typedef struct gxy_request_stats {

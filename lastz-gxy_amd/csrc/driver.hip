@@ -1495,7 +1495,9 @@ int gxy_server_create(int device, const gxy_dseq *const *targets, uint64_t n_tar
             S->sep = (uint32_t)std::max<uint64_t>(64, walk);
             S->align = (uint32_t)(32 * cfg->step / std::__gcd<uint64_t>(32, cfg->step));
         }
-        const int nw = cfg->gapped_enabled && params->co_batch ? (params->dp_workers > 0 ? params->dp_workers : 4) : 0;
+        // one search stream and three gapped workers by default: the device's
+        // four hardware queues, one stream each
+        const int nw = cfg->gapped_enabled && params->co_batch ? (params->dp_workers > 0 ? params->dp_workers : 3) : 0;
         size_t free_b = 0, total_b = 0;
         GXY_HIP(S->ctx, hipMemGetInfo(&free_b, &total_b));
         for (int w = 0; w < nw; w++) {
