@@ -16,13 +16,14 @@ using namespace gxy;
 
 namespace {
 
+// (word, position) of windows w0 .. w0 + nwin - 1 into items[0 .. nwin)
 __global__ void k_table_items(const Blk *__restrict__ seq, uint64_t len, DevPattern pat,
-                              uint64_t step, uint64_t nwin, int respect_mask,
+                              uint64_t step, uint64_t w0, uint64_t nwin, int respect_mask,
                               uint64_t *__restrict__ items, unsigned long long *__restrict__ nvalid) {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = false;
     if (w < nwin) {
-        const uint64_t pos = w * step;
+        const uint64_t pos = (w0 + w) * step;
         const uint64_t sentinel = 1ull << (2 * pat.weight);
         uint64_t word = sentinel;
         // SeedExtractor::word_at (src/seeds.rs:133-147)
@@ -49,7 +50,24 @@ __global__ void k_table_starts(const uint64_t *__restrict__ items, uint64_t nval
     const uint64_t key = i < nvalid ? (items[i] >> 32) : num_words;
     const int64_t prev = i == 0 ? -1 : (int64_t)(items[i - 1] >> 32);
     for (int64_t w = prev + 1; w <= (int64_t)key; w++) starts[w] = (uint32_t)i;
-    if (i < nvalid) positions[i] = (uint32_t)items[i];
+    if (positions && i < nvalid) positions[i] = (uint32_t)items[i];
+}
+
+// Tables of more than 2^31 windows are sorted as two halves A (the first
+// windows) and B; A's positions precede B's, so word w's bucket is A's part
+// then B's: starts = startsA + startsB, A's item i of word w goes to
+// startsB[w] + i and B's item j to startsA[w + 1] + j.
+__global__ void k_starts_sum(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b, uint64_t n,
+                             uint32_t *__restrict__ out) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < n) out[w] = a[w] + b[w];
+}
+__global__ void k_place_half(const uint64_t *__restrict__ items, uint64_t nvalid, const uint32_t *__restrict__ other,
+                             uint32_t shift, uint32_t *__restrict__ positions) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvalid) return;
+    const uint64_t it = items[i];
+    positions[other[(it >> 32) + shift] + i] = (uint32_t)it;
 }
 
 // Bitmap of non-empty words: one thread per word (coalesced reads of
@@ -151,8 +169,10 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     if (t->len > 0xFFFFFFFFull)
         return set_err(ctx, GXY_ERANGE, "pos_table overflow (> 4 Gbp)");  // src/pos_table.rs:56
     const uint64_t max_start = t->len >= pat.span ? t->len - pat.span : 0;
-    const uint64_t nwin = max_start / step + 1;
-    if (nwin > 0x7FFFFFFFull) return set_err(ctx, GXY_ERANGE, "too many windows");
+    const uint64_t nwin = max_start / step + 1;  // < 2^32: len < 2^32
+    // the radix sort takes at most 2^31 - 1 items: more windows sort as two halves
+    const bool split = nwin > 0x7FFFFFFFull;
+    const uint64_t nA = split ? nwin / 2 : nwin, nB = nwin - nA;
     auto *tab = new (std::nothrow) gxy_postable();
     if (!tab) return set_err(ctx, GXY_ENOMEM, "host allocation failed");
     tab->pattern = pat;
@@ -163,7 +183,8 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     auto *items = (uint64_t *)scratch(ctx, kSlotHits, nwin * sizeof(uint64_t));
     auto *items2 = (uint64_t *)scratch(ctx, kSlotPassKeys, nwin * sizeof(uint64_t));
     auto *cnt = (unsigned long long *)scratch(ctx, kSlotSpread, kSpreadBytes);
-    if (!items || !items2 || !cnt) {
+    auto *cntB = split ? (unsigned long long *)scratch(ctx, kSlotCounters, kSpreadBytes) : nullptr;
+    if (!items || !items2 || !cnt || (split && !cntB)) {
         delete tab;
         return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
     }
@@ -172,17 +193,21 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
         return set_err(ctx, code, "%s: %s", msg, hipGetErrorString(e));
     };
     hipError_t e = hipMemsetAsync(cnt, 0, kSpreadBytes, ctx->stream);
+    if (e == hipSuccess && split) e = hipMemsetAsync(cntB, 0, kSpreadBytes, ctx->stream);
     if (e != hipSuccess) return fail(GXY_EDEVICE, "memset", e);
     {
         KernelTimer kt_(ctx, "table_items");
-        k_table_items<<<grid_for(nwin, 256), 256, 0, ctx->stream>>>(t->d, t->len, pat, step, nwin, respect_mask,
-                                                                    items, cnt);
+        k_table_items<<<grid_for(nA, 256), 256, 0, ctx->stream>>>(t->d, t->len, pat, step, 0, nA, respect_mask,
+                                                                  items, cnt);
+        if (split)
+            k_table_items<<<grid_for(nB, 256), 256, 0, ctx->stream>>>(t->d, t->len, pat, step, nA, nB, respect_mask,
+                                                                      items + nA, cntB);
     }
     if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_table_items", e);
     size_t tmp_bytes = 0;
     const int end_bit = 32 + 2 * (int)pat.weight + 1;
-    (void)hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, items, items2, (int)nwin, 32, end_bit,
-                                      ctx->stream);
+    (void)hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, items, items2, (int)std::max(nA, nB), 32, end_bit,
+                                            ctx->stream);
     void *tmp = scratch(ctx, kSlotTmp, tmp_bytes);
     if (!tmp) {
         gxy_postable_free(tab);
@@ -190,26 +215,47 @@ int gxy::postable_build_impl(gxy_ctx *ctx, const gxy_dseq *t, const gxy_seed_pat
     }
     {
         KernelTimer kt_(ctx, "table_sort");
-        e = hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, items, items2, (int)nwin, 32, end_bit,
-                                              ctx->stream);
+        e = hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, items, items2, (int)nA, 32, end_bit, ctx->stream);
+        if (e == hipSuccess && split)
+            e = hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, items + nA, items2 + nA, (int)nB, 32, end_bit,
+                                                  ctx->stream);
     }
     if (e != hipSuccess) return fail(GXY_EDEVICE, "radix sort", e);
-    uint64_t nvalid = 0;
-    if (spread_sum(ctx, cnt, &nvalid) != GXY_OK) {
+    uint64_t nvalid = 0, nvalidB = 0;
+    if (spread_sum(ctx, cnt, &nvalid) != GXY_OK || (split && spread_sum(ctx, cntB, &nvalidB) != GXY_OK)) {
         gxy_postable_free(tab);
         return GXY_EDEVICE;
     }
-    tab->npos = nvalid;
+    tab->npos = nvalid + nvalidB;
     tab->d_starts = (uint32_t *)dev_alloc(ctx, (tab->num_words + 1) * sizeof(uint32_t), &tab->pool);
-    tab->d_positions = (uint32_t *)dev_alloc(ctx, std::max<uint64_t>(1, nvalid) * sizeof(uint32_t), &tab->pool);
+    tab->d_positions = (uint32_t *)dev_alloc(ctx, std::max<uint64_t>(1, tab->npos) * sizeof(uint32_t), &tab->pool);
     if (!tab->d_starts || !tab->d_positions) {
         gxy_postable_free(tab);
         return set_err(ctx, GXY_ENOMEM, "table allocation failed");
     }
-    {
+    if (!split) {
         KernelTimer kt_(ctx, "table_starts");
         k_table_starts<<<grid_for(nvalid + 1, 256), 256, 0, ctx->stream>>>(items2, nvalid, tab->num_words,
                                                                             tab->d_starts, tab->d_positions);
+    } else {
+        KernelTimer kt_(ctx, "table_starts");
+        const uint64_t ns = tab->num_words + 1;
+        auto *sA = (uint32_t *)scratch(ctx, kSlotCounts, ns * sizeof(uint32_t));
+        auto *sB = (uint32_t *)scratch(ctx, kSlotOffsets, ns * sizeof(uint32_t));
+        if (!sA || !sB) {
+            gxy_postable_free(tab);
+            return set_err(ctx, GXY_ENOMEM, "scratch allocation failed");
+        }
+        k_table_starts<<<grid_for(nvalid + 1, 256), 256, 0, ctx->stream>>>(items2, nvalid, tab->num_words, sA,
+                                                                            nullptr);
+        k_table_starts<<<grid_for(nvalidB + 1, 256), 256, 0, ctx->stream>>>(items2 + nA, nvalidB, tab->num_words,
+                                                                             sB, nullptr);
+        k_starts_sum<<<grid_for(ns, 256), 256, 0, ctx->stream>>>(sA, sB, ns, tab->d_starts);
+        if (nvalid)
+            k_place_half<<<grid_for(nvalid, 256), 256, 0, ctx->stream>>>(items2, nvalid, sB, 0, tab->d_positions);
+        if (nvalidB)
+            k_place_half<<<grid_for(nvalidB, 256), 256, 0, ctx->stream>>>(items2 + nA, nvalidB, sA, 1,
+                                                                           tab->d_positions);
     }
     if ((e = hipGetLastError()) != hipSuccess) return fail(GXY_EDEVICE, "k_table_starts", e);
     const uint64_t nbw = (tab->num_words + 31) / 32;
@@ -371,9 +417,9 @@ int gxy_postable_export(gxy_ctx *ctx, const gxy_postable *tab, uint32_t *starts,
         if (!ctx) return GXY_EINVAL;
         if (!tab) return set_err(ctx, GXY_EINVAL, "null table");
         const uint64_t nw = tab->num_words;
-        std::vector<uint32_t> ds(nw + 1), dp(std::max<uint64_t>(1, tab->npos));
+        std::vector<uint32_t> ds(nw + 1), dp(positions ? std::max<uint64_t>(1, tab->npos) : 0);
         GXY_HIP(ctx, hipMemcpyAsync(ds.data(), tab->d_starts, (nw + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
-        if (tab->npos)
+        if (tab->npos && positions)
             GXY_HIP(ctx, hipMemcpyAsync(dp.data(), tab->d_positions, tab->npos * 4, hipMemcpyDeviceToHost, ctx->stream));
         GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         uint64_t acc = 0;
@@ -381,8 +427,12 @@ int gxy_postable_export(gxy_ctx *ctx, const gxy_postable *tab, uint32_t *starts,
             const uint64_t idx = word_to_index(w, tab->pattern.weight);
             const uint32_t b = ds[idx], e = ds[idx + 1];
             if (starts && w < starts_cap) starts[w] = (uint32_t)acc;
-            for (uint32_t k = b; k < e; k++, acc++)
-                if (positions && acc < positions_cap) positions[acc] = dp[k];
+            if (positions) {
+                for (uint32_t k = b; k < e; k++, acc++)
+                    if (acc < positions_cap) positions[acc] = dp[k];
+            } else {
+                acc += e - b;
+            }
         }
         if (starts && nw < starts_cap) starts[nw] = (uint32_t)acc;
         return GXY_OK;
