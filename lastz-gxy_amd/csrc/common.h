@@ -372,6 +372,9 @@ struct gxy_ctx {
     // A second context for the driver's other-strand searches (created on
     // first use, destroyed with this one).
     gxy_ctx *helper = nullptr;
+    // A second context (with its own helper) for the driver's second chunk
+    // lane (created on first use, destroyed with this one).
+    gxy_ctx *lane = nullptr;
     uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(96 GiB, device / 3))
     uint64_t dp_pool_rows = 0; // traceback pool rows set by the caller (0: no row cap)
     int dp_kernel = 0;         // DP forward kernel (GXY_DP_*), 0: chosen by the library

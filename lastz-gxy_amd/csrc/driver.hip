@@ -828,56 +828,96 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 helper->xdrop_single = ctx->xdrop_single;
             }
         }
-        HostJobs helper_job(helper ? 1 : 0);
-
-        clk.lap(kTrSetup);
-        for (size_t c = 0; c < chunks.size(); c++) {
+        // A second lane (its own context and helper, so four streams in all)
+        // takes every other chunk: two chunks' searches run at once and fill
+        // each other's host round trips and table builds.  Each lane keeps the
+        // results of its chunks; they are taken here in chunk order, as one
+        // lane would give them.
+        gxy_ctx *lane = nullptr, *lane_helper = nullptr;
+        if (chunks.size() >= 2 && helper) {
+            if (!ctx->lane && gxy_ctx_create(ctx->device, &ctx->lane) == GXY_OK) ctx->lane->pool_on = true;
+            if (ctx->lane && !ctx->lane->helper && gxy_ctx_create(ctx->device, &ctx->lane->helper) == GXY_OK)
+                ctx->lane->helper->pool_on = true;
+            bool ok = ctx->lane && ctx->lane->helper;
+            for (gxy_ctx *c : {ctx->lane, ok ? ctx->lane->helper : nullptr}) {
+                if (ok && !c->stream && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+                    (void)hipGetLastError();
+                    ok = false;
+                }
+            }
+            if (ok) {
+                lane = ctx->lane;
+                lane_helper = lane->helper;
+                for (gxy_ctx *c : {lane, lane_helper}) {
+                    c->timing = ctx->timing;
+                    c->xdrop_single = ctx->xdrop_single;
+                }
+                // the chunk sequences, reverse complements and query byte copies
+                // were written on ctx's stream
+                GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            }
+        }
+        struct ChunkOut {
+            std::vector<std::vector<gxy_hsp>> shs;   // [query * 2 + strand slot]
+            std::vector<gxy_search_stats> sst;
+            int rc = GXY_OK;
+            bool done = false;
+        };
+        std::vector<ChunkOut> couts(chunks.size());
+        std::mutex cmu;
+        std::condition_variable ccv;
+        // chunk c's table and searches on a lane (lc: plus strand, on this
+        // thread; lh: the other strand, on the lane's helper thread)
+        auto search_chunk = [&](size_t c, gxy_ctx *lc, gxy_ctx *lh, HostJobs &hj) -> int {
             const Chunk &ch = chunks[c];
+            ChunkOut &co = couts[c];
+            co.shs.assign((size_t)n_queries * 2, {});
+            co.sst.assign((size_t)n_queries * 2, gxy_search_stats{});
             gxy_postable *tp;
-            clk.lap(kTrRecords);
-            int rc = postable_build_impl(ctx, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
-            clk.lap(kTrTable);
+            PhaseClock lclk(lc);
+            int rc = postable_build_impl(lc, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
+            lclk.lap(kTrTable);
             if (rc) return rc;
             Table table(tp);
             if (cfg->max_word_count > 0) {
-                rc = gxy_postable_prune(ctx, tp, cfg->max_word_count);
+                rc = gxy_postable_prune(lc, tp, cfg->max_word_count);
                 if (rc) return rc;
             }
             for (uint32_t qi = 0; qi < n_queries; qi++) {
-                std::vector<gxy_hsp> shs[2];
-                gxy_search_stats sst[2];
                 int src[2] = {GXY_OK, GXY_OK};
-                // Ungapped: Vec::dedup runs on a host worker with the records.
-                auto run_search = [&](size_t k, gxy_ctx *c) {
+                auto run_search = [&](size_t k, gxy_ctx *sc) {
                     const uint32_t strand = strands[k];
                     const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
                     const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
                     const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
-                    src[k] = search_dev_impl(c, tp, qs, &cfg->scoring, &sp, respect, &shs[k], &sst[k], qb,
-                                             cfg->gapped_enabled != 0);
+                    src[k] = search_dev_impl(sc, tp, qs, &cfg->scoring, &sp, respect, &co.shs[2 * qi + k],
+                                             &co.sst[2 * qi + k], qb, cfg->gapped_enabled != 0);
                 };
-                clk.lap(kTrRecords);
-                if (helper) {
-                    helper_job.submit([&] {
-                        (void)hipSetDevice(ctx->device);
-                        run_search(1, helper);
+                if (lh) {
+                    hj.submit([&] {
+                        (void)hipSetDevice(lc->device);
+                        run_search(1, lh);
                     });
-                    run_search(0, ctx);
-                    helper_job.wait_all();
-                    if (!src[0] && src[1]) {
-                        set_err(ctx, src[1], "%s", helper->err.c_str());
-                        return src[1];
-                    }
+                    run_search(0, lc);
+                    hj.wait_all();
+                    if (!src[0] && src[1]) return set_err(lc, src[1], "%s", lh->err.c_str());
                 } else {
-                    for (size_t k = 0; k < strands.size() && !src[0]; k++) run_search(k, ctx);
+                    for (size_t k = 0; k < strands.size() && !src[0]; k++) run_search(k, lc);
                 }
                 if (src[0]) return src[0];
-                clk = PhaseClock(ctx);
+            }
+            return GXY_OK;  // the chunk's table is released here; gapped work needs only the sequences
+        };
+        // the searches' HSPs of chunk c, taken in chunk order
+        auto take_chunk = [&](size_t c) {
+            const Chunk &ch = chunks[c];
+            ChunkOut &co = couts[c];
+            for (uint32_t qi = 0; qi < n_queries; qi++) {
                 for (size_t k = 0; k < strands.size(); k++) {
                     const uint32_t strand = strands[k];
                     const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
-                    std::vector<gxy_hsp> hsps = std::move(shs[k]);
-                    const gxy_search_stats &st = sst[k];
+                    std::vector<gxy_hsp> hsps = std::move(co.shs[2 * qi + k]);
+                    const gxy_search_stats &st = co.sst[2 * qi + k];
                     rs.hits += st.hits;
                     rs.columns += st.columns;
                     rs.hsps += st.emitted;
@@ -920,13 +960,66 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                         if (cfg->chain_enabled) gg->kept = best_chain(gg->kept);  // src/driver.rs:258-260
                         std::stable_sort(gg->kept.begin(), gg->kept.end(),
                                          [](const gxy_hsp &a, const gxy_hsp &b) { return a.score > b.score; });
-                        for (size_t k = 1; k < gg->kept.size(); k++)
-                            if (gg->kept[k].score == gg->kept[k - 1].score) rs.score_ties++;
+                        for (size_t k2 = 1; k2 < gg->kept.size(); k2++)
+                            if (gg->kept[k2].score == gg->kept[k2 - 1].score) rs.score_ties++;
                         gap_groups.push_back(std::move(gg));
                     }
                 }
             }
-            // the chunk's table is released here; gapped work needs only the sequences
+            co = ChunkOut{};
+        };
+
+        clk.lap(kTrSetup);
+        if (!lane) {
+            HostJobs helper_job(helper ? 1 : 0);
+            for (size_t c = 0; c < chunks.size(); c++) {
+                const int rc = search_chunk(c, ctx, helper, helper_job);
+                if (rc) return rc;
+                take_chunk(c);
+            }
+        } else {
+            // lane 0 (ctx, its helper) takes the even chunks, lane 1 the odd ones
+            std::atomic<bool> abort_lanes{false};
+            auto lane_loop = [&](int L, gxy_ctx *lc, gxy_ctx *lh) {
+                (void)hipSetDevice(lc->device);
+                HostJobs hj(1);
+                for (size_t c = (size_t)L; c < chunks.size() && !abort_lanes.load(); c += 2) {
+                    const int rc = search_chunk(c, lc, lh, hj);
+                    {
+                        std::lock_guard<std::mutex> lk(cmu);
+                        couts[c].rc = rc;
+                        couts[c].done = true;
+                    }
+                    ccv.notify_all();
+                    if (rc) break;
+                }
+            };
+            std::thread t0(lane_loop, 0, ctx, helper), t1(lane_loop, 1, lane, lane_helper);
+            int rc = GXY_OK;
+            for (size_t c = 0; c < chunks.size(); c++) {
+                {
+                    std::unique_lock<std::mutex> lk(cmu);
+                    ccv.wait(lk, [&] { return couts[c].done; });
+                    rc = couts[c].rc;
+                }
+                if (rc) {
+                    if (c % 2) set_err(ctx, rc, "%s", lane->err.c_str());
+                    abort_lanes = true;
+                    // a lane stopped by abort_lanes leaves later chunks undone
+                    for (size_t d = c + 1; d < chunks.size(); d++) {
+                        std::lock_guard<std::mutex> lk(cmu);
+                        if (!couts[d].done) {
+                            couts[d].done = true;
+                            couts[d].rc = rc;
+                        }
+                    }
+                    break;
+                }
+                take_chunk(c);
+            }
+            t0.join();
+            t1.join();
+            if (rc) return rc;
         }
         if (!gap_groups.empty()) {
             int rc = gapped_groups_dev(ctx, gap_groups, cfg, &run_store, &rs);
@@ -941,15 +1034,17 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             if (rc) return rc;
             for (auto &tg : tween_groups) finalize_group(*tg.g);
         }
-        if (helper) {  // the helper's x-drop counts as this context's
-            ctx->xd_first_cols += helper->xd_first_cols;
-            ctx->xd_conts += helper->xd_conts;
-            ctx->xd_first_pass += helper->xd_first_pass;
-            helper->xd_first_cols = helper->xd_conts = helper->xd_first_pass = 0;
-        }
-        if (helper && helper->timing) {  // the helper's kernel times count as this context's
-            timing_flush(helper);
-            for (const auto &t : helper->totals) {
+        // the helpers' and the second lane's x-drop counts and kernel times
+        // count as this context's
+        for (gxy_ctx *aux : {helper, lane, lane_helper}) {
+            if (!aux) continue;
+            ctx->xd_first_cols += aux->xd_first_cols;
+            ctx->xd_conts += aux->xd_conts;
+            ctx->xd_first_pass += aux->xd_first_pass;
+            aux->xd_first_cols = aux->xd_conts = aux->xd_first_pass = 0;
+            if (!aux->timing) continue;
+            timing_flush(aux);
+            for (const auto &t : aux->totals) {
                 bool found = false;
                 for (auto &u : ctx->totals)
                     if (u.first == t.first) {
@@ -960,17 +1055,18 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                     }
                 if (!found) ctx->totals.push_back(t);
             }
-            helper->totals.clear();
+            aux->totals.clear();
         }
-        // The helper's stream ends with the run (its scratch and block cache
-        // stay for the next run): a stream kept would take a share of the
-        // device's four hardware queues from streams that other contexts create
-        // later (a server started after a two-strand run answered the
-        // many-contig workload ~25 % slower).
-        if (ctx->helper && ctx->helper->stream) {
-            (void)hipStreamSynchronize(ctx->helper->stream);
-            (void)hipStreamDestroy(ctx->helper->stream);
-            ctx->helper->stream = nullptr;
+        // The helpers' and the lane's streams end with the run (their scratch
+        // and block caches stay for the next run): streams kept would take
+        // shares of the device's four hardware queues from streams that other
+        // contexts create later.
+        for (gxy_ctx *aux : {ctx->helper, ctx->lane, ctx->lane ? ctx->lane->helper : nullptr}) {
+            if (aux && aux->stream) {
+                (void)hipStreamSynchronize(aux->stream);
+                (void)hipStreamDestroy(aux->stream);
+                aux->stream = nullptr;
+            }
         }
         rs.tie_pairs += ties.load();
         for (uint64_t qi = 0; qi < n_queries; qi++) rs.query_bp += queries[qi]->len * strands.size();

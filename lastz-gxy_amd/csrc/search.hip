@@ -256,26 +256,39 @@ __global__ __launch_bounds__(256) void k_qexpand_w(uint64_t w0, uint64_t w1, uin
     const uint64_t orel = xc - c;  // this window's first slot, relative to the wavefront
     if (active && fits && m) {
         const uint32_t word = words[w];
-        uint32_t cum = (uint32_t)orel, k = kfirst;
-        while (m) {
-            const int i = (__ffs(m) - 1) >> 1;
-            const uint32_t cw = (m >> (2 * i)) & 3u;
-            m &= ~(3u << (2 * i));
-            const uint64_t v = variant1(word, weight, i);
-            uint32_t bk, len;
-            if (cw == 1) {  // one position: take it directly
-                bk = spos[v];
-                len = 1;
-            } else {
-                bk = starts[v];
-                len = starts[v + 1] - bk;
-            }
-            kb[wv][k] = bk;
-            kc[wv][k] = cum;
-            kl[wv][k] = (uint8_t)(lane | (cw == 1 ? 0x80 : 0));
-            cum += len;
-            k++;
+        // every non-empty variant's bucket: all loads issued before the first
+        // use, so a window pays one memory round trip, not one per bucket
+        // (the variant indices first, outside the branches: the word's load is
+        // then waited for once, not in every branch)
+        uint32_t vix[kMaxV1];
+#pragma unroll
+        for (int i = 0; i < kMaxV1; i++) {
+            vix[i] = (uint32_t)variant1(word, weight, i);
+            asm volatile("" : "+v"(vix[i]));  // kept here, not sunk into the branches
         }
+        uint32_t r0[kMaxV1] = {}, r1[kMaxV1] = {};
+#pragma unroll
+        for (int i = 0; i < kMaxV1; i++) {
+            const uint32_t cw = (m >> (2 * i)) & 3u;
+            if (cw) {
+                // one position: take it directly; else the bucket's bounds
+                r0[i] = *(cw == 1 ? spos + vix[i] : starts + vix[i]);
+                if (cw > 1) r1[i] = starts[vix[i] + 1];
+            }
+        }
+        uint32_t cum = (uint32_t)orel, k = kfirst;
+#pragma unroll
+        for (int i = 0; i < kMaxV1; i++) {
+            const uint32_t cw = (m >> (2 * i)) & 3u;
+            if (cw) {
+                kb[wv][k] = r0[i];
+                kc[wv][k] = cum;
+                kl[wv][k] = (uint8_t)(lane | (cw == 1 ? 0x80 : 0));
+                cum += cw == 1 ? 1u : r1[i] - r0[i];
+                k++;
+            }
+        }
+        m = 0;
     }
     __syncthreads();
     if (active) {

@@ -463,7 +463,7 @@ int gxy_ctx_release_cache(gxy_ctx *ctx) {
     return guarded(ctx, [&]() -> int {
         if (!ctx) return GXY_EINVAL;
         GXY_HIP(ctx, hipSetDevice(ctx->device));
-        for (gxy_ctx *c : {ctx, ctx->helper}) {
+        for (gxy_ctx *c : {ctx, ctx->helper, ctx->lane, ctx->lane ? ctx->lane->helper : nullptr}) {
             if (!c) continue;
             // a helper holds no stream between runs: its work has completed
             if (c->stream) GXY_HIP(ctx, hipStreamSynchronize(c->stream));
@@ -574,6 +574,7 @@ int gxy_ctx_reset_timing(gxy_ctx *ctx) {
 void gxy_ctx_destroy(gxy_ctx *ctx) {
     if (!ctx) return;
     if (ctx->helper) gxy_ctx_destroy(ctx->helper);
+    if (ctx->lane) gxy_ctx_destroy(ctx->lane);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     timing_flush(ctx);
