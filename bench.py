@@ -224,6 +224,21 @@ def hbm_xdrop_roofline(hits, xds, kernel_ms, launches, traffic=None):
             "bytes_model": "8 B/hit + 0.75 B/first-step column + 36 B/resumed hit + 24 B/first-step passing HSP"}
 
 
+def hbm_qscan_roofline(lookups, hits, kernel_ms, launches, traffic=None):
+    """HBM roofline of k_qexpand_w, the query scan's expansion (timed as
+    "qexpand"): the reference's bucket reads for every lookup (8 B: a word's two
+    `starts` entries, src/pos_table.rs:102-108), 4 B per hit position and 8 B
+    per hit record written — the lookups' bytes counted on this kernel, though
+    the count pass (k_qcount_v) does part of that work."""
+    alg = 8 * lookups + 12 * hits
+    achieved = alg / (kernel_ms / 1e3) / 1e9 if kernel_ms else None
+    return {"bound": "hbm", "kernel": "k_qexpand_w", "achieved": round(achieved, 2) if achieved else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
+            "launches": launches, "avg_launch_ms": round(kernel_ms / max(1, launches), 4) if kernel_ms else None,
+            "algorithmic_bytes_per_launch": round(alg / max(1, launches)), "traffic": traffic,
+            "bytes_model": "8 B/lookup (query windows x seed variants) + 4 B/hit position + 8 B/hit record"}
+
+
 def dominant_roofline(ktimes, st, xds):
     """Roofline of whichever of the DP forward and x-drop first-step kernels took longer."""
     dp_ms, dp_n = ktimes.get("dp_forward", (0.0, 0))
@@ -629,18 +644,43 @@ def pair_leg(args, workload, steps, warmup):
     xds = tuple(b - a for a, b in zip(xd0, ctx.xdrop_stats()))
     dom_name, (dom_ms, dom_launches) = max(ktime.items(), key=lambda kv: kv[1][0])
     roof = None
-    if dom_name == "extend":
-        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-        # runs of this workload (scaled by a 1 GiB copy), when present
-        traffic = None
-        prof = os.path.join(ROOT, "profiles", "k_xdrop_first_chrom48M.json")
+
+    def prof_of(name):
+        # the committed rocprofv3 summary of this workload: HBM bytes per launch
+        # (FETCH_SIZE/WRITE_SIZE, scaled by a 1 GiB copy) and the kernel's
+        # duration in the counter runs, where kernels run one at a time
+        prof = os.path.join(ROOT, "profiles", name)
         if workload == "chrom" and os.path.exists(prof):
-            traffic = json.load(open(prof))["traffic_bytes_per_launch"]
-        roof = hbm_xdrop_roofline(st.hits * steps, xds, dom_ms, dom_launches, traffic)
+            return json.load(open(prof))
+        return {}
+
+    def with_serialized(r, prof):
+        r["traffic"] = prof.get("traffic_bytes_per_launch")
+        ser = prof.get("avg_ns_serialized")
+        if ser and r.get("algorithmic_bytes_per_launch"):
+            # the same bytes over the kernel's own duration (no other stream on the device)
+            r["serialized"] = {"avg_launch_ms": round(ser / 1e6, 4),
+                               "achieved": round(r["algorithmic_bytes_per_launch"] / (ser / 1e9) / 1e9, 2),
+                               "source": prof.get("source")}
+            r["serialized"]["frac"] = round(r["serialized"]["achieved"] / HBM_PEAK_GBS, 5)
+        return r
+
+    def xdrop_roof():
+        xms, xn = ktime.get("extend", (0.0, 0))
+        r = with_serialized(hbm_xdrop_roofline(st.hits * steps, xds, xms, xn),
+                            prof_of("k_xdrop_first_chrom48M.json"))
         ems, en = ktime.get("extend_cont", (0.0, 0))
         if en:
-            roof["resume_kernel"] = {"kernel": "k_xdrop (resumed walks)", "launches": en,
-                                     "avg_launch_ms": round(ems / en, 4)}
+            r["resume_kernel"] = {"kernel": "k_xdrop (resumed walks)", "launches": en,
+                                  "avg_launch_ms": round(ems / en, 4)}
+        return r
+
+    if dom_name == "extend":
+        roof = xdrop_roof()
+    elif dom_name == "qexpand":
+        roof = with_serialized(hbm_qscan_roofline(st.lookups * steps, st.hits * steps, dom_ms, dom_launches),
+                               prof_of("k_qexpand_w_chrom48M.json"))
+        roof["second_kernel"] = xdrop_roof()
     elif dom_name == "dp_forward":
         roof = valu_roofline(st.dp_cells * steps, dom_ms, dom_launches)
     else:
@@ -648,6 +688,9 @@ def pair_leg(args, workload, steps, warmup):
                 "frac": None, "traffic": None, "kernel": dom_name,
                 "avg_launch_ms": round(dom_ms / max(1, dom_launches), 4), "launches": dom_launches}
 
+    # kernels of the driver's four streams overlap: their launch intervals
+    # (the `avg_launch_ms` above) are shared with the other streams' kernels
+    roof["stream_concurrency"] = round(sum(v[0] for v in ktime.values()) / max(1e-9, 1000.0 * elapsed), 2)
     if roof["bound"] == "hbm":
         # `peak` stays the published 8 TB/s; a measured streaming-copy rate rides beside it
         copy_gbs = measured_copy_gbs()

@@ -464,6 +464,7 @@ typedef struct gxy_run_stats {
     uint64_t tie_pairs;      /* order-sensitive ties (see gxy_search_stats) */
     uint64_t score_ties;     /* equal-score neighbours in the gapped order  */
     uint64_t passing;        /* x-drop extensions reaching hsp_threshold    */
+    uint64_t lookups;        /* table lookups: valid query windows x seed variants */
 } gxy_run_stats;
 
 /* Config::default() (src/driver.rs:91-111) with HOXD70 (src/scoring.rs:44-58). */

@@ -923,6 +923,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                     rs.hsps += st.emitted;
                     rs.tie_pairs += st.tie_pairs;
                     rs.passing += st.passing;
+                    rs.lookups += st.lookups;
                     if (hsps.empty()) continue;
                     groups.push_back(std::make_unique<Group>(Group{ch.parent, qi, strand, c, {}}));
                     Group &g = *groups.back();

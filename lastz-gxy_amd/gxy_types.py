@@ -114,7 +114,7 @@ class RecordSet(C.Structure):
 class RunStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("chunks", "query_bp", "hits", "columns", "hsps",
                                            "dp_problems", "dp_cells", "tie_pairs",
-                                           "score_ties", "passing")]
+                                           "score_ties", "passing", "lookups")]
 
 
 # ---------------------------------------------------------------------------
