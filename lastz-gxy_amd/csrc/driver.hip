@@ -829,12 +829,14 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             }
         }
         // A second lane (its own context and helper, so four streams in all)
-        // takes every other chunk: two chunks' searches run at once and fill
-        // each other's host round trips and table builds.  Each lane keeps the
-        // results of its chunks; they are taken here in chunk order, as one
+        // takes every other (chunk, query) item: two items' searches run at
+        // once and fill each other's host round trips and table builds.  Each
+        // lane builds the tables of its items' chunks and keeps its items'
+        // results; they are taken here in item order (chunk-major), as one
         // lane would give them.
+        const size_t nitems = chunks.size() * (size_t)n_queries;
         gxy_ctx *lane = nullptr, *lane_helper = nullptr;
-        if (chunks.size() >= 2 && helper) {
+        if (nitems >= 2 && helper) {
             if (!ctx->lane && gxy_ctx_create(ctx->device, &ctx->lane) == GXY_OK) ctx->lane->pool_on = true;
             if (ctx->lane && !ctx->lane->helper && gxy_ctx_create(ctx->device, &ctx->lane->helper) == GXY_OK)
                 ctx->lane->helper->pool_on = true;
@@ -857,67 +859,79 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 GXY_HIP(ctx, hipStreamSynchronize(ctx->stream));
             }
         }
-        struct ChunkOut {
-            std::vector<std::vector<gxy_hsp>> shs;   // [query * 2 + strand slot]
-            std::vector<gxy_search_stats> sst;
+        struct ItemOut {
+            std::vector<gxy_hsp> shs[2];  // per strand slot
+            gxy_search_stats sst[2] = {};
             int rc = GXY_OK;
             bool done = false;
         };
-        std::vector<ChunkOut> couts(chunks.size());
+        std::vector<ItemOut> iouts(nitems);
         std::mutex cmu;
         std::condition_variable ccv;
-        // chunk c's table and searches on a lane (lc: plus strand, on this
-        // thread; lh: the other strand, on the lane's helper thread)
-        auto search_chunk = [&](size_t c, gxy_ctx *lc, gxy_ctx *lh, HostJobs &hj) -> int {
-            const Chunk &ch = chunks[c];
-            ChunkOut &co = couts[c];
-            co.shs.assign((size_t)n_queries * 2, {});
-            co.sst.assign((size_t)n_queries * 2, gxy_search_stats{});
-            gxy_postable *tp;
-            PhaseClock lclk(lc);
-            int rc = postable_build_impl(lc, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
-            lclk.lap(kTrTable);
-            if (rc) return rc;
-            Table table(tp);
-            if (cfg->max_word_count > 0) {
-                rc = gxy_postable_prune(lc, tp, cfg->max_word_count);
-                if (rc) return rc;
-            }
-            for (uint32_t qi = 0; qi < n_queries; qi++) {
-                int src[2] = {GXY_OK, GXY_OK};
-                auto run_search = [&](size_t k, gxy_ctx *sc) {
-                    const uint32_t strand = strands[k];
-                    const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
-                    const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
-                    const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
-                    src[k] = search_dev_impl(sc, tp, qs, &cfg->scoring, &sp, respect, &co.shs[2 * qi + k],
-                                             &co.sst[2 * qi + k], qb, cfg->gapped_enabled != 0);
-                };
-                if (lh) {
-                    hj.submit([&] {
-                        (void)hipSetDevice(lc->device);
-                        run_search(1, lh);
-                    });
-                    run_search(0, lc);
-                    hj.wait_all();
-                    if (!src[0] && src[1]) return set_err(lc, src[1], "%s", lh->err.c_str());
-                } else {
-                    for (size_t k = 0; k < strands.size() && !src[0]; k++) run_search(k, lc);
-                }
-                if (src[0]) return src[0];
-            }
-            return GXY_OK;  // the chunk's table is released here; gapped work needs only the sequences
+        // A lane's table: built for the chunk of its next item when that
+        // chunk changes (the chunk's table is released then; gapped work needs
+        // only the sequences).
+        struct LaneTable {
+            Table table;
+            size_t chunk = SIZE_MAX;
         };
-        // the searches' HSPs of chunk c, taken in chunk order
-        auto take_chunk = [&](size_t c) {
+        // item k = (chunk k / n_queries, query k % n_queries) on a lane (lc:
+        // plus strand, on this thread; lh: the other strand, on the lane's
+        // helper thread)
+        auto search_item = [&](size_t k, gxy_ctx *lc, gxy_ctx *lh, HostJobs &hj, LaneTable &lt) -> int {
+            const size_t c = k / n_queries;
+            const uint32_t qi = (uint32_t)(k % n_queries);
             const Chunk &ch = chunks[c];
-            ChunkOut &co = couts[c];
-            for (uint32_t qi = 0; qi < n_queries; qi++) {
+            ItemOut &io = iouts[k];
+            if (lt.chunk != c) {
+                lt.table.reset();
+                gxy_postable *tp;
+                PhaseClock lclk(lc);
+                int rc = postable_build_impl(lc, ch.seq, &cfg->pattern, cfg->step, respect, &tp);
+                lclk.lap(kTrTable);
+                if (rc) return rc;
+                lt.table.reset(tp);
+                lt.chunk = c;
+                if (cfg->max_word_count > 0) {
+                    rc = gxy_postable_prune(lc, tp, cfg->max_word_count);
+                    if (rc) return rc;
+                }
+            }
+            gxy_postable *tp = lt.table.get();
+            int src[2] = {GXY_OK, GXY_OK};
+            auto run_search = [&](size_t s2, gxy_ctx *sc) {
+                const uint32_t strand = strands[s2];
+                const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
+                const int si = strand == GXY_STRAND_PLUS ? 0 : 1;
+                const uint8_t *qb = byte_ok ? qbytes[2 * qi + si].p + kBytePad : nullptr;
+                src[s2] = search_dev_impl(sc, tp, qs, &cfg->scoring, &sp, respect, &io.shs[s2], &io.sst[s2], qb,
+                                          cfg->gapped_enabled != 0);
+            };
+            if (lh) {
+                hj.submit([&] {
+                    (void)hipSetDevice(lc->device);
+                    run_search(1, lh);
+                });
+                run_search(0, lc);
+                hj.wait_all();
+                if (!src[0] && src[1]) return set_err(lc, src[1], "%s", lh->err.c_str());
+            } else {
+                for (size_t s2 = 0; s2 < strands.size() && !src[0]; s2++) run_search(s2, lc);
+            }
+            return src[0];
+        };
+        // the searches' HSPs of item k, taken in item order
+        auto take_item = [&](size_t ik) {
+            const size_t c = ik / n_queries;
+            const Chunk &ch = chunks[c];
+            ItemOut &io = iouts[ik];
+            {
+                const uint32_t qi = (uint32_t)(ik % n_queries);
                 for (size_t k = 0; k < strands.size(); k++) {
                     const uint32_t strand = strands[k];
                     const gxy_dseq *qs = strand == GXY_STRAND_PLUS ? queries[qi] : drc[qi].get();
-                    std::vector<gxy_hsp> hsps = std::move(co.shs[2 * qi + k]);
-                    const gxy_search_stats &st = co.sst[2 * qi + k];
+                    std::vector<gxy_hsp> hsps = std::move(io.shs[k]);
+                    const gxy_search_stats &st = io.sst[k];
                     rs.hits += st.hits;
                     rs.columns += st.columns;
                     rs.hsps += st.emitted;
@@ -967,29 +981,32 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                     }
                 }
             }
-            co = ChunkOut{};
+            io.shs[0] = {};
+            io.shs[1] = {};
         };
 
         clk.lap(kTrSetup);
         if (!lane) {
             HostJobs helper_job(helper ? 1 : 0);
-            for (size_t c = 0; c < chunks.size(); c++) {
-                const int rc = search_chunk(c, ctx, helper, helper_job);
+            LaneTable lt;
+            for (size_t k = 0; k < nitems; k++) {
+                const int rc = search_item(k, ctx, helper, helper_job, lt);
                 if (rc) return rc;
-                take_chunk(c);
+                take_item(k);
             }
         } else {
-            // lane 0 (ctx, its helper) takes the even chunks, lane 1 the odd ones
+            // lane 0 (ctx, its helper) takes the even items, lane 1 the odd ones
             std::atomic<bool> abort_lanes{false};
             auto lane_loop = [&](int L, gxy_ctx *lc, gxy_ctx *lh) {
                 (void)hipSetDevice(lc->device);
                 HostJobs hj(1);
-                for (size_t c = (size_t)L; c < chunks.size() && !abort_lanes.load(); c += 2) {
-                    const int rc = search_chunk(c, lc, lh, hj);
+                LaneTable lt;
+                for (size_t k = (size_t)L; k < nitems && !abort_lanes.load(); k += 2) {
+                    const int rc = search_item(k, lc, lh, hj, lt);
                     {
                         std::lock_guard<std::mutex> lk(cmu);
-                        couts[c].rc = rc;
-                        couts[c].done = true;
+                        iouts[k].rc = rc;
+                        iouts[k].done = true;
                     }
                     ccv.notify_all();
                     if (rc) break;
@@ -997,26 +1014,26 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             };
             std::thread t0(lane_loop, 0, ctx, helper), t1(lane_loop, 1, lane, lane_helper);
             int rc = GXY_OK;
-            for (size_t c = 0; c < chunks.size(); c++) {
+            for (size_t k = 0; k < nitems; k++) {
                 {
                     std::unique_lock<std::mutex> lk(cmu);
-                    ccv.wait(lk, [&] { return couts[c].done; });
-                    rc = couts[c].rc;
+                    ccv.wait(lk, [&] { return iouts[k].done; });
+                    rc = iouts[k].rc;
                 }
                 if (rc) {
-                    if (c % 2) set_err(ctx, rc, "%s", lane->err.c_str());
+                    if (k % 2) set_err(ctx, rc, "%s", lane->err.c_str());
                     abort_lanes = true;
-                    // a lane stopped by abort_lanes leaves later chunks undone
-                    for (size_t d = c + 1; d < chunks.size(); d++) {
+                    // a lane stopped by abort_lanes leaves later items undone
+                    for (size_t d = k + 1; d < nitems; d++) {
                         std::lock_guard<std::mutex> lk(cmu);
-                        if (!couts[d].done) {
-                            couts[d].done = true;
-                            couts[d].rc = rc;
+                        if (!iouts[d].done) {
+                            iouts[d].done = true;
+                            iouts[d].rc = rc;
                         }
                     }
                     break;
                 }
-                take_chunk(c);
+                take_item(k);
             }
             t0.join();
             t1.join();
