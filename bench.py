@@ -655,9 +655,17 @@ def pair_leg(args, workload, steps, warmup):
         return {}
 
     def with_serialized(r, prof):
+        # the summary's per-launch figures hold only for launches of the same
+        # size (the same hit batch): otherwise they are left out
+        pb, lb = prof.get("algorithmic_bytes_per_launch"), r.get("algorithmic_bytes_per_launch")
+        if not pb or not lb or abs(pb - lb) > 0.02 * pb:
+            r["traffic"] = None
+            if prof:
+                r["profile_mismatch"] = f"{prof.get('source')} was taken on launches of {pb} algorithmic bytes"
+            return r
         r["traffic"] = prof.get("traffic_bytes_per_launch")
         ser = prof.get("avg_ns_serialized")
-        if ser and r.get("algorithmic_bytes_per_launch"):
+        if ser:
             # the same bytes over the kernel's own duration (no other stream on the device)
             r["serialized"] = {"avg_launch_ms": round(ser / 1e6, 4),
                                "achieved": round(r["algorithmic_bytes_per_launch"] / (ser / 1e9) / 1e9, 2),

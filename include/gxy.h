@@ -239,6 +239,13 @@ int gxy_ctx_dp_persist_stats(gxy_ctx *ctx, uint64_t *launches, uint64_t *sides, 
  * hits whose walks go on; 1: the persistent kernel alone, one 32-column chunk
  * of one walk per lane per step.  Results do not depend on it. */
 int gxy_ctx_set_xdrop_form(gxy_ctx *ctx, int form);
+/* Seed hits per expansion batch of a search (no counterpart in the reference,
+ * which extends each hit as it is found: src/seed_search.rs:95-113).  A
+ * search whose hits exceed the batch runs them in several batches of whole
+ * query-window blocks, each sized to about `hits` (0: the default, 2^27), which
+ * bounds its scratch memory to a few tens of bytes per batch hit.  Results do
+ * not depend on it. */
+int gxy_ctx_set_hit_batch(gxy_ctx *ctx, uint64_t hits);
 /* First steps of the x-drop (form 0) since the context was created, the
  * minus-strand helper's included: columns they evaluated, hits they handed on
  * to be resumed and passing extensions they finished. */

@@ -403,6 +403,8 @@ struct gxy_ctx {
     // x-drop in one persistent kernel per hit batch instead of a first step
     // for every hit and a kernel resuming the longer walks (for measurement)
     bool xdrop_single = false;
+    // hits per expansion batch of a search (0: the default, 2^27)
+    uint64_t hit_batch = 0;
     // first steps since creation: x-drop columns evaluated, hits handed on to
     // the resuming kernel, passing extensions finished
     uint64_t xd_first_cols = 0, xd_conts = 0, xd_first_pass = 0;

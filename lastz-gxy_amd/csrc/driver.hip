@@ -826,6 +826,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
             if (helper) {
                 helper->timing = ctx->timing;
                 helper->xdrop_single = ctx->xdrop_single;
+                helper->hit_batch = ctx->hit_batch;
             }
         }
         // A second lane (its own context and helper, so four streams in all)
@@ -853,6 +854,7 @@ int gxy_run_dev(gxy_ctx *ctx, const gxy_dseq *const *targets, uint64_t n_targets
                 for (gxy_ctx *c : {lane, lane_helper}) {
                     c->timing = ctx->timing;
                     c->xdrop_single = ctx->xdrop_single;
+                    c->hit_batch = ctx->hit_batch;
                 }
                 // the chunk sequences, reverse complements and query byte copies
                 // were written on ctx's stream

@@ -1454,9 +1454,13 @@ int search_dev_impl(gxy_ctx *ctx, const gxy_postable *tab, const gxy_dseq *query
     uint64_t npass = 0;
     clk.lap(kTrSearchOther);
     if (H > 0) {
-        // Hits are expanded and extended in batches of at most kCap.
-        const uint64_t kCap = 1ull << 27;
-        const uint32_t nb = (uint32_t)std::max<uint64_t>(1, (H + kCap / 2 - 1) / (kCap / 2));
+        // Hits are expanded and extended in batches of about `half` hits
+        // (whole 256-window blocks; under 2 half): one batch for a search of
+        // up to 2^27 hits, whose launches run larger and fewer.
+        const uint64_t half = ctx->hit_batch ? ctx->hit_batch : (1ull << 27);
+        // (no more batches than window blocks, and at most 2^20)
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(
+            std::min<uint64_t>(std::max<uint64_t>(1, (H + half - 1) / half), std::max<uint64_t>(noff, 1)), 1u << 20);
         std::vector<uint64_t> bounds(nb + 1);  // in units of unit_win windows
         if (nb == 1) {
             bounds[0] = 0;

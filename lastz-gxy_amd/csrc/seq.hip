@@ -537,6 +537,12 @@ int gxy_ctx_set_xdrop_form(gxy_ctx *ctx, int form) {
     return GXY_OK;
 }
 
+int gxy_ctx_set_hit_batch(gxy_ctx *ctx, uint64_t hits) {
+    if (!ctx) return GXY_EINVAL;
+    ctx->hit_batch = hits;
+    return GXY_OK;
+}
+
 int gxy_ctx_xdrop_stats(gxy_ctx *ctx, uint64_t *first_step_columns, uint64_t *resumed, uint64_t *first_step_passing) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !first_step_columns || !resumed || !first_step_passing) return GXY_EINVAL;

@@ -39,6 +39,7 @@ _SIGS = {
     "gxy_ctx_dp_pool_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_set_dp_persist": (C.c_int, [_vp, C.c_int]),
     "gxy_ctx_set_xdrop_form": (C.c_int, [_vp, C.c_int]),
+    "gxy_ctx_set_hit_batch": (C.c_int, [_vp, C.c_uint64]),
     "gxy_ctx_xdrop_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_dp_persist_stats": (C.c_int, [_vp, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "gxy_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _P(C.c_double), _P(C.c_uint64)]),
@@ -188,6 +189,10 @@ class Context:
     def set_xdrop_form(self, form: int):
         """x-drop form of the search: 0 first step + resumed walks (default), 1 one persistent kernel."""
         self.check(lib().gxy_ctx_set_xdrop_form(self._h, form), "gxy_ctx_set_xdrop_form")
+
+    def set_hit_batch(self, hits: int):
+        """Seed hits per expansion batch of a search (0: the default, 2^27)."""
+        self.check(lib().gxy_ctx_set_hit_batch(self._h, hits), "gxy_ctx_set_hit_batch")
 
     def xdrop_stats(self):
         """First steps of the x-drop since creation: (columns evaluated, hits

@@ -225,6 +225,47 @@ def test_search_xdrop_forms_agree(trans, tlen, qlen):
     assert h0 == oh and s0.emitted == ost.emitted
 
 
+@pytest.mark.parametrize("trans", [1, 2])
+def test_search_in_many_hit_batches(trans):
+    """A search whose hits exceed the context's hit batch runs them in several
+    batches (gxy_ctx_set_hit_batch): the passing extensions of each batch are
+    kept across batches and admitted together, so HSPs and statistics equal one
+    batch's and the oracle's; the driver's helper and lane contexts take the
+    setting too."""
+    say("hit batches, transitions", trans)
+    rng = np.random.default_rng(515 + trans)
+    t, q = homologous_pair(rng, 150000, 0.12, 0.01, 0.2)
+    pat = T.pattern_from_spec("12of19")
+    sp = T.SearchParams(1, T.HspParams(910, 2000), trans, 0, 2000)
+    res = {}
+    for hb in (0, 100):
+        c = gxy.Context(0)
+        c.set_hit_batch(hb)
+        tab = gxy.PosTable.build(t, pat, 1, c)
+        res[hb] = gxy.search(tab, t, q, HOX, sp)
+        c.close()
+    (h0, s0), (h1, s1) = res[0], res[100]
+    assert s0.hits > 10 * 100
+    assert h0 == h1
+    assert (s0.hits, s0.columns, s0.passing, s0.emitted) == (s1.hits, s1.columns, s1.passing, s1.emitted)
+    oh, ost = O.search(O.PosTable(t, pat, 1), t, q, HOX, sp)
+    assert h0 == oh and s0.emitted == ost.emitted
+    # through the driver: two chunks, both strands, HSP-only
+    targets, queries = [("t", t)], [("q", q)]
+    cfg = T.config_default()
+    cfg.gapped_enabled = 0
+    cfg.chunk_primary_bp = 80_000
+    cfg.chunk_halo_bp = 5_000
+    recs = {}
+    for hb in (0, 100):
+        c = gxy.Context(0)
+        c.set_hit_batch(hb)
+        recs[hb] = gxy.run(targets, queries, cfg, c)[0].tuples()
+        c.close()
+    assert recs[0] == recs[100]
+    assert recs[0] == O.run(targets, queries, cfg)[0].tuples()
+
+
 def test_search_dense_table(ctx):
     """A dense table (match8 on 300 kbp unmasked: ~4.6 positions per word, ~9
     non-empty variant buckets per window) overflows the expansion kernel's
