@@ -138,6 +138,18 @@ void dev_free(gxy_ctx *owner, void *p) {
     owner->pool_used.erase(it);
 }
 
+// Returns the device memory pool's unused memory to the device: stream-ordered
+// frees (hipFreeAsync) leave it in the pool, where allocations of other
+// processes, and plain allocations of this one, cannot use it.
+static void trim_device_pool(gxy_ctx *ctx) {
+    if (!ctx->async_alloc) return;
+    hipMemPool_t mp;
+    if (hipDeviceGetDefaultMemPool(&mp, ctx->device) == hipSuccess)
+        (void)hipMemPoolTrimTo(mp, 0);
+    else
+        (void)hipGetLastError();
+}
+
 void pool_trim(gxy_ctx *ctx) {
     if (ctx->pool_free.empty()) return;
     (void)hipStreamSynchronize(ctx->stream);
@@ -471,6 +483,7 @@ int gxy_ctx_release_cache(gxy_ctx *ctx) {
             pool_trim(c);
             if (c->stream) GXY_HIP(ctx, hipStreamSynchronize(c->stream));
         }
+        trim_device_pool(ctx);
         return GXY_OK;
     });
 }
@@ -591,6 +604,7 @@ void gxy_ctx_destroy(gxy_ctx *ctx) {
         if (ctx->slot[k]) (void)(ctx->async_alloc ? hipFreeAsync(ctx->slot[k], ctx->stream) : hipFree(ctx->slot[k]));
     (void)hipStreamSynchronize(ctx->stream);
     pool_trim(ctx);
+    trim_device_pool(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);  // a run's helper may hold none between runs
     delete ctx;
