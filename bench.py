@@ -859,6 +859,11 @@ def main():
                                     f"about {LEG_ESTIMATE_S.get(name, 60)} s"}
             log(f"{name}: skipped ({left:.0f} s left)")
             continue
+        # each configuration starts on a device without the scratch memory the
+        # previous one left cached in the default context (its lanes' search
+        # buffers, its traceback pool): gxy_ctx_release_cache
+        import gxy
+        gxy.default_context().release_cache()
         t1 = time.time()
         key = {"dp": "batched_dp"}.get(name, name)
         if name == "pair30k":

@@ -176,7 +176,7 @@ int gxy_ctx_reset_timing(gxy_ctx *ctx);
 /* Traceback-pool cap of the gapped extension (no counterpart in the
  * reference, whose DP keeps every row of a side: src/gapped_extend.rs:233-242).
  * The library keeps the from-states of the rows being swept in a pool of
- * 64-row chunks, by default up to half the device (at most 144 GiB).
+ * 64-row chunks, by default up to a third of the device (at most 96 GiB).
  * With rows > 0 the pool holds at most max(16, rows / 64) chunks: sides that
  * find the pool empty stop and are swept again in a later pass, a pass that
  * finishes nothing is followed by one that runs its first side alone, and a
@@ -185,7 +185,7 @@ int gxy_ctx_reset_timing(gxy_ctx *ctx);
 int gxy_ctx_set_dp_pool_rows(gxy_ctx *ctx, uint64_t rows);
 /* Returns the context's cached device memory to the device (no counterpart in
  * the reference): the scratch buffers that calls grow and keep for the next
- * call — the traceback pool among them, up to half the device after a
+ * call — the traceback pool among them, up to a third of the device after a
  * large gapped batch — and freed sequence blocks kept for reuse.  Sequences,
  * tables and results stay valid; the next call allocates what it needs again. */
 int gxy_ctx_release_cache(gxy_ctx *ctx);

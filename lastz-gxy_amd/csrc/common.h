@@ -375,7 +375,7 @@ struct gxy_ctx {
     // A second context (with its own helper) for the driver's second chunk
     // lane (created on first use, destroyed with this one).
     gxy_ctx *lane = nullptr;
-    uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(144 GiB, device / 2))
+    uint64_t dp_pool_cap = 0;  // traceback pool bytes (0: min(96 GiB, device / 3))
     uint64_t dp_pool_rows = 0; // traceback pool rows set by the caller (0: no row cap)
     int dp_kernel = 0;         // DP forward kernel (GXY_DP_*), 0: chosen by the library
     // DP autotuner (gapped.hip): per class of pass size, the forward kernel

@@ -2100,12 +2100,12 @@ int gapped_batch_dev(gxy_ctx *ctx, const std::vector<GappedItem> &items, const g
         // longest sides first, so the long sweeps start early instead of
         // forming the tail of a pass
         longest_first(pending);
-        // The traceback pool: up to half the device (at most 144 GiB),
+        // The traceback pool: up to a third of the device (at most 96 GiB),
         // no more than every side could use.  Its size bounds how many long
         // sweeps run at once (rows in flight x kRowBytes).
         size_t free_b = 0, total_b = 0;
         GXY_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-        const uint64_t cap_b = ctx->dp_pool_cap ? ctx->dp_pool_cap : std::min<uint64_t>(144ull << 30, total_b / 2);
+        const uint64_t cap_b = ctx->dp_pool_cap ? ctx->dp_pool_cap : std::min<uint64_t>(96ull << 30, total_b / 3);
         // a row cap set by the caller (gxy_ctx_set_dp_pool_rows) bounds the pool
         // even where the slot is larger from earlier calls
         const uint64_t row_cap = ctx->dp_pool_rows ? std::max<uint64_t>(16, ctx->dp_pool_rows / kChunkRows) : ~0ull;
